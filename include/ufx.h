@@ -1,0 +1,218 @@
+/*
+ * ufx.h -- C-ABI of the MI355X (gfx950) residual hot path of UNSFlow2D's
+ * cell-centred finite-volume solver.
+ *
+ * The reference's hot path is a set of parameterless Fortran subroutines that
+ * read and write module globals (CCFVM/types/grid.F90:7-77 and friends).  This
+ * interface replaces them one for one: a context owns the device copy of that
+ * state, every entry point named after a reference routine performs that
+ * routine on the device, and the state can be injected and read back between
+ * calls (ufx_cc_set / ufx_cc_get) so a Fortran host can interleave reference
+ * and device routines.  Plain pointers and sizes only; no torch types.
+ *
+ * Conventions
+ *  - Return value: UFX_OK (0) or a positive error code; ufx_last_error() gives
+ *    the message (thread-local).  Errors the reference reports with STOP are
+ *    returned as codes; device-detected ones (negative Roe sound speed,
+ *    non-positive density/pressure) surface at the next synchronising call.
+ *  - Index arrays passed in are 1-based, in the reference's numbering; arrays
+ *    read back with ufx_mesh_get are 1-based as well.
+ *  - All calls on a context are asynchronous on the context's own HIP stream
+ *    except those documented as synchronising.  A context is not thread-safe.
+ *  - Host pointers are owned by the caller and copied; device memory is owned
+ *    by the context.
+ */
+#ifndef UFX_H
+#define UFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UFX_OK 0
+#define UFX_ERR_ARG 1          /* invalid argument or unknown name */
+#define UFX_ERR_MESH 2         /* mesh rejected (the reference STOPs here) */
+#define UFX_ERR_HIP 3          /* HIP runtime failure */
+#define UFX_ERR_STATE 4        /* device-detected invalid flow state */
+#define UFX_ERR_UNSUPPORTED 5  /* option outside what this build implements */
+
+#define UFX_KIND_CC 1
+
+/* flux_type, CCFVM/read_input.F90:93-98 */
+#define UFX_FLUX_ROE 1
+#define UFX_FLUX_AUSM 2
+#define UFX_FLUX_VANLEER 3
+#define UFX_FLUX_RUSANOV 4
+/* grad_type, CCFVM/read_input.F90:87-91 */
+#define UFX_GRAD_LSQR 1
+#define UFX_GRAD_GG 2
+#define UFX_GRAD_GGFC 3
+/* flow_type, CCFVM/read_input.F90:61-66 */
+#define UFX_FLOW_INVISCID 1
+#define UFX_FLOW_LAMINAR 2
+/* timemode, CCFVM/read_input.F90:74-79 */
+#define UFX_TIME_RK3 1
+#define UFX_TIME_LUSGS 2
+/* arithmetic of the bulk flux/gradient/limiter work */
+#define UFX_PREC_FP64 0
+#define UFX_PREC_MIXED 1
+
+typedef struct ufx_mesh ufx_mesh;
+typedef struct ufx_cc ufx_cc;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* ufx_last_error(void);
+
+/* ------------------------------------------------------------------ mesh */
+
+/* Preprocess a geometry.inp mesh into the reference's post-`geometric` state.
+ * Replaces read_grid's ghost mirror (CCFVM/read_input.F90:179-203) and
+ * geometric (CCFVM/geometric.F90:1-57): face normals and orientation,
+ * breadth-first cell renumbering, faces-of-cell order, vertex->cell lists,
+ * co-volumes, least-squares factors and pseudo-Laplacian weights.
+ * Inputs follow CCFVM/read_input.F90:160-173: nop vertices (px, py), noc cells
+ * (centroid ccx, ccy and area cv, used as given), nfi interior faces
+ * (fi_p1, fi_p2, fi_in, fi_out; fi_bc may be NULL) and nbf boundary faces
+ * (fb_p1, fb_p2, fb_in, fb_nx, fb_ny, fb_bc). */
+int ufx_cc_geometric(int nop, int noc, int nfi, int nbf,
+                     const double* px, const double* py,
+                     const double* ccx, const double* ccy, const double* cv,
+                     const int* fi_p1, const int* fi_p2, const int* fi_in,
+                     const int* fi_out, const int* fi_bc,
+                     const int* fb_p1, const int* fb_p2, const int* fb_in,
+                     const double* fb_nx, const double* fb_ny, const int* fb_bc,
+                     ufx_mesh** mesh);
+
+/* Build a mesh directly from arrays the reference holds after its own
+ * `geometric` (CCFVM/types/grid.F90 fields fc%pt,in,out,bc,nx,ny,area,cen,cov;
+ * cell%cen,cv,cov,r11,r12,r22,nc2c,c2f; pt%x,y,v2c,wt).  nof = nfi + nbf
+ * faces, ncell = noc + nbf cells (ghosts last).  c2f_ptr has noc+1 entries
+ * (0-based offsets), v2c_ptr nop+1.  Lets a Fortran host hand over the
+ * reference's own preprocessing unchanged. */
+int ufx_cc_mesh_import(int nop, int noc, int nfi, int nbf,
+                       const double* px, const double* py,
+                       const int* fc_p1, const int* fc_p2, const int* fc_in,
+                       const int* fc_out, const int* fc_bc,
+                       const double* fc_nx, const double* fc_ny, const double* fc_area,
+                       const double* fc_cenx, const double* fc_ceny, const double* fc_cov,
+                       const double* cell_cenx, const double* cell_ceny,
+                       const double* cell_cv, const double* cell_cov,
+                       const double* cell_r11, const double* cell_r12, const double* cell_r22,
+                       const int* cell_nc2c, const int* c2f_ptr, const int* c2f,
+                       const int* v2c_ptr, const int* v2c, const double* v2c_wt,
+                       ufx_mesh** mesh);
+
+/* Element count of a named mesh array (see ufx_mesh_get). */
+int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count);
+
+/* Copy a named mesh array out.  int32 arrays: "fc_p1","fc_p2","fc_in",
+ * "fc_out","fc_bc","cell_nc2c","cell_nc2f","cell_c2f","pt_nv2c","pt_v2c",
+ * "oldnum","counts" (nop,noc,nof,nbf); float64 arrays: "fc_nx","fc_ny",
+ * "fc_area","fc_cenx","fc_ceny","fc_cov","cell_cenx","cell_ceny","cell_cv",
+ * "cell_cov","cell_r11","cell_r12","cell_r22","pt_x","pt_y","pt_wt".
+ * Indices are returned 1-based. */
+int ufx_mesh_get(const ufx_mesh* mesh, const char* name, void* dst);
+
+void ufx_mesh_destroy(ufx_mesh* mesh);
+
+/* --------------------------------------------------------- solver context */
+
+/* Run parameters of the input deck (CCFVM/read_input.F90:31-50). */
+typedef struct {
+  double mach;         /* Mach */
+  double aoa_deg;      /* AOA in degrees */
+  double reynolds;     /* Reynolds (laminar only) */
+  double t_wall;       /* FlowType wall temperature (isothermal wall, K) */
+  int flow_type;       /* UFX_FLOW_* */
+  int iwall;           /* 0 isothermal, 1 adiabatic no-slip wall */
+  int flux;            /* UFX_FLUX_* */
+  int grad;            /* UFX_GRAD_* */
+  int ilimit;          /* 1: Venkatakrishnan limiter after the gradient */
+  int irs;             /* 1: central implicit residual smoothing */
+  int timemode;        /* UFX_TIME_* */
+  int precision;       /* UFX_PREC_* */
+  double cfl_max;      /* needed by residual smoothing (CCFVM/ImpResiSmoothening.F90:17) */
+  double xref, yref;   /* moment reference point (FarBC line) */
+} ufx_cc_params;
+
+/* Create a context on the current HIP device: uploads the mesh in SoA layout
+ * and allocates the solution state.  The state starts as the reference's
+ * `initialize` leaves it (CCFVM/solution.F90:18-84): free stream in every
+ * cell's qc including ghosts, limiter value 1, ghost primitives 0. */
+int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* params, ufx_cc** ctx);
+int ufx_cc_destroy(ufx_cc* ctx);
+
+/* Inject / read back a named state array (host <-> device, synchronising).
+ * Layout is cell-major with the variables of one cell contiguous, exactly as
+ * the reference's per-cell records hold them:
+ *   "qc"   4 x ncell  conserved [E, rho, rho u, rho v] (ghosts last)
+ *   "qp"   5 x ncell  primitives [rho, u, v, p, T]
+ *   "qold" 4 x noc    "res" 4 x noc   "phi" 4 x noc
+ *   "grad" 10 x noc   (d/dx, d/dy) per variable, variable-major
+ *   "dt","la" noc;  "fc_la" nof;  "pt_prim" 5 x nop;  "fc_qp" 5 x nof
+ *   "fc_mu" nof;  "cell_mul" ncell;  "fc_grad" 10 x nof (viscous)
+ * Not every name is writable ("fc_qp" is derived). */
+int ufx_cc_set(ufx_cc* ctx, const char* name, const double* src);
+int ufx_cc_get(ufx_cc* ctx, const char* name, double* dst);
+
+/* ---- the routines of the hot path, one entry per reference routine ---- */
+
+/* Interior primitive refresh, CCFVM/main.F90:62-67 (con2prim, types/pri.F90:13-37). */
+int ufx_cc_prim(ufx_cc* ctx);
+/* Vertex and face averages of the primitives, CCFVM/average.F90:1-39. */
+int ufx_cc_avg_c2v(ufx_cc* ctx);
+/* Face spectral radii and local time step, CCFVM/timestep.F90:98-177. */
+int ufx_cc_time_step(ufx_cc* ctx, double cfl);
+/* qold = qc, CCFVM/solution.F90:118-132. */
+int ufx_cc_save_old(ufx_cc* ctx);
+/* Gradient of the primitives by the context's method (CCFVM/gradient.F90:
+ * LSQR :2-93, GG :95-156, GG_FC :159-323), followed by the limiter when
+ * ilimit is set, as the reference does inside each gradient routine. */
+int ufx_cc_gradient(ufx_cc* ctx);
+/* Venkatakrishnan limiter with the ratchet, CCFVM/limiter.F90:3-17,123-296. */
+int ufx_cc_limit(ufx_cc* ctx);
+/* Finite-volume residual, CCFVM/fvresidual.F90:4-187 (boundary faces first,
+ * then interior faces, then viscous flux and residual smoothing). */
+int ufx_cc_fvresidual(ufx_cc* ctx);
+/* Runge-Kutta stage update of CCFVM/main.F90:92-101 for stage 1..3. */
+int ufx_cc_rk_update(ufx_cc* ctx, int stage);
+/* fvresidual followed by the stage update in one pass (same results). */
+int ufx_cc_rk_stage(ufx_cc* ctx, int stage);
+/* Matrix-free LU-SGS step, CCFVM/lusgs.F90:5-10,207-379 (includes its
+ * fvresidual call). */
+int ufx_cc_lusgs(ufx_cc* ctx);
+/* L2 residual norm, CCFVM/solution.F90:136-180; synchronising.  `iter` == 1
+ * stores the normalisation as the reference does; *fres receives the
+ * normalised value and *fres_raw the unnormalised one (either may be NULL). */
+int ufx_cc_residue(ufx_cc* ctx, int iter, double* fres, double* fres_raw);
+/* Positivity check of the interior primitives, CCFVM/main.F90:358-414;
+ * synchronising.  *bad_cell is 0 when all cells pass, else the 1-based
+ * index of the first failing cell (and UFX_ERR_STATE is returned). */
+int ufx_cc_check_positivity(ufx_cc* ctx, int* bad_cell);
+/* Force coefficients of CCFVM/main.F90:263-355 (find_cl_cd); synchronising.
+ * out[0..7] = cl, cd (pressure integral), cl1, cd1 (flux-based, what the
+ * reference prints), cm, fx, fy, dtglobal.  Optional cp (nwall) receives -Cp
+ * per wall face in boundary order; cf likewise (NULL to skip). */
+int ufx_cc_forces(ufx_cc* ctx, double* out, double* cp, double* cf);
+
+/* One or more whole iterations of CCFVM/main.F90:57-128 on the device:
+ * prim refresh, avg_c2v, time step, save_old, gradient (+limiter), then three
+ * RK stages or one LU-SGS step, then the residual norm.  `iter0` is the
+ * reference iteration counter before the first one; cfl[k] is the CFL of
+ * iteration iter0+1+k (the host evaluates the reference's ramp).  hist, if
+ * not NULL, receives the normalised residual of each iteration; that makes the
+ * call synchronising.  Without hist no host synchronisation happens. */
+int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double* hist);
+
+/* Block until all queued work has finished; returns the first device-detected
+ * error, if any. */
+int ufx_cc_sync(ufx_cc* ctx);
+/* The context's HIP stream (hipStream_t), for timing with HIP events. */
+void* ufx_cc_stream(ufx_cc* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UFX_H */

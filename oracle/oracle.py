@@ -1,0 +1,180 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only the tests, the smoke check and the benchmark's CPU-baseline leg use this
+module, as the checker.  The solver state lives in numpy arrays in the
+reference's per-cell record layout (variables of one cell contiguous).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    return C.CDLL(LIB_PATH)
+
+
+_lib = _load()
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class _State(C.Structure):
+    _fields_ = (
+        [(n, C.c_int) for n in ("nop", "noc", "nfi", "nbf")]
+        + [(n, _dp) for n in ("px", "py")]
+        + [(n, _ip) for n in ("fp1", "fp2", "fin", "fout", "fbc")]
+        + [(n, _dp) for n in ("fnx", "fny", "farea", "fcx", "fcy")]
+        + [(n, _dp) for n in ("ccx", "ccy", "ccv", "ccov", "r11", "r12", "r22")]
+        + [("nc2c", _ip), ("c2f_ptr", _ip), ("c2f", _ip), ("v2c_ptr", _ip), ("v2c", _ip),
+           ("wt", _dp)]
+        + [(n, C.c_double) for n in ("mach", "aoa_deg", "reynolds", "t_wall")]
+        + [(n, C.c_int) for n in ("flow_type", "iwall", "flux", "grad_type", "ilimit", "irs")]
+        + [(n, C.c_double) for n in ("cfl", "cfl_max")]
+        + [(n, _dp) for n in ("qc", "qp", "qold", "res", "grad", "phi", "dt", "la", "fla", "fmu",
+                              "fqp", "fgrad", "cmul", "pprim", "pmu")]
+        + [(n, C.c_double) for n in ("fx1", "fy1", "dtglobal", "fres", "fres1")]
+        + [("error", C.c_int), ("error_cell", C.c_int)]
+    )
+
+
+for _n in ("initialize", "prim", "avg_c2v", "time_step", "save_old", "gradient", "limit",
+           "fvresidual", "lusgs"):
+    getattr(_lib, f"orc_cc_{_n}").argtypes = [C.POINTER(_State)]
+_lib.orc_cc_rk_update.argtypes = [C.POINTER(_State), C.c_int]
+_lib.orc_cc_residue.argtypes = [C.POINTER(_State), C.c_int]
+_lib.orc_cc_check_positivity.argtypes = [C.POINTER(_State)]
+_lib.orc_cc_iteration.argtypes = [C.POINTER(_State), C.c_int, C.c_double, C.c_int]
+_lib.orc_flux.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
+_lib.orc_farfield_state.argtypes = [C.c_double, C.c_double, _dp, C.c_double, C.c_double, _dp]
+
+FLUX = {"roe": 1, "ausm": 2, "vanleer": 3, "rusanov": 4}
+GRAD = {"lsqr": 1, "gg": 2, "ggfc": 3}
+
+
+class CCOracle:
+    """Reference-order CPU restatement of the cell-centred hot path.
+
+    ``mesh`` is the dict of post-``geometric`` arrays (1-based indices, as
+    ``unsflow2d_amd.cc.Mesh.arrays()`` or the reference's own dump gives).
+    """
+
+    def __init__(self, mesh: dict, *, mach=0.5, aoa_deg=0.0, reynolds=1000.0, t_wall=300.0,
+                 flow_type="inviscid", iwall=1, flux="roe", grad="lsqr", ilimit=1, irs=0,
+                 cfl_max=0.8):
+        i0 = lambda n: np.ascontiguousarray(mesh[n], dtype=np.int32) - 1
+        i1 = lambda n: np.ascontiguousarray(mesh[n], dtype=np.int32)
+        f = lambda n: np.ascontiguousarray(mesh[n], dtype=np.float64)
+        self.nop = len(mesh["pt_x"])
+        self.noc = len(mesh["cell_cv"])
+        nof = len(mesh["fc_p1"])
+        self.ncell = len(mesh["cell_cenx"])
+        self.nbf = self.ncell - self.noc
+        self.nfi = nof - self.nbf
+        self.nof = nof
+        c2f_ptr = np.zeros(self.noc + 1, dtype=np.int32)
+        np.cumsum(mesh["cell_nc2f"], out=c2f_ptr[1:])
+        v2c_ptr = np.zeros(self.nop + 1, dtype=np.int32)
+        np.cumsum(mesh["pt_nv2c"], out=v2c_ptr[1:])
+        self._keep = dict(
+            px=f("pt_x"), py=f("pt_y"), fp1=i0("fc_p1"), fp2=i0("fc_p2"), fin=i0("fc_in"),
+            fout=i0("fc_out"), fbc=i1("fc_bc"), fnx=f("fc_nx"), fny=f("fc_ny"),
+            farea=f("fc_area"), fcx=f("fc_cenx"), fcy=f("fc_ceny"), ccx=f("cell_cenx"),
+            ccy=f("cell_ceny"), ccv=f("cell_cv"), ccov=f("cell_cov"), r11=f("cell_r11"),
+            r12=f("cell_r12"), r22=f("cell_r22"), nc2c=i1("cell_nc2c"), c2f_ptr=c2f_ptr,
+            c2f=i0("cell_c2f"), v2c_ptr=v2c_ptr, v2c=i0("pt_v2c"), wt=f("pt_wt"))
+        z = lambda *s: np.zeros(s, dtype=np.float64)
+        self.qc = z(self.ncell, 4)
+        self.qp = z(self.ncell, 5)
+        self.qold = z(self.noc, 4)
+        self.res = z(self.noc, 4)
+        self.grad = z(self.noc, 10)
+        self.phi = z(self.noc, 4)
+        self.dt = z(self.noc)
+        self.la = z(self.noc)
+        self.fla = z(nof)
+        self.fmu = z(nof)
+        self.fqp = z(nof, 5)
+        self.fgrad = z(nof, 10)
+        self.cmul = z(self.ncell)
+        self.pprim = z(self.nop, 5)
+        self.pmu = z(self.nop)
+        s = _State()
+        s.nop, s.noc, s.nfi, s.nbf = self.nop, self.noc, self.nfi, self.nbf
+        for k, v in self._keep.items():
+            setattr(s, k, v.ctypes.data_as(_ip if v.dtype == np.int32 else _dp))
+        for k in ("qc", "qp", "qold", "res", "grad", "phi", "dt", "la", "fla", "fmu", "fqp",
+                  "fgrad", "cmul", "pprim", "pmu"):
+            setattr(s, k, getattr(self, k).ctypes.data_as(_dp))
+        s.mach, s.aoa_deg, s.reynolds, s.t_wall = mach, aoa_deg, reynolds, t_wall
+        s.flow_type = 1 if flow_type == "inviscid" else 2
+        s.iwall, s.flux, s.grad_type, s.ilimit, s.irs = iwall, FLUX[flux], GRAD[grad], ilimit, irs
+        s.cfl_max = cfl_max
+        self.s = s
+        _lib.orc_cc_initialize(C.byref(s))
+
+    def _call(self, name, *args):
+        getattr(_lib, f"orc_cc_{name}")(C.byref(self.s), *args)
+        if self.s.error:
+            raise RuntimeError(f"oracle error {self.s.error} at {self.s.error_cell}")
+
+    def prim(self): self._call("prim")
+    def avg_c2v(self): self._call("avg_c2v")
+
+    def time_step(self, cfl):
+        self.s.cfl = cfl
+        self._call("time_step")
+
+    def save_old(self): self._call("save_old")
+    def gradient(self): self._call("gradient")
+    def limit(self): self._call("limit")
+    def fvresidual(self): self._call("fvresidual")
+    def rk_update(self, k): self._call("rk_update", k)
+    def lusgs(self): self._call("lusgs")
+
+    def residue(self, it):
+        _lib.orc_cc_residue(C.byref(self.s), it)
+        return self.s.fres
+
+    def check_positivity(self) -> int:
+        return _lib.orc_cc_check_positivity(C.byref(self.s))
+
+    def iteration(self, it, cfl, timemode="rk3"):
+        _lib.orc_cc_iteration(C.byref(self.s), it, cfl, 1 if timemode == "rk3" else 2)
+        if self.s.error:
+            raise RuntimeError(f"oracle error {self.s.error} at {self.s.error_cell}")
+        return self.s.fres
+
+    @property
+    def forces(self):
+        return self.s.fx1, self.s.fy1
+
+
+def flux(kind: str, mach: float, ql, qr, nx: float, ny: float, area: float):
+    """One face flux; returns (flux, ok, ql_after, qr_after)."""
+    a = np.array(ql, dtype=np.float64)
+    b = np.array(qr, dtype=np.float64)
+    out = np.zeros(4)
+    bad = _lib.orc_flux(FLUX[kind], mach, a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), nx, ny,
+                        area, out.ctypes.data_as(_dp))
+    return out, bad == 0, a, b
+
+
+def farfield_state(mach: float, aoa_deg: float, ql, nx: float, ny: float):
+    a = np.array(ql, dtype=np.float64)
+    out = np.zeros(4)
+    _lib.orc_farfield_state(mach, aoa_deg, a.ctypes.data_as(_dp), nx, ny, out.ctypes.data_as(_dp))
+    return out

@@ -1,0 +1,52 @@
+// Internal host-side types of the ufx library (not part of the C-ABI).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/ufx.h"
+
+namespace ufx {
+
+// Raw mesh as the reference reads it from geometry.inp
+// (CCFVM/read_input.F90:160-173); indices 1-based.
+struct RawInput {
+  int nop = 0, noc = 0, nfi = 0, nbf = 0;
+  const double *px = nullptr, *py = nullptr;
+  const double *ccx = nullptr, *ccy = nullptr, *cv = nullptr;
+  const int *fi_p1 = nullptr, *fi_p2 = nullptr, *fi_in = nullptr, *fi_out = nullptr,
+            *fi_bc = nullptr;
+  const int *fb_p1 = nullptr, *fb_p2 = nullptr, *fb_in = nullptr, *fb_bc = nullptr;
+  const double *fb_nx = nullptr, *fb_ny = nullptr;
+};
+
+// A preprocessed mesh, 0-based.  Cells 0..noc-1 are interior (reference
+// numbering after its breadth-first renumbering), noc..noc+nbf-1 are ghost
+// cells, ghost k belonging to boundary face nfi+k.  Faces 0..nfi-1 interior,
+// nfi..nfi+nbf-1 boundary, in the reference's face order.
+struct Mesh {
+  int kind = 0;
+  int nop = 0, noc = 0, nfi = 0, nbf = 0;
+  std::vector<double> px, py;
+  std::vector<int> fp1, fp2, fin, fout, fbc;
+  std::vector<double> fnx, fny, farea, fcx, fcy, fcov;
+  std::vector<double> ccx, ccy, ccov;   // ncell
+  std::vector<double> ccv, r11, r12, r22;  // noc
+  std::vector<int> nc2c;                // noc
+  std::vector<int> c2f_ptr, c2f;        // LU-SGS face order per cell
+  std::vector<int> v2c_ptr, v2c;        // vertex -> cells (ghosts included)
+  std::vector<double> wt;               // pseudo-Laplacian weights per v2c entry
+  std::vector<int> oldnum;              // new cell -> input cell (0-based)
+  int nof() const { return nfi + nbf; }
+  int ncell() const { return noc + nbf; }
+};
+
+int cc_geometric(const RawInput& in, Mesh& m, std::string& err);
+
+void set_error(const std::string& msg);
+
+}  // namespace ufx
+
+struct ufx_mesh {
+  ufx::Mesh m;
+};
