@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the cell-centred residual hot path on one MI355X.
+
+Workload: the second-order Euler case on the seeded bump channel at
+1000 x 1000 quads (2 000 000 triangles), bump height 0.08, Mach 0.75,
+least-squares gradients with the Venkatakrishnan limiter, Roe flux, three
+Runge-Kutta stages per iteration, CFL 0.8 with local time stepping.  A step
+is one iteration of the reference's main loop (CCFVM/main.F90:57-128) on the
+device: primitive refresh, vertex averages, time step, gradient + limiter and
+three residual stages with their updates, then the residual norm.  Inputs are
+resident in HBM before the timed region.
+
+Metric: million cell-RK3-stages per second = 3 * cells * iterations / s / 1e6.
+
+Prints one JSON line with, besides the contract keys, the roofline of the
+dominant kernel (the residual stage, timed with HIP events on the library's
+stream) and a CPU baseline (the reference solver compiled from its own
+sources, timed on this host on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(noc: int, nof: int, w: int, d: int = 8, i: int = 4) -> float:
+    """Algorithmic bytes of one residual-stage launch, averaged over the three
+    stages: face connectivity and geometry once per face, cell state once per
+    cell (primitives, gradients, limiter values, centroid), the old and the
+    current conserved state, dt/area, and the written state."""
+    faces = nof * (2 * i + 5 * w)
+    cells = noc * (2 * w + 4 * w + 8 * w + 4 * w) + noc * 4 * d + noc * w + noc * 4 * d
+    prev_q = noc * 4 * d * 2.0 / 3.0
+    return faces + cells + prev_q
+
+
+def iteration_bytes(noc: int, nof: int, nop: int, nv2c: int, w: int, d: int = 8, i: int = 4) -> float:
+    """Algorithmic bytes of one RK3 iteration (all passes)."""
+    p1 = noc * (4 * d + 4 * w)
+    p2 = nop * i + nv2c * (i + w) + noc * 4 * w + nop * 4 * w
+    p3 = nof * (4 * i + 3 * w) + nop * 4 * w + noc * 2 * w
+    p4 = nof * 2 * i + noc * (2 * w + 3 * w + 4 * w + 4 * w) + nof * 2 * w + noc * (8 * w + 4 * w)
+    return p1 + p2 + p3 + p4 + 3 * stage_bytes(noc, nof, w, d, i)
+
+
+def cpu_baseline(budget_s: float = 20.0) -> dict | None:
+    """Time the reference solver (oracle/_ref, built from the reference's own
+    sources) on a bounded sample of the workload: the same generator, flow
+    and scheme on a 200 x 200-quad channel (80 000 triangles), warm-up then a
+    timed window, single thread."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cc_fast", "ref_cc_driver")
+    from unsflow2d_amd import meshgen
+    nq = 200
+    raw = meshgen.bump_channel(nq, nq, 2, h=0.08)
+    cells = raw.noc
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as tmp:
+            meshgen.write_geometry_inp(raw, os.path.join(tmp, "geometry.inp"))
+            deck = meshgen.Deck(mach=0.75, maxiter=10)
+            with open(os.path.join(tmp, "flo.inp"), "w") as f:
+                f.write(deck.text())
+            # a short probe sizes the window to the budget
+            warm = 3
+            out = subprocess.run([exe, "flo.inp", ".", str(warm + 3), "0", "-1", str(warm)],
+                                 cwd=tmp, capture_output=True, text=True, timeout=600)
+            ms = float(out.stdout.split("ms_per_iteration")[-1].split()[0])
+            nit = max(5, int(budget_s * 1000.0 / ms))
+            out = subprocess.run([exe, "flo.inp", ".", str(warm + nit), "0", "-1", str(warm)],
+                                 cwd=tmp, capture_output=True, text=True, timeout=1200)
+            ms = float(out.stdout.split("ms_per_iteration")[-1].split()[0])
+        kind, what = "reference", "reference CCFVM (amdflang -O3 -march=x86-64-v3)"
+    else:
+        from oracle.oracle import CCOracle
+        from unsflow2d_amd import cc
+        mesh = cc.Mesh.from_raw(raw)
+        orc = CCOracle(mesh.arrays(), mach=0.75)
+        for it in range(1, 4):
+            orc.iteration(it, 0.8)
+        t0 = time.perf_counter()
+        nit = 0
+        while time.perf_counter() - t0 < budget_s:
+            nit += 1
+            orc.iteration(3 + nit, 0.8)
+        ms = (time.perf_counter() - t0) * 1000.0 / nit
+        kind, what = "port", "C restatement of the reference (gcc -O2)"
+    value = 3.0 * cells / (ms / 1000.0) / 1e6
+    return {"value": round(value, 3), "unit": "M cell-RK3-stages/s", "cores": 1, "kind": kind,
+            "sample": f"{what}, bump channel {nq}x{nq} quads ({cells} triangles), M=0.75, LSQ + "
+                      f"limiter, Roe, RK3, {nit} timed iterations after 3 warm-up, "
+                      f"{ms:.2f} ms/iteration, 1 thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--quads", type=int, default=1000, help="quads per side of the channel")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
+              "shards by mesh partition, which this build does not implement)", file=sys.stderr)
+        return 2
+
+    from unsflow2d_amd import cc, meshgen
+    t0 = time.time()
+    raw = meshgen.bump_channel(args.quads, args.quads, 2, h=0.08)
+    mesh = cc.Mesh.from_raw(raw)
+    counts = mesh.counts
+    nv2c = int(mesh.get("pt_nv2c").sum())
+    cfg = cc.Config(mach=0.75, flux="roe", grad="lsqr", ilimit=1, precision=args.precision)
+    solver = cc.Solver(mesh, cfg)
+    print(f"# setup {time.time() - t0:.1f}s: {counts['noc']} cells, {counts['nof']} faces, "
+          f"{counts['nop']} vertices", file=sys.stderr, flush=True)
+
+    solver.iterate(args.warmup, hist=True)
+    solver.sync()
+    # timed region: K iterations, HIP events on the library's stream per kernel
+    solver.profile(True)
+    solver.sync()
+    t1 = time.perf_counter()
+    solver.iterate(args.steps, hist=False)
+    solver.sync()
+    wall = time.perf_counter() - t1
+    prof = solver.profile_read()
+    solver.profile(False)
+    hist = solver.iterate(1, hist=True)
+    if not np.all(np.isfinite(hist)):
+        print("# residual is not finite", file=sys.stderr)
+        return 1
+
+    noc, nof, nop = counts["noc"], counts["nof"], counts["nop"]
+    ms_step = wall * 1000.0 / args.steps
+    value = 3.0 * noc * args.steps / wall / 1e6
+    w = 8 if args.precision == "fp64" else 4
+    st_ms, st_n = prof["stage"]
+    st_avg = st_ms / max(1, st_n)
+    sb = stage_bytes(noc, nof, w)
+    achieved = sb / (st_avg / 1000.0) / 1e9
+    it_bytes = iteration_bytes(noc, nof, nop, nv2c, w)
+    kern = {k: round(v[0] / max(1, v[1]), 4) for k, v in prof.items() if v[1]}
+    print(f"# kernel avg ms: {kern}; iteration {ms_step:.3f} ms, "
+          f"{it_bytes / (ms_step / 1000.0) / 1e9:.0f} GB/s algorithmic over the whole iteration",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_budget)
+        except Exception as e:  # the baseline is reported, never required
+            print(f"# cpu baseline failed: {e}", file=sys.stderr)
+
+    line = {
+        "metric": "M cell-RK3-stages/s @2M cells, FP64 vs mixed at CL/CD<=1e-6; batched solves/s",
+        "value": round(value, 2),
+        "unit": "M cell-RK3-stages/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"CCFVM 2nd-order Euler, seeded bump channel {args.quads}x{args.quads} "
+                               f"quads ({noc} triangles), h=0.08, M=0.75, LSQ+Venkatakrishnan, Roe, "
+                               f"RK3, CFL 0.8 local time step, {args.precision}",
+                   "cells": noc, "faces": nof, "precision": args.precision},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_residual (stage)", "bytes_per_launch": int(sb),
+                     "avg_launch_ms": round(st_avg, 5)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -206,6 +206,23 @@ int ufx_cc_forces(ufx_cc* ctx, double* out, double* cp, double* cf);
  * call synchronising.  Without hist no host synchronisation happens. */
 int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double* hist);
 
+/* Kernel timing (no reference counterpart; instrumentation for benchmarks).
+ * When enabled, a pair of HIP events is recorded on the context's stream
+ * around every kernel launch; ufx_cc_profile_read (synchronising) returns,
+ * per category, the summed milliseconds and the number of launches since the
+ * last read, then resets.  Categories: */
+#define UFX_K_PRIM 0      /* primitive refresh + range partials */
+#define UFX_K_RANGE 1     /* global range final reduction */
+#define UFX_K_AVG 2       /* vertex averages */
+#define UFX_K_TSTEP 3     /* spectral radius + time step */
+#define UFX_K_GRAD 4      /* gradient (+ limiter) */
+#define UFX_K_STAGE 5     /* residual (+ stage update) */
+#define UFX_K_NORM 6      /* residual norm */
+#define UFX_K_OTHER 7
+#define UFX_K_NCAT 8
+int ufx_cc_profile(ufx_cc* ctx, int enable);
+int ufx_cc_profile_read(ufx_cc* ctx, double* ms, int* count);
+
 /* Block until all queued work has finished; returns the first device-detected
  * error, if any. */
 int ufx_cc_sync(ufx_cc* ctx);

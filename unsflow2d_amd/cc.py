@@ -90,6 +90,10 @@ _sig("ufx_cc_check_positivity", [_vp, _ip])
 _sig("ufx_cc_forces", [_vp, _dp, _dp, _dp])
 _sig("ufx_cc_iterate", [_vp, C.c_int, C.c_int, _dp, _dp])
 _sig("ufx_cc_stream", [_vp], C.c_void_p)
+_sig("ufx_cc_profile", [_vp, C.c_int])
+_sig("ufx_cc_profile_read", [_vp, _dp, _ip])
+
+KCAT = ["prim", "range", "avg_c2v", "time_step", "gradient", "stage", "norm", "other"]
 
 EXPORTED = [
     "ufx_last_error", "ufx_cc_geometric", "ufx_cc_mesh_import", "ufx_mesh_size", "ufx_mesh_get",
@@ -97,7 +101,7 @@ EXPORTED = [
     "ufx_cc_prim", "ufx_cc_avg_c2v", "ufx_cc_time_step", "ufx_cc_save_old", "ufx_cc_gradient",
     "ufx_cc_limit", "ufx_cc_fvresidual", "ufx_cc_rk_update", "ufx_cc_rk_stage", "ufx_cc_lusgs",
     "ufx_cc_residue", "ufx_cc_check_positivity", "ufx_cc_forces", "ufx_cc_iterate",
-    "ufx_cc_sync", "ufx_cc_stream",
+    "ufx_cc_sync", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
 ]
 
 
@@ -324,6 +328,17 @@ class Solver:
         self.iter += niter
         self.cfl = c
         return h
+
+    def profile(self, enable: bool = True) -> None:
+        """Start (or stop) per-kernel HIP-event timing on the context's stream."""
+        _check(lib.ufx_cc_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self) -> dict:
+        """{category: (milliseconds, launches)} since the last read."""
+        ms = np.zeros(len(KCAT))
+        cnt = np.zeros(len(KCAT), dtype=np.int32)
+        _check(lib.ufx_cc_profile_read(self._h, _d(ms), _i(cnt)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KCAT)}
 
     @property
     def stream(self) -> int:

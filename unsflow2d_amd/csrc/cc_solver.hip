@@ -526,7 +526,52 @@ struct ufx_cc {
   int hist_cap = 0;
   double fres1_host = 0.0;
   std::vector<int> wall_faces;     // boundary ordinals of wall faces
+  // kernel timing: event pool and (category, begin, end) records
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Rec { int cat; size_t a, b; };
+  std::vector<Rec> recs;
 };
+
+namespace {
+
+hipEvent_t prof_event(ufx_cc* x) {
+  if (x->ev_used == x->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    x->ev_pool.push_back(e);
+  }
+  return x->ev_pool[x->ev_used++];
+}
+
+int prof_begin(ufx_cc* x, int cat) {
+  if (!x->prof) return -1;
+  size_t i = x->ev_used;
+  hipEvent_t e = prof_event(x);
+  if (!e || hipEventRecord(e, x->stream) != hipSuccess) return -1;
+  x->recs.push_back({cat, i, i});
+  return (int)x->recs.size() - 1;
+}
+
+void prof_end(ufx_cc* x, int r) {
+  if (r < 0) return;
+  size_t i = x->ev_used;
+  hipEvent_t e = prof_event(x);
+  if (!e || hipEventRecord(e, x->stream) != hipSuccess) return;
+  x->recs[r].b = i;
+}
+
+}  // namespace
+
+// Launch `kern` on the context's stream, timed under category `cat` when
+// profiling is on.
+#define LAUNCH(x, cat, kern, grid, ...)                                  \
+  do {                                                                   \
+    int r_ = prof_begin((x), (cat));                                     \
+    kern<<<(grid), BLK, 0, (x)->stream>>>(__VA_ARGS__);                  \
+    prof_end((x), r_);                                                   \
+  } while (0)
 
 namespace {
 
@@ -781,6 +826,7 @@ int ufx_cc_destroy(ufx_cc* x) {
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   for (void* p : x->allocs) (void)hipFree(p);
   if (x->d_hist) (void)hipFree(x->d_hist);
+  for (hipEvent_t e : x->ev_pool) (void)hipEventDestroy(e);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return UFX_OK;
@@ -857,7 +903,8 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     if ((rc = dalloc(x, &er, 4))) { ufx_cc_destroy(x); return rc; }
     d.err = er;
     int init[4] = {0, 0, 0x7fffffff, 0};
-    if (hipMemcpyAsync(er, init, sizeof(init), hipMemcpyHostToDevice, x->stream) != hipSuccess) {
+    if (hipMemcpyAsync(er, init, sizeof(init), hipMemcpyHostToDevice, x->stream) != hipSuccess ||
+        hipStreamSynchronize(x->stream) != hipSuccess) {
       set_error("upload of the error word failed");
       ufx_cc_destroy(x);
       return UFX_ERR_HIP;
@@ -943,38 +990,38 @@ int ufx_cc_get(ufx_cc* x, const char* name, double* dst) {
 
 int ufx_cc_prim(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  k_prim<<<x->nblk_all, BLK, 0, x->stream>>>(x->d, x->g, 1, 0, 0);
+  LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, x->d, x->g, 1, 0, 0);
   return launch_check();
 }
 
 int ufx_cc_save_old(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  k_prim<<<x->nblk_all, BLK, 0, x->stream>>>(x->d, x->g, 0, 1, 0);
+  LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, x->d, x->g, 0, 1, 0);
   return launch_check();
 }
 
 int ufx_cc_avg_c2v(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  k_avg_c2v<<<nblocks(x->d.nop), BLK, 0, x->stream>>>(x->d);
+  LAUNCH(x, UFX_K_AVG, k_avg_c2v, nblocks(x->d.nop), x->d);
   return launch_check();
 }
 
 int ufx_cc_time_step(ufx_cc* x, double cfl) {
   if (int rc = check_ctx(x)) return rc;
-  k_timestep<<<x->nblk_cell, BLK, 0, x->stream>>>(x->d, x->g, x->p.vll, cfl);
+  LAUNCH(x, UFX_K_TSTEP, k_timestep, x->nblk_cell, x->d, x->g, x->p.vll, cfl);
   return launch_check();
 }
 
 static int range_pass(ufx_cc* x) {
-  k_prim<<<x->nblk_all, BLK, 0, x->stream>>>(x->d, x->g, 0, 0, 1);
-  k_range_final<<<1, BLK, 0, x->stream>>>(x->d, x->nblk_all);
+  LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, x->d, x->g, 0, 0, 1);
+  LAUNCH(x, UFX_K_RANGE, k_range_final, 1, x->d, x->nblk_all);
   return launch_check();
 }
 
 int ufx_cc_limit(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
   if (int rc = range_pass(x)) return rc;
-  k_grad_lsq<<<x->nblk_cell, BLK, 0, x->stream>>>(x->d, 0, 1);
+  LAUNCH(x, UFX_K_GRAD, k_grad_lsq, x->nblk_cell, x->d, 0, 1);
   return launch_check();
 }
 
@@ -982,7 +1029,7 @@ int ufx_cc_gradient(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
   if (x->p.ilimit)
     if (int rc = range_pass(x)) return rc;
-  k_grad_lsq<<<x->nblk_cell, BLK, 0, x->stream>>>(x->d, 1, x->p.ilimit);
+  LAUNCH(x, UFX_K_GRAD, k_grad_lsq, x->nblk_cell, x->d, 1, x->p.ilimit);
   return launch_check();
 }
 
@@ -991,16 +1038,16 @@ static int residual(ufx_cc* x, int mode, double ak, double bk) {
   int viscous = x->p.flow != UFX_FLOW_INVISCID;
   switch (x->p.flux) {
     case UFX_FLUX_ROE:
-      k_residual<UFX_FLUX_ROE><<<x->nblk_cell, BLK, 0, x->stream>>>(d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
+      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_ROE>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
       break;
     case UFX_FLUX_AUSM:
-      k_residual<UFX_FLUX_AUSM><<<x->nblk_cell, BLK, 0, x->stream>>>(d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
+      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_AUSM>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
       break;
     case UFX_FLUX_VANLEER:
-      k_residual<UFX_FLUX_VANLEER><<<x->nblk_cell, BLK, 0, x->stream>>>(d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
+      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_VANLEER>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
       break;
     default:
-      k_residual<UFX_FLUX_RUSANOV><<<x->nblk_cell, BLK, 0, x->stream>>>(d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
+      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_RUSANOV>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
       break;
   }
   return launch_check();
@@ -1017,7 +1064,7 @@ int ufx_cc_fvresidual(ufx_cc* x) {
 int ufx_cc_rk_update(ufx_cc* x, int stage) {
   if (int rc = check_ctx(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
-  k_rk_update<<<x->nblk_cell, BLK, 0, x->stream>>>(x->d, RK_A[stage - 1], RK_B[stage - 1]);
+  LAUNCH(x, UFX_K_STAGE, k_rk_update, x->nblk_cell, x->d, RK_A[stage - 1], RK_B[stage - 1]);
   return launch_check();
 }
 
@@ -1045,8 +1092,8 @@ static int ensure_hist(ufx_cc* x, int n) {
 int ufx_cc_residue(ufx_cc* x, int iter, double* fres, double* fres_raw) {
   if (int rc = check_ctx(x)) return rc;
   if (int rc = ensure_hist(x, 1)) return rc;
-  k_sumsq<<<x->nblk_cell, BLK, 0, x->stream>>>(x->d);
-  k_norm_final<<<1, BLK, 0, x->stream>>>(x->d, x->nblk_cell, iter == 1, x->d_hist, 0);
+  LAUNCH(x, UFX_K_NORM, k_sumsq, x->nblk_cell, x->d);
+  LAUNCH(x, UFX_K_NORM, k_norm_final, 1, x->d, x->nblk_cell, iter == 1, x->d_hist, 0);
   if (int rc = launch_check()) return rc;
   double h = 0.0, raw = 0.0;
   HIPCHK(hipMemcpyAsync(&h, x->d_hist, 8, hipMemcpyDeviceToHost, x->stream));
@@ -1061,7 +1108,7 @@ int ufx_cc_check_positivity(ufx_cc* x, int* bad) {
   if (int rc = check_ctx(x)) return rc;
   int big = 0x7fffffff;
   HIPCHK(hipMemcpyAsync(x->d.err + 2, &big, sizeof(int), hipMemcpyHostToDevice, x->stream));
-  k_positivity<<<x->nblk_cell, BLK, 0, x->stream>>>(x->d);
+  LAUNCH(x, UFX_K_OTHER, k_positivity, x->nblk_cell, x->d);
   if (int rc = launch_check()) return rc;
   int e = 0;
   HIPCHK(hipMemcpyAsync(&e, x->d.err + 2, sizeof(int), hipMemcpyDeviceToHost, x->stream));
@@ -1079,7 +1126,7 @@ int ufx_cc_forces(ufx_cc* x, double* out, double* cp, double* cf) {
   if (!out) { set_error("ufx_cc_forces: null output"); return UFX_ERR_ARG; }
   const Dev& d = x->d;
   const Params& p = x->p;
-  k_dtmin<<<1, BLK, 0, x->stream>>>(x->d);
+  LAUNCH(x, UFX_K_OTHER, k_dtmin, 1, x->d);
   if (int rc = launch_check()) return rc;
   int nbf = d.nbf;
   std::vector<double> bf(2 * (size_t)std::max(1, nbf)), dtmin(1);
@@ -1143,14 +1190,15 @@ int ufx_cc_iterate(ufx_cc* x, int iter0, int niter, const double* cfl, double* h
   for (int k = 0; k < niter; ++k) {
     int iter = iter0 + 1 + k;
     // prim refresh + qold (save_old commutes with avg_c2v/time_step) + range
-    k_prim<<<x->nblk_all, BLK, 0, x->stream>>>(d, x->g, 1, 1, x->p.ilimit);
-    k_avg_c2v<<<nblocks(d.nop), BLK, 0, x->stream>>>(d);
-    k_timestep<<<x->nblk_cell, BLK, 0, x->stream>>>(d, x->g, x->p.vll, cfl[k]);
-    if (x->p.ilimit) k_range_final<<<1, BLK, 0, x->stream>>>(d, x->nblk_all);
-    k_grad_lsq<<<x->nblk_cell, BLK, 0, x->stream>>>(d, 1, x->p.ilimit);
+    LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, d, x->g, 1, 1, x->p.ilimit);
+    LAUNCH(x, UFX_K_AVG, k_avg_c2v, nblocks(d.nop), d);
+    LAUNCH(x, UFX_K_TSTEP, k_timestep, x->nblk_cell, d, x->g, x->p.vll, cfl[k]);
+    if (x->p.ilimit)
+      LAUNCH(x, UFX_K_RANGE, k_range_final, 1, d, x->nblk_all);
+    LAUNCH(x, UFX_K_GRAD, k_grad_lsq, x->nblk_cell, d, 1, x->p.ilimit);
     for (int s = 0; s < 3; ++s)
       if (int rc = residual(x, s == 2 ? 2 : 1, RK_A[s], RK_B[s])) return rc;
-    k_norm_final<<<1, BLK, 0, x->stream>>>(d, x->nblk_cell, iter == 1, x->d_hist, k);
+    LAUNCH(x, UFX_K_NORM, k_norm_final, 1, d, x->nblk_cell, iter == 1, x->d_hist, k);
     if (int rc = launch_check()) return rc;
   }
   if (hist && niter > 0) {
@@ -1158,6 +1206,32 @@ int ufx_cc_iterate(ufx_cc* x, int iter0, int niter, const double* cfl, double* h
     HIPCHK(hipStreamSynchronize(x->stream));
     return device_error(x);
   }
+  return UFX_OK;
+}
+
+int ufx_cc_profile(ufx_cc* x, int enable) {
+  if (int rc = check_ctx(x)) return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->prof = enable != 0;
+  x->recs.clear();
+  x->ev_used = 0;
+  return UFX_OK;
+}
+
+int ufx_cc_profile_read(ufx_cc* x, double* ms, int* count) {
+  if (int rc = check_ctx(x)) return rc;
+  if (!ms || !count) { set_error("ufx_cc_profile_read: null output"); return UFX_ERR_ARG; }
+  HIPCHK(hipStreamSynchronize(x->stream));
+  for (int k = 0; k < UFX_K_NCAT; ++k) { ms[k] = 0.0; count[k] = 0; }
+  for (const auto& r : x->recs) {
+    if (r.a == r.b) continue;
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, x->ev_pool[r.a], x->ev_pool[r.b]));
+    ms[r.cat] += t;
+    count[r.cat] += 1;
+  }
+  x->recs.clear();
+  x->ev_used = 0;
   return UFX_OK;
 }
 
