@@ -219,7 +219,8 @@ int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double*
 #define UFX_K_STAGE 5     /* residual (+ stage update) */
 #define UFX_K_NORM 6      /* residual norm */
 #define UFX_K_OTHER 7
-#define UFX_K_NCAT 8
+#define UFX_K_BFACE 8     /* boundary faces of a residual */
+#define UFX_K_NCAT 9
 int ufx_cc_profile(ufx_cc* ctx, int enable);
 int ufx_cc_profile_read(ufx_cc* ctx, double* ms, int* count);
 

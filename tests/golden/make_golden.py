@@ -1,0 +1,120 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of the cell-centred hot path from the reference.
+
+Runs the reference solver built from its own sources (oracle/_ref/cc, no
+floating-point contraction; `make -C oracle ref`) through the harness driver
+oracle/ref_cc_driver.F90 on seeded meshes from unsflow2d_amd.meshgen, and
+stores what it dumps -- the preprocessed mesh, every routine's output for the
+first iterations, the residual history and the final state -- in one .npz
+per case under tests/golden/.  Needs /root/reference (this container only);
+the fixtures themselves are data and travel with the repository.
+
+    python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from unsflow2d_amd import meshgen  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "cc", "ref_cc_driver")
+
+# name -> (mesh kwargs, deck kwargs, niter, ndump, phi0, keep_mesh, keep_final)
+TINY = dict(nx=12, ny=6, seed=7, h=0.08)
+VISC = dict(nx=16, ny=10, seed=4, h=0.05, first=2e-3, growth=1.3, jitter_y=False)
+CASES = {
+    "cc_tiny_roe_lsqr": (TINY, dict(mach=0.5), 30, 3, -1.0, True, True),
+    "cc_tiny_first_order": (TINY, dict(mach=0.5), 30, 2, 0.0, False, True),
+    "cc_tiny_vanleer": (TINY, dict(mach=0.5, flux_type="vanleer"), 10, 2, -1.0, False, True),
+    "cc_tiny_rusanov": (TINY, dict(mach=0.5, flux_type="rusanov"), 2, 2, -1.0, False, True),
+    "cc_tiny_ausm": (TINY, dict(mach=0.5, flux_type="ausm", cfl_min=0.05, cfl_max=0.05),
+                     10, 2, -1.0, False, True),
+    "cc_tiny_gg": (TINY, dict(mach=0.5, grad_type="gg"), 10, 2, -1.0, False, True),
+    "cc_tiny_ggfc": (TINY, dict(mach=0.5, grad_type="ggfc"), 10, 2, -1.0, False, True),
+    "cc_visc_ggfc_irs": (VISC, dict(mach=0.3, flow_type="laminar", reynolds=1000.0, iwall=1,
+                                    grad_type="ggfc", irs=1, cfl_min=0.5, cfl_max=0.8,
+                                    cfl_ramp_steps=5), 10, 2, -1.0, True, True),
+    "cc_visc_lsqr_iso": (VISC, dict(mach=0.3, flow_type="laminar", reynolds=1000.0, iwall=0,
+                                    grad_type="lsqr", irs=0, cfl_min=0.5, cfl_max=0.8,
+                                    cfl_ramp_steps=5), 10, 2, -1.0, False, True),
+    "cc_lusgs_gg": (dict(nx=20, ny=10, seed=8, h=0.08),
+                    dict(mach=0.5, grad_type="gg", timemode="lusgs", cfl_min=1.0, cfl_max=5.0,
+                         cfl_ramp_steps=10), 15, 2, -1.0, True, True),
+    # the configuration-1 workload: 100 x 100 quads, first order (limiter values
+    # start at 0), and the same with the second-order scheme
+    "cc_c1_first_order": (dict(nx=100, ny=100, seed=1, h=0.05), dict(mach=0.5), 1000, 0, 0.0,
+                          False, ("final_qc",)),
+    "cc_c1_second_order": (dict(nx=100, ny=100, seed=1, h=0.05), dict(mach=0.5), 1000, 0, -1.0,
+                           False, ("final_qc",)),
+}
+
+MESH_I = ["fc_p1", "fc_p2", "fc_in", "fc_out", "fc_bc", "cell_nc2c", "cell_nc2f", "cell_c2f",
+          "pt_nv2c", "pt_v2c", "counts"]
+MESH_F = ["fc_nx", "fc_ny", "fc_area", "fc_cenx", "fc_ceny", "fc_cov", "cell_cenx", "cell_ceny",
+          "cell_cv", "cell_cov", "cell_r11", "cell_r12", "cell_r22", "pt_x", "pt_y", "pt_wt"]
+
+
+def raw_sha(raw: meshgen.RawMesh) -> str:
+    h = hashlib.sha256()
+    for a in (raw.x, raw.y, raw.cen, raw.area, raw.fint, raw.fbnd, raw.fbn, raw.fbc):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def make(name: str) -> str:
+    mkw, dkw, niter, ndump, phi0, keep_mesh, keep_final = CASES[name]
+    mkw = dict(mkw)
+    raw = meshgen.bump_channel(mkw.pop("nx"), mkw.pop("ny"), mkw.pop("seed"), **mkw)
+    deck = meshgen.Deck(**dkw)
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        meshgen.write_geometry_inp(raw, os.path.join(tmp, "geometry.inp"))
+        with open(os.path.join(tmp, "flo.inp"), "w") as f:
+            f.write(deck.text())
+        os.makedirs(os.path.join(tmp, "d"))
+        r = subprocess.run([DRIVER, "flo.inp", "d", str(niter), str(ndump), repr(phi0)], cwd=tmp,
+                           capture_output=True, text=True, timeout=3600)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: reference failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        ddir = os.path.join(tmp, "d")
+        for fn in sorted(os.listdir(ddir)):
+            base, ext = os.path.splitext(fn)
+            dt = np.int32 if ext == ".i32" else np.float64
+            a = np.fromfile(os.path.join(ddir, fn), dtype=dt)
+            is_mesh = base in MESH_I or base in MESH_F
+            if is_mesh and not keep_mesh:
+                out["sha_" + base] = np.frombuffer(hashlib.sha256(a.tobytes()).digest(), np.uint8)
+                continue
+            if base.startswith("final_") and not (keep_final is True or base in keep_final):
+                continue
+            out[base] = a
+    meta = dict(mesh=CASES[name][0], deck=dkw, niter=niter, ndump=ndump, phi0=phi0,
+                raw_sha=raw_sha(raw), reference="CCFVM, amdflang -O2 -ffp-contract=off",
+                driver="oracle/ref_cc_driver.F90")
+    out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
+def main(argv):
+    if not os.path.exists(DRIVER):
+        raise SystemExit("build the reference first: make -C oracle ref")
+    for name in (argv or list(CASES)):
+        p = make(name)
+        print(f"{name}: {os.path.getsize(p) / 1024:.0f} KiB", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -110,3 +110,131 @@ def test_first_order_start():
     ref = np.array([orc.iteration(it, 0.8) for it in range(1, 26)])
     _check("history", hist, ref, 1e-11)
     assert np.all(dev.get("phi") == 0.0)
+
+
+@pytest.mark.parametrize("grad", ["gg", "ggfc"])
+def test_gradient_methods_match_oracle(grad):
+    print(f"case: gradient={grad} mesh 18x8 seed 6", flush=True)
+    dev, orc = _setup(18, 8, 6, mach=0.5, grad=grad)
+    noc = orc.noc
+    for it in (1, 2, 3):
+        dev.prim(); orc.prim()
+        dev.avg_c2v(); orc.avg_c2v()
+        dev.time_step(0.8); orc.time_step(0.8)
+        dev.save_old(); orc.save_old()
+        dev.gradient(); orc.gradient()
+        _check("grad", dev.get("grad"), orc.grad)
+        _check("phi", dev.get("phi"), orc.phi)
+        if grad == "ggfc":
+            _check("fc_grad", dev.get("fc_grad"), orc.fgrad)
+        for k in (1, 2, 3):
+            dev.fvresidual(); orc.fvresidual()
+            _check("res", dev.get("res"), orc.res)
+            dev.rk_update(k); orc.rk_update(k)
+            _check("qc", dev.get("qc")[:noc], orc.qc[:noc])
+        _check("fres", dev.residue(it)[0], orc.residue(it))
+    dev.iter, dev.cfl = 3, 0.8
+    hist = dev.iterate(10)
+    ref = np.array([orc.iteration(it, 0.8) for it in range(4, 14)])
+    _check("history", hist, ref, 1e-11)
+
+
+def _setup_visc(grad, irs, iwall=1):
+    from unsflow2d_amd import cc, meshgen
+    from oracle.oracle import CCOracle
+    raw = meshgen.bump_channel(16, 10, 4, h=0.05, first=2e-3, growth=1.3, jitter_y=False)
+    mesh = cc.Mesh.from_raw(raw)
+    c = cc.Config(mach=0.3, flow_type="laminar", reynolds=1000.0, iwall=iwall, grad=grad,
+                  irs=irs, cfl_min=0.5, cfl_max=0.8, cfl_ramp_steps=5)
+    dev = cc.Solver(mesh, c)
+    orc = CCOracle(mesh.arrays(), mach=c.mach, flow_type="laminar", reynolds=c.reynolds,
+                   iwall=iwall, grad=grad, irs=irs, cfl_max=c.cfl_max)
+    return dev, orc, c
+
+
+@pytest.mark.parametrize("grad,irs,iwall", [("ggfc", 1, 1), ("lsqr", 0, 1), ("ggfc", 0, 0)])
+def test_viscous_matches_oracle(grad, irs, iwall):
+    """Laminar Navier-Stokes: Sutherland viscosity, diamond-path face
+    gradients, no-slip wall, residual smoothing."""
+    print(f"case: viscous grad={grad} irs={irs} iwall={iwall}", flush=True)
+    dev, orc, c = _setup_visc(grad, irs, iwall)
+    noc = orc.noc
+    cfl = float("nan")
+    for it in (1, 2):
+        cfl = c.cfl(it, cfl)
+        dev.prim(); orc.prim()
+        dev.avg_c2v(); orc.avg_c2v()
+        dev.time_step(cfl); orc.time_step(cfl)
+        _check("dt", dev.get("dt"), orc.dt)
+        dev.save_old(); orc.save_old()
+        dev.gradient(); orc.gradient()
+        _check("grad", dev.get("grad"), orc.grad)
+        for k in (1, 2, 3):
+            dev.fvresidual(); orc.fvresidual()
+            _check("res", dev.get("res"), orc.res)
+            _check("fc_mu", dev.get("fc_mu"), orc.fmu)
+            _check("phi", dev.get("phi"), orc.phi)
+            f = dev.forces()
+            _check("force", [f["fx1"], f["fy1"]], list(orc.forces))
+            dev.rk_update(k); orc.rk_update(k)
+            _check("qc", dev.get("qc")[:noc], orc.qc[:noc])
+        _check("fres", dev.residue(it)[0], orc.residue(it))
+    dev.cfl = cfl
+    dev.iter = 2
+    hist = dev.iterate(8)
+    ref = []
+    for it in range(3, 11):
+        cfl = c.cfl(it, cfl)
+        ref.append(orc.iteration(it, cfl))
+    _check("history", hist, np.array(ref), 1e-11)
+
+
+def test_lusgs_matches_oracle():
+    """Matrix-free LU-SGS: level-scheduled sweeps reproduce the reference's
+    sequential breadth-first order."""
+    print("case: lusgs gg mesh 20x10 seed 8", flush=True)
+    from unsflow2d_amd import cc, meshgen
+    from oracle.oracle import CCOracle
+    raw = meshgen.bump_channel(20, 10, 8, h=0.08)
+    mesh = cc.Mesh.from_raw(raw)
+    c = cc.Config(mach=0.5, grad="gg", timemode="lusgs", cfl_type="ramp", cfl_min=1.0,
+                  cfl_max=5.0, cfl_ramp_steps=10)
+    dev = cc.Solver(mesh, c)
+    orc = CCOracle(mesh.arrays(), mach=0.5, grad="gg", cfl_max=5.0)
+    noc = orc.noc
+    cfl = c.cfl(1)
+    dev.prim(); orc.prim()
+    dev.avg_c2v(); orc.avg_c2v()
+    dev.time_step(cfl); orc.time_step(cfl)
+    dev.save_old(); orc.save_old()
+    dev.gradient(); orc.gradient()
+    dev.lusgs(); orc.lusgs()
+    _check("res", dev.get("res"), orc.res)
+    _check("qc", dev.get("qc")[:noc], orc.qc[:noc])
+    orc.residue(1)
+    dev.residue(1)
+    dev.iter, dev.cfl = 1, cfl
+    hist = dev.iterate(12)
+    ref = []
+    for it in range(2, 14):
+        cfl = c.cfl(it, cfl)
+        ref.append(orc.iteration(it, cfl, "lusgs"))
+    _check("history", hist, np.array(ref), 1e-10)
+    _check("final qc", dev.get("qc")[:noc], orc.qc[:noc], 1e-10)
+
+
+def test_mixed_precision_tracks_fp64():
+    """The mixed form (FP32 flux/gradient/limiter data and arithmetic, FP64
+    state, residual sums and norms) stays close to the FP64 form."""
+    print("case: mixed vs fp64 mesh 40x20 seed 9", flush=True)
+    from unsflow2d_amd import cc, meshgen
+    raw = meshgen.bump_channel(40, 20, 9, h=0.08)
+    mesh = cc.Mesh.from_raw(raw)
+    a = cc.Solver(mesh, cc.Config(mach=0.6))
+    b = cc.Solver(mesh, cc.Config(mach=0.6, precision="mixed"))
+    ha = a.iterate(60)
+    hb = b.iterate(60)
+    e_h = relerr(hb, ha)
+    e_q = relerr(b.get("qc"), a.get("qc"))
+    print(f"  history rel {e_h:.2e} state rel {e_q:.2e}", flush=True)
+    assert e_q < 1e-4 and e_h < 1e-2

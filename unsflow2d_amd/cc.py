@@ -93,7 +93,7 @@ _sig("ufx_cc_stream", [_vp], C.c_void_p)
 _sig("ufx_cc_profile", [_vp, C.c_int])
 _sig("ufx_cc_profile_read", [_vp, _dp, _ip])
 
-KCAT = ["prim", "range", "avg_c2v", "time_step", "gradient", "stage", "norm", "other"]
+KCAT = ["prim", "range", "avg_c2v", "time_step", "gradient", "stage", "norm", "other", "bface"]
 
 EXPORTED = [
     "ufx_last_error", "ufx_cc_geometric", "ufx_cc_mesh_import", "ufx_mesh_size", "ufx_mesh_get",

@@ -1,0 +1,144 @@
+// Device data layout of the cell-centred solver and small device helpers.
+//
+// HBM layout (one context):
+//   rec  [ncell][RW] R   per-cell record gathered by neighbours: primitives
+//                        rho,u,v,p | gradient d/dx,d/dy of each | limiter
+//                        values | centroid (interior cells then ghosts)
+//   qT   [ncell] R       temperature (fifth primitive)
+//   gT   [noc][2] R      its gradient (only when a consumer needs it)
+//   qc   [ncell][4] f64  conserved state [E, rho, rho u, rho v] (ghosts last)
+//   qold [noc][4] f64, res [noc][4] f64, dt/dta/la [noc] f64
+//   frec [nof][FW] R     face record nx, ny, area, centre x, y
+//   fpt  [nof] int2      face endpoints (storage numbering of points)
+//   prec [nop][PW] R     vertex record: averaged primitives rho,u,v,p,T, mu
+//   sl_face/sl_nbr [MAXF][noc] int  the faces of each cell in ascending
+//                        reference face order, and the cell across each
+//                        (bit-inverted when the cell is the face's `out`)
+// R is double for the FP64 form and float for the bulk data of the mixed
+// form; states, residuals and their sums stay double in both.  Face and
+// vertex storage follows the cell order (first touch) so gathers stay local;
+// the reference order of every per-cell sum is carried by the slot order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ufx {
+
+constexpr int MAXF = 4;     // faces per cell (triangles, quadrilaterals)
+constexpr int BLK = 256;    // threads per block for cell/point/face kernels
+constexpr int NXCD = 8;     // MI355X: 8 accelerator complex dies, own L2 each
+constexpr int WALL = 1001, FARF = 2001;
+
+template <typename R> struct Layout;
+template <> struct Layout<double> { static constexpr int RW = 18, FW = 6, PW = 6; };
+template <> struct Layout<float> { static constexpr int RW = 20, FW = 6, PW = 6; };
+// record offsets
+constexpr int R_QP = 0, R_G = 4, R_PHI = 12, R_CEN = 16;
+constexpr int F_NX = 0, F_NY = 1, F_AREA = 2, F_CX = 3, F_CY = 4;
+constexpr int P_MU = 5;
+
+// error word codes
+constexpr int DERR_ROE = 1, DERR_POS = 2, DERR_AREA = 3;
+
+// scalar slots (double) and ordered-integer range accumulators
+constexpr int SC_NORM = 0, SC_FRES1 = 1, SC_DTMIN = 2, SC_NSCAL = 8;
+
+template <typename R>
+struct DevT {
+  using Real = R;
+  int nop, noc, nfi, nbf, nof, ncell;
+  // mesh
+  const R* frec;
+  const int2* fpt;
+  const int* fbc;            // bc code per boundary ordinal (nbf)
+  const int *bface, *bcell;  // storage face and inner cell per boundary ordinal
+  double* bres;              // [nbf][4] boundary-face fluxes of the current stage
+  const int* sl_face;        // [MAXF][noc]
+  const int* sl_nbr;         // [MAXF][noc]
+  const int* cnf;            // nf | nb << 8
+  const double *r11, *r12, *r22, *cv, *cov;  // noc (cov: ncell)
+  const int* nc2c;
+  const int *v2c_ptr, *v2c;  // point storage order
+  const double* wt;
+  const double *px, *py;     // point storage order
+  const int *fin, *fout;     // face sides (storage numbering of faces)
+  double* fw;                // a1 + a2 of the diamond path per face (GG_FC)
+  // LU-SGS
+  const int *lu_face, *lu_cnt;   // [MAXF][noc] reference c2f order
+  // state
+  R* rec;
+  R* qT;
+  R* gT;
+  double* qc;
+  double* qold;
+  double* res;
+  double* dt;
+  double* dta;
+  double* la;
+  double* fla;               // nof (LU-SGS)
+  double* fmu;               // nof (viscous)
+  R* cmul;                   // ncell (viscous)
+  R* prec;                   // nop * PW
+  R* fgrad;                  // nof * 10 (GG_FC)
+  double* bflux;             // [2][nbf]
+  double* part;              // reduction partials
+  double* scal;              // SC_* slots
+  unsigned long long* range; // [8]: max of q0..3 (ordered), min of q0..3
+  int* err;                  // [0] code [1] cell [2] first non-positive cell
+  double* lower;             // LU-SGS work [noc][4]
+  double* upper;
+};
+
+// Block index remapped so that each XCD (blocks are dealt to the 8 dies
+// round-robin) works on one contiguous slice of the cells: neighbour data
+// then mostly sits in the die's own L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  int q = nblk / NXCD, r = nblk % NXCD;
+  int x = b % NXCD, i = b / NXCD;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+__device__ __forceinline__ int cell_index() {
+  return xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
+}
+
+// Monotone map of doubles onto unsigned integers, for atomic max/min.
+__device__ __forceinline__ unsigned long long ord_of(double x) {
+  long long i = __double_as_longlong(x);
+  return i >= 0 ? (unsigned long long)i ^ 0x8000000000000000ULL : ~(unsigned long long)i;
+}
+__host__ __device__ inline double of_ord(unsigned long long u) {
+  long long i = (u & 0x8000000000000000ULL) ? (long long)(u ^ 0x8000000000000000ULL)
+                                            : (long long)~u;
+  double d;
+  static_assert(sizeof(d) == sizeof(i), "");
+  __builtin_memcpy(&d, &i, sizeof d);
+  return d;
+}
+
+// Deterministic block reduction (fixed shared-memory tree).
+template <int N, typename Op>
+__device__ __forceinline__ void block_reduce(double (*sh)[BLK], double* v, Op op) {
+  int t = threadIdx.x;
+  __syncthreads();
+  for (int k = 0; k < N; ++k) sh[k][t] = v[k];
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < N; ++k) sh[k][t] = op(sh[k][t], sh[k][t + s]);
+    __syncthreads();
+  }
+  for (int k = 0; k < N; ++k) v[k] = sh[k][0];
+}
+struct OpMax { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
+struct OpMin { __device__ double operator()(double a, double b) const { return a < b ? a : b; } };
+struct OpAdd { __device__ double operator()(double a, double b) const { return a + b; } };
+
+template <typename R>
+__device__ __forceinline__ void set_err(const DevT<R>& d, int code, int cell) {
+  if (atomicCAS(d.err, 0, code) == 0) d.err[1] = cell;
+}
+
+}  // namespace ufx

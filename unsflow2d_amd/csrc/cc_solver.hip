@@ -1,45 +1,21 @@
-// Device residual hot path of the cell-centred solver for gfx950, and its
-// C-ABI (include/ufx.h).
-//
-// Layout in HBM (structure of arrays, one array per variable):
-//   cells: qc[4][ncell], qp[5][ncell] (interior then ghost cells),
-//          qold/res/phi[4][noc], grad[10][noc] (d/dx, d/dy per variable),
-//          dt, la, dtA (= dt / area) [noc], centroids [2][ncell];
-//   faces: p1, p2, in, out, bc, nx, ny, area, centre [nof];
-//   per-cell face slots: sl_face[4][noc] (faces of the cell in ascending face
-//          order) and sl_nbr[4][noc] (the cell across the face, bit-inverted
-//          when the cell is the face's `out` side), plus cnf[noc] = number of
-//          faces | number of boundary faces << 8.
-// Every per-cell sum is a gather over the cell's slots in the order the
-// reference accumulates it (ascending face index for gradients and spectral
-// radii; boundary faces first, then interior faces, for the residual), so
-// the sums are deterministic and reproduce the reference's rounding order.
-// Interior face fluxes are evaluated by both adjacent cells with identical
-// inputs, which trades a second flux evaluation for the atomics and scatter
-// of a face loop.
+// Host side of the device residual hot path (gfx950) and its C-ABI
+// (include/ufx.h): context creation (storage layout, slots, LU-SGS level
+// schedule), one entry point per reference routine, whole iterations, and
+// state injection / read-back in the reference's per-cell record layout.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
-#include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
-#include "cc_physics.h"
+#include "cc_kernels.h"
 #include "ufx_internal.h"
 
-namespace ufx {
-
-static thread_local std::string g_err;
-void set_error(const std::string& msg) { g_err = msg; }
-
-constexpr int MAXF = 4;       // faces per cell (triangles and quadrilaterals)
-constexpr int BLK = 256;      // threads per block of the cell/point kernels
-constexpr int WALL = 1001, FARF = 2001;
-
-// error word codes (device)
-constexpr int DERR_ROE = 1, DERR_POS = 2, DERR_AREA = 3;
+using namespace ufx;
 
 #define HIPCHK(x)                                                                \
   do {                                                                           \
@@ -51,482 +27,42 @@ constexpr int DERR_ROE = 1, DERR_POS = 2, DERR_AREA = 3;
     }                                                                            \
   } while (0)
 
-// Device view of a context (passed by value to kernels).
-struct Dev {
-  int nop, noc, nfi, nbf, nof, ncell;
-  // mesh
-  const double *px, *py;
-  const int *fp1, *fp2, *fin, *fout, *fbc;
-  const double *fnx, *fny, *farea, *fcx, *fcy;
-  const double *ccx, *ccy, *ccv, *ccov, *r11, *r12, *r22;
-  const int *nc2c;
-  const int *sl_face, *sl_nbr, *cnf;
-  const int *lu_face, *lu_nf;
-  const int *v2c_ptr, *v2c;
-  const double *wt;
-  // state
-  double *qc, *qp, *qold, *res, *grad, *phi, *dt, *la, *dta;
-  double *fla, *fmu, *cmul, *pprim, *pmu, *fgrad;
-  double *bflux;     // [2][nbf] x/y momentum flux of each boundary face
-  double *part;      // reduction partials
-  double *scal;      // small scalars (see SC_*)
-  int *err;          // [0] code, [1] cell, [2] first non-positive cell (INT_MAX if none)
-};
-
-// scal[] slots
-constexpr int SC_QMAX = 0, SC_QMIN = 4, SC_NORM = 8, SC_FRES1 = 9, SC_DTMIN = 10, SC_NSCAL = 16;
+namespace {
 
 struct Params {
   double gamma, gamma1, znd, r_inf, u_inf, v_inf, p_inf, mref, t_wall, m_inf, rey, vll;
-  double aoa, xref, yref;
+  double aoa, xref, yref, cfl_max;
+  double qinf[4];
   int flow, iwall, flux, grad, ilimit, irs, timemode, precision;
-  double cfl_max;
 };
 
-// ------------------------------------------------------------- reductions
-// Deterministic block reduction: fixed shared-memory tree.
-template <int N, typename Op>
-__device__ __forceinline__ void block_reduce(double (*sh)[BLK], double* v, Op op) {
-  int t = threadIdx.x;
-  for (int k = 0; k < N; ++k) sh[k][t] = v[k];
-  __syncthreads();
-  for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (t < s)
-      for (int k = 0; k < N; ++k) sh[k][t] = op(sh[k][t], sh[k][t + s]);
-    __syncthreads();
-  }
-  for (int k = 0; k < N; ++k) v[k] = sh[k][0];
-}
+}  // namespace
 
-struct OpMax { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
-struct OpMin { __device__ double operator()(double a, double b) const { return a < b ? a : b; } };
-struct OpAdd { __device__ double operator()(double a, double b) const { return a + b; } };
-
-// ---------------------------------------------------------------- kernels
-
-// Primitive refresh of interior cells (CCFVM/main.F90:62-67), optionally
-// qold = qc (CCFVM/solution.F90:118-132), and per-block max/min of the four
-// primitives over all cells including ghosts (CCFVM/limiter.F90:146-149).
-__global__ void k_prim(Dev d, Gas<double> g, int do_prim, int do_old, int do_range) {
-  __shared__ double sh[4][BLK];
-  int c = blockIdx.x * BLK + threadIdx.x;
-  double mx[4], mn[4];
-  for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
-  if (c < d.ncell && !d.err[0]) {
-    if (c < d.noc && do_prim) {
-      double q[4], p[5];
-      for (int k = 0; k < 4; ++k) q[k] = d.qc[k * d.ncell + c];
-      con2prim(g, q, p);
-      for (int k = 0; k < 5; ++k) d.qp[k * d.ncell + c] = p[k];
-      // positivity of the refreshed primitives, CCFVM/main.F90:384-411
-      // (the reference checks them after the update; they are unchanged)
-      if (do_old && (p[0] <= 0.0 || p[3] <= 0.0)) {
-        atomicMin(&d.err[2], c + 1);
-        atomicCAS(d.err, 0, DERR_POS);
-      }
-      if (do_old)
-        for (int k = 0; k < 4; ++k) d.qold[k * d.noc + c] = q[k];
-      for (int k = 0; k < 4; ++k) mx[k] = mn[k] = p[k];
-    } else {
-      if (c < d.noc && do_old)
-        for (int k = 0; k < 4; ++k) d.qold[k * d.noc + c] = d.qc[k * d.ncell + c];
-      for (int k = 0; k < 4; ++k) mx[k] = mn[k] = d.qp[k * d.ncell + c];
-    }
-  }
-  if (!do_range) return;
-  block_reduce<4>(sh, mx, OpMax());
-  block_reduce<4>(sh, mn, OpMin());
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) {
-      d.part[blockIdx.x * 8 + k] = mx[k];
-      d.part[blockIdx.x * 8 + 4 + k] = mn[k];
-    }
-}
-
-// Final reduction of k_prim's partials into scal[SC_QMAX..SC_QMIN+3].
-__global__ void k_range_final(Dev d, int nblk) {
-  __shared__ double sh[4][BLK];
-  double mx[4], mn[4];
-  for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
-  for (int b = threadIdx.x; b < nblk; b += BLK)
-    for (int k = 0; k < 4; ++k) {
-      double a = d.part[b * 8 + k], m = d.part[b * 8 + 4 + k];
-      mx[k] = a > mx[k] ? a : mx[k];
-      mn[k] = m < mn[k] ? m : mn[k];
-    }
-  block_reduce<4>(sh, mx, OpMax());
-  block_reduce<4>(sh, mn, OpMin());
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) {
-      d.scal[SC_QMAX + k] = mx[k];
-      d.scal[SC_QMIN + k] = mn[k];
-    }
-}
-
-// Vertex averages with pseudo-Laplacian weights, CCFVM/average.F90:11-29.
-__global__ void k_avg_c2v(Dev d) {
-  int p = blockIdx.x * BLK + threadIdx.x;
-  if (p >= d.nop || d.err[0]) return;
-  double w = 0.0, con[6] = {0, 0, 0, 0, 0, 0};
-  for (int j = d.v2c_ptr[p]; j < d.v2c_ptr[p + 1]; ++j) {
-    int c = d.v2c[j];
-    double cw = d.wt[j];
-    w = w + cw;
-    for (int k = 0; k < 5; ++k) con[k] = con[k] + d.qp[k * d.ncell + c] * cw;
-    con[5] = con[5] + d.cmul[c] * cw;
-  }
-  for (int k = 0; k < 5; ++k) d.pprim[k * d.nop + p] = con[k] / w;
-  d.pmu[p] = con[5] / w;
-}
-
-// Face-averaged primitive (CCFVM/average.F90:32-36) computed where needed.
-__device__ __forceinline__ double face_prim(const Dev& d, int f, int k) {
-  return 0.5 * (d.pprim[k * d.nop + d.fp1[f]] + d.pprim[k * d.nop + d.fp2[f]]);
-}
-
-// Spectral radii and local time step, CCFVM/timestep.F90:98-177: each cell
-// sums its faces' radii in ascending face order; the `in` cell also stores
-// the face radius used by LU-SGS.
-__global__ void k_timestep(Dev d, Gas<double> g, double vll, double cfl) {
-  int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= d.noc || d.err[0]) return;
-  int nf = d.cnf[c] & 0xff;
-  double la = 0.0;
-  for (int s = 0; s < nf; ++s) {
-    int f = d.sl_face[s * d.noc + c];
-    double r0 = face_prim(d, f, 0), u0 = face_prim(d, f, 1), v0 = face_prim(d, f, 2),
-           p0 = face_prim(d, f, 3);
-    double a0 = sqrt(g.gamma * p0 / r0);
-    double un = u0 * d.fnx[f] + v0 * d.fny[f];
-    double ll = d.farea[f] * (fabs(un) + a0);
-    ll = ll + vll * d.fmu[f] / r0;
-    la = la + ll;
-    if (d.sl_nbr[s * d.noc + c] >= 0) d.fla[f] = ll;
-  }
-  d.la[c] = la;
-  double dt = cfl * d.ccv[c] / la;
-  d.dt[c] = dt;
-  d.dta[c] = dt / d.ccv[c];
-}
-
-__device__ __forceinline__ double vk_limit(double a, double b, double tol) {
-  double nr = a * a + 2.0 * a * b + tol * tol;
-  double dr = a * a + 2.0 * b * b + a * b + tol * tol;
-  return nr / dr;
-}
-
-// Venkatakrishnan limiter of one cell with the ratchet,
-// CCFVM/limiter.F90:123-296 (kappa 0.05 of the global range).
-__device__ __forceinline__ void limit_cell(const Dev& d, int c, const double* qpc,
-                                           const double* gc, double* phi) {
-  int info = d.cnf[c], nf = info & 0xff, nb = info >> 8;
-  double umax[4], umin[4];
-  for (int j = 0; j < 4; ++j) umax[j] = umin[j] = qpc[j];
-  for (int s = 0; s < nf; ++s) {
-    int n = d.sl_nbr[s * d.noc + c];
-    if (s < nf - nb) {
-      n = n >= 0 ? n : ~n;
-      for (int j = 0; j < 4; ++j) {
-        double q = d.qp[j * d.ncell + n];
-        umax[j] = umax[j] > q ? umax[j] : q;
-        umin[j] = umin[j] < q ? umin[j] : q;
-      }
-    } else {
-      int f = d.sl_face[s * d.noc + c];
-      for (int j = 0; j < 4; ++j) {
-        double v = face_prim(d, f, j);
-        umax[j] = umax[j] > v ? umax[j] : v;
-        umin[j] = umin[j] < v ? umin[j] : v;
-      }
-    }
-  }
-  const double kappa = 0.05;
-  double cx = d.ccx[c], cy = d.ccy[c];
-  for (int j = 0; j < 4; ++j) phi[j] = d.phi[j * d.noc + c];
-  for (int s = 0; s < nf; ++s) {
-    int f = d.sl_face[s * d.noc + c];
-    double dx = d.fcx[f] - cx, dy = d.fcy[f] - cy;
-    for (int j = 0; j < 4; ++j) {
-      double tol = kappa * (d.scal[SC_QMAX + j] - d.scal[SC_QMIN + j]);
-      double dl = gc[2 * j] * dx + gc[2 * j + 1] * dy;
-      double p0 = 1.0;
-      if (dl > 0.0) p0 = vk_limit(umax[j] - qpc[j], dl, tol);
-      else if (dl < 0.0) p0 = vk_limit(umin[j] - qpc[j], dl, tol);
-      phi[j] = phi[j] < p0 ? phi[j] : p0;
-    }
-  }
-}
-
-// Least-squares gradient (CCFVM/gradient.F90:2-93) of one cell, optionally
-// followed by the limiter; ascending face order, boundary faces last.
-__global__ void k_grad_lsq(Dev d, int do_grad, int do_limit) {
-  int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= d.noc || d.err[0]) return;
-  double qpc[5];
-  for (int k = 0; k < 5; ++k) qpc[k] = d.qp[k * d.ncell + c];
-  double gc[10];
-  if (do_grad) {
-    for (int k = 0; k < 10; ++k) gc[k] = 0.0;
-    int nf = d.cnf[c] & 0xff;
-    double cx = d.ccx[c], cy = d.ccy[c];
-    double r11 = d.r11[c], r12 = d.r12[c], r22 = d.r22[c];
-    for (int s = 0; s < nf; ++s) {
-      int f = d.sl_face[s * d.noc + c];
-      int n = d.sl_nbr[s * d.noc + c];
-      bool in_side = n >= 0;
-      n = in_side ? n : ~n;
-      double dx = d.ccx[n] - cx, dy = d.ccy[n] - cy;
-      double alfa1 = dx / r11 / r11;
-      double alfa2 = f < d.nfi ? (dy - dx * r12 / r11) / (r22 * r22)
-                               : (dy - dx * r12 / r11) / r22 / r22;
-      double wx = alfa1 - alfa2 * r12 / r11;
-      double wy = alfa2;
-      if (in_side) {
-        for (int k = 0; k < 5; ++k) {
-          double var = d.qp[k * d.ncell + n] - qpc[k];
-          gc[2 * k] = gc[2 * k] + var * wx;
-          gc[2 * k + 1] = gc[2 * k + 1] + var * wy;
-        }
-      } else {
-        for (int k = 0; k < 5; ++k) {
-          double var = qpc[k] - d.qp[k * d.ncell + n];
-          gc[2 * k] = gc[2 * k] - var * wx;
-          gc[2 * k + 1] = gc[2 * k + 1] - var * wy;
-        }
-      }
-    }
-    for (int k = 0; k < 10; ++k) d.grad[k * d.noc + c] = gc[k];
-  } else {
-    for (int k = 0; k < 10; ++k) gc[k] = d.grad[k * d.noc + c];
-  }
-  if (do_limit) {
-    double phi[4];
-    limit_cell(d, c, qpc, gc, phi);
-    for (int j = 0; j < 4; ++j) d.phi[j * d.noc + c] = phi[j];
-  }
-}
-
-template <int FLUX>
-__device__ __forceinline__ bool face_flux(const Gas<double>& g, double* ql, double* qr, double nx,
-                                          double ny, double area, double* F) {
-  if (FLUX == UFX_FLUX_ROE) return roe_flux(g, ql, qr, nx, ny, area, F);
-  if (FLUX == UFX_FLUX_AUSM) ausm_flux(g, ql, qr, nx, ny, F);
-  if (FLUX == UFX_FLUX_VANLEER) vanleer_flux(g, ql, qr, nx, ny, area, F);
-  if (FLUX == UFX_FLUX_RUSANOV) rusanov_flux(g, ql, qr, nx, ny, area, F);
-  return true;
-}
-
-__device__ __forceinline__ void set_err(const Dev& d, int code, int cell) {
-  if (atomicCAS(d.err, 0, code) == 0) d.err[1] = cell;
-}
-
-// Residual of one cell (CCFVM/fvresidual.F90:16-171) as a gather over its
-// faces: boundary faces first (farfield state or wall ghost, flux, ghost
-// update, wall force), then interior faces with the MUSCL reconstruction of
-// recon (:190-252) on both sides.  mode 0 stores the residual; mode 1 also
-// applies the Runge-Kutta stage update of CCFVM/main.F90:92-101 in place
-// (no other cell reads this cell's qc during a stage); mode 2 does both and
-// adds the squared residual to a block partial for the norm.
-template <int FLUX>
-__global__ void k_residual(Dev d, Gas<double> g, int viscous_wall, int iwall, double t_wall,
-                           int mode, double ak, double bk) {
-  __shared__ double sh[1][BLK];
-  int c = blockIdx.x * BLK + threadIdx.x;
-  double sq = 0.0;
-  if (c < d.noc && !d.err[0]) {
-    int info = d.cnf[c], nf = info & 0xff, nb = info >> 8;
-    double qpc[4], gc[8], phic[4];
-    for (int k = 0; k < 4; ++k) qpc[k] = d.qp[k * d.ncell + c];
-    for (int k = 0; k < 8; ++k) gc[k] = d.grad[k * d.noc + c];
-    for (int k = 0; k < 4; ++k) phic[k] = d.phi[k * d.noc + c];
-    double cx = d.ccx[c], cy = d.ccy[c];
-    double r[4] = {0.0, 0.0, 0.0, 0.0};
-    bool ok = true;
-    // boundary faces, CCFVM/fvresidual.F90:27-130
-    for (int s = nf - nb; s < nf; ++s) {
-      int f = d.sl_face[s * d.noc + c];
-      int gh = d.sl_nbr[s * d.noc + c];
-      int bc = d.fbc[f];
-      if (bc != FARF && bc != WALL) continue;
-      double nx = d.fnx[f], ny = d.fny[f], area = d.farea[f];
-      double dx = d.fcx[f] - cx, dy = d.fcy[f] - cy;
-      double ql[4], qr[4], F[4];
-      for (int i = 0; i < 4; ++i) ql[i] = qpc[i] + phic[i] * (gc[2 * i] * dx + gc[2 * i + 1] * dy);
-      double gq[4], gp[5];
-      if (bc == FARF) {
-        farfield_state(g, ql, nx, ny, qr);
-        ok = face_flux<FLUX>(g, ql, qr, nx, ny, area, F) && ok;
-        prim2con(g, qr, gq);
-        con2prim(g, gq, gp);
-      } else {
-        double q[4];
-        for (int k = 0; k < 4; ++k) q[k] = d.qc[k * d.ncell + c];
-        if (!viscous_wall) {
-          // slip_wall, CCFVM/boundCond.F90:46-81
-          double rho = q[1], u = q[2] / q[1], v = q[3] / q[1];
-          double un = u * nx + v * ny;
-          gq[0] = q[0];
-          gq[1] = q[1];
-          gq[2] = rho * u - 2.0 * un * nx;
-          gq[3] = rho * v - 2.0 * un * ny;
-          con2prim(g, gq, gp);
-        } else if (iwall == 1) {
-          // adiabatic no-slip, CCFVM/boundCond.F90:31-37
-          gq[0] = q[0];
-          gq[1] = q[1];
-          gq[2] = -q[2];
-          gq[3] = -q[3];
-          con2prim(g, gq, gp);
-        } else {
-          // isothermal no-slip, CCFVM/boundCond.F90:19-30
-          double pin[5];
-          con2prim(g, q, pin);
-          double tg = 2.0 * t_wall - pin[4];
-          if (tg <= 0.0) tg = 0.01 * t_wall;
-          gp[0] = pin[3] / (pin[0] * g.znd);
-          gp[1] = -pin[1];
-          gp[2] = -pin[2];
-          gp[3] = pin[3];
-          gp[4] = tg;
-          prim2con(g, gp, gq);
-        }
-        for (int i = 0; i < 4; ++i) qr[i] = gp[i];
-        ok = face_flux<FLUX>(g, ql, qr, nx, ny, area, F) && ok;
-        int k = f - d.nfi;
-        d.bflux[k] = F[2];
-        d.bflux[d.nbf + k] = F[3];
-      }
-      for (int i = 0; i < 4; ++i) d.qc[i * d.ncell + gh] = gq[i];
-      for (int i = 0; i < 5; ++i) d.qp[i * d.ncell + gh] = gp[i];
-      for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i];
-    }
-    // interior faces, CCFVM/fvresidual.F90:133-171
-    const double zhi = 1.0 / 3.0;
-    for (int s = 0; s < nf - nb; ++s) {
-      int f = d.sl_face[s * d.noc + c];
-      int n = d.sl_nbr[s * d.noc + c];
-      bool in_side = n >= 0;
-      n = in_side ? n : ~n;
-      double qpn[4], gn[8], phin[4];
-      for (int k = 0; k < 4; ++k) qpn[k] = d.qp[k * d.ncell + n];
-      for (int k = 0; k < 8; ++k) gn[k] = d.grad[k * d.noc + n];
-      for (int k = 0; k < 4; ++k) phin[k] = d.phi[k * d.noc + n];
-      double fx = d.fcx[f], fy = d.fcy[f];
-      const double *qa = in_side ? qpc : qpn, *qb = in_side ? qpn : qpc;
-      const double *ga = in_side ? gc : gn, *gb = in_side ? gn : gc;
-      const double *pa = in_side ? phic : phin, *pb = in_side ? phin : phic;
-      double ax = in_side ? cx : d.ccx[n], ay = in_side ? cy : d.ccy[n];
-      double bx = in_side ? d.ccx[n] : cx, by = in_side ? d.ccy[n] : cy;
-      double dlx = fx - ax, dly = fy - ay, drx = fx - bx, dry = fy - by;
-      double ql[4], qr[4], F[4];
-      for (int i = 0; i < 4; ++i) {
-        double alfa = ga[2 * i] * dlx + ga[2 * i + 1] * dly;
-        double beta = qb[i] - qa[i];
-        ql[i] = qa[i] + 0.5 * pa[i] * (zhi * beta + (1.0 - zhi) * alfa);
-        alfa = gb[2 * i] * drx + gb[2 * i + 1] * dry;
-        beta = qa[i] - qb[i];
-        qr[i] = qb[i] + 0.5 * pb[i] * (zhi * beta + (1.0 - zhi) * alfa);
-      }
-      ok = face_flux<FLUX>(g, ql, qr, d.fnx[f], d.fny[f], d.farea[f], F) && ok;
-      if (in_side)
-        for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i];
-      else
-        for (int i = 0; i < 4; ++i) r[i] = r[i] - F[i];
-    }
-    if (!ok) set_err(d, DERR_ROE, c + 1);
-    for (int i = 0; i < 4; ++i) d.res[i * d.noc + c] = r[i];
-    if (mode >= 1) {
-      double dta = d.dta[c];
-      for (int i = 0; i < 4; ++i) {
-        double c1 = d.qold[i * d.noc + c], c2 = d.qc[i * d.ncell + c];
-        d.qc[i * d.ncell + c] = ak * c1 + bk * (c2 - dta * r[i]);
-      }
-    }
-    if (mode == 2)
-      for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];
-  }
-  if (mode == 2) {
-    block_reduce<1>(sh, &sq, OpAdd());
-    if (threadIdx.x == 0) d.part[blockIdx.x] = sq;
-  }
-}
-
-// Runge-Kutta stage update alone, CCFVM/main.F90:92-101.
-__global__ void k_rk_update(Dev d, double ak, double bk) {
-  int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= d.noc || d.err[0]) return;
-  double dta = d.dta[c];
-  for (int i = 0; i < 4; ++i) {
-    double c1 = d.qold[i * d.noc + c], c2 = d.qc[i * d.ncell + c];
-    d.qc[i * d.ncell + c] = ak * c1 + bk * (c2 - dta * d.res[i * d.noc + c]);
-  }
-}
-
-// Sum of squared residuals per block (CCFVM/solution.F90:152-157).
-__global__ void k_sumsq(Dev d) {
-  __shared__ double sh[1][BLK];
-  int c = blockIdx.x * BLK + threadIdx.x;
-  double sq = 0.0;
-  if (c < d.noc)
-    for (int i = 0; i < 4; ++i) {
-      double r = d.res[i * d.noc + c];
-      sq = sq + r * r;
-    }
-  block_reduce<1>(sh, &sq, OpAdd());
-  if (threadIdx.x == 0) d.part[blockIdx.x] = sq;
-}
-
-// L2 norm from the partials (CCFVM/solution.F90:159-166): stores the raw norm,
-// the iteration-1 normalisation and the normalised value of iteration k.
-__global__ void k_norm_final(Dev d, int nblk, int first, double* hist, int k) {
-  __shared__ double sh[1][BLK];
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
-  block_reduce<1>(sh, &s, OpAdd());
-  if (threadIdx.x == 0) {
-    double fres = sqrt(s / d.noc / 4);
-    d.scal[SC_NORM] = fres;
-    if (first) d.scal[SC_FRES1] = fres;
-    double f1 = d.scal[SC_FRES1];
-    if (hist) hist[k] = f1 >= 2.220446049250313e-16 ? fres / f1 : fres;
-  }
-}
-
-// Positivity of the interior primitives, CCFVM/main.F90:384-411: the first
-// failing cell (lowest index) goes to err[2].
-__global__ void k_positivity(Dev d) {
-  int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= d.noc) return;
-  if (d.qp[c] <= 0.0 || d.qp[3 * d.ncell + c] <= 0.0) atomicMin(&d.err[2], c + 1);
-}
-
-// Minimum local time step (the screen's global dt), CCFVM/timestep.F90:111-119.
-__global__ void k_dtmin(Dev d) {
-  __shared__ double sh[1][BLK];
-  double m = INFINITY;
-  for (int c = threadIdx.x; c < d.noc; c += BLK) m = d.dt[c] < m ? d.dt[c] : m;
-  block_reduce<1>(sh, &m, OpMin());
-  if (threadIdx.x == 0) d.scal[SC_DTMIN] = m;
-}
-
-}  // namespace ufx
-
-using namespace ufx;
-
-// --------------------------------------------------------------- context
 struct ufx_cc {
-  Dev d{};
   Params p{};
   Gas<double> g{};
+  Gas<float> gf{};
+  DevT<double> d64{};
+  DevT<float> d32{};
+  int nop = 0, noc = 0, nfi = 0, nbf = 0, nof = 0, ncell = 0;
+  double cfl = 0.0;              // CFL of the current iteration (param's CFL)
+  bool store_fla = false;        // store face spectral radii in any mode
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
-  int nblk_cell = 0, nblk_all = 0;
-  int fres1_set = 0;
+  int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
   double* d_hist = nullptr;
   int hist_cap = 0;
-  double fres1_host = 0.0;
-  std::vector<int> wall_faces;     // boundary ordinals of wall faces
-  // kernel timing: event pool and (category, begin, end) records
+  double* cirs_tmp = nullptr;    // [2][noc][4]
+  double* bfluxv = nullptr;      // viscous wall force terms [2][nbf]
+  std::vector<int> face_of_ref;  // reference face -> storage face
+  std::vector<int> pt_of_ref;    // reference vertex -> storage vertex
+  std::vector<int> wall_k;       // boundary ordinals of wall faces
+  std::vector<int> fin_h, fp1_h, fp2_h;  // boundary faces (reference numbering)
+  std::vector<double> fnx_h, fny_h, farea_h, px_h, py_h;
+  // LU-SGS level schedule: cells grouped by level, forward then backward
+  int* lv_cells = nullptr;
+  std::vector<int> lv_fwd_off, lv_bwd_off;
+  // kernel timing
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
@@ -575,27 +111,7 @@ void prof_end(ufx_cc* x, int r) {
 
 namespace {
 
-template <typename T>
-int dalloc(ufx_cc* x, T** p, size_t n) {
-  if (n == 0) n = 1;
-  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
-  x->allocs.push_back((void*)*p);
-  HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), x->stream));
-  return UFX_OK;
-}
-
-template <typename T>
-int dupload(ufx_cc* x, const T** dst, const std::vector<T>& src) {
-  T* p = nullptr;
-  int rc = dalloc(x, &p, src.size());
-  if (rc) return rc;
-  if (!src.empty())
-    HIPCHK(hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, x->stream));
-  *dst = p;
-  return UFX_OK;
-}
-
-int nblocks(int n) { return (n + BLK - 1) / BLK; }
+int nblocks(int n) { return std::max(1, (n + BLK - 1) / BLK); }
 
 int launch_check() {
   hipError_t e = hipGetLastError();
@@ -614,212 +130,505 @@ int check_ctx(ufx_cc* x) {
   return UFX_OK;
 }
 
+template <typename T>
+int dalloc(ufx_cc* x, T** p, size_t n) {
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+  x->allocs.push_back((void*)*p);
+  HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(T), x->stream));
+  return UFX_OK;
+}
+
+template <typename T>
+int dupload(ufx_cc* x, T** dst, const std::vector<T>& src) {
+  T* p = nullptr;
+  if (int rc = dalloc(x, &p, src.size())) return rc;
+  if (!src.empty())
+    HIPCHK(hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, x->stream));
+  *dst = p;
+  return UFX_OK;
+}
+
+template <typename T>
+int dupload(ufx_cc* x, const T** dst, const std::vector<T>& src) {
+  T* p = nullptr;
+  int rc = dupload(x, &p, src);
+  *dst = p;
+  return rc;
+}
+
+int* err_word(ufx_cc* x) { return x->p.precision == UFX_PREC_FP64 ? x->d64.err : x->d32.err; }
+
 int device_error(ufx_cc* x) {
   int e[3] = {0, 0, 0};
-  HIPCHK(hipMemcpyAsync(e, x->d.err, sizeof(e), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(e, err_word(x), sizeof(e), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
-  if (e[0] == DERR_POS) e[1] = e[2];
-  if (e[0] == DERR_ROE) {
-    set_error("Roe scheme: sonic speed is negative (cell " + std::to_string(e[1]) + ")");
-    return UFX_ERR_STATE;
+  if (e[0] == 0) return UFX_OK;
+  if (e[0] == DERR_ROE) set_error("Roe scheme: sonic speed is negative (cell " + std::to_string(e[1]) + ")");
+  else if (e[0] == DERR_POS) set_error("Density/pressure is negative at point " + std::to_string(e[2]));
+  else if (e[0] == DERR_AREA) set_error("ggfc,gradtrixy,  -ve area (face " + std::to_string(e[1]) + ")");
+  else set_error("device error " + std::to_string(e[0]));
+  return UFX_ERR_STATE;
+}
+
+// Run f(d) on the device view of the context's precision.
+template <typename F>
+int with_dev(ufx_cc* x, F&& f) {
+  if (x->p.precision == UFX_PREC_FP64) return f(x->d64);
+  return f(x->d32);
+}
+
+template <typename R>
+const Gas<R>& gas_of(ufx_cc* x);
+template <> const Gas<double>& gas_of<double>(ufx_cc* x) { return x->g; }
+template <> const Gas<float>& gas_of<float>(ufx_cc* x) { return x->gf; }
+
+// ---- per-routine launchers (templated on the bulk type)
+
+template <typename R>
+int run_prim(ufx_cc* x, DevT<R>& d, int do_prim, int do_old, int do_range) {
+  LAUNCH(x, UFX_K_PRIM, k_prim<R>, x->nblk_all, d, x->g, do_prim, do_old, do_range);
+  return launch_check();
+}
+
+template <typename R>
+int run_range_final(ufx_cc* x, DevT<R>& d) {
+  LAUNCH(x, UFX_K_RANGE, k_range_final<R>, 1, d, x->nblk_all);
+  return launch_check();
+}
+
+template <typename R>
+int run_range(ufx_cc* x, DevT<R>& d) {
+  if (int rc = run_prim(x, d, 0, 0, 1)) return rc;
+  return run_range_final(x, d);
+}
+
+template <typename R>
+int run_avg(ufx_cc* x, DevT<R>& d) {
+  LAUNCH(x, UFX_K_AVG, k_avg_c2v<R>, x->nblk_pt, d, x->p.flow != UFX_FLOW_INVISCID);
+  return launch_check();
+}
+
+// k_cell with the options of one call; with_t when the temperature gradient
+// has a consumer (the drop-in gradient routines always produce it).
+template <typename R, int GR>
+int run_cell_g(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl) {
+  // face spectral radii are only consumed by LU-SGS; the drop-in time step
+  // routine always stores them, as the reference does
+  int fla = (x->p.timemode == UFX_TIME_LUSGS || x->store_fla) ? 1 : 0;
+  double vll = x->p.vll;
+  int cat = GR != G_NONE ? UFX_K_GRAD : (tstep ? UFX_K_TSTEP : UFX_K_GRAD);
+#define KC(WT, LM, TS) \
+  LAUNCH(x, cat, (k_cell<R, GR, WT, LM, TS>), x->nblk_cell, d, x->g, vll, cfl, fla)
+  if (with_t) {
+    if (limit) { if (tstep) KC(true, true, true); else KC(true, true, false); }
+    else { if (tstep) KC(true, false, true); else KC(true, false, false); }
+  } else {
+    if (limit) { if (tstep) KC(false, true, true); else KC(false, true, false); }
+    else { if (tstep) KC(false, false, true); else KC(false, false, false); }
   }
-  if (e[0] == DERR_POS) {
-    set_error("Density/pressure is negative at cell " + std::to_string(e[1]));
-    return UFX_ERR_STATE;
+#undef KC
+  return launch_check();
+}
+
+template <typename R>
+int run_ggfc_face(ufx_cc* x, DevT<R>& d) {
+  LAUNCH(x, UFX_K_GRAD, k_ggfc_face<R>, x->nblk_face, d);
+  return launch_check();
+}
+
+// The gradient of the context's method, optionally fused with the limiter
+// and the time step (the reference calls limit from inside each gradient
+// routine when ILIMIT is set).
+template <typename R>
+int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl) {
+  switch (x->p.grad) {
+    case UFX_GRAD_GG: return run_cell_g<R, G_GG>(x, d, with_t, limit, tstep, cfl);
+    case UFX_GRAD_GGFC:
+      if (int rc = run_ggfc_face(x, d)) return rc;
+      return run_cell_g<R, G_GGFC>(x, d, with_t, limit, tstep, cfl);
+    default: return run_cell_g<R, G_LSQR>(x, d, with_t, limit, tstep, cfl);
   }
-  if (e[0] != 0) {
-    set_error("device error " + std::to_string(e[0]) + " at cell " + std::to_string(e[1]));
-    return UFX_ERR_STATE;
+}
+
+template <typename R>
+int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int write_res) {
+  int visc = x->p.flow != UFX_FLOW_INVISCID;
+  const Gas<R>& gr = gas_of<R>(x);
+  int nbb = nblocks(x->nbf);
+#define KS(F)                                                                                   \
+  do {                                                                                          \
+    if (x->nbf > 0)                                                                             \
+      LAUNCH(x, UFX_K_BFACE, (k_bface<R, F>), nbb, d, x->g, visc, x->p.iwall, x->p.t_wall);    \
+    LAUNCH(x, UFX_K_STAGE, (k_stage<R, F>), x->nblk_cell, d, gr, mode, ak, bk, write_res);      \
+  } while (0)
+  switch (x->p.flux) {
+    case UFX_FLUX_ROE: KS(1); break;
+    case UFX_FLUX_AUSM: KS(2); break;
+    case UFX_FLUX_VANLEER: KS(3); break;
+    default: KS(4); break;
+  }
+#undef KS
+  return launch_check();
+}
+
+// Two Jacobi sweeps of the residual smoothing with the smoothing factor of
+// the current CFL, CCFVM/ImpResiSmoothening.F90:13-44 (OldRes = residual on
+// entry).
+template <typename R>
+int run_cirs(ufx_cc* x, DevT<R>& d) {
+  double rr = x->cfl / x->p.cfl_max;
+  double eps = std::max(0.25 * (rr * rr - 1.0), 0.0);
+  double* old = x->cirs_tmp;
+  double* tmp = x->cirs_tmp + (size_t)4 * x->noc;
+  HIPCHK(hipMemcpyAsync(old, d.res, (size_t)32 * x->noc, hipMemcpyDeviceToDevice, x->stream));
+  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, d.res, old, tmp);
+  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, tmp, old, d.res);
+  return launch_check();
+}
+
+// fvresidual, CCFVM/fvresidual.F90:4-187: inviscid faces, then the viscous
+// flux (which first recomputes the diamond-path gradient and the limiter
+// unless that is the gradient method, CCFVM/viscous_flux.F90:17) and the
+// residual smoothing.
+template <typename R>
+int run_residual_full(ufx_cc* x, DevT<R>& d) {
+  if (int rc = run_stage_kernel(x, d, 0, 0.0, 0.0, 1)) return rc;
+  if (x->p.flow != UFX_FLOW_INVISCID) {
+    if (x->p.grad != UFX_GRAD_GGFC) {
+      if (int rc = run_ggfc_face(x, d)) return rc;
+      if (x->p.ilimit)
+        if (int rc = run_range(x, d)) return rc;
+      if (int rc = run_cell_g<R, G_GGFC>(x, d, true, x->p.ilimit != 0, false, 0.0)) return rc;
+    }
+    LAUNCH(x, UFX_K_STAGE, k_visc_mul<R>, x->nblk_cell, d, x->g);
+    LAUNCH(x, UFX_K_STAGE, k_visc_res<R>, x->nblk_cell, d, x->g, x->p.rey, x->p.m_inf, x->bfluxv);
+    if (int rc = launch_check()) return rc;
+  }
+  if (x->p.irs)
+    if (int rc = run_cirs(x, d)) return rc;
+  return UFX_OK;
+}
+
+template <typename R>
+int run_rk_update(ufx_cc* x, DevT<R>& d, double ak, double bk) {
+  LAUNCH(x, UFX_K_STAGE, k_rk_update<R>, x->nblk_cell, d, ak, bk);
+  return launch_check();
+}
+
+const double RK_A[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+const double RK_B[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+
+// One Runge-Kutta stage: residual and update fused in one pass when nothing
+// else contributes to the residual, else the full residual then the update.
+template <typename R>
+int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res) {
+  double ak = RK_A[stage - 1], bk = RK_B[stage - 1];
+  if (x->p.flow == UFX_FLOW_INVISCID && !x->p.irs)
+    return run_stage_kernel(x, d, norm ? 2 : 1, ak, bk, write_res ? 1 : 0);
+  if (int rc = run_residual_full(x, d)) return rc;
+  if (int rc = run_rk_update(x, d, ak, bk)) return rc;
+  if (norm) {
+    LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
+    return launch_check();
   }
   return UFX_OK;
 }
 
-// Per-name descriptor of a state array for set/get: device pointer, number of
-// variables, stride between variables (entities), entity count.
-struct Field {
-  double* ptr;
-  int nvar;
-  int stride;
-  int count;
-  bool writable;
-};
+// lusgs2, CCFVM/lusgs.F90:207-379: residual, level-scheduled forward and
+// backward sweeps, update.
+template <typename R>
+int run_lusgs(ufx_cc* x, DevT<R>& d) {
+  if (int rc = run_residual_full(x, d)) return rc;
+  int nlv = (int)x->lv_fwd_off.size() - 1;
+  for (int l = 0; l < nlv; ++l) {
+    int a = x->lv_fwd_off[l], n = x->lv_fwd_off[l + 1] - a;
+    LAUNCH(x, UFX_K_OTHER, k_lusgs_level<R>, nblocks(n), d, x->g, x->lv_cells + a, n, 0);
+  }
+  nlv = (int)x->lv_bwd_off.size() - 1;
+  for (int l = 0; l < nlv; ++l) {
+    int a = x->lv_bwd_off[l], n = x->lv_bwd_off[l + 1] - a;
+    LAUNCH(x, UFX_K_OTHER, k_lusgs_level<R>, nblocks(n), d, x->g, x->lv_cells + a, n, 1);
+  }
+  LAUNCH(x, UFX_K_OTHER, k_lusgs_final<R>, x->nblk_cell, d);
+  return launch_check();
+}
 
-bool field(ufx_cc* x, const char* name, Field& f) {
-  Dev& d = x->d;
-  std::string n(name);
-  if (n == "qc") f = {d.qc, 4, d.ncell, d.ncell, true};
-  else if (n == "qp") f = {d.qp, 5, d.ncell, d.ncell, true};
-  else if (n == "qold") f = {d.qold, 4, d.noc, d.noc, true};
-  else if (n == "res") f = {d.res, 4, d.noc, d.noc, true};
-  else if (n == "phi") f = {d.phi, 4, d.noc, d.noc, true};
-  else if (n == "grad") f = {d.grad, 10, d.noc, d.noc, true};
-  else if (n == "dt") f = {d.dt, 1, d.noc, d.noc, true};
-  else if (n == "la") f = {d.la, 1, d.noc, d.noc, true};
-  else if (n == "fc_la") f = {d.fla, 1, d.nof, d.nof, true};
-  else if (n == "fc_mu") f = {d.fmu, 1, d.nof, d.nof, true};
-  else if (n == "cell_mul") f = {d.cmul, 1, d.ncell, d.ncell, true};
-  else if (n == "pt_prim") f = {d.pprim, 5, d.nop, d.nop, true};
-  else if (n == "fc_grad") f = {d.fgrad, 10, d.nof, d.nof, true};
-  else return false;
-  return true;
+int ensure_hist(ufx_cc* x, int n) {
+  if (n <= x->hist_cap) return UFX_OK;
+  if (x->d_hist) HIPCHK(hipFree(x->d_hist));
+  x->d_hist = nullptr;
+  HIPCHK(hipMalloc((void**)&x->d_hist, (size_t)n * sizeof(double)));
+  x->hist_cap = n;
+  return UFX_OK;
+}
+
+// One whole iteration of CCFVM/main.F90:59-115.
+template <typename R>
+int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
+  bool lim = x->p.ilimit != 0;
+  bool visc = x->p.flow != UFX_FLOW_INVISCID;
+  // prim refresh, qold (save_old commutes with avg_c2v and the time step),
+  // positivity and the global range (re-armed by the previous norm)
+  if (int rc = run_prim(x, d, 1, 1, lim ? 1 : 0)) return rc;
+  if (lim)
+    if (int rc = run_range_final(x, d)) return rc;
+  if (int rc = run_avg(x, d)) return rc;
+  // gradient, limiter and time step in one pass over the cells; the
+  // temperature gradient only when the viscous flux needs it
+  if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
+  if (x->p.timemode == UFX_TIME_RK3) {
+    for (int s = 1; s <= 3; ++s)
+      if (int rc = run_stage(x, d, s, s == 3, false)) return rc;
+  } else {
+    if (int rc = run_lusgs(x, d)) return rc;
+    LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
+  }
+  LAUNCH(x, UFX_K_NORM, k_norm_final<R>, 1, d, x->nblk_cell, iter == 1, x->d_hist, k);
+  return launch_check();
+}
+
+// ---- state transfer in the reference's per-cell record layout
+
+template <typename R>
+int get_field(ufx_cc* x, DevT<R>& d, const std::string& n, double* dst) {
+  constexpr int RW = Layout<R>::RW, PW = Layout<R>::PW;
+  auto copy = [&](void* h, const void* dv, size_t bytes) {
+    return hipMemcpyAsync(h, dv, bytes, hipMemcpyDeviceToHost, x->stream);
+  };
+  const int noc = x->noc, ncell = x->ncell, nof = x->nof, nop = x->nop;
+  if (n == "qc" || n == "qold" || n == "res") {
+    size_t cnt = n == "qc" ? (size_t)ncell : (size_t)noc;
+    const double* src = n == "qc" ? d.qc : (n == "qold" ? d.qold : d.res);
+    HIPCHK(copy(dst, src, cnt * 32));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  if (n == "dt" || n == "la") {
+    HIPCHK(copy(dst, n == "dt" ? d.dt : d.la, (size_t)noc * 8));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  if (n == "qp" || n == "grad" || n == "phi") {
+    std::vector<R> rec((size_t)ncell * RW), t((size_t)ncell), gt((size_t)noc * 2);
+    HIPCHK(copy(rec.data(), d.rec, rec.size() * sizeof(R)));
+    HIPCHK(copy(t.data(), d.qT, t.size() * sizeof(R)));
+    HIPCHK(copy(gt.data(), d.gT, gt.size() * sizeof(R)));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (n == "qp")
+      for (int c = 0; c < ncell; ++c) {
+        for (int k = 0; k < 4; ++k) dst[(size_t)c * 5 + k] = rec[(size_t)c * RW + R_QP + k];
+        dst[(size_t)c * 5 + 4] = t[c];
+      }
+    else if (n == "grad")
+      for (int c = 0; c < noc; ++c) {
+        for (int k = 0; k < 8; ++k) dst[(size_t)c * 10 + k] = rec[(size_t)c * RW + R_G + k];
+        dst[(size_t)c * 10 + 8] = gt[(size_t)c * 2];
+        dst[(size_t)c * 10 + 9] = gt[(size_t)c * 2 + 1];
+      }
+    else
+      for (int c = 0; c < noc; ++c)
+        for (int k = 0; k < 4; ++k) dst[(size_t)c * 4 + k] = rec[(size_t)c * RW + R_PHI + k];
+    return UFX_OK;
+  }
+  if (n == "fc_la" || n == "fc_mu") {
+    std::vector<double> t(nof);
+    HIPCHK(copy(t.data(), n == "fc_la" ? d.fla : d.fmu, (size_t)nof * 8));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int f = 0; f < nof; ++f) dst[f] = t[x->face_of_ref[f]];
+    return UFX_OK;
+  }
+  if (n == "fc_grad") {
+    std::vector<R> t((size_t)nof * 10);
+    HIPCHK(copy(t.data(), d.fgrad, t.size() * sizeof(R)));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int f = 0; f < nof; ++f)
+      for (int k = 0; k < 10; ++k) dst[(size_t)f * 10 + k] = t[(size_t)x->face_of_ref[f] * 10 + k];
+    return UFX_OK;
+  }
+  if (n == "pt_prim") {
+    std::vector<R> t((size_t)nop * PW);
+    HIPCHK(copy(t.data(), d.prec, t.size() * sizeof(R)));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int i = 0; i < nop; ++i)
+      for (int k = 0; k < 5; ++k) dst[(size_t)i * 5 + k] = t[(size_t)x->pt_of_ref[i] * PW + k];
+    return UFX_OK;
+  }
+  if (n == "cell_mul") {
+    std::vector<R> t((size_t)ncell);
+    HIPCHK(copy(t.data(), d.cmul, t.size() * sizeof(R)));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int c = 0; c < ncell; ++c) dst[c] = t[c];
+    return UFX_OK;
+  }
+  set_error("ufx_cc_get: unknown field " + n);
+  return UFX_ERR_ARG;
+}
+
+template <typename R>
+int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
+  constexpr int RW = Layout<R>::RW;
+  auto up = [&](void* dv, const void* h, size_t bytes) {
+    return hipMemcpyAsync(dv, h, bytes, hipMemcpyHostToDevice, x->stream);
+  };
+  const int noc = x->noc, ncell = x->ncell, nof = x->nof;
+  if (n == "qc" || n == "qold" || n == "res") {
+    size_t cnt = n == "qc" ? (size_t)ncell : (size_t)noc;
+    double* dv = n == "qc" ? d.qc : (n == "qold" ? d.qold : d.res);
+    HIPCHK(up(dv, src, cnt * 32));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  if (n == "dt" || n == "la") {
+    HIPCHK(up(n == "dt" ? d.dt : d.la, src, (size_t)noc * 8));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (n == "dt") {
+      // dt / area, which the stage update uses
+      std::vector<double> cv(noc), dta(noc);
+      HIPCHK(hipMemcpyAsync(cv.data(), d.cv, (size_t)noc * 8, hipMemcpyDeviceToHost, x->stream));
+      HIPCHK(hipStreamSynchronize(x->stream));
+      for (int c = 0; c < noc; ++c) dta[c] = src[c] / cv[c];
+      HIPCHK(up(d.dta, dta.data(), (size_t)noc * 8));
+      HIPCHK(hipStreamSynchronize(x->stream));
+    }
+    return UFX_OK;
+  }
+  if (n == "qp" || n == "grad" || n == "phi") {
+    std::vector<R> rec((size_t)ncell * RW), t((size_t)ncell), gt((size_t)noc * 2);
+    HIPCHK(hipMemcpyAsync(rec.data(), d.rec, rec.size() * sizeof(R), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(t.data(), d.qT, t.size() * sizeof(R), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(gt.data(), d.gT, gt.size() * sizeof(R), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (n == "qp")
+      for (int c = 0; c < ncell; ++c) {
+        for (int k = 0; k < 4; ++k) rec[(size_t)c * RW + R_QP + k] = (R)src[(size_t)c * 5 + k];
+        t[c] = (R)src[(size_t)c * 5 + 4];
+      }
+    else if (n == "grad")
+      for (int c = 0; c < noc; ++c) {
+        for (int k = 0; k < 8; ++k) rec[(size_t)c * RW + R_G + k] = (R)src[(size_t)c * 10 + k];
+        gt[(size_t)c * 2] = (R)src[(size_t)c * 10 + 8];
+        gt[(size_t)c * 2 + 1] = (R)src[(size_t)c * 10 + 9];
+      }
+    else
+      for (int c = 0; c < noc; ++c)
+        for (int k = 0; k < 4; ++k) rec[(size_t)c * RW + R_PHI + k] = (R)src[(size_t)c * 4 + k];
+    HIPCHK(up(d.rec, rec.data(), rec.size() * sizeof(R)));
+    HIPCHK(up(d.qT, t.data(), t.size() * sizeof(R)));
+    HIPCHK(up(d.gT, gt.data(), gt.size() * sizeof(R)));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  if (n == "fc_la" || n == "fc_mu") {
+    std::vector<double> t(nof);
+    for (int f = 0; f < nof; ++f) t[x->face_of_ref[f]] = src[f];
+    HIPCHK(up(n == "fc_la" ? d.fla : d.fmu, t.data(), (size_t)nof * 8));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  if (n == "cell_mul") {
+    std::vector<R> t((size_t)ncell);
+    for (int c = 0; c < ncell; ++c) t[c] = (R)src[c];
+    HIPCHK(up(d.cmul, t.data(), t.size() * sizeof(R)));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  set_error("ufx_cc_set: unknown or read-only field " + n);
+  return UFX_ERR_ARG;
+}
+
+// ---- context construction
+
+template <typename R>
+int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_face,
+              const std::vector<int>& sl_nbr, const std::vector<int>& cnf,
+              const std::vector<int>& lu_face, const std::vector<int>& lu_cnt) {
+  constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW, PW = Layout<R>::PW;
+  const int noc = m.noc, ncell = m.ncell(), nof = m.nof(), nop = m.nop, nbf = m.nbf;
+  d.nop = nop; d.noc = noc; d.nfi = m.nfi; d.nbf = nbf; d.nof = nof; d.ncell = ncell;
+  // faces in storage order
+  std::vector<R> frec((size_t)nof * FW, R(0));
+  std::vector<int2> fpt(nof);
+  std::vector<int> fin(nof), fout(nof);
+  for (int f = 0; f < nof; ++f) {
+    int s = x->face_of_ref[f];
+    R* r = &frec[(size_t)s * FW];
+    r[F_NX] = (R)m.fnx[f]; r[F_NY] = (R)m.fny[f]; r[F_AREA] = (R)m.farea[f];
+    r[F_CX] = (R)m.fcx[f]; r[F_CY] = (R)m.fcy[f];
+    fpt[s] = make_int2(x->pt_of_ref[m.fp1[f]], x->pt_of_ref[m.fp2[f]]);
+    fin[s] = m.fin[f];
+    fout[s] = m.fout[f];
+  }
+  std::vector<int> fbc(std::max(1, nbf)), bface(std::max(1, nbf)), bcell(std::max(1, nbf));
+  for (int k = 0; k < nbf; ++k) {
+    fbc[k] = m.fbc[m.nfi + k];
+    bface[k] = x->face_of_ref[m.nfi + k];
+    bcell[k] = m.fin[m.nfi + k];
+  }
+  // vertices in storage order
+  std::vector<int> v2c_ptr(nop + 1, 0), v2c;
+  std::vector<double> wt, px(nop), py(nop);
+  std::vector<int> ref_of_pt(nop);
+  for (int i = 0; i < nop; ++i) ref_of_pt[x->pt_of_ref[i]] = i;
+  v2c.reserve(m.v2c.size());
+  wt.reserve(m.wt.size());
+  for (int s = 0; s < nop; ++s) {
+    int i = ref_of_pt[s];
+    for (int j = m.v2c_ptr[i]; j < m.v2c_ptr[i + 1]; ++j) {
+      v2c.push_back(m.v2c[j]);
+      wt.push_back(m.wt[j]);
+    }
+    v2c_ptr[s + 1] = (int)v2c.size();
+    px[s] = m.px[i];
+    py[s] = m.py[i];
+  }
+  // cell records: primitives 0 (ghost primitives start at 0, as the
+  // reference leaves them), gradients 0, limiter 1, centroid
+  std::vector<R> rec((size_t)ncell * RW, R(0));
+  for (int c = 0; c < ncell; ++c) {
+    R* r = &rec[(size_t)c * RW];
+    if (c < noc)
+      for (int k = 0; k < 4; ++k) r[R_PHI + k] = R(1);
+    r[R_CEN] = (R)m.ccx[c];
+    r[R_CEN + 1] = (R)m.ccy[c];
+  }
+  std::vector<double> qc((size_t)ncell * 4);
+  for (int c = 0; c < ncell; ++c)
+    for (int k = 0; k < 4; ++k) qc[(size_t)c * 4 + k] = x->p.qinf[k];
+  int rc;
+#define UP(dst, src) if ((rc = dupload(x, &d.dst, src))) return rc
+#define AL(dst, n) if ((rc = dalloc(x, &d.dst, (size_t)(n)))) return rc
+  UP(frec, frec);
+  {
+    int2* p = nullptr;
+    if ((rc = dupload(x, &p, fpt))) return rc;
+    d.fpt = p;
+  }
+  UP(fbc, fbc); UP(bface, bface); UP(bcell, bcell);
+  UP(sl_face, sl_face); UP(sl_nbr, sl_nbr); UP(cnf, cnf);
+  UP(r11, m.r11); UP(r12, m.r12); UP(r22, m.r22); UP(cv, m.ccv); UP(cov, m.ccov);
+  UP(nc2c, m.nc2c); UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py);
+  UP(fin, fin); UP(fout, fout); UP(lu_face, lu_face); UP(lu_cnt, lu_cnt);
+  UP(rec, rec); UP(qc, qc);
+  AL(fw, nof); AL(qT, ncell); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc);
+  AL(res, 4 * (size_t)noc); AL(dt, noc); AL(dta, noc); AL(la, noc); AL(fla, nof); AL(fmu, nof);
+  AL(cmul, ncell); AL(prec, (size_t)nop * PW); AL(fgrad, (size_t)nof * 10);
+  AL(bflux, 2 * (size_t)std::max(1, nbf)); AL(bres, 4 * (size_t)std::max(1, nbf));
+  AL(part, (size_t)std::max(x->nblk_all, 1) * 8);
+  AL(scal, SC_NSCAL); AL(range, 8); AL(err, 4); AL(lower, 4 * (size_t)noc); AL(upper, 4 * (size_t)noc);
+#undef UP
+#undef AL
+  int init[4] = {0, 0, INT_MAX, 0};
+  HIPCHK(hipMemcpyAsync(d.err, init, sizeof(init), hipMemcpyHostToDevice, x->stream));
+  unsigned long long rg[8] = {0, 0, 0, 0, ~0ULL, ~0ULL, ~0ULL, ~0ULL};
+  HIPCHK(hipMemcpyAsync(d.range, rg, sizeof(rg), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return UFX_OK;
 }
 
 }  // namespace
 
 extern "C" {
-
-const char* ufx_last_error(void) { return g_err.c_str(); }
-
-int ufx_cc_geometric(int nop, int noc, int nfi, int nbf, const double* px, const double* py,
-                     const double* ccx, const double* ccy, const double* cv, const int* fi_p1,
-                     const int* fi_p2, const int* fi_in, const int* fi_out, const int* fi_bc,
-                     const int* fb_p1, const int* fb_p2, const int* fb_in, const double* fb_nx,
-                     const double* fb_ny, const int* fb_bc, ufx_mesh** mesh) {
-  if (!mesh || !px || !py || !ccx || !ccy || !cv || (nfi > 0 && (!fi_p1 || !fi_p2 || !fi_in || !fi_out)) ||
-      (nbf > 0 && (!fb_p1 || !fb_p2 || !fb_in || !fb_nx || !fb_ny || !fb_bc))) {
-    set_error("ufx_cc_geometric: null argument");
-    return UFX_ERR_ARG;
-  }
-  RawInput in;
-  in.nop = nop; in.noc = noc; in.nfi = nfi; in.nbf = nbf;
-  in.px = px; in.py = py; in.ccx = ccx; in.ccy = ccy; in.cv = cv;
-  in.fi_p1 = fi_p1; in.fi_p2 = fi_p2; in.fi_in = fi_in; in.fi_out = fi_out; in.fi_bc = fi_bc;
-  in.fb_p1 = fb_p1; in.fb_p2 = fb_p2; in.fb_in = fb_in; in.fb_nx = fb_nx; in.fb_ny = fb_ny;
-  in.fb_bc = fb_bc;
-  ufx_mesh* m = new ufx_mesh();
-  std::string err;
-  int rc = cc_geometric(in, m->m, err);
-  if (rc) {
-    delete m;
-    set_error(err);
-    return rc;
-  }
-  *mesh = m;
-  return UFX_OK;
-}
-
-int ufx_cc_mesh_import(int nop, int noc, int nfi, int nbf, const double* px, const double* py,
-                       const int* fc_p1, const int* fc_p2, const int* fc_in, const int* fc_out,
-                       const int* fc_bc, const double* fc_nx, const double* fc_ny,
-                       const double* fc_area, const double* fc_cenx, const double* fc_ceny,
-                       const double* fc_cov, const double* cell_cenx, const double* cell_ceny,
-                       const double* cell_cv, const double* cell_cov, const double* cell_r11,
-                       const double* cell_r12, const double* cell_r22, const int* cell_nc2c,
-                       const int* c2f_ptr, const int* c2f, const int* v2c_ptr, const int* v2c,
-                       const double* v2c_wt, ufx_mesh** mesh) {
-  if (!mesh || nop <= 0 || noc <= 0 || nfi < 0 || nbf < 0) {
-    set_error("ufx_cc_mesh_import: invalid argument");
-    return UFX_ERR_ARG;
-  }
-  const int nof = nfi + nbf, ncell = noc + nbf;
-  ufx_mesh* h = new ufx_mesh();
-  Mesh& m = h->m;
-  m.kind = UFX_KIND_CC;
-  m.nop = nop; m.noc = noc; m.nfi = nfi; m.nbf = nbf;
-  m.px.assign(px, px + nop);
-  m.py.assign(py, py + nop);
-  auto idx = [](const int* a, int n, int base) {
-    std::vector<int> v(n);
-    for (int i = 0; i < n; ++i) v[i] = a[i] - base;
-    return v;
-  };
-  m.fp1 = idx(fc_p1, nof, 1); m.fp2 = idx(fc_p2, nof, 1);
-  m.fin = idx(fc_in, nof, 1); m.fout = idx(fc_out, nof, 1);
-  m.fbc.assign(fc_bc, fc_bc + nof);
-  m.fnx.assign(fc_nx, fc_nx + nof); m.fny.assign(fc_ny, fc_ny + nof);
-  m.farea.assign(fc_area, fc_area + nof);
-  m.fcx.assign(fc_cenx, fc_cenx + nof); m.fcy.assign(fc_ceny, fc_ceny + nof);
-  m.fcov.assign(fc_cov, fc_cov + nof);
-  m.ccx.assign(cell_cenx, cell_cenx + ncell); m.ccy.assign(cell_ceny, cell_ceny + ncell);
-  m.ccov.assign(cell_cov, cell_cov + ncell);
-  m.ccv.assign(cell_cv, cell_cv + noc);
-  m.r11.assign(cell_r11, cell_r11 + noc); m.r12.assign(cell_r12, cell_r12 + noc);
-  m.r22.assign(cell_r22, cell_r22 + noc);
-  m.nc2c.assign(cell_nc2c, cell_nc2c + noc);
-  m.c2f_ptr.assign(c2f_ptr, c2f_ptr + noc + 1);
-  m.c2f = idx(c2f, c2f_ptr[noc], 1);
-  m.v2c_ptr.assign(v2c_ptr, v2c_ptr + nop + 1);
-  m.v2c = idx(v2c, v2c_ptr[nop], 1);
-  m.wt.assign(v2c_wt, v2c_wt + v2c_ptr[nop]);
-  m.oldnum.resize(noc);
-  for (int i = 0; i < noc; ++i) m.oldnum[i] = i;
-  for (int i = 0; i < nof; ++i) {
-    if (m.fin[i] < 0 || m.fin[i] >= noc || m.fout[i] < 0 || m.fout[i] >= ncell ||
-        m.fp1[i] < 0 || m.fp1[i] >= nop || m.fp2[i] < 0 || m.fp2[i] >= nop ||
-        (i < nfi && m.fout[i] >= noc) || (i >= nfi && m.fout[i] != noc + (i - nfi))) {
-      delete h;
-      set_error("ufx_cc_mesh_import: face " + std::to_string(i + 1) + " has an invalid index");
-      return UFX_ERR_MESH;
-    }
-  }
-  for (int c : m.c2f)
-    if (c < 0 || c >= nof) { delete h; set_error("ufx_cc_mesh_import: bad c2f"); return UFX_ERR_MESH; }
-  for (int c : m.v2c)
-    if (c < 0 || c >= ncell) { delete h; set_error("ufx_cc_mesh_import: bad v2c"); return UFX_ERR_MESH; }
-  *mesh = h;
-  return UFX_OK;
-}
-
-int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count) {
-  if (!mesh || !name || !count) { set_error("ufx_mesh_size: null argument"); return UFX_ERR_ARG; }
-  const Mesh& m = mesh->m;
-  std::string n(name);
-  int64_t v = -1;
-  if (n.rfind("fc_", 0) == 0) v = m.nof();
-  if (n == "cell_cenx" || n == "cell_ceny" || n == "cell_cov") v = m.ncell();
-  if (n == "cell_cv" || n == "cell_r11" || n == "cell_r12" || n == "cell_r22" ||
-      n == "cell_nc2c" || n == "cell_nc2f" || n == "oldnum") v = m.noc;
-  if (n == "cell_c2f") v = (int64_t)m.c2f.size();
-  if (n == "pt_x" || n == "pt_y" || n == "pt_nv2c") v = m.nop;
-  if (n == "pt_v2c" || n == "pt_wt") v = (int64_t)m.v2c.size();
-  if (n == "counts") v = 4;
-  if (v < 0) { set_error("ufx_mesh_size: unknown array " + n); return UFX_ERR_ARG; }
-  *count = v;
-  return UFX_OK;
-}
-
-int ufx_mesh_get(const ufx_mesh* mesh, const char* name, void* dst) {
-  if (!mesh || !name || !dst) { set_error("ufx_mesh_get: null argument"); return UFX_ERR_ARG; }
-  const Mesh& m = mesh->m;
-  std::string n(name);
-  int* di = (int*)dst;
-  double* dd = (double*)dst;
-  auto put1 = [&](const std::vector<int>& v) { for (size_t i = 0; i < v.size(); ++i) di[i] = v[i] + 1; };
-  auto put0 = [&](const std::vector<int>& v) { for (size_t i = 0; i < v.size(); ++i) di[i] = v[i]; };
-  auto putd = [&](const std::vector<double>& v) { if (!v.empty()) std::memcpy(dd, v.data(), v.size() * 8); };
-  if (n == "fc_p1") put1(m.fp1);
-  else if (n == "fc_p2") put1(m.fp2);
-  else if (n == "fc_in") put1(m.fin);
-  else if (n == "fc_out") put1(m.fout);
-  else if (n == "fc_bc") put0(m.fbc);
-  else if (n == "fc_nx") putd(m.fnx);
-  else if (n == "fc_ny") putd(m.fny);
-  else if (n == "fc_area") putd(m.farea);
-  else if (n == "fc_cenx") putd(m.fcx);
-  else if (n == "fc_ceny") putd(m.fcy);
-  else if (n == "fc_cov") putd(m.fcov);
-  else if (n == "cell_cenx") putd(m.ccx);
-  else if (n == "cell_ceny") putd(m.ccy);
-  else if (n == "cell_cov") putd(m.ccov);
-  else if (n == "cell_cv") putd(m.ccv);
-  else if (n == "cell_r11") putd(m.r11);
-  else if (n == "cell_r12") putd(m.r12);
-  else if (n == "cell_r22") putd(m.r22);
-  else if (n == "cell_nc2c") put0(m.nc2c);
-  else if (n == "cell_nc2f") { for (int i = 0; i < m.noc; ++i) di[i] = m.c2f_ptr[i + 1] - m.c2f_ptr[i]; }
-  else if (n == "cell_c2f") put1(m.c2f);
-  else if (n == "pt_x") putd(m.px);
-  else if (n == "pt_y") putd(m.py);
-  else if (n == "pt_nv2c") { for (int i = 0; i < m.nop; ++i) di[i] = m.v2c_ptr[i + 1] - m.v2c_ptr[i]; }
-  else if (n == "pt_v2c") put1(m.v2c);
-  else if (n == "pt_wt") putd(m.wt);
-  else if (n == "oldnum") put1(m.oldnum);
-  else if (n == "counts") { di[0] = m.nop; di[1] = m.noc; di[2] = m.nof(); di[3] = m.nbf; }
-  else { set_error("ufx_mesh_get: unknown array " + n); return UFX_ERR_ARG; }
-  return UFX_OK;
-}
-
-void ufx_mesh_destroy(ufx_mesh* mesh) { delete mesh; }
 
 int ufx_cc_destroy(ufx_cc* x) {
   if (!x) return UFX_OK;
@@ -836,82 +645,125 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   if (!mesh || !prm || !out) { set_error("ufx_cc_create: null argument"); return UFX_ERR_ARG; }
   const Mesh& m = mesh->m;
   if (m.kind != UFX_KIND_CC) { set_error("ufx_cc_create: not a cell-centred mesh"); return UFX_ERR_ARG; }
-  if (prm->flux < UFX_FLUX_ROE || prm->flux > UFX_FLUX_RUSANOV) { set_error("unknown flux"); return UFX_ERR_ARG; }
-  if (prm->grad != UFX_GRAD_LSQR) {
-    set_error("gradient method not implemented on the device yet");
+  if (prm->flux < UFX_FLUX_ROE || prm->flux > UFX_FLUX_RUSANOV) { set_error("Unknown flux"); return UFX_ERR_ARG; }
+  if (prm->grad < UFX_GRAD_LSQR || prm->grad > UFX_GRAD_GGFC) { set_error("Unknown gradient type"); return UFX_ERR_ARG; }
+  if (prm->flow_type != UFX_FLOW_INVISCID && prm->flow_type != UFX_FLOW_LAMINAR) {
+    set_error("unsupported flow type (the reference's RANS is a stub)");
     return UFX_ERR_UNSUPPORTED;
   }
-  if (prm->flow_type != UFX_FLOW_INVISCID || prm->irs || prm->timemode != UFX_TIME_RK3 ||
-      prm->precision != UFX_PREC_FP64) {
-    set_error("option not implemented on the device yet");
-    return UFX_ERR_UNSUPPORTED;
+  if (prm->iwall != 0 && prm->iwall != 1) { set_error("Unknown no-slip wall BC"); return UFX_ERR_ARG; }
+  if (prm->timemode != UFX_TIME_RK3 && prm->timemode != UFX_TIME_LUSGS) { set_error("Unknown time integration scheme"); return UFX_ERR_ARG; }
+  if (prm->precision != UFX_PREC_FP64 && prm->precision != UFX_PREC_MIXED) { set_error("unknown precision"); return UFX_ERR_ARG; }
+  if (prm->irs && !(prm->cfl_max > 0.0)) { set_error("residual smoothing needs cfl_max > 0"); return UFX_ERR_ARG; }
+  const int noc = m.noc, nof = m.nof(), nop = m.nop;
+  // storage order of faces and vertices: first touch walking the cells in
+  // order and their faces in ascending reference order
+  std::vector<std::vector<int>> cf(noc);
+  for (int f = 0; f < nof; ++f) {
+    cf[m.fin[f]].push_back(f);
+    if (f < m.nfi) cf[m.fout[f]].push_back(f);
   }
-  // per-cell face slots (ascending face order), boundary-face count
-  const int noc = m.noc, nof = m.nof(), ncell = m.ncell();
-  std::vector<int> sl_face(MAXF * (size_t)noc, -1), sl_nbr(MAXF * (size_t)noc, 0), cnf(noc, 0);
-  {
-    std::vector<int> cnt(noc, 0), nb(noc, 0);
-    for (int f = 0; f < nof; ++f) {
-      int sides[2] = {m.fin[f], f < m.nfi ? m.fout[f] : -1};
-      for (int s = 0; s < 2; ++s) {
-        int c = sides[s];
-        if (c < 0) continue;
-        if (cnt[c] >= MAXF) {
-          set_error("cell " + std::to_string(c + 1) + " has more than 4 faces");
-          return UFX_ERR_UNSUPPORTED;
-        }
-        int other = s == 0 ? m.fout[f] : m.fin[f];
-        sl_face[(size_t)cnt[c] * noc + c] = f;
-        sl_nbr[(size_t)cnt[c] * noc + c] = s == 0 ? other : ~other;
-        cnt[c]++;
-        if (f >= m.nfi) nb[c]++;
-      }
+  for (int c = 0; c < noc; ++c) {
+    if (cf[c].size() > (size_t)MAXF) {
+      set_error("cell " + std::to_string(c + 1) + " has more than 4 faces");
+      return UFX_ERR_UNSUPPORTED;
     }
-    for (int c = 0; c < noc; ++c) cnf[c] = cnt[c] | (nb[c] << 8);
+    std::sort(cf[c].begin(), cf[c].end());
   }
   ufx_cc* x = new ufx_cc();
-  x->p = Params{};
+  x->face_of_ref.assign(nof, -1);
+  x->pt_of_ref.assign(nop, -1);
+  {
+    int nf = 0, np = 0;
+    for (int c = 0; c < noc; ++c)
+      for (int f : cf[c]) {
+        if (x->face_of_ref[f] < 0) x->face_of_ref[f] = nf++;
+        for (int p : {m.fp1[f], m.fp2[f]})
+          if (x->pt_of_ref[p] < 0) x->pt_of_ref[p] = np++;
+      }
+    for (int f = 0; f < nof; ++f)
+      if (x->face_of_ref[f] < 0) x->face_of_ref[f] = nf++;
+    for (int p = 0; p < nop; ++p)
+      if (x->pt_of_ref[p] < 0) x->pt_of_ref[p] = np++;
+  }
+  std::vector<int> sl_face((size_t)MAXF * noc, 0), sl_nbr((size_t)MAXF * noc, 0), cnf(noc);
+  for (int c = 0; c < noc; ++c) {
+    int nb = 0;
+    for (size_t s = 0; s < cf[c].size(); ++s) {
+      int f = cf[c][s];
+      bool in_side = m.fin[f] == c;
+      int other = in_side ? m.fout[f] : m.fin[f];
+      sl_face[s * noc + c] = x->face_of_ref[f];
+      sl_nbr[s * noc + c] = in_side ? other : ~other;
+      if (f >= m.nfi) ++nb;
+    }
+    cnf[c] = (int)cf[c].size() | (nb << 8);
+  }
+  // LU-SGS: the reference's faces-of-cell order and the level schedule of
+  // its sequential sweeps (a cell's level is one more than the highest level
+  // among the neighbours it waits for)
+  std::vector<int> lu_face((size_t)MAXF * noc, 0), lu_cnt(noc);
+  std::vector<int> lvf(noc, 0), lvb(noc, 0);
+  for (int c = 0; c < noc; ++c) {
+    int n = m.c2f_ptr[c + 1] - m.c2f_ptr[c];
+    if (n > MAXF) {
+      delete x;
+      set_error("cell " + std::to_string(c + 1) + " has more than 4 faces");
+      return UFX_ERR_UNSUPPORTED;
+    }
+    lu_cnt[c] = n;
+    for (int j = 0; j < n; ++j) lu_face[(size_t)j * noc + c] = x->face_of_ref[m.c2f[m.c2f_ptr[c] + j]];
+  }
+  auto nbr_of = [&](int c, int j) {
+    int f = m.c2f[m.c2f_ptr[c] + j];
+    return m.fout[f] == c ? m.fin[f] : m.fout[f];
+  };
+  int nlf = 0, nlb = 0;
+  for (int c = 0; c < noc; ++c) {
+    int l = 0;
+    for (int j = 0; j < lu_cnt[c]; ++j) {
+      int o = nbr_of(c, j);
+      if (o < c && o >= 0) l = std::max(l, lvf[o] + 1);
+    }
+    lvf[c] = l;
+    nlf = std::max(nlf, l + 1);
+  }
+  for (int c = noc - 1; c >= 0; --c) {
+    int l = 0;
+    for (int j = 0; j < lu_cnt[c]; ++j) {
+      int o = nbr_of(c, j);
+      if (o > c && o < noc) l = std::max(l, lvb[o] + 1);
+    }
+    lvb[c] = l;
+    nlb = std::max(nlb, l + 1);
+  }
+  std::vector<int> lv_cells(2 * (size_t)noc);
+  {
+    std::vector<int> cnt(nlf + 1, 0);
+    for (int c = 0; c < noc; ++c) cnt[lvf[c] + 1]++;
+    for (int l = 0; l < nlf; ++l) cnt[l + 1] += cnt[l];
+    x->lv_fwd_off = cnt;
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int c = 0; c < noc; ++c) lv_cells[pos[lvf[c]]++] = c;
+    std::vector<int> cb(nlb + 1, 0);
+    for (int c = 0; c < noc; ++c) cb[lvb[c] + 1]++;
+    for (int l = 0; l < nlb; ++l) cb[l + 1] += cb[l];
+    for (int l = 0; l <= nlb; ++l) cb[l] += noc;
+    x->lv_bwd_off = cb;
+    std::vector<int> pb(cb.begin(), cb.end() - 1);
+    for (int c = noc - 1; c >= 0; --c) lv_cells[pb[lvb[c]]++] = c;
+  }
   hipError_t e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
     delete x;
     return UFX_ERR_HIP;
   }
-  Dev& d = x->d;
-  d.nop = m.nop; d.noc = noc; d.nfi = m.nfi; d.nbf = m.nbf; d.nof = nof; d.ncell = ncell;
-  int rc = 0;
-#define UP(dst, src) if ((rc = dupload(x, &d.dst, src))) { ufx_cc_destroy(x); return rc; }
-#define AL(dst, n) if ((rc = dalloc(x, &d.dst, (size_t)(n)))) { ufx_cc_destroy(x); return rc; }
-  UP(px, m.px); UP(py, m.py);
-  UP(fp1, m.fp1); UP(fp2, m.fp2); UP(fin, m.fin); UP(fout, m.fout); UP(fbc, m.fbc);
-  UP(fnx, m.fnx); UP(fny, m.fny); UP(farea, m.farea); UP(fcx, m.fcx); UP(fcy, m.fcy);
-  UP(ccx, m.ccx); UP(ccy, m.ccy); UP(ccv, m.ccv); UP(ccov, m.ccov);
-  UP(r11, m.r11); UP(r12, m.r12); UP(r22, m.r22); UP(nc2c, m.nc2c);
-  UP(sl_face, sl_face); UP(sl_nbr, sl_nbr); UP(cnf, cnf);
-  UP(v2c_ptr, m.v2c_ptr); UP(v2c, m.v2c); UP(wt, m.wt);
-  AL(qc, 4 * (size_t)ncell); AL(qp, 5 * (size_t)ncell); AL(qold, 4 * (size_t)noc);
-  AL(res, 4 * (size_t)noc); AL(grad, 10 * (size_t)noc); AL(phi, 4 * (size_t)noc);
-  AL(dt, noc); AL(la, noc); AL(dta, noc); AL(fla, nof); AL(fmu, nof); AL(cmul, ncell);
-  AL(pprim, 5 * (size_t)m.nop); AL(pmu, m.nop); AL(fgrad, 10 * (size_t)nof);
-  AL(bflux, 2 * (size_t)std::max(1, m.nbf));
+  x->nop = nop; x->noc = noc; x->nfi = m.nfi; x->nbf = m.nbf; x->nof = nof; x->ncell = m.ncell();
   x->nblk_cell = nblocks(noc);
-  x->nblk_all = nblocks(ncell);
-  AL(part, 8 * (size_t)std::max(x->nblk_all, 1));
-  AL(scal, SC_NSCAL);
-  {
-    int* er = nullptr;
-    if ((rc = dalloc(x, &er, 4))) { ufx_cc_destroy(x); return rc; }
-    d.err = er;
-    int init[4] = {0, 0, 0x7fffffff, 0};
-    if (hipMemcpyAsync(er, init, sizeof(init), hipMemcpyHostToDevice, x->stream) != hipSuccess ||
-        hipStreamSynchronize(x->stream) != hipSuccess) {
-      set_error("upload of the error word failed");
-      ufx_cc_destroy(x);
-      return UFX_ERR_HIP;
-    }
-  }
-#undef UP
-#undef AL
+  x->nblk_all = nblocks(m.ncell());
+  x->nblk_pt = nblocks(nop);
+  x->nblk_face = nblocks(nof);
   // constants and free stream, CCFVM/solution.F90:4-49
   Params& p = x->p;
   p.gamma = 1.4;
@@ -922,15 +774,14 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   p.aoa = aoa;
   p.xref = prm->xref;
   p.yref = prm->yref;
-  double qinf[4];
-  qinf[1] = 1.0;
-  qinf[2] = std::cos(aoa) * 1.0;
-  qinf[3] = std::sin(aoa) * 1.0;
+  p.qinf[1] = 1.0;
+  p.qinf[2] = std::cos(aoa) * 1.0;
+  p.qinf[3] = std::sin(aoa) * 1.0;
   p.p_inf = 1.0 / (p.gamma * p.m_inf * p.m_inf);
-  qinf[0] = p.p_inf / (p.gamma - 1.0) + 0.5;
-  p.r_inf = qinf[1];
-  p.u_inf = qinf[2] / qinf[1];
-  p.v_inf = qinf[3] / qinf[1];
+  p.qinf[0] = p.p_inf / (p.gamma - 1.0) + 0.5;
+  p.r_inf = p.qinf[1];
+  p.u_inf = p.qinf[2] / p.qinf[1];
+  p.v_inf = p.qinf[3] / p.qinf[1];
   p.znd = 1.0 / (p.gamma * p.m_inf * p.m_inf);
   p.mref = std::max((double)0.32f, 0.5 * p.m_inf);
   p.t_wall = prm->t_wall / 300.0;
@@ -940,164 +791,133 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   p.precision = prm->precision; p.cfl_max = prm->cfl_max;
   p.vll = 0.0;
   if (p.flow != UFX_FLOW_INVISCID) p.vll = 2.0 * p.gamma / (0.72 * p.rey * 0.25);
-  Gas<double>& g = x->g;
-  g.gamma = p.gamma; g.gamma1 = p.gamma1; g.znd = p.znd; g.r_inf = p.r_inf; g.u_inf = p.u_inf;
-  g.v_inf = p.v_inf; g.p_inf = p.p_inf; g.mref = p.mref;
-  // initialize: free stream everywhere, limiter 1 (CCFVM/solution.F90:78-81)
-  {
-    std::vector<double> q(4 * (size_t)ncell), ph(4 * (size_t)noc, 1.0);
-    for (int k = 0; k < 4; ++k)
-      for (int c = 0; c < ncell; ++c) q[(size_t)k * ncell + c] = qinf[k];
-    e = hipMemcpyAsync(d.qc, q.data(), q.size() * 8, hipMemcpyHostToDevice, x->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d.phi, ph.data(), ph.size() * 8, hipMemcpyHostToDevice, x->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(x->stream);
-    if (e != hipSuccess) {
-      set_error(std::string("upload: ") + hipGetErrorString(e));
-      ufx_cc_destroy(x);
-      return UFX_ERR_HIP;
-    }
+  x->g = Gas<double>{p.gamma, p.gamma1, p.znd, p.r_inf, p.u_inf, p.v_inf, p.p_inf, p.mref};
+  x->gf = Gas<float>{(float)p.gamma, (float)p.gamma1, (float)p.znd, (float)p.r_inf,
+                     (float)p.u_inf, (float)p.v_inf, (float)p.p_inf, (float)p.mref};
+  int rc = p.precision == UFX_PREC_FP64
+               ? build_dev(x, x->d64, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt)
+               : build_dev(x, x->d32, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt);
+  if (!rc) rc = dupload(x, &x->lv_cells, lv_cells);
+  if (!rc) rc = dalloc(x, &x->cirs_tmp, 8 * (size_t)noc);
+  if (!rc) rc = dalloc(x, &x->bfluxv, 2 * (size_t)std::max(1, m.nbf));
+  if (!rc) {
+    e = hipStreamSynchronize(x->stream);
+    if (e != hipSuccess) { set_error(std::string("upload: ") + hipGetErrorString(e)); rc = UFX_ERR_HIP; }
+  }
+  if (rc) {
+    ufx_cc_destroy(x);
+    return rc;
   }
   for (int k = 0; k < m.nbf; ++k)
-    if (m.fbc[m.nfi + k] == WALL) x->wall_faces.push_back(k);
+    if (m.fbc[m.nfi + k] == WALL) x->wall_k.push_back(k);
+  x->fin_h.resize(m.nbf); x->fp1_h.resize(m.nbf); x->fp2_h.resize(m.nbf);
+  x->fnx_h.resize(m.nbf); x->fny_h.resize(m.nbf); x->farea_h.resize(m.nbf);
+  for (int k = 0; k < m.nbf; ++k) {
+    int f = m.nfi + k;
+    x->fin_h[k] = m.fin[f]; x->fp1_h[k] = m.fp1[f]; x->fp2_h[k] = m.fp2[f];
+    x->fnx_h[k] = m.fnx[f]; x->fny_h[k] = m.fny[f]; x->farea_h[k] = m.farea[f];
+  }
+  x->px_h = m.px;
+  x->py_h = m.py;
   *out = x;
   return UFX_OK;
 }
 
 int ufx_cc_set(ufx_cc* x, const char* name, const double* src) {
   if (int rc = check_ctx(x)) return rc;
-  Field f;
-  if (!name || !src || !field(x, name, f)) { set_error(std::string("ufx_cc_set: unknown field ") + (name ? name : "")); return UFX_ERR_ARG; }
-  std::vector<double> t((size_t)f.nvar * f.stride, 0.0);
-  for (int c = 0; c < f.count; ++c)
-    for (int k = 0; k < f.nvar; ++k) t[(size_t)k * f.stride + c] = src[(size_t)c * f.nvar + k];
-  HIPCHK(hipMemcpyAsync(f.ptr, t.data(), t.size() * 8, hipMemcpyHostToDevice, x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
-  return UFX_OK;
+  if (!name || !src) { set_error("ufx_cc_set: null argument"); return UFX_ERR_ARG; }
+  std::string n(name);
+  return with_dev(x, [&](auto& d) { return set_field(x, d, n, src); });
 }
 
 int ufx_cc_get(ufx_cc* x, const char* name, double* dst) {
   if (int rc = check_ctx(x)) return rc;
-  Field f;
-  if (!name || !dst || !field(x, name, f)) { set_error(std::string("ufx_cc_get: unknown field ") + (name ? name : "")); return UFX_ERR_ARG; }
-  std::vector<double> t((size_t)f.nvar * f.stride);
-  HIPCHK(hipMemcpyAsync(t.data(), f.ptr, t.size() * 8, hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
-  for (int c = 0; c < f.count; ++c)
-    for (int k = 0; k < f.nvar; ++k) dst[(size_t)c * f.nvar + k] = t[(size_t)k * f.stride + c];
-  return UFX_OK;
+  if (!name || !dst) { set_error("ufx_cc_get: null argument"); return UFX_ERR_ARG; }
+  std::string n(name);
+  return with_dev(x, [&](auto& d) { return get_field(x, d, n, dst); });
 }
 
 int ufx_cc_prim(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, x->d, x->g, 1, 0, 0);
-  return launch_check();
+  return with_dev(x, [&](auto& d) { return run_prim(x, d, 1, 0, 0); });
 }
 
 int ufx_cc_save_old(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, x->d, x->g, 0, 1, 0);
-  return launch_check();
+  return with_dev(x, [&](auto& d) { return run_prim(x, d, 0, 1, 0); });
 }
 
 int ufx_cc_avg_c2v(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  LAUNCH(x, UFX_K_AVG, k_avg_c2v, nblocks(x->d.nop), x->d);
-  return launch_check();
+  return with_dev(x, [&](auto& d) { return run_avg(x, d); });
 }
 
 int ufx_cc_time_step(ufx_cc* x, double cfl) {
   if (int rc = check_ctx(x)) return rc;
-  LAUNCH(x, UFX_K_TSTEP, k_timestep, x->nblk_cell, x->d, x->g, x->p.vll, cfl);
-  return launch_check();
-}
-
-static int range_pass(ufx_cc* x) {
-  LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, x->d, x->g, 0, 0, 1);
-  LAUNCH(x, UFX_K_RANGE, k_range_final, 1, x->d, x->nblk_all);
-  return launch_check();
-}
-
-int ufx_cc_limit(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
-  if (int rc = range_pass(x)) return rc;
-  LAUNCH(x, UFX_K_GRAD, k_grad_lsq, x->nblk_cell, x->d, 0, 1);
-  return launch_check();
+  x->cfl = cfl;
+  x->store_fla = true;
+  int rc = with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    return run_cell_g<R, G_NONE>(x, d, false, false, true, cfl);
+  });
+  x->store_fla = false;
+  return rc;
 }
 
 int ufx_cc_gradient(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  if (x->p.ilimit)
-    if (int rc = range_pass(x)) return rc;
-  LAUNCH(x, UFX_K_GRAD, k_grad_lsq, x->nblk_cell, x->d, 1, x->p.ilimit);
-  return launch_check();
+  return with_dev(x, [&](auto& d) {
+    if (x->p.ilimit)
+      if (int rc = run_range(x, d)) return rc;
+    return run_gradient(x, d, true, x->p.ilimit != 0, false, 0.0);
+  });
 }
 
-static int residual(ufx_cc* x, int mode, double ak, double bk) {
-  const Dev& d = x->d;
-  int viscous = x->p.flow != UFX_FLOW_INVISCID;
-  switch (x->p.flux) {
-    case UFX_FLUX_ROE:
-      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_ROE>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
-      break;
-    case UFX_FLUX_AUSM:
-      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_AUSM>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
-      break;
-    case UFX_FLUX_VANLEER:
-      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_VANLEER>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
-      break;
-    default:
-      LAUNCH(x, UFX_K_STAGE, k_residual<UFX_FLUX_RUSANOV>, x->nblk_cell, d, x->g, viscous, x->p.iwall, x->p.t_wall, mode, ak, bk);
-      break;
-  }
-  return launch_check();
+int ufx_cc_limit(ufx_cc* x) {
+  if (int rc = check_ctx(x)) return rc;
+  return with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    if (int rc = run_range(x, d)) return rc;
+    return run_cell_g<R, G_NONE>(x, d, false, true, false, 0.0);
+  });
 }
-
-static const double RK_A[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
-static const double RK_B[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
 
 int ufx_cc_fvresidual(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  return residual(x, 0, 0.0, 0.0);
+  return with_dev(x, [&](auto& d) { return run_residual_full(x, d); });
 }
 
 int ufx_cc_rk_update(ufx_cc* x, int stage) {
   if (int rc = check_ctx(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
-  LAUNCH(x, UFX_K_STAGE, k_rk_update, x->nblk_cell, x->d, RK_A[stage - 1], RK_B[stage - 1]);
-  return launch_check();
+  return with_dev(x, [&](auto& d) { return run_rk_update(x, d, RK_A[stage - 1], RK_B[stage - 1]); });
 }
 
 int ufx_cc_rk_stage(ufx_cc* x, int stage) {
   if (int rc = check_ctx(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
-  return residual(x, 1, RK_A[stage - 1], RK_B[stage - 1]);
+  return with_dev(x, [&](auto& d) { return run_stage(x, d, stage, false, true); });
 }
 
 int ufx_cc_lusgs(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
-  set_error("LU-SGS is not implemented on the device yet");
-  return UFX_ERR_UNSUPPORTED;
-}
-
-static int ensure_hist(ufx_cc* x, int n) {
-  if (n <= x->hist_cap) return UFX_OK;
-  if (x->d_hist) HIPCHK(hipFree(x->d_hist));
-  x->d_hist = nullptr;
-  HIPCHK(hipMalloc((void**)&x->d_hist, (size_t)n * sizeof(double)));
-  x->hist_cap = n;
-  return UFX_OK;
+  return with_dev(x, [&](auto& d) { return run_lusgs(x, d); });
 }
 
 int ufx_cc_residue(ufx_cc* x, int iter, double* fres, double* fres_raw) {
   if (int rc = check_ctx(x)) return rc;
   if (int rc = ensure_hist(x, 1)) return rc;
-  LAUNCH(x, UFX_K_NORM, k_sumsq, x->nblk_cell, x->d);
-  LAUNCH(x, UFX_K_NORM, k_norm_final, 1, x->d, x->nblk_cell, iter == 1, x->d_hist, 0);
-  if (int rc = launch_check()) return rc;
+  int rc = with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
+    LAUNCH(x, UFX_K_NORM, k_norm_final<R>, 1, d, x->nblk_cell, iter == 1, x->d_hist, 0);
+    return launch_check();
+  });
+  if (rc) return rc;
   double h = 0.0, raw = 0.0;
+  const double* scal = x->p.precision == UFX_PREC_FP64 ? x->d64.scal : x->d32.scal;
   HIPCHK(hipMemcpyAsync(&h, x->d_hist, 8, hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipMemcpyAsync(&raw, x->d.scal + SC_NORM, 8, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipMemcpyAsync(&raw, scal + SC_NORM, 8, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   if (fres) *fres = h;
   if (fres_raw) *fres_raw = raw;
@@ -1106,15 +926,21 @@ int ufx_cc_residue(ufx_cc* x, int iter, double* fres, double* fres_raw) {
 
 int ufx_cc_check_positivity(ufx_cc* x, int* bad) {
   if (int rc = check_ctx(x)) return rc;
-  int big = 0x7fffffff;
-  HIPCHK(hipMemcpyAsync(x->d.err + 2, &big, sizeof(int), hipMemcpyHostToDevice, x->stream));
-  LAUNCH(x, UFX_K_OTHER, k_positivity, x->nblk_cell, x->d);
-  if (int rc = launch_check()) return rc;
-  int e = 0;
-  HIPCHK(hipMemcpyAsync(&e, x->d.err + 2, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  int* err = err_word(x);
+  int big = INT_MAX;
+  HIPCHK(hipMemcpyAsync(err + 2, &big, sizeof(int), hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
-  if (bad) *bad = e == 0x7fffffff ? 0 : e;
-  if (e != 0x7fffffff) {
+  int rc = with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    LAUNCH(x, UFX_K_OTHER, k_positivity<R>, x->nblk_cell, d);
+    return launch_check();
+  });
+  if (rc) return rc;
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, err + 2, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (bad) *bad = e == INT_MAX ? 0 : e;
+  if (e != INT_MAX) {
     set_error("Density/pressure is negative at point " + std::to_string(e));
     return UFX_ERR_STATE;
   }
@@ -1124,52 +950,73 @@ int ufx_cc_check_positivity(ufx_cc* x, int* bad) {
 int ufx_cc_forces(ufx_cc* x, double* out, double* cp, double* cf) {
   if (int rc = check_ctx(x)) return rc;
   if (!out) { set_error("ufx_cc_forces: null output"); return UFX_ERR_ARG; }
-  const Dev& d = x->d;
   const Params& p = x->p;
-  LAUNCH(x, UFX_K_OTHER, k_dtmin, 1, x->d);
-  if (int rc = launch_check()) return rc;
-  int nbf = d.nbf;
-  std::vector<double> bf(2 * (size_t)std::max(1, nbf)), dtmin(1);
-  std::vector<int> fin(nbf), fp1(nbf), fp2(nbf);
-  std::vector<double> fnx(nbf), fny(nbf), fa(nbf), px(d.nop), py(d.nop);
-  std::vector<double> qp4((size_t)d.ncell);
-  if (nbf > 0) {
-    HIPCHK(hipMemcpyAsync(bf.data(), d.bflux, 2 * (size_t)nbf * 8, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(fin.data(), d.fin + d.nfi, nbf * 4, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(fp1.data(), d.fp1 + d.nfi, nbf * 4, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(fp2.data(), d.fp2 + d.nfi, nbf * 4, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(fnx.data(), d.fnx + d.nfi, nbf * 8, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(fny.data(), d.fny + d.nfi, nbf * 8, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(fa.data(), d.farea + d.nfi, nbf * 8, hipMemcpyDeviceToHost, x->stream));
+  const int nbf = x->nbf;
+  std::vector<double> bf(2 * (size_t)std::max(1, nbf)), bv(2 * (size_t)std::max(1, nbf));
+  std::vector<double> pin(std::max(1, nbf)), dtmin(1);
+  int rc = with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    constexpr int RW = Layout<R>::RW;
+    LAUNCH(x, UFX_K_OTHER, k_dtmin<R>, 1, d);
+    if (int r = launch_check()) return r;
+    std::vector<R> rec((size_t)x->ncell * RW);
+    HIPCHK(hipMemcpyAsync(rec.data(), d.rec, rec.size() * sizeof(R), hipMemcpyDeviceToHost, x->stream));
+    if (nbf > 0) {
+      HIPCHK(hipMemcpyAsync(bf.data(), d.bflux, 2 * (size_t)nbf * 8, hipMemcpyDeviceToHost, x->stream));
+      HIPCHK(hipMemcpyAsync(bv.data(), x->bfluxv, 2 * (size_t)nbf * 8, hipMemcpyDeviceToHost, x->stream));
+    }
+    HIPCHK(hipMemcpyAsync(dtmin.data(), d.scal + SC_DTMIN, 8, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int k = 0; k < nbf; ++k) pin[k] = rec[(size_t)x->fin_h[k] * RW + R_QP + 3];
+    return (int)UFX_OK;
+  });
+  if (rc) return rc;
+  std::vector<double> fg, fmu;
+  bool visc = p.flow != UFX_FLOW_INVISCID;
+  if (visc) {
+    fg.resize((size_t)x->nof * 10);
+    fmu.resize(x->nof);
+    if ((rc = ufx_cc_get(x, "fc_grad", fg.data()))) return rc;
+    if ((rc = ufx_cc_get(x, "fc_mu", fmu.data()))) return rc;
   }
-  HIPCHK(hipMemcpyAsync(px.data(), d.px, d.nop * 8, hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipMemcpyAsync(py.data(), d.py, d.nop * 8, hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipMemcpyAsync(qp4.data(), d.qp + 3 * (size_t)d.ncell, d.ncell * 8, hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipMemcpyAsync(dtmin.data(), d.scal + SC_DTMIN, 8, hipMemcpyDeviceToHost, x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
   // find_cl_cd, CCFVM/main.F90:263-355: pressure of the adjacent cell for
-  // the pressure integral, momentum fluxes of the last residual for the
-  // flux-based coefficients the reference prints (:351-353)
+  // the surface integral; the momentum fluxes of the last residual for the
+  // coefficients the reference prints (:351-353)
   double dyna = 0.5 * p.gamma * p.p_inf * p.m_inf * p.m_inf;
   double fx = 0.0, fy = 0.0, cm = 0.0, fx1 = 0.0, fy1 = 0.0;
+  const double two = 2.0, two_third = two / 3.0;
   int iw = 0;
-  for (int k : x->wall_faces) {
-    double pwall = qp4[fin[k]];
-    double xi = 0.5 * (px[fp1[k]] + px[fp2[k]]);
-    double yi = 0.5 * (py[fp1[k]] + py[fp2[k]]);
+  for (int k : x->wall_k) {
+    double pwall = pin[k];
+    double xi = 0.5 * (x->px_h[x->fp1_h[k]] + x->px_h[x->fp2_h[k]]);
+    double yi = 0.5 * (x->py_h[x->fp1_h[k]] + x->py_h[x->fp2_h[k]]);
     double xa = xi - p.xref, ya = yi - p.yref;
-    double area = fa[k], nx = fnx[k], ny = fny[k];
+    double area = x->farea_h[k], nx = x->fnx_h[k], ny = x->fny_h[k];
     double cpv = (pwall - p.p_inf) / dyna;
     double dcx = cpv * nx * area, dcy = cpv * ny * area;
     fx = fx + pwall * nx * area;
     fy = fy + pwall * ny * area;
+    double cfv = 0.0;
+    if (visc) {
+      int f = x->nfi + k;
+      double mu = fmu[f] / p.rey;
+      double ux = fg[(size_t)f * 10 + 2], uy = fg[(size_t)f * 10 + 3];
+      double vx = fg[(size_t)f * 10 + 4], vy = fg[(size_t)f * 10 + 5];
+      double tauxx = two_third * (two * ux - vy), tauxy = uy + vx, tauyy = two_third * (two * vy - ux);
+      fx = fx - mu * (tauxx * nx + tauxy * ny) * area;
+      fy = fy - mu * (tauxy * nx + tauyy * ny) * area;
+      double tx = ny, ty = -nx;
+      double tw = mu * ((ux * tx + vx * ty) * nx + (uy * tx + vy * ty) * ny);
+      cfv = tw * area / dyna;
+    }
     cm = cm + dcx * ya - dcy * xa;
     if (cp) cp[iw] = -cpv;
-    if (cf) cf[iw] = 0.0;
-    fx1 = fx1 + bf[k];
-    fy1 = fy1 + bf[nbf + k];
+    if (cf) cf[iw] = cfv;
     ++iw;
   }
+  for (int k : x->wall_k) { fx1 = fx1 + bf[k]; fy1 = fy1 + bf[nbf + k]; }
+  if (visc)
+    for (int k : x->wall_k) { fx1 = fx1 + bv[k]; fy1 = fy1 + bv[nbf + k]; }
   double aoa = p.aoa;
   out[0] = (fy * std::cos(aoa) - fx * std::sin(aoa)) / dyna;
   out[1] = (fy * std::sin(aoa) + fx * std::cos(aoa)) / dyna;
@@ -1186,20 +1033,10 @@ int ufx_cc_iterate(ufx_cc* x, int iter0, int niter, const double* cfl, double* h
   if (int rc = check_ctx(x)) return rc;
   if (niter < 0 || (niter > 0 && !cfl)) { set_error("ufx_cc_iterate: bad arguments"); return UFX_ERR_ARG; }
   if (int rc = ensure_hist(x, std::max(1, niter))) return rc;
-  const Dev& d = x->d;
   for (int k = 0; k < niter; ++k) {
-    int iter = iter0 + 1 + k;
-    // prim refresh + qold (save_old commutes with avg_c2v/time_step) + range
-    LAUNCH(x, UFX_K_PRIM, k_prim, x->nblk_all, d, x->g, 1, 1, x->p.ilimit);
-    LAUNCH(x, UFX_K_AVG, k_avg_c2v, nblocks(d.nop), d);
-    LAUNCH(x, UFX_K_TSTEP, k_timestep, x->nblk_cell, d, x->g, x->p.vll, cfl[k]);
-    if (x->p.ilimit)
-      LAUNCH(x, UFX_K_RANGE, k_range_final, 1, d, x->nblk_all);
-    LAUNCH(x, UFX_K_GRAD, k_grad_lsq, x->nblk_cell, d, 1, x->p.ilimit);
-    for (int s = 0; s < 3; ++s)
-      if (int rc = residual(x, s == 2 ? 2 : 1, RK_A[s], RK_B[s])) return rc;
-    LAUNCH(x, UFX_K_NORM, k_norm_final, 1, d, x->nblk_cell, iter == 1, x->d_hist, k);
-    if (int rc = launch_check()) return rc;
+    x->cfl = cfl[k];
+    int rc = with_dev(x, [&](auto& d) { return run_iteration(x, d, iter0 + 1 + k, k); });
+    if (rc) return rc;
   }
   if (hist && niter > 0) {
     HIPCHK(hipMemcpyAsync(hist, x->d_hist, (size_t)niter * 8, hipMemcpyDeviceToHost, x->stream));
