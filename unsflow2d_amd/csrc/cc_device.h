@@ -60,6 +60,7 @@ struct DevT {
   const int* cnf;            // nf | nb << 8
   const double *r11, *r12, *r22, *cv, *cov;  // noc (cov: ncell)
   const int* nc2c;
+  const int* cref;           // storage cell -> reference cell number (ncell)
   const int *v2c_ptr, *v2c;  // point storage order
   const double* wt;
   const double *px, *py;     // point storage order

@@ -83,6 +83,24 @@ __device__ __forceinline__ void face_prims(const DevT<R>& d, int f, R* fq) {
   for (int k = 0; k < NV; ++k) fq[k] = R(0.5) * (a[k] + b[k]);
 }
 
+// Spectral radius of every face, CCFVM/timestep.F90:147-171 (spectral02):
+// inviscid |u.n| + a from the face-averaged primitives, times the face
+// length, plus the viscous term.
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, double vll) {
+  int f = cell_index();
+  if (f >= d.nof || d.err[0]) return;
+  R fq[4], fr[Layout<R>::FW];
+  face_prims<R, 4>(d, f, fq);
+  face_rec(d, f, fr);
+  double r0 = fq[0], u0 = fq[1], v0 = fq[2], p0 = fq[3];
+  double a0 = sqrt(g.gamma * p0 / r0);
+  double un = u0 * (double)fr[F_NX] + v0 * (double)fr[F_NY];
+  double ll = (double)fr[F_AREA] * (fabs(un) + a0);
+  ll = ll + vll * d.fmu[f] / r0;
+  d.fla[f] = ll;
+}
+
 // ------------------------------------------------- primitive refresh + range
 // CCFVM/main.F90:62-67 (con2prim of interior cells), qold = qc
 // (CCFVM/solution.F90:118-132), positivity (CCFVM/main.F90:384-411) and the
@@ -111,7 +129,7 @@ __global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_p
         stv<R, 4>(d.rec + (size_t)c * RW + R_QP, q4);
         d.qT[c] = (R)p[4];
         if (do_old && (p[0] <= 0.0 || p[3] <= 0.0)) {
-          atomicMin(&d.err[2], c + 1);
+          atomicMin(&d.err[2], d.cref[c] + 1);
           atomicCAS(d.err, 0, DERR_POS);
         }
       } else {
@@ -201,8 +219,7 @@ enum { G_NONE = 0, G_LSQR = 1, G_GG = 2, G_GGFC = 3 };
 //   limiter   Venkatakrishnan with the ratchet, CCFVM/limiter.F90:123-296;
 //   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177.
 template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
-__global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl,
-                                              int store_fla) {
+__global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   int c = cell_index();
   if (c >= d.noc || d.err[0]) return;
@@ -235,20 +252,13 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
     fcx[s] = fr[F_CX];
     fcy[s] = fr[F_CY];
     R fq[5];
-    if (TSTEP || GRAD == G_GG || (LIMIT && bnd)) {
+    if (GRAD == G_GG || (LIMIT && bnd)) {
       if (GRAD == G_GG) face_prims<R, 5>(d, f, fq);
       else face_prims<R, 4>(d, f, fq);
     }
-    if (TSTEP) {
-      // spectral radius of the face, CCFVM/timestep.F90:147-174
-      double r0 = fq[0], u0 = fq[1], v0 = fq[2], p0 = fq[3];
-      double a0 = sqrt(g.gamma * p0 / r0);
-      double un = u0 * (double)fr[F_NX] + v0 * (double)fr[F_NY];
-      double ll = (double)fr[F_AREA] * (fabs(un) + a0);
-      ll = ll + vll * d.fmu[f] / r0;
-      la = la + ll;
-      if (store_fla && in_side) d.fla[f] = ll;
-    }
+    // cell spectral radius: the face radii of k_face_lambda summed in
+    // ascending face order, CCFVM/timestep.F90:171-173
+    if (TSTEP) la = la + d.fla[f];
     R nq[4];
     if (GRAD == G_LSQR || LIMIT) ldv<R, 4>(d.rec + (size_t)n * RW + R_QP, nq);
     if (GRAD == G_LSQR) {
@@ -511,7 +521,7 @@ __global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int vis
       d.bflux[k] = F[2];
       d.bflux[d.nbf + k] = F[3];
     }
-    if (!ok) set_err(d, DERR_ROE, c + 1);
+    if (!ok) set_err(d, DERR_ROE, d.cref[c] + 1);
     st4d(d.qc + (size_t)gh * 4, gq);
     R gp4[4] = {(R)gp[0], (R)gp[1], (R)gp[2], (R)gp[3]};
     stv<R, 4>(d.rec + (size_t)gh * RW + R_QP, gp4);
@@ -591,7 +601,7 @@ __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, d
         for (int i = 0; i < 4; ++i) r[i] = r[i] - (double)F[i];
       }
     }
-    if (!ok) set_err(d, DERR_ROE, c + 1);
+    if (!ok) set_err(d, DERR_ROE, d.cref[c] + 1);
     if (mode == 0 || write_res) st4d(d.res + (size_t)c * 4, r);
     if (mode >= 1) {
       double dta = d.dta[c], q[4], o[4];
@@ -755,7 +765,7 @@ __global__ __launch_bounds__(BLK) void k_positivity(DevT<R> d) {
   if (c >= d.noc) return;
   R q[4];
   ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q);
-  if (q[0] <= R(0) || q[3] <= R(0)) atomicMin(&d.err[2], c + 1);
+  if (q[0] <= R(0) || q[3] <= R(0)) atomicMin(&d.err[2], d.cref[c] + 1);
 }
 
 template <typename R>
@@ -814,7 +824,8 @@ __global__ __launch_bounds__(BLK) void k_lusgs_level(DevT<R> d, Gas<double> g, c
     double nx = fr[F_NX], ny = fr[F_NY];
     int in = d.fin[f], out = d.fout[f];
     if (out == it) { out = in; nx = -(double)fr[F_NX]; ny = -(double)fr[F_NY]; }
-    bool use = dir == 0 ? (out < it && out >= 0 && out < noc) : (out > it && out < noc);
+    // lower/upper neighbour in the reference's cell numbering
+    bool use = out < noc && (dir == 0 ? d.cref[out] < d.cref[it] : d.cref[out] > d.cref[it]);
     if (use) {
       double qo[4], dq[4], df[4];
       ld4d(d.qold + (size_t)out * 4, qo);

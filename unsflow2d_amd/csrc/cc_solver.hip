@@ -46,7 +46,6 @@ struct ufx_cc {
   DevT<float> d32{};
   int nop = 0, noc = 0, nfi = 0, nbf = 0, nof = 0, ncell = 0;
   double cfl = 0.0;              // CFL of the current iteration (param's CFL)
-  bool store_fla = false;        // store face spectral radii in any mode
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
@@ -54,6 +53,9 @@ struct ufx_cc {
   int hist_cap = 0;
   double* cirs_tmp = nullptr;    // [2][noc][4]
   double* bfluxv = nullptr;      // viscous wall force terms [2][nbf]
+  std::vector<int> sto_of_ref;   // reference cell -> storage cell (ncell)
+  std::vector<int> ref_of_sto;   // storage cell -> reference cell
+  std::vector<int> ref_of_face;  // storage face -> reference face
   std::vector<int> face_of_ref;  // reference face -> storage face
   std::vector<int> pt_of_ref;    // reference vertex -> storage vertex
   std::vector<int> wall_k;       // boundary ordinals of wall faces
@@ -113,6 +115,45 @@ namespace {
 
 int nblocks(int n) { return std::max(1, (n + BLK - 1) / BLK); }
 
+// Position along a Hilbert curve of order 16 of the point (x, y) of the
+// 65536 x 65536 grid.
+uint32_t hilbert_d(uint32_t x, uint32_t y) {
+  uint32_t d = 0;
+  for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+    uint32_t rx = (x & s) ? 1 : 0, ry = (y & s) ? 1 : 0;
+    d += s * s * ((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = 65535u - x;
+        y = 65535u - y;
+      }
+      uint32_t t = x; x = y; y = t;
+    }
+  }
+  return d;
+}
+
+// Cells ordered along the Hilbert curve through their centroids (ties keep
+// the reference numbering).
+std::vector<int> hilbert_order(const double* cx, const double* cy, int n) {
+  double x0 = cx[0], x1 = cx[0], y0 = cy[0], y1 = cy[0];
+  for (int i = 0; i < n; ++i) {
+    x0 = std::min(x0, cx[i]); x1 = std::max(x1, cx[i]);
+    y0 = std::min(y0, cy[i]); y1 = std::max(y1, cy[i]);
+  }
+  double sc = 65535.0 / std::max(std::max(x1 - x0, y1 - y0), 1e-300);
+  std::vector<std::pair<uint32_t, int>> key(n);
+  for (int i = 0; i < n; ++i) {
+    uint32_t ix = (uint32_t)std::min(65535.0, std::max(0.0, (cx[i] - x0) * sc));
+    uint32_t iy = (uint32_t)std::min(65535.0, std::max(0.0, (cy[i] - y0) * sc));
+    key[i] = {hilbert_d(ix, iy), i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = key[i].second;
+  return order;
+}
+
 int launch_check() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -166,7 +207,9 @@ int device_error(ufx_cc* x) {
   if (e[0] == 0) return UFX_OK;
   if (e[0] == DERR_ROE) set_error("Roe scheme: sonic speed is negative (cell " + std::to_string(e[1]) + ")");
   else if (e[0] == DERR_POS) set_error("Density/pressure is negative at point " + std::to_string(e[2]));
-  else if (e[0] == DERR_AREA) set_error("ggfc,gradtrixy,  -ve area (face " + std::to_string(e[1]) + ")");
+  else if (e[0] == DERR_AREA)
+    set_error("ggfc,gradtrixy,  -ve area (face " +
+              std::to_string(e[1] >= 1 && e[1] <= x->nof ? x->ref_of_face[e[1] - 1] + 1 : e[1]) + ")");
   else set_error("device error " + std::to_string(e[0]));
   return UFX_ERR_STATE;
 }
@@ -213,13 +256,14 @@ int run_avg(ufx_cc* x, DevT<R>& d) {
 // has a consumer (the drop-in gradient routines always produce it).
 template <typename R, int GR>
 int run_cell_g(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl) {
-  // face spectral radii are only consumed by LU-SGS; the drop-in time step
-  // routine always stores them, as the reference does
-  int fla = (x->p.timemode == UFX_TIME_LUSGS || x->store_fla) ? 1 : 0;
   double vll = x->p.vll;
   int cat = GR != G_NONE ? UFX_K_GRAD : (tstep ? UFX_K_TSTEP : UFX_K_GRAD);
+  if (tstep) {
+    LAUNCH(x, UFX_K_TSTEP, k_face_lambda<R>, x->nblk_face, d, x->g, vll);
+    if (int rc = launch_check()) return rc;
+  }
 #define KC(WT, LM, TS) \
-  LAUNCH(x, cat, (k_cell<R, GR, WT, LM, TS>), x->nblk_cell, d, x->g, vll, cfl, fla)
+  LAUNCH(x, cat, (k_cell<R, GR, WT, LM, TS>), x->nblk_cell, d, x->g, vll, cfl)
   if (with_t) {
     if (limit) { if (tstep) KC(true, true, true); else KC(true, true, false); }
     else { if (tstep) KC(true, false, true); else KC(true, false, false); }
@@ -397,16 +441,22 @@ int get_field(ufx_cc* x, DevT<R>& d, const std::string& n, double* dst) {
     return hipMemcpyAsync(h, dv, bytes, hipMemcpyDeviceToHost, x->stream);
   };
   const int noc = x->noc, ncell = x->ncell, nof = x->nof, nop = x->nop;
+  const std::vector<int>& sto = x->sto_of_ref;
   if (n == "qc" || n == "qold" || n == "res") {
-    size_t cnt = n == "qc" ? (size_t)ncell : (size_t)noc;
+    int cnt = n == "qc" ? ncell : noc;
     const double* src = n == "qc" ? d.qc : (n == "qold" ? d.qold : d.res);
-    HIPCHK(copy(dst, src, cnt * 32));
+    std::vector<double> t((size_t)cnt * 4);
+    HIPCHK(copy(t.data(), src, t.size() * 8));
     HIPCHK(hipStreamSynchronize(x->stream));
+    for (int c = 0; c < cnt; ++c)
+      for (int k = 0; k < 4; ++k) dst[(size_t)c * 4 + k] = t[(size_t)sto[c] * 4 + k];
     return UFX_OK;
   }
   if (n == "dt" || n == "la") {
-    HIPCHK(copy(dst, n == "dt" ? d.dt : d.la, (size_t)noc * 8));
+    std::vector<double> t(noc);
+    HIPCHK(copy(t.data(), n == "dt" ? d.dt : d.la, (size_t)noc * 8));
     HIPCHK(hipStreamSynchronize(x->stream));
+    for (int c = 0; c < noc; ++c) dst[c] = t[sto[c]];
     return UFX_OK;
   }
   if (n == "qp" || n == "grad" || n == "phi") {
@@ -417,18 +467,20 @@ int get_field(ufx_cc* x, DevT<R>& d, const std::string& n, double* dst) {
     HIPCHK(hipStreamSynchronize(x->stream));
     if (n == "qp")
       for (int c = 0; c < ncell; ++c) {
-        for (int k = 0; k < 4; ++k) dst[(size_t)c * 5 + k] = rec[(size_t)c * RW + R_QP + k];
-        dst[(size_t)c * 5 + 4] = t[c];
+        size_t s = sto[c];
+        for (int k = 0; k < 4; ++k) dst[(size_t)c * 5 + k] = rec[s * RW + R_QP + k];
+        dst[(size_t)c * 5 + 4] = t[s];
       }
     else if (n == "grad")
       for (int c = 0; c < noc; ++c) {
-        for (int k = 0; k < 8; ++k) dst[(size_t)c * 10 + k] = rec[(size_t)c * RW + R_G + k];
-        dst[(size_t)c * 10 + 8] = gt[(size_t)c * 2];
-        dst[(size_t)c * 10 + 9] = gt[(size_t)c * 2 + 1];
+        size_t s = sto[c];
+        for (int k = 0; k < 8; ++k) dst[(size_t)c * 10 + k] = rec[s * RW + R_G + k];
+        dst[(size_t)c * 10 + 8] = gt[s * 2];
+        dst[(size_t)c * 10 + 9] = gt[s * 2 + 1];
       }
     else
       for (int c = 0; c < noc; ++c)
-        for (int k = 0; k < 4; ++k) dst[(size_t)c * 4 + k] = rec[(size_t)c * RW + R_PHI + k];
+        for (int k = 0; k < 4; ++k) dst[(size_t)c * 4 + k] = rec[(size_t)sto[c] * RW + R_PHI + k];
     return UFX_OK;
   }
   if (n == "fc_la" || n == "fc_mu") {
@@ -458,7 +510,7 @@ int get_field(ufx_cc* x, DevT<R>& d, const std::string& n, double* dst) {
     std::vector<R> t((size_t)ncell);
     HIPCHK(copy(t.data(), d.cmul, t.size() * sizeof(R)));
     HIPCHK(hipStreamSynchronize(x->stream));
-    for (int c = 0; c < ncell; ++c) dst[c] = t[c];
+    for (int c = 0; c < ncell; ++c) dst[c] = t[sto[c]];
     return UFX_OK;
   }
   set_error("ufx_cc_get: unknown field " + n);
@@ -472,22 +524,28 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
     return hipMemcpyAsync(dv, h, bytes, hipMemcpyHostToDevice, x->stream);
   };
   const int noc = x->noc, ncell = x->ncell, nof = x->nof;
+  const std::vector<int>& sto = x->sto_of_ref;
   if (n == "qc" || n == "qold" || n == "res") {
-    size_t cnt = n == "qc" ? (size_t)ncell : (size_t)noc;
+    int cnt = n == "qc" ? ncell : noc;
     double* dv = n == "qc" ? d.qc : (n == "qold" ? d.qold : d.res);
-    HIPCHK(up(dv, src, cnt * 32));
+    std::vector<double> t((size_t)cnt * 4);
+    for (int c = 0; c < cnt; ++c)
+      for (int k = 0; k < 4; ++k) t[(size_t)sto[c] * 4 + k] = src[(size_t)c * 4 + k];
+    HIPCHK(up(dv, t.data(), t.size() * 8));
     HIPCHK(hipStreamSynchronize(x->stream));
     return UFX_OK;
   }
   if (n == "dt" || n == "la") {
-    HIPCHK(up(n == "dt" ? d.dt : d.la, src, (size_t)noc * 8));
+    std::vector<double> t(noc);
+    for (int c = 0; c < noc; ++c) t[sto[c]] = src[c];
+    HIPCHK(up(n == "dt" ? d.dt : d.la, t.data(), (size_t)noc * 8));
     HIPCHK(hipStreamSynchronize(x->stream));
     if (n == "dt") {
       // dt / area, which the stage update uses
       std::vector<double> cv(noc), dta(noc);
       HIPCHK(hipMemcpyAsync(cv.data(), d.cv, (size_t)noc * 8, hipMemcpyDeviceToHost, x->stream));
       HIPCHK(hipStreamSynchronize(x->stream));
-      for (int c = 0; c < noc; ++c) dta[c] = src[c] / cv[c];
+      for (int s = 0; s < noc; ++s) dta[s] = t[s] / cv[s];
       HIPCHK(up(d.dta, dta.data(), (size_t)noc * 8));
       HIPCHK(hipStreamSynchronize(x->stream));
     }
@@ -501,18 +559,20 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
     HIPCHK(hipStreamSynchronize(x->stream));
     if (n == "qp")
       for (int c = 0; c < ncell; ++c) {
-        for (int k = 0; k < 4; ++k) rec[(size_t)c * RW + R_QP + k] = (R)src[(size_t)c * 5 + k];
-        t[c] = (R)src[(size_t)c * 5 + 4];
+        size_t s = sto[c];
+        for (int k = 0; k < 4; ++k) rec[s * RW + R_QP + k] = (R)src[(size_t)c * 5 + k];
+        t[s] = (R)src[(size_t)c * 5 + 4];
       }
     else if (n == "grad")
       for (int c = 0; c < noc; ++c) {
-        for (int k = 0; k < 8; ++k) rec[(size_t)c * RW + R_G + k] = (R)src[(size_t)c * 10 + k];
-        gt[(size_t)c * 2] = (R)src[(size_t)c * 10 + 8];
-        gt[(size_t)c * 2 + 1] = (R)src[(size_t)c * 10 + 9];
+        size_t s = sto[c];
+        for (int k = 0; k < 8; ++k) rec[s * RW + R_G + k] = (R)src[(size_t)c * 10 + k];
+        gt[s * 2] = (R)src[(size_t)c * 10 + 8];
+        gt[s * 2 + 1] = (R)src[(size_t)c * 10 + 9];
       }
     else
       for (int c = 0; c < noc; ++c)
-        for (int k = 0; k < 4; ++k) rec[(size_t)c * RW + R_PHI + k] = (R)src[(size_t)c * 4 + k];
+        for (int k = 0; k < 4; ++k) rec[(size_t)sto[c] * RW + R_PHI + k] = (R)src[(size_t)c * 4 + k];
     HIPCHK(up(d.rec, rec.data(), rec.size() * sizeof(R)));
     HIPCHK(up(d.qT, t.data(), t.size() * sizeof(R)));
     HIPCHK(up(d.gT, gt.data(), gt.size() * sizeof(R)));
@@ -528,7 +588,7 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
   }
   if (n == "cell_mul") {
     std::vector<R> t((size_t)ncell);
-    for (int c = 0; c < ncell; ++c) t[c] = (R)src[c];
+    for (int c = 0; c < ncell; ++c) t[sto[c]] = (R)src[c];
     HIPCHK(up(d.cmul, t.data(), t.size() * sizeof(R)));
     HIPCHK(hipStreamSynchronize(x->stream));
     return UFX_OK;
@@ -546,6 +606,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW, PW = Layout<R>::PW;
   const int noc = m.noc, ncell = m.ncell(), nof = m.nof(), nop = m.nop, nbf = m.nbf;
   d.nop = nop; d.noc = noc; d.nfi = m.nfi; d.nbf = nbf; d.nof = nof; d.ncell = ncell;
+  const std::vector<int>& sto = x->sto_of_ref;
   // faces in storage order
   std::vector<R> frec((size_t)nof * FW, R(0));
   std::vector<int2> fpt(nof);
@@ -556,15 +617,25 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     r[F_NX] = (R)m.fnx[f]; r[F_NY] = (R)m.fny[f]; r[F_AREA] = (R)m.farea[f];
     r[F_CX] = (R)m.fcx[f]; r[F_CY] = (R)m.fcy[f];
     fpt[s] = make_int2(x->pt_of_ref[m.fp1[f]], x->pt_of_ref[m.fp2[f]]);
-    fin[s] = m.fin[f];
-    fout[s] = m.fout[f];
+    fin[s] = sto[m.fin[f]];
+    fout[s] = sto[m.fout[f]];
   }
   std::vector<int> fbc(std::max(1, nbf)), bface(std::max(1, nbf)), bcell(std::max(1, nbf));
   for (int k = 0; k < nbf; ++k) {
     fbc[k] = m.fbc[m.nfi + k];
     bface[k] = x->face_of_ref[m.nfi + k];
-    bcell[k] = m.fin[m.nfi + k];
+    bcell[k] = sto[m.fin[m.nfi + k]];
   }
+  // per-cell mesh data in storage order
+  auto perm = [&](const std::vector<double>& a, int n) {
+    std::vector<double> o(n);
+    for (int s = 0; s < n; ++s) o[s] = a[x->ref_of_sto[s]];
+    return o;
+  };
+  std::vector<double> r11 = perm(m.r11, noc), r12 = perm(m.r12, noc), r22 = perm(m.r22, noc);
+  std::vector<double> cv = perm(m.ccv, noc), cov = perm(m.ccov, ncell);
+  std::vector<int> nc2c(noc);
+  for (int s = 0; s < noc; ++s) nc2c[s] = m.nc2c[x->ref_of_sto[s]];
   // vertices in storage order
   std::vector<int> v2c_ptr(nop + 1, 0), v2c;
   std::vector<double> wt, px(nop), py(nop);
@@ -575,7 +646,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   for (int s = 0; s < nop; ++s) {
     int i = ref_of_pt[s];
     for (int j = m.v2c_ptr[i]; j < m.v2c_ptr[i + 1]; ++j) {
-      v2c.push_back(m.v2c[j]);
+      v2c.push_back(sto[m.v2c[j]]);
       wt.push_back(m.wt[j]);
     }
     v2c_ptr[s + 1] = (int)v2c.size();
@@ -589,8 +660,8 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     R* r = &rec[(size_t)c * RW];
     if (c < noc)
       for (int k = 0; k < 4; ++k) r[R_PHI + k] = R(1);
-    r[R_CEN] = (R)m.ccx[c];
-    r[R_CEN + 1] = (R)m.ccy[c];
+    r[R_CEN] = (R)m.ccx[x->ref_of_sto[c]];
+    r[R_CEN + 1] = (R)m.ccy[x->ref_of_sto[c]];
   }
   std::vector<double> qc((size_t)ncell * 4);
   for (int c = 0; c < ncell; ++c)
@@ -606,8 +677,9 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   }
   UP(fbc, fbc); UP(bface, bface); UP(bcell, bcell);
   UP(sl_face, sl_face); UP(sl_nbr, sl_nbr); UP(cnf, cnf);
-  UP(r11, m.r11); UP(r12, m.r12); UP(r22, m.r22); UP(cv, m.ccv); UP(cov, m.ccov);
-  UP(nc2c, m.nc2c); UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py);
+  UP(r11, r11); UP(r12, r12); UP(r22, r22); UP(cv, cv); UP(cov, cov);
+  UP(nc2c, nc2c); UP(cref, x->ref_of_sto);
+  UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py);
   UP(fin, fin); UP(fout, fout); UP(lu_face, lu_face); UP(lu_cnt, lu_cnt);
   UP(rec, rec); UP(qc, qc);
   AL(fw, nof); AL(qT, ncell); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc);
@@ -656,27 +728,40 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   if (prm->precision != UFX_PREC_FP64 && prm->precision != UFX_PREC_MIXED) { set_error("unknown precision"); return UFX_ERR_ARG; }
   if (prm->irs && !(prm->cfl_max > 0.0)) { set_error("residual smoothing needs cfl_max > 0"); return UFX_ERR_ARG; }
   const int noc = m.noc, nof = m.nof(), nop = m.nop;
-  // storage order of faces and vertices: first touch walking the cells in
-  // order and their faces in ascending reference order
+  // faces of each cell in ascending reference order (the order of every
+  // per-cell sum)
   std::vector<std::vector<int>> cf(noc);
   for (int f = 0; f < nof; ++f) {
     cf[m.fin[f]].push_back(f);
     if (f < m.nfi) cf[m.fout[f]].push_back(f);
   }
   for (int c = 0; c < noc; ++c) {
-    if (cf[c].size() > (size_t)MAXF) {
+    if (cf[c].size() > (size_t)MAXF || m.c2f_ptr[c + 1] - m.c2f_ptr[c] > MAXF) {
       set_error("cell " + std::to_string(c + 1) + " has more than 4 faces");
       return UFX_ERR_UNSUPPORTED;
     }
     std::sort(cf[c].begin(), cf[c].end());
   }
   ufx_cc* x = new ufx_cc();
+  // storage order of the interior cells: Hilbert curve through the centroids
+  // (compact thread blocks, local neighbour gathers); ghosts keep their
+  // reference position after the interior cells
+  x->ref_of_sto = hilbert_order(m.ccx.data(), m.ccy.data(), noc);
+  x->sto_of_ref.assign(m.ncell(), 0);
+  for (int s = 0; s < noc; ++s) x->sto_of_ref[x->ref_of_sto[s]] = s;
+  for (int g = noc; g < m.ncell(); ++g) {
+    x->ref_of_sto.push_back(g);
+    x->sto_of_ref[g] = g;
+  }
+  const std::vector<int>& sto = x->sto_of_ref;
+  // storage order of faces and vertices: first touch walking the cells in
+  // storage order and their faces in ascending reference order
   x->face_of_ref.assign(nof, -1);
   x->pt_of_ref.assign(nop, -1);
   {
     int nf = 0, np = 0;
-    for (int c = 0; c < noc; ++c)
-      for (int f : cf[c]) {
+    for (int s = 0; s < noc; ++s)
+      for (int f : cf[x->ref_of_sto[s]]) {
         if (x->face_of_ref[f] < 0) x->face_of_ref[f] = nf++;
         for (int p : {m.fp1[f], m.fp2[f]})
           if (x->pt_of_ref[p] < 0) x->pt_of_ref[p] = np++;
@@ -685,34 +770,33 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
       if (x->face_of_ref[f] < 0) x->face_of_ref[f] = nf++;
     for (int p = 0; p < nop; ++p)
       if (x->pt_of_ref[p] < 0) x->pt_of_ref[p] = np++;
+    x->ref_of_face.assign(nof, 0);
+    for (int f = 0; f < nof; ++f) x->ref_of_face[x->face_of_ref[f]] = f;
   }
   std::vector<int> sl_face((size_t)MAXF * noc, 0), sl_nbr((size_t)MAXF * noc, 0), cnf(noc);
-  for (int c = 0; c < noc; ++c) {
+  for (int s = 0; s < noc; ++s) {
+    int c = x->ref_of_sto[s];
     int nb = 0;
-    for (size_t s = 0; s < cf[c].size(); ++s) {
-      int f = cf[c][s];
+    for (size_t j = 0; j < cf[c].size(); ++j) {
+      int f = cf[c][j];
       bool in_side = m.fin[f] == c;
-      int other = in_side ? m.fout[f] : m.fin[f];
-      sl_face[s * noc + c] = x->face_of_ref[f];
-      sl_nbr[s * noc + c] = in_side ? other : ~other;
+      int other = sto[in_side ? m.fout[f] : m.fin[f]];
+      sl_face[j * noc + s] = x->face_of_ref[f];
+      sl_nbr[j * noc + s] = in_side ? other : ~other;
       if (f >= m.nfi) ++nb;
     }
-    cnf[c] = (int)cf[c].size() | (nb << 8);
+    cnf[s] = (int)cf[c].size() | (nb << 8);
   }
   // LU-SGS: the reference's faces-of-cell order and the level schedule of
   // its sequential sweeps (a cell's level is one more than the highest level
-  // among the neighbours it waits for)
+  // among the neighbours it waits for), in the reference's cell numbering
   std::vector<int> lu_face((size_t)MAXF * noc, 0), lu_cnt(noc);
   std::vector<int> lvf(noc, 0), lvb(noc, 0);
   for (int c = 0; c < noc; ++c) {
     int n = m.c2f_ptr[c + 1] - m.c2f_ptr[c];
-    if (n > MAXF) {
-      delete x;
-      set_error("cell " + std::to_string(c + 1) + " has more than 4 faces");
-      return UFX_ERR_UNSUPPORTED;
-    }
-    lu_cnt[c] = n;
-    for (int j = 0; j < n; ++j) lu_face[(size_t)j * noc + c] = x->face_of_ref[m.c2f[m.c2f_ptr[c] + j]];
+    lu_cnt[sto[c]] = n;
+    for (int j = 0; j < n; ++j)
+      lu_face[(size_t)j * noc + sto[c]] = x->face_of_ref[m.c2f[m.c2f_ptr[c] + j]];
   }
   auto nbr_of = [&](int c, int j) {
     int f = m.c2f[m.c2f_ptr[c] + j];
@@ -721,7 +805,7 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   int nlf = 0, nlb = 0;
   for (int c = 0; c < noc; ++c) {
     int l = 0;
-    for (int j = 0; j < lu_cnt[c]; ++j) {
+    for (int j = 0; j < m.c2f_ptr[c + 1] - m.c2f_ptr[c]; ++j) {
       int o = nbr_of(c, j);
       if (o < c && o >= 0) l = std::max(l, lvf[o] + 1);
     }
@@ -730,28 +814,28 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   }
   for (int c = noc - 1; c >= 0; --c) {
     int l = 0;
-    for (int j = 0; j < lu_cnt[c]; ++j) {
+    for (int j = 0; j < m.c2f_ptr[c + 1] - m.c2f_ptr[c]; ++j) {
       int o = nbr_of(c, j);
       if (o > c && o < noc) l = std::max(l, lvb[o] + 1);
     }
     lvb[c] = l;
     nlb = std::max(nlb, l + 1);
   }
-  std::vector<int> lv_cells(2 * (size_t)noc);
+  std::vector<int> lv_cells(2 * (size_t)noc);  // storage indices, by level
   {
     std::vector<int> cnt(nlf + 1, 0);
     for (int c = 0; c < noc; ++c) cnt[lvf[c] + 1]++;
     for (int l = 0; l < nlf; ++l) cnt[l + 1] += cnt[l];
     x->lv_fwd_off = cnt;
     std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-    for (int c = 0; c < noc; ++c) lv_cells[pos[lvf[c]]++] = c;
+    for (int c = 0; c < noc; ++c) lv_cells[pos[lvf[c]]++] = sto[c];
     std::vector<int> cb(nlb + 1, 0);
     for (int c = 0; c < noc; ++c) cb[lvb[c] + 1]++;
     for (int l = 0; l < nlb; ++l) cb[l + 1] += cb[l];
     for (int l = 0; l <= nlb; ++l) cb[l] += noc;
     x->lv_bwd_off = cb;
     std::vector<int> pb(cb.begin(), cb.end() - 1);
-    for (int c = noc - 1; c >= 0; --c) lv_cells[pb[lvb[c]]++] = c;
+    for (int c = noc - 1; c >= 0; --c) lv_cells[pb[lvb[c]]++] = sto[c];
   }
   hipError_t e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -855,13 +939,10 @@ int ufx_cc_avg_c2v(ufx_cc* x) {
 int ufx_cc_time_step(ufx_cc* x, double cfl) {
   if (int rc = check_ctx(x)) return rc;
   x->cfl = cfl;
-  x->store_fla = true;
-  int rc = with_dev(x, [&](auto& d) {
+  return with_dev(x, [&](auto& d) {
     using R = typename std::remove_reference_t<decltype(d)>::Real;
     return run_cell_g<R, G_NONE>(x, d, false, false, true, cfl);
   });
-  x->store_fla = false;
-  return rc;
 }
 
 int ufx_cc_gradient(ufx_cc* x) {
@@ -967,7 +1048,7 @@ int ufx_cc_forces(ufx_cc* x, double* out, double* cp, double* cf) {
     }
     HIPCHK(hipMemcpyAsync(dtmin.data(), d.scal + SC_DTMIN, 8, hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    for (int k = 0; k < nbf; ++k) pin[k] = rec[(size_t)x->fin_h[k] * RW + R_QP + 3];
+    for (int k = 0; k < nbf; ++k) pin[k] = rec[(size_t)x->sto_of_ref[x->fin_h[k]] * RW + R_QP + 3];
     return (int)UFX_OK;
   });
   if (rc) return rc;
