@@ -102,6 +102,20 @@ def cpu_baseline(budget_s: float = 20.0) -> dict | None:
                       f"{ms:.2f} ms/iteration, 1 thread"}
 
 
+def copy_probe(mib: int) -> int:
+    """A plain device-to-device copy of known size, five times: the reference
+    point that scales the profiler's HBM read/write counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"# copy probe: {mib} MiB read + {mib} MiB written per launch, 5 launches", flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,7 +125,11 @@ def main() -> int:
     ap.add_argument("--quads", type=int, default=1000, help="quads per side of the channel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
               "shards by mesh partition, which this build does not implement)", file=sys.stderr)
@@ -184,7 +202,7 @@ def main() -> int:
                    "cells": noc, "faces": nof, "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_residual (stage)", "bytes_per_launch": int(sb),
+                     "kernel": "k_stage", "bytes_per_launch": int(sb),
                      "avg_launch_ms": round(st_avg, 5)},
         "cpu_baseline": cpu,
     }

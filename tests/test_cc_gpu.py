@@ -238,3 +238,31 @@ def test_mixed_precision_tracks_fp64():
     e_q = relerr(b.get("qc"), a.get("qc"))
     print(f"  history rel {e_h:.2e} state rel {e_q:.2e}", flush=True)
     assert e_q < 1e-4 and e_h < 1e-2
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_shared_flux_stage_matches_gather(precision, monkeypatch):
+    """The block-shared residual stage (each interior flux once per block,
+    kept in LDS) against the per-cell gather stage (each flux once per
+    adjacent cell), on a mesh of many blocks, and the FP64 form against the
+    oracle."""
+    print(f"case: shared vs gather stage {precision} mesh 80x40 seed 13", flush=True)
+    from unsflow2d_amd import cc, meshgen
+    from oracle.oracle import CCOracle
+    raw = meshgen.bump_channel(80, 40, 13, h=0.08)
+    mesh = cc.Mesh.from_raw(raw)
+    cfg = cc.Config(mach=0.7, precision=precision)
+    monkeypatch.setenv("UFX_STAGE_KERNEL", "gather")
+    g = cc.Solver(mesh, cfg)
+    monkeypatch.setenv("UFX_STAGE_KERNEL", "shared")
+    s = cc.Solver(mesh, cfg)
+    monkeypatch.delenv("UFX_STAGE_KERNEL")
+    hg = g.iterate(12)
+    hs = s.iterate(12)
+    _check("history", hs, hg, 1e-13 if precision == "fp64" else 1e-6)
+    _check("qc", s.get("qc"), g.get("qc"), 1e-13 if precision == "fp64" else 1e-6)
+    if precision == "fp64":
+        orc = CCOracle(mesh.arrays(), mach=0.7)
+        ref = np.array([orc.iteration(it, 0.8) for it in range(1, 13)])
+        _check("history vs oracle", hs, ref, 1e-11)
+        _check("qc vs oracle", s.get("qc"), orc.qc, 1e-11)

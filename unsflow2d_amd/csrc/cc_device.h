@@ -14,6 +14,8 @@
 //   sl_face/sl_nbr [MAXF][noc] int  the faces of each cell in ascending
 //                        reference face order, and the cell across each
 //                        (bit-inverted when the cell is the face's `out`)
+//   bf_ptr/bf_face       per block of BLK cells, the interior faces it touches
+//   sl_loc [MAXF][noc]   per cell slot, the face's position in its block's list
 // R is double for the FP64 form and float for the bulk data of the mixed
 // form; states, residuals and their sums stay double in both.  Face and
 // vertex storage follows the cell order (first touch) so gathers stay local;
@@ -58,6 +60,11 @@ struct DevT {
   const int* sl_face;        // [MAXF][noc]
   const int* sl_nbr;         // [MAXF][noc]
   const int* cnf;            // nf | nb << 8
+  // shared-flux residual: the interior faces touched by each block of BLK
+  // cells (ascending storage order) and, per cell slot, the position of its
+  // face in that list (bit-inverted when the cell is the face's `out`)
+  const int *bf_ptr, *bf_face, *sl_loc;
+  int bf_max;
   const double *r11, *r12, *r22, *cv, *cov;  // noc (cov: ncell)
   const int* nc2c;
   const int* cref;           // storage cell -> reference cell number (ncell)

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -36,6 +37,12 @@ struct Params {
   int flow, iwall, flux, grad, ilimit, irs, timemode, precision;
 };
 
+// Host copies of the shared-flux residual's block face lists.
+struct Slots {
+  std::vector<int> bf_ptr, bf_face, sl_loc;
+  int bf_max = 0;
+};
+
 }  // namespace
 
 struct ufx_cc {
@@ -46,6 +53,8 @@ struct ufx_cc {
   DevT<float> d32{};
   int nop = 0, noc = 0, nfi = 0, nbf = 0, nof = 0, ncell = 0;
   double cfl = 0.0;              // CFL of the current iteration (param's CFL)
+  bool stage_shared = true;      // residual stage with block-shared fluxes
+  size_t stage_lds = 0;          // its LDS bytes per block
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
@@ -108,6 +117,13 @@ void prof_end(ufx_cc* x, int r) {
   do {                                                                   \
     int r_ = prof_begin((x), (cat));                                     \
     kern<<<(grid), BLK, 0, (x)->stream>>>(__VA_ARGS__);                  \
+    prof_end((x), r_);                                                   \
+  } while (0)
+// The same with `lds` bytes of dynamic shared memory per block.
+#define LAUNCH_LDS(x, cat, kern, grid, lds, ...)                         \
+  do {                                                                   \
+    int r_ = prof_begin((x), (cat));                                     \
+    kern<<<(grid), BLK, (lds), (x)->stream>>>(__VA_ARGS__);              \
     prof_end((x), r_);                                                   \
   } while (0)
 
@@ -304,7 +320,11 @@ int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int 
   do {                                                                                          \
     if (x->nbf > 0)                                                                             \
       LAUNCH(x, UFX_K_BFACE, (k_bface<R, F>), nbb, d, x->g, visc, x->p.iwall, x->p.t_wall);    \
-    LAUNCH(x, UFX_K_STAGE, (k_stage<R, F>), x->nblk_cell, d, gr, mode, ak, bk, write_res);      \
+    if (x->stage_shared)                                                                        \
+      LAUNCH_LDS(x, UFX_K_STAGE, (k_stage_blk<R, F>), x->nblk_cell, x->stage_lds, d, gr, mode,  \
+                 ak, bk, write_res);                                                            \
+    else                                                                                        \
+      LAUNCH(x, UFX_K_STAGE, (k_stage<R, F>), x->nblk_cell, d, gr, mode, ak, bk, write_res);    \
   } while (0)
   switch (x->p.flux) {
     case UFX_FLUX_ROE: KS(1); break;
@@ -602,7 +622,7 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
 template <typename R>
 int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_face,
               const std::vector<int>& sl_nbr, const std::vector<int>& cnf,
-              const std::vector<int>& lu_face, const std::vector<int>& lu_cnt) {
+              const std::vector<int>& lu_face, const std::vector<int>& lu_cnt, const Slots& sl) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW, PW = Layout<R>::PW;
   const int noc = m.noc, ncell = m.ncell(), nof = m.nof(), nop = m.nop, nbf = m.nbf;
   d.nop = nop; d.noc = noc; d.nfi = m.nfi; d.nbf = nbf; d.nof = nof; d.ncell = ncell;
@@ -677,6 +697,8 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   }
   UP(fbc, fbc); UP(bface, bface); UP(bcell, bcell);
   UP(sl_face, sl_face); UP(sl_nbr, sl_nbr); UP(cnf, cnf);
+  UP(bf_ptr, sl.bf_ptr); UP(bf_face, sl.bf_face); UP(sl_loc, sl.sl_loc);
+  d.bf_max = sl.bf_max;
   UP(r11, r11); UP(r12, r12); UP(r22, r22); UP(cv, cv); UP(cov, cov);
   UP(nc2c, nc2c); UP(cref, x->ref_of_sto);
   UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py);
@@ -787,6 +809,39 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     }
     cnf[s] = (int)cf[c].size() | (nb << 8);
   }
+  // shared-flux residual: per block of BLK storage cells, the interior faces
+  // it touches (ascending storage order) and each slot's position among them
+  Slots sl;
+  {
+    const int nblk = nblocks(noc);
+    sl.bf_ptr.assign(nblk + 1, 0);
+    sl.sl_loc.assign((size_t)MAXF * noc, 0);
+    sl.bf_face.reserve((size_t)noc * 2);
+    std::vector<int> loc(nof, 0), list;
+    for (int b = 0; b < nblk; ++b) {
+      int c0 = b * BLK, c1 = std::min(noc, c0 + BLK);
+      list.clear();
+      for (int s = c0; s < c1; ++s) {
+        int ni = (cnf[s] & 0xff) - (cnf[s] >> 8);
+        for (int j = 0; j < ni; ++j) list.push_back(sl_face[(size_t)j * noc + s]);
+      }
+      std::sort(list.begin(), list.end());
+      list.erase(std::unique(list.begin(), list.end()), list.end());
+      for (size_t i = 0; i < list.size(); ++i) {
+        loc[list[i]] = (int)i;
+        sl.bf_face.push_back(list[i]);
+      }
+      sl.bf_ptr[b + 1] = (int)sl.bf_face.size();
+      sl.bf_max = std::max(sl.bf_max, (int)list.size());
+      for (int s = c0; s < c1; ++s) {
+        int ni = (cnf[s] & 0xff) - (cnf[s] >> 8);
+        for (int j = 0; j < ni; ++j) {
+          size_t q = (size_t)j * noc + s;
+          sl.sl_loc[q] = sl_nbr[q] >= 0 ? loc[sl_face[q]] : ~loc[sl_face[q]];
+        }
+      }
+    }
+  }
   // LU-SGS: the reference's faces-of-cell order and the level schedule of
   // its sequential sweeps (a cell's level is one more than the highest level
   // among the neighbours it waits for), in the reference's cell numbering
@@ -878,9 +933,23 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   x->g = Gas<double>{p.gamma, p.gamma1, p.znd, p.r_inf, p.u_inf, p.v_inf, p.p_inf, p.mref};
   x->gf = Gas<float>{(float)p.gamma, (float)p.gamma1, (float)p.znd, (float)p.r_inf,
                      (float)p.u_inf, (float)p.v_inf, (float)p.p_inf, (float)p.mref};
+  // residual stage form: the shared-flux kernel for the mixed form (its FP32
+  // flux arithmetic is the larger cost there), the per-cell gather for FP64
+  // (measured faster: that form is bound by its gathers, not its flux
+  // arithmetic); UFX_STAGE_KERNEL=gather|shared overrides.  The shared form
+  // needs its block flux list to fit the LDS.
+  {
+    size_t w = p.precision == UFX_PREC_FP64 ? sizeof(double) : sizeof(float);
+    x->stage_lds = (size_t)sl.bf_max * 4 * w;
+    const char* env = std::getenv("UFX_STAGE_KERNEL");
+    bool shared = p.precision == UFX_PREC_MIXED;
+    if (env && std::strcmp(env, "gather") == 0) shared = false;
+    if (env && std::strcmp(env, "shared") == 0) shared = true;
+    x->stage_shared = shared && x->stage_lds <= 48 * 1024;
+  }
   int rc = p.precision == UFX_PREC_FP64
-               ? build_dev(x, x->d64, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt)
-               : build_dev(x, x->d32, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt);
+               ? build_dev(x, x->d64, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt, sl)
+               : build_dev(x, x->d32, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt, sl);
   if (!rc) rc = dupload(x, &x->lv_cells, lv_cells);
   if (!rc) rc = dalloc(x, &x->cirs_tmp, 8 * (size_t)noc);
   if (!rc) rc = dalloc(x, &x->bfluxv, 2 * (size_t)std::max(1, m.nbf));
