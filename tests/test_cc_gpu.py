@@ -241,28 +241,35 @@ def test_mixed_precision_tracks_fp64():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_shared_flux_stage_matches_gather(precision, monkeypatch):
-    """The block-shared residual stage (each interior flux once per block,
-    kept in LDS) against the per-cell gather stage (each flux once per
-    adjacent cell), on a mesh of many blocks, and the FP64 form against the
-    oracle."""
-    print(f"case: shared vs gather stage {precision} mesh 80x40 seed 13", flush=True)
+def test_stage_kernel_forms_agree(precision, monkeypatch):
+    """The four forms of the residual stage -- per-cell gather or block-shared
+    fluxes (each interior flux once per block, kept in LDS), each with or
+    without the block's records staged in LDS -- agree on a mesh of many
+    blocks, and the FP64 form follows the oracle."""
+    print(f"case: stage kernel forms {precision} mesh 80x40 seed 13", flush=True)
     from unsflow2d_amd import cc, meshgen
     from oracle.oracle import CCOracle
     raw = meshgen.bump_channel(80, 40, 13, h=0.08)
     mesh = cc.Mesh.from_raw(raw)
     cfg = cc.Config(mach=0.7, precision=precision)
-    monkeypatch.setenv("UFX_STAGE_KERNEL", "gather")
-    g = cc.Solver(mesh, cfg)
-    monkeypatch.setenv("UFX_STAGE_KERNEL", "shared")
-    s = cc.Solver(mesh, cfg)
+    runs = {}
+    for kern in ("gather", "shared"):
+        for lds in ("0", "1"):
+            monkeypatch.setenv("UFX_STAGE_KERNEL", kern)
+            monkeypatch.setenv("UFX_STAGE_LDS", lds)
+            s = cc.Solver(mesh, cfg)
+            h = s.iterate(12)
+            runs[(kern, lds)] = (h, s.get("qc"))
     monkeypatch.delenv("UFX_STAGE_KERNEL")
-    hg = g.iterate(12)
-    hs = s.iterate(12)
-    _check("history", hs, hg, 1e-13 if precision == "fp64" else 1e-6)
-    _check("qc", s.get("qc"), g.get("qc"), 1e-13 if precision == "fp64" else 1e-6)
+    monkeypatch.delenv("UFX_STAGE_LDS")
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    h0, q0 = runs[("gather", "0")]
+    for key, (h, q) in runs.items():
+        _check(f"history {key[0]} lds={key[1]}", h, h0, tol)
+        _check(f"qc {key[0]} lds={key[1]}", q, q0, tol)
     if precision == "fp64":
         orc = CCOracle(mesh.arrays(), mach=0.7)
         ref = np.array([orc.iteration(it, 0.8) for it in range(1, 13)])
-        _check("history vs oracle", hs, ref, 1e-11)
-        _check("qc vs oracle", s.get("qc"), orc.qc, 1e-11)
+        for key, (h, q) in runs.items():
+            _check(f"history vs oracle {key}", h, ref, 1e-11)
+            _check(f"qc vs oracle {key}", q, orc.qc, 1e-11)

@@ -65,6 +65,23 @@ __device__ __forceinline__ void st4d(double* p, const double* v) {
   q[1] = make_double2(v[2], v[3]);
 }
 
+// Streaming (non-temporal) forms for data a kernel touches exactly once: they
+// keep such data from displacing the neighbour records in L2.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld4d_nt(const double* p, double* o) {
+  const nt_d2* q = reinterpret_cast<const nt_d2*>(p);
+  nt_d2 a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1);
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4d_nt(double* p, const double* v) {
+  nt_d2* q = reinterpret_cast<nt_d2*>(p);
+  nt_d2 a = {v[0], v[1]}, b = {v[2], v[3]};
+  __builtin_nontemporal_store(a, q);
+  __builtin_nontemporal_store(b, q + 1);
+}
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+
 template <typename R>
 __device__ __forceinline__ void face_rec(const DevT<R>& d, int f, R* fr) {
   ldv<R, Layout<R>::FW>(d.frec + (size_t)f * Layout<R>::FW, fr);
@@ -530,6 +547,28 @@ __global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int vis
   st4d(d.bres + (size_t)k * 4, F);
 }
 
+// Copy the records of the block's cells [base, base + cnt) into LDS with
+// coalesced 16-byte loads (records are 16-byte multiples).
+template <typename R>
+__device__ __forceinline__ void stage_block_recs(const DevT<R>& d, R* srec, int base, int cnt) {
+  constexpr int RW = Layout<R>::RW;
+  static_assert(RW * sizeof(R) % 16 == 0, "records are 16-byte multiples");
+  const int4* src = reinterpret_cast<const int4*>(d.rec + (size_t)base * RW);
+  int4* dst = reinterpret_cast<int4*>(srec);
+  const int n16 = cnt * RW * (int)sizeof(R) / 16;
+  for (int i = threadIdx.x; i < n16; i += BLK) dst[i] = src[i];
+}
+
+// Record of cell n: from the block's LDS copy when n is one of its cells
+// (SL), else from HBM.
+template <typename R, bool SL>
+__device__ __forceinline__ void load_rec(const DevT<R>& d, const R* srec, int base, int n, R* out) {
+  constexpr int RW = Layout<R>::RW;
+  unsigned l = (unsigned)(n - base);
+  if (SL && l < (unsigned)BLK) ldv<R, RW>(srec + l * RW, out);
+  else ldv<R, RW>(d.rec + (size_t)n * RW, out);
+}
+
 // Residual of one cell, CCFVM/fvresidual.F90:16-171, as a gather over its
 // faces: the boundary-face fluxes from k_bface first, then the interior
 // faces with recon (:190-252) on both sides, each flux evaluated from the
@@ -538,22 +577,32 @@ __global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int vis
 //   mode 1: apply the stage update of CCFVM/main.F90:92-101 in place (no
 //           other cell reads this cell's qc during a stage);
 //   mode 2: mode 1 and a block partial of the squared residual.
-template <typename R, int FLUX>
+// SL: the block's records are first copied to LDS, so the neighbours inside
+// the block (most of them, the cells being Hilbert-ordered) are read from
+// there instead of through L2.
+template <typename R, int FLUX, bool SL>
 __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, double ak,
                                                double bk, int write_res) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
+  extern __shared__ double lds_raw[];
+  R* srec = reinterpret_cast<R*>(lds_raw);  // [BLK][RW] when SL
   __shared__ double sh[1][BLK];
-  int c = cell_index();
+  const int base = xcd_block(blockIdx.x, gridDim.x) * BLK;
+  const int c = base + threadIdx.x;
+  if (SL) {
+    stage_block_recs(d, srec, base, min(BLK, d.noc - base));
+    __syncthreads();
+  }
   double sq = 0.0;
   if (c < d.noc && !d.err[0]) {
     const int noc = d.noc;
-    int info = d.cnf[c], nf = info & 0xff, nb = info >> 8;
+    int info = ld_nt(d.cnf + c), nf = info & 0xff, nb = info >> 8;
     R own[RW];
-    ldv<R, RW>(d.rec + (size_t)c * RW, own);
+    load_rec<R, SL>(d, srec, base, c, own);
     double r[4] = {0.0, 0.0, 0.0, 0.0};
     bool ok = true;
     for (int s = nf - nb; s < nf; ++s) {
-      int k = d.sl_nbr[s * noc + c] - noc;
+      int k = ld_nt(d.sl_nbr + s * noc + c) - noc;
       double F[4];
       ld4d(d.bres + (size_t)k * 4, F);
       for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i];
@@ -563,12 +612,12 @@ __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, d
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
       if (s >= nf - nb) break;
-      int f = d.sl_face[s * noc + c];
-      int n = d.sl_nbr[s * noc + c];
+      int f = ld_nt(d.sl_face + s * noc + c);
+      int n = ld_nt(d.sl_nbr + s * noc + c);
       bool in_side = n >= 0;
       n = in_side ? n : ~n;
       R nr[RW], fr[FW];
-      ldv<R, RW>(d.rec + (size_t)n * RW, nr);
+      load_rec<R, SL>(d, srec, base, n, nr);
       face_rec(d, f, fr);
       R ql[4], qr[4], F[4];
       // side a = the face's `in` cell, side b = its `out` cell
@@ -602,14 +651,14 @@ __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, d
       }
     }
     if (!ok) set_err(d, DERR_ROE, d.cref[c] + 1);
-    if (mode == 0 || write_res) st4d(d.res + (size_t)c * 4, r);
+    if (mode == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
     if (mode >= 1) {
-      double dta = d.dta[c], q[4], o[4];
-      ld4d(d.qc + (size_t)c * 4, q);
-      ld4d(d.qold + (size_t)c * 4, o);
+      double dta = ld_nt(d.dta + c), q[4], o[4];
+      ld4d_nt(d.qc + (size_t)c * 4, q);
+      ld4d_nt(d.qold + (size_t)c * 4, o);
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dta * r[i]);
-      st4d(d.qc + (size_t)c * 4, q);
+      st4d_nt(d.qc + (size_t)c * 4, q);
     }
     if (mode == 2)
       for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];
@@ -649,40 +698,43 @@ __device__ __forceinline__ bool face_flux_ab(const Gas<R>& gr, const R* ra, cons
 // about 1.65 faces per cell instead of 3), evaluate each flux once and keep
 // it in LDS.  Phase 2: each cell sums the boundary fluxes and then its
 // interior fluxes from LDS in its slot order (the reference's accumulation
-// order), then applies the stage update as k_stage does.
-template <typename R, int FLUX>
+// order), then applies the stage update as k_stage does.  SL as in k_stage.
+// LDS: [BLK][RW] records when SL, then [bf_max][4] fluxes.
+template <typename R, int FLUX, bool SL>
 __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mode, double ak,
                                                    double bk, int write_res) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
-  R* fl = reinterpret_cast<R*>(lds_raw);  // [bf_max][4]
+  R* srec = reinterpret_cast<R*>(lds_raw);
+  R* fl = srec + (SL ? BLK * RW : 0);
   __shared__ double sh[1][BLK];
   __shared__ int s_err;
   if (threadIdx.x == 0) s_err = d.err[0];
+  const int b = xcd_block(blockIdx.x, gridDim.x), base = b * BLK;
+  if (SL) stage_block_recs(d, srec, base, min(BLK, d.noc - base));
   __syncthreads();
   const bool run = s_err == 0;  // uniform over the block
-  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int j0 = d.bf_ptr[b], nj = run ? d.bf_ptr[b + 1] - j0 : 0;
   for (int j = threadIdx.x; j < nj; j += BLK) {
     int f = d.bf_face[j0 + j];
     int ca = d.fin[f], cb = d.fout[f];
     R ra[RW], rb[RW], fr[FW], F[4];
-    ldv<R, RW>(d.rec + (size_t)ca * RW, ra);
-    ldv<R, RW>(d.rec + (size_t)cb * RW, rb);
+    load_rec<R, SL>(d, srec, base, ca, ra);
+    load_rec<R, SL>(d, srec, base, cb, rb);
     face_rec(d, f, fr);
     if (!face_flux_ab<R, FLUX>(gr, ra, rb, fr, F)) set_err(d, DERR_ROE, d.cref[ca] + 1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) fl[j * 4 + i] = F[i];
   }
   __syncthreads();
-  const int c = b * BLK + threadIdx.x;
+  const int c = base + threadIdx.x;
   double sq = 0.0;
   if (run && c < d.noc) {
     const int noc = d.noc;
-    int info = d.cnf[c], nf = info & 0xff, nb = info >> 8;
+    int info = ld_nt(d.cnf + c), nf = info & 0xff, nb = info >> 8;
     double r[4] = {0.0, 0.0, 0.0, 0.0};
     for (int s = nf - nb; s < nf; ++s) {
-      int k = d.sl_nbr[s * noc + c] - noc;
+      int k = ld_nt(d.sl_nbr + s * noc + c) - noc;
       double F[4];
       ld4d(d.bres + (size_t)k * 4, F);
       for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i];
@@ -690,7 +742,7 @@ __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mod
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
       if (s >= nf - nb) break;
-      int l = d.sl_loc[s * noc + c];
+      int l = ld_nt(d.sl_loc + s * noc + c);
       bool in_side = l >= 0;
       l = in_side ? l : ~l;
       if (in_side) {
@@ -701,14 +753,14 @@ __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mod
         for (int i = 0; i < 4; ++i) r[i] = r[i] - (double)fl[l * 4 + i];
       }
     }
-    if (mode == 0 || write_res) st4d(d.res + (size_t)c * 4, r);
+    if (mode == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
     if (mode >= 1) {
-      double dta = d.dta[c], q[4], o[4];
-      ld4d(d.qc + (size_t)c * 4, q);
-      ld4d(d.qold + (size_t)c * 4, o);
+      double dta = ld_nt(d.dta + c), q[4], o[4];
+      ld4d_nt(d.qc + (size_t)c * 4, q);
+      ld4d_nt(d.qold + (size_t)c * 4, o);
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dta * r[i]);
-      st4d(d.qc + (size_t)c * 4, q);
+      st4d_nt(d.qc + (size_t)c * 4, q);
     }
     if (mode == 2)
       for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];

@@ -54,7 +54,8 @@ struct ufx_cc {
   int nop = 0, noc = 0, nfi = 0, nbf = 0, nof = 0, ncell = 0;
   double cfl = 0.0;              // CFL of the current iteration (param's CFL)
   bool stage_shared = true;      // residual stage with block-shared fluxes
-  size_t stage_lds = 0;          // its LDS bytes per block
+  bool stage_local = true;       // block records staged in LDS
+  size_t stage_lds = 0;          // LDS bytes per block of the stage kernel
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
@@ -320,11 +321,18 @@ int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int 
   do {                                                                                          \
     if (x->nbf > 0)                                                                             \
       LAUNCH(x, UFX_K_BFACE, (k_bface<R, F>), nbb, d, x->g, visc, x->p.iwall, x->p.t_wall);    \
-    if (x->stage_shared)                                                                        \
-      LAUNCH_LDS(x, UFX_K_STAGE, (k_stage_blk<R, F>), x->nblk_cell, x->stage_lds, d, gr, mode,  \
-                 ak, bk, write_res);                                                            \
+    if (x->stage_shared && x->stage_local)                                                      \
+      LAUNCH_LDS(x, UFX_K_STAGE, (k_stage_blk<R, F, true>), x->nblk_cell, x->stage_lds, d, gr,  \
+                 mode, ak, bk, write_res);                                                      \
+    else if (x->stage_shared)                                                                   \
+      LAUNCH_LDS(x, UFX_K_STAGE, (k_stage_blk<R, F, false>), x->nblk_cell, x->stage_lds, d, gr, \
+                 mode, ak, bk, write_res);                                                      \
+    else if (x->stage_local)                                                                    \
+      LAUNCH_LDS(x, UFX_K_STAGE, (k_stage<R, F, true>), x->nblk_cell, x->stage_lds, d, gr,      \
+                 mode, ak, bk, write_res);                                                      \
     else                                                                                        \
-      LAUNCH(x, UFX_K_STAGE, (k_stage<R, F>), x->nblk_cell, d, gr, mode, ak, bk, write_res);    \
+      LAUNCH(x, UFX_K_STAGE, (k_stage<R, F, false>), x->nblk_cell, d, gr, mode, ak, bk,         \
+             write_res);                                                                        \
   } while (0)
   switch (x->p.flux) {
     case UFX_FLUX_ROE: KS(1); break;
@@ -934,18 +942,24 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   x->gf = Gas<float>{(float)p.gamma, (float)p.gamma1, (float)p.znd, (float)p.r_inf,
                      (float)p.u_inf, (float)p.v_inf, (float)p.p_inf, (float)p.mref};
   // residual stage form: the shared-flux kernel for the mixed form (its FP32
-  // flux arithmetic is the larger cost there), the per-cell gather for FP64
-  // (measured faster: that form is bound by its gathers, not its flux
-  // arithmetic); UFX_STAGE_KERNEL=gather|shared overrides.  The shared form
-  // needs its block flux list to fit the LDS.
+  // flux arithmetic is the larger cost there), the per-cell gather for FP64;
+  // both with the block's cell records staged in LDS.  UFX_STAGE_KERNEL=
+  // gather|shared and UFX_STAGE_LDS=0|1 override.  LDS per block: records
+  // BLK x RW, and for the shared form the block's flux list.
   {
     size_t w = p.precision == UFX_PREC_FP64 ? sizeof(double) : sizeof(float);
-    x->stage_lds = (size_t)sl.bf_max * 4 * w;
+    size_t rw = p.precision == UFX_PREC_FP64 ? Layout<double>::RW : Layout<float>::RW;
     const char* env = std::getenv("UFX_STAGE_KERNEL");
     bool shared = p.precision == UFX_PREC_MIXED;
     if (env && std::strcmp(env, "gather") == 0) shared = false;
     if (env && std::strcmp(env, "shared") == 0) shared = true;
-    x->stage_shared = shared && x->stage_lds <= 48 * 1024;
+    const char* envl = std::getenv("UFX_STAGE_LDS");
+    bool local = !(envl && std::strcmp(envl, "0") == 0);
+    size_t lds_fl = (size_t)sl.bf_max * 4 * w, lds_rec = (size_t)BLK * rw * w;
+    if (shared && lds_fl + (local ? lds_rec : 0) > 64 * 1024) shared = false;
+    x->stage_shared = shared;
+    x->stage_local = local;
+    x->stage_lds = (shared ? lds_fl : 0) + (local ? lds_rec : 0);
   }
   int rc = p.precision == UFX_PREC_FP64
                ? build_dev(x, x->d64, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt, sl)
