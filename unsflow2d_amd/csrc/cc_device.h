@@ -69,6 +69,7 @@ struct DevT {
   const int* nc2c;
   const int* cref;           // storage cell -> reference cell number (ncell)
   const int *v2c_ptr, *v2c;  // point storage order
+  const int* avg_w0;         // per block of BLK vertices: its lowest interior cell
   const double* wt;
   const double *px, *py;     // point storage order
   const int *fin, *fout;     // face sides (storage numbering of faces)

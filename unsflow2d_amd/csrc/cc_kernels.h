@@ -190,24 +190,50 @@ __global__ __launch_bounds__(BLK) void k_range_final(DevT<R> d, int nblk) {
 
 // ------------------------------------------------------- vertex averages
 // CCFVM/average.F90:11-29: pseudo-Laplacian weighted average over the cells
-// around a vertex (ghosts included), accumulated in double.
-template <typename R>
-__global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d, int with_mu) {
+// around a vertex (ghosts included), accumulated in double.  The cells
+// around a block of vertices (numbered by first touch in cell order) lie
+// mostly in a window of AVG_WIN cells starting at the block's lowest cell
+// (avg_w0); their primitives are staged in LDS, the rest read from HBM.
+constexpr int AVG_WIN = 768;
+template <typename R, bool VISC>
+__global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
   constexpr int RW = Layout<R>::RW, PW = Layout<R>::PW;
-  int p = cell_index();
+  __shared__ R s_q[AVG_WIN * 4];
+  __shared__ R s_t[AVG_WIN];
+  __shared__ R s_mu[VISC ? AVG_WIN : 1];
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  const int p = vb * BLK + threadIdx.x;
+  const int w0 = d.avg_w0[vb], wn = min(AVG_WIN, d.noc - w0);
+  for (int i = threadIdx.x; i < wn; i += BLK) {
+    R q[4];
+    ldv<R, 4>(d.rec + (size_t)(w0 + i) * RW + R_QP, q);
+    stv<R, 4>(s_q + i * 4, q);
+    s_t[i] = d.qT[w0 + i];
+    if (VISC) s_mu[i] = d.cmul[w0 + i];
+  }
+  __syncthreads();
   if (p >= d.nop || d.err[0]) return;
   double w = 0.0, con[6] = {0, 0, 0, 0, 0, 0};
   int j0 = d.v2c_ptr[p], j1 = d.v2c_ptr[p + 1];
   for (int j = j0; j < j1; ++j) {
-    int c = d.v2c[j];
-    double cw = d.wt[j];
-    R q[4];
-    ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q);
+    int c = ld_nt(d.v2c + j);
+    double cw = ld_nt(d.wt + j);
+    R q[4], t, mu = R(0);
+    unsigned l = (unsigned)(c - w0);
+    if (l < (unsigned)wn) {
+      ldv<R, 4>(s_q + l * 4, q);
+      t = s_t[l];
+      if (VISC) mu = s_mu[l];
+    } else {
+      ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q);
+      t = d.qT[c];
+      if (VISC) mu = d.cmul[c];
+    }
     w = w + cw;
 #pragma unroll
     for (int k = 0; k < 4; ++k) con[k] = con[k] + (double)q[k] * cw;
-    con[4] = con[4] + (double)d.qT[c] * cw;
-    if (with_mu) con[5] = con[5] + (double)d.cmul[c] * cw;
+    con[4] = con[4] + (double)t * cw;
+    if (VISC) con[5] = con[5] + (double)mu * cw;
   }
   R o[PW];
 #pragma unroll
@@ -238,14 +264,33 @@ enum { G_NONE = 0, G_LSQR = 1, G_GG = 2, G_GGFC = 3 };
 template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
 __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
-  int c = cell_index();
+  // the primitives, centroids (and temperatures) of the block's cells are
+  // staged in LDS: most neighbours lie in the same block
+  constexpr bool SL = GRAD == G_LSQR || LIMIT;
+  __shared__ R s_qp[SL ? BLK * 4 : 1];
+  __shared__ R s_cen[SL ? BLK * 2 : 1];
+  __shared__ R s_t[SL && WITH_T ? BLK : 1];
+  const int base = xcd_block(blockIdx.x, gridDim.x) * BLK;
+  const int c = base + threadIdx.x;
+  const int cnt = min(BLK, d.noc - base);
+  R own[RW];
+  R tc = R(0);
+  if (c < d.noc) {
+    ldv<R, RW>(d.rec + (size_t)c * RW, own);
+    tc = d.qT[c];
+  }
+  if (SL) {
+    if (c < d.noc) {
+      stv<R, 4>(s_qp + threadIdx.x * 4, own + R_QP);
+      stv<R, 2>(s_cen + threadIdx.x * 2, own + R_CEN);
+      if (WITH_T) s_t[threadIdx.x] = tc;
+    }
+    __syncthreads();
+  }
   if (c >= d.noc || d.err[0]) return;
   const int noc = d.noc;
-  int info = d.cnf[c], nf = info & 0xff, nb = info >> 8;
-  R own[RW];
-  ldv<R, RW>(d.rec + (size_t)c * RW, own);
+  int info = ld_nt(d.cnf + c), nf = info & 0xff, nb = info >> 8;
   R cx = own[R_CEN], cy = own[R_CEN + 1];
-  R tc = d.qT[c];
   R gr[10];
 #pragma unroll
   for (int k = 0; k < 10; ++k) gr[k] = R(0);
@@ -259,8 +304,8 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
 #pragma unroll
   for (int s = 0; s < MAXF; ++s) {
     if (s >= nf) break;
-    int f = d.sl_face[s * noc + c];
-    int n = d.sl_nbr[s * noc + c];
+    int f = ld_nt(d.sl_face + s * noc + c);
+    int n = ld_nt(d.sl_nbr + s * noc + c);
     bool in_side = n >= 0;
     n = in_side ? n : ~n;
     bool bnd = n >= noc;
@@ -277,15 +322,21 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
     // ascending face order, CCFVM/timestep.F90:171-173
     if (TSTEP) la = la + d.fla[f];
     R nq[4];
-    if (GRAD == G_LSQR || LIMIT) ldv<R, 4>(d.rec + (size_t)n * RW + R_QP, nq);
+    const unsigned l = (unsigned)(n - base);
+    const bool loc = SL && l < (unsigned)cnt;
+    if (GRAD == G_LSQR || LIMIT) {
+      if (loc) ldv<R, 4>(s_qp + l * 4, nq);
+      else ldv<R, 4>(d.rec + (size_t)n * RW + R_QP, nq);
+    }
     if (GRAD == G_LSQR) {
       R ncen[2];
-      ldv<R, 2>(d.rec + (size_t)n * RW + R_CEN, ncen);
+      if (loc) ldv<R, 2>(s_cen + l * 2, ncen);
+      else ldv<R, 2>(d.rec + (size_t)n * RW + R_CEN, ncen);
       R dx = ncen[0] - cx, dy = ncen[1] - cy;
       R alfa1 = dx / r11 / r11;
       R alfa2 = bnd ? (dy - dx * r12 / r11) / r22 / r22 : (dy - dx * r12 / r11) / (r22 * r22);
       R wx = alfa1 - alfa2 * r12 / r11, wy = alfa2;
-      R nt = WITH_T ? d.qT[n] : R(0);
+      R nt = WITH_T ? (loc ? s_t[l] : d.qT[n]) : R(0);
       if (in_side) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {

@@ -265,7 +265,8 @@ int run_range(ufx_cc* x, DevT<R>& d) {
 
 template <typename R>
 int run_avg(ufx_cc* x, DevT<R>& d) {
-  LAUNCH(x, UFX_K_AVG, k_avg_c2v<R>, x->nblk_pt, d, x->p.flow != UFX_FLOW_INVISCID);
+  if (x->p.flow != UFX_FLOW_INVISCID) LAUNCH(x, UFX_K_AVG, (k_avg_c2v<R, true>), x->nblk_pt, d);
+  else LAUNCH(x, UFX_K_AVG, (k_avg_c2v<R, false>), x->nblk_pt, d);
   return launch_check();
 }
 
@@ -681,6 +682,12 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     px[s] = m.px[i];
     py[s] = m.py[i];
   }
+  // window start of the staged vertex averages: per block of BLK vertices,
+  // the lowest interior cell around any of them (noc if none)
+  std::vector<int> avg_w0(nblocks(nop), noc);
+  for (int s = 0; s < nop; ++s)
+    for (int j = v2c_ptr[s]; j < v2c_ptr[s + 1]; ++j)
+      if (v2c[j] < noc) avg_w0[s / BLK] = std::min(avg_w0[s / BLK], v2c[j]);
   // cell records: primitives 0 (ghost primitives start at 0, as the
   // reference leaves them), gradients 0, limiter 1, centroid
   std::vector<R> rec((size_t)ncell * RW, R(0));
@@ -709,7 +716,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   d.bf_max = sl.bf_max;
   UP(r11, r11); UP(r12, r12); UP(r22, r22); UP(cv, cv); UP(cov, cov);
   UP(nc2c, nc2c); UP(cref, x->ref_of_sto);
-  UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py);
+  UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py); UP(avg_w0, avg_w0);
   UP(fin, fin); UP(fout, fout); UP(lu_face, lu_face); UP(lu_cnt, lu_cnt);
   UP(rec, rec); UP(qc, qc);
   AL(fw, nof); AL(qT, ncell); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc);
@@ -941,9 +948,10 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   x->g = Gas<double>{p.gamma, p.gamma1, p.znd, p.r_inf, p.u_inf, p.v_inf, p.p_inf, p.mref};
   x->gf = Gas<float>{(float)p.gamma, (float)p.gamma1, (float)p.znd, (float)p.r_inf,
                      (float)p.u_inf, (float)p.v_inf, (float)p.p_inf, (float)p.mref};
-  // residual stage form: the shared-flux kernel for the mixed form (its FP32
-  // flux arithmetic is the larger cost there), the per-cell gather for FP64;
-  // both with the block's cell records staged in LDS.  UFX_STAGE_KERNEL=
+  // residual stage form (measured on MI355X at 2M cells): FP64, the
+  // per-cell gather with the block's records staged in LDS; mixed, the
+  // shared-flux kernel reading records through L2 (its 80-byte records cache
+  // well, and staging would cost it occupancy).  UFX_STAGE_KERNEL=
   // gather|shared and UFX_STAGE_LDS=0|1 override.  LDS per block: records
   // BLK x RW, and for the shared form the block's flux list.
   {
@@ -954,7 +962,9 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     if (env && std::strcmp(env, "gather") == 0) shared = false;
     if (env && std::strcmp(env, "shared") == 0) shared = true;
     const char* envl = std::getenv("UFX_STAGE_LDS");
-    bool local = !(envl && std::strcmp(envl, "0") == 0);
+    bool local = p.precision == UFX_PREC_FP64;
+    if (envl && std::strcmp(envl, "0") == 0) local = false;
+    if (envl && std::strcmp(envl, "1") == 0) local = true;
     size_t lds_fl = (size_t)sl.bf_max * 4 * w, lds_rec = (size_t)BLK * rw * w;
     if (shared && lds_fl + (local ? lds_rec : 0) > 64 * 1024) shared = false;
     x->stage_shared = shared;
