@@ -39,6 +39,7 @@ extern "C" {
 #define UFX_ERR_UNSUPPORTED 5  /* option outside what this build implements */
 
 #define UFX_KIND_CC 1
+#define UFX_KIND_NC 2
 
 /* flux_type, CCFVM/read_input.F90:93-98 */
 #define UFX_FLUX_ROE 1
@@ -103,6 +104,22 @@ int ufx_cc_mesh_import(int nop, int noc, int nfi, int nbf,
                        const int* cell_nc2c, const int* c2f_ptr, const int* c2f,
                        const int* v2c_ptr, const int* v2c, const double* v2c_wt,
                        ufx_mesh** mesh);
+
+/* Preprocess a vertex-centred geometry.inp mesh (NCFVM/read_input.f90:101-112:
+ * nop vertices px, py with tag pbc, noc cells with centroid and area, nof
+ * edges p1 p2 in out bc, 1-based, 0 for a missing cell) into the reference's
+ * post-`geometric` state (NCFVM/geometric.f90:1-46): breadth-first vertex
+ * numbering, dual-face normals and lengths, median-dual volumes, vertex
+ * neighbour and edge lists, least-squares weights.  Fails like the
+ * reference's STOPs (disconnected mesh, non-positive face co-volume).
+ * Arrays read back with ufx_mesh_get: "pt_x","pt_y","pt_cv","pt_bc",
+ * "pt_nv2v","pt_v2v","pt_wx","pt_wy","pt_nv2f","pt_v2f","oldnum","fc_p1",
+ * "fc_p2","fc_in","fc_out","fc_bc","fc_nx","fc_ny","fc_area","fc_cenx",
+ * "fc_ceny","cell_cenx","cell_ceny","cell_cv","counts" (nop,noc,nof). */
+int ufx_nc_geometric(int nop, int noc, int nof, const double* px, const double* py,
+                     const int* pbc, const double* ccx, const double* ccy, const double* cv,
+                     const int* e_p1, const int* e_p2, const int* e_in, const int* e_out,
+                     const int* e_bc, ufx_mesh** mesh);
 
 /* Element count of a named mesh array (see ufx_mesh_get). */
 int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count);

@@ -60,7 +60,7 @@ def build_lib() -> str:
         if tg != {ARCH}:
             raise RuntimeError(f"{src}: expected code for {ARCH} only, found {sorted(tg)}")
         objs.append(obj)
-    for src in ("cc_mesh.cpp", "cc_mesh_api.cpp"):
+    for src in ("cc_mesh.cpp", "nc_mesh.cpp", "cc_mesh_api.cpp"):
         obj = os.path.join(bdir, src + ".o")
         _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
               "-Wall", *inc, "-c", os.path.join(CSRC, src), "-o", obj])

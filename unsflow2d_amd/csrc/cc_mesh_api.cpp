@@ -46,6 +46,30 @@ int ufx_cc_geometric(int nop, int noc, int nfi, int nbf, const double* px, const
   return UFX_OK;
 }
 
+int ufx_nc_geometric(int nop, int noc, int nof, const double* px, const double* py,
+                     const int* pbc, const double* ccx, const double* ccy, const double* cv,
+                     const int* e_p1, const int* e_p2, const int* e_in, const int* e_out,
+                     const int* e_bc, ufx_mesh** mesh) {
+  if (!mesh || !px || !py || !ccx || !ccy || !cv || !e_p1 || !e_p2 || !e_in || !e_out || !e_bc) {
+    set_error("ufx_nc_geometric: null argument");
+    return UFX_ERR_ARG;
+  }
+  NCRawInput in;
+  in.nop = nop; in.noc = noc; in.nof = nof;
+  in.px = px; in.py = py; in.pbc = pbc; in.ccx = ccx; in.ccy = ccy; in.cv = cv;
+  in.e_p1 = e_p1; in.e_p2 = e_p2; in.e_in = e_in; in.e_out = e_out; in.e_bc = e_bc;
+  ufx_mesh* m = new ufx_mesh();
+  std::string err;
+  int rc = nc_geometric(in, m->m, err);
+  if (rc) {
+    delete m;
+    set_error(err);
+    return rc;
+  }
+  *mesh = m;
+  return UFX_OK;
+}
+
 int ufx_cc_mesh_import(int nop, int noc, int nfi, int nbf, const double* px, const double* py,
                        const int* fc_p1, const int* fc_p2, const int* fc_in, const int* fc_out,
                        const int* fc_bc, const double* fc_nx, const double* fc_ny,
@@ -121,6 +145,17 @@ int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count) {
   if (n == "pt_x" || n == "pt_y" || n == "pt_nv2c") v = m.nop;
   if (n == "pt_v2c" || n == "pt_wt") v = (int64_t)m.v2c.size();
   if (n == "counts") v = 4;
+  if (m.kind == UFX_KIND_NC) {
+    v = -1;
+    if (n.rfind("fc_", 0) == 0) v = m.nof();
+    if (n == "pt_x" || n == "pt_y" || n == "pt_cv" || n == "pt_bc" || n == "pt_nv2v" ||
+        n == "pt_nv2f" || n == "oldnum")
+      v = m.nop;
+    if (n == "pt_v2v" || n == "pt_wx" || n == "pt_wy") v = (int64_t)m.v2v.size();
+    if (n == "pt_v2f") v = (int64_t)m.v2f.size();
+    if (n == "cell_cenx" || n == "cell_ceny" || n == "cell_cv") v = m.noc;
+    if (n == "counts") v = 3;
+  }
   if (v < 0) { set_error("ufx_mesh_size: unknown array " + n); return UFX_ERR_ARG; }
   *count = v;
   return UFX_OK;
@@ -135,6 +170,35 @@ int ufx_mesh_get(const ufx_mesh* mesh, const char* name, void* dst) {
   auto put1 = [&](const std::vector<int>& v) { for (size_t i = 0; i < v.size(); ++i) di[i] = v[i] + 1; };
   auto put0 = [&](const std::vector<int>& v) { for (size_t i = 0; i < v.size(); ++i) di[i] = v[i]; };
   auto putd = [&](const std::vector<double>& v) { if (!v.empty()) std::memcpy(dd, v.data(), v.size() * 8); };
+  if (m.kind == UFX_KIND_NC) {
+    if (n == "fc_p1") put1(m.fp1);
+    else if (n == "fc_p2") put1(m.fp2);
+    else if (n == "fc_in") put1(m.fin);
+    else if (n == "fc_out") put1(m.fout);
+    else if (n == "fc_bc") put0(m.fbc);
+    else if (n == "fc_nx") putd(m.fnx);
+    else if (n == "fc_ny") putd(m.fny);
+    else if (n == "fc_area") putd(m.farea);
+    else if (n == "fc_cenx") putd(m.fcx);
+    else if (n == "fc_ceny") putd(m.fcy);
+    else if (n == "pt_x") putd(m.px);
+    else if (n == "pt_y") putd(m.py);
+    else if (n == "pt_cv") putd(m.pcv);
+    else if (n == "pt_bc") put0(m.pbc);
+    else if (n == "pt_nv2v") { for (int i = 0; i < m.nop; ++i) di[i] = m.v2v_ptr[i + 1] - m.v2v_ptr[i]; }
+    else if (n == "pt_v2v") put1(m.v2v);
+    else if (n == "pt_wx") putd(m.wx);
+    else if (n == "pt_wy") putd(m.wy);
+    else if (n == "pt_nv2f") { for (int i = 0; i < m.nop; ++i) di[i] = m.v2f_ptr[i + 1] - m.v2f_ptr[i]; }
+    else if (n == "pt_v2f") put1(m.v2f);
+    else if (n == "oldnum") put1(m.pt_oldnum);
+    else if (n == "cell_cenx") putd(m.ccx);
+    else if (n == "cell_ceny") putd(m.ccy);
+    else if (n == "cell_cv") putd(m.ccv);
+    else if (n == "counts") { di[0] = m.nop; di[1] = m.noc; di[2] = m.nof(); }
+    else { set_error("ufx_mesh_get: unknown array " + n); return UFX_ERR_ARG; }
+    return UFX_OK;
+  }
   if (n == "fc_p1") put1(m.fp1);
   else if (n == "fc_p2") put1(m.fp2);
   else if (n == "fc_in") put1(m.fin);

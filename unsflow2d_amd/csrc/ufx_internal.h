@@ -37,11 +37,33 @@ struct Mesh {
   std::vector<int> v2c_ptr, v2c;        // vertex -> cells (ghosts included)
   std::vector<double> wt;               // pseudo-Laplacian weights per v2c entry
   std::vector<int> oldnum;              // new cell -> input cell (0-based)
+  // vertex-centred meshes (kind UFX_KIND_NC): every edge is a face (nfi =
+  // nof, nbf = 0), fin/fout are the cells left/right of p1->p2 (-1 = none),
+  // vertices are in the reference's breadth-first order
+  std::vector<int> pbc;                 // vertex tag
+  std::vector<double> pcv;              // median-dual volume
+  std::vector<int> v2v_ptr, v2v;        // vertex neighbours (edge order)
+  std::vector<double> wx, wy;           // least-squares weights per v2v entry
+  std::vector<int> v2f_ptr, v2f;        // edges of a vertex (edge order)
+  std::vector<int> pt_oldnum;           // new vertex -> input vertex (0-based)
   int nof() const { return nfi + nbf; }
   int ncell() const { return noc + nbf; }
 };
 
 int cc_geometric(const RawInput& in, Mesh& m, std::string& err);
+
+// Raw vertex-centred mesh as NCFVM/read_input.f90:101-112 reads it (1-based,
+// 0 = no cell on that side of an edge).
+struct NCRawInput {
+  int nop = 0, noc = 0, nof = 0;
+  const double *px = nullptr, *py = nullptr;
+  const int* pbc = nullptr;
+  const double *ccx = nullptr, *ccy = nullptr, *cv = nullptr;
+  const int *e_p1 = nullptr, *e_p2 = nullptr, *e_in = nullptr, *e_out = nullptr,
+            *e_bc = nullptr;
+};
+
+int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err);
 
 void set_error(const std::string& msg);
 
