@@ -1,0 +1,55 @@
+/*
+ * CPU oracle of the vertex-centred (median-dual) hot path -- TEST
+ * INFRASTRUCTURE ONLY.
+ *
+ * A plain C restatement of the NCFVM routines in the reference's own loop
+ * structure (edge loops scattering into both end points, in edge order),
+ * compiled without FMA contraction.  Only the tests, the smoke check and the
+ * benchmark's CPU leg use it, as the checker.  Each function cites the
+ * reference routine it restates.
+ *
+ * Per-vertex records keep the variables of one vertex contiguous:
+ * qc[p*4+k] = [E, rho, rho u, rho v], qp[p*4+k] = [rho, u, v, p],
+ * grad[p*8 + 2*k + d] (d = 0: x, 1: y), phi/dumin/dumax[p*4+k].
+ * Indices are 0-based; vertices are in the reference's breadth-first order.
+ */
+#ifndef UFX_NC_ORACLE_H
+#define UFX_NC_ORACLE_H
+
+typedef struct {
+  /* mesh (post-geometric) */
+  int nop, nof;
+  const double *px, *py, *pcv;
+  const int *fp1, *fp2, *fbc;
+  const double *fnx, *fny, *farea;
+  const int *v2v_ptr, *v2v;
+  const double *wx, *wy;
+  const int *v2f_ptr, *v2f;
+  /* parameters */
+  double mach, aoa_deg, cfl_max;
+  int grad_kind;               /* 1: Gradient_GG, 2: Gradient_LSQR */
+  double cfl;                  /* CFL of the current iteration */
+  /* state (caller-owned) */
+  double *qc, *qp, *qold, *res;  /* 4*nop */
+  double *grad;                  /* 8*nop */
+  double *phi, *dumin, *dumax;   /* 4*nop */
+  double *la, *dt;               /* nop */
+  double *fla;                   /* nof */
+  /* outputs */
+  double dtglobal, fres, fres1, fresi;
+  int iresi;
+} orc_nc;
+
+void orc_nc_initialize(orc_nc* s);
+void orc_nc_save_old(orc_nc* s);
+void orc_nc_gradient(orc_nc* s);
+void orc_nc_limiter(orc_nc* s);
+void orc_nc_fvresidual(orc_nc* s);
+void orc_nc_rk_update(orc_nc* s, int irk);
+void orc_nc_lusgs(orc_nc* s);
+void orc_nc_residue(orc_nc* s, int iter);
+double orc_nc_cfl(double cfl_max, int iter);
+/* One iteration of NCFVM/main.f90:51-88 (timemode 1: rk3, 2: lusgs). */
+void orc_nc_iteration(orc_nc* s, int iter, int timemode);
+
+#endif

@@ -178,3 +178,92 @@ def farfield_state(mach: float, aoa_deg: float, ql, nx: float, ny: float):
     out = np.zeros(4)
     _lib.orc_farfield_state(mach, aoa_deg, a.ctypes.data_as(_dp), nx, ny, out.ctypes.data_as(_dp))
     return out
+
+
+class _NCState(C.Structure):
+    _fields_ = (
+        [(n, C.c_int) for n in ("nop", "nof")]
+        + [(n, _dp) for n in ("px", "py", "pcv")]
+        + [(n, _ip) for n in ("fp1", "fp2", "fbc")]
+        + [(n, _dp) for n in ("fnx", "fny", "farea")]
+        + [("v2v_ptr", _ip), ("v2v", _ip), ("wx", _dp), ("wy", _dp), ("v2f_ptr", _ip),
+           ("v2f", _ip)]
+        + [(n, C.c_double) for n in ("mach", "aoa_deg", "cfl_max")]
+        + [("grad_kind", C.c_int), ("cfl", C.c_double)]
+        + [(n, _dp) for n in ("qc", "qp", "qold", "res", "grad", "phi", "dumin", "dumax", "la",
+                              "dt", "fla")]
+        + [(n, C.c_double) for n in ("dtglobal", "fres", "fres1", "fresi")]
+        + [("iresi", C.c_int)]
+    )
+
+
+for _n in ("initialize", "save_old", "gradient", "limiter", "fvresidual", "lusgs"):
+    getattr(_lib, f"orc_nc_{_n}").argtypes = [C.POINTER(_NCState)]
+_lib.orc_nc_rk_update.argtypes = [C.POINTER(_NCState), C.c_int]
+_lib.orc_nc_residue.argtypes = [C.POINTER(_NCState), C.c_int]
+_lib.orc_nc_iteration.argtypes = [C.POINTER(_NCState), C.c_int, C.c_int]
+_lib.orc_nc_cfl.argtypes = [C.c_double, C.c_int]
+_lib.orc_nc_cfl.restype = C.c_double
+
+
+class NCOracle:
+    """Reference-order CPU restatement of the vertex-centred hot path.
+
+    ``mesh`` is the dict of post-``geometric`` arrays (1-based indices) of a
+    vertex-centred mesh: pt_x, pt_y, pt_cv, fc_p1, fc_p2, fc_bc, fc_nx,
+    fc_ny, fc_area, pt_nv2v, pt_v2v, pt_wx, pt_wy, pt_nv2f, pt_v2f.
+    """
+
+    def __init__(self, mesh: dict, *, mach=0.4, aoa_deg=0.0, cfl_max=50.0, grad="gg"):
+        i0 = lambda n: np.ascontiguousarray(mesh[n], dtype=np.int32) - 1
+        i1 = lambda n: np.ascontiguousarray(mesh[n], dtype=np.int32)
+        f = lambda n: np.ascontiguousarray(mesh[n], dtype=np.float64)
+        self.nop = len(mesh["pt_x"])
+        self.nof = len(mesh["fc_p1"])
+        ptr = lambda n: np.r_[0, np.cumsum(mesh[n])].astype(np.int32)
+        self._keep = dict(px=f("pt_x"), py=f("pt_y"), pcv=f("pt_cv"), fp1=i0("fc_p1"),
+                          fp2=i0("fc_p2"), fbc=i1("fc_bc"), fnx=f("fc_nx"), fny=f("fc_ny"),
+                          farea=f("fc_area"), v2v_ptr=ptr("pt_nv2v"), v2v=i0("pt_v2v"),
+                          wx=f("pt_wx"), wy=f("pt_wy"), v2f_ptr=ptr("pt_nv2f"), v2f=i0("pt_v2f"))
+        z = lambda *s: np.zeros(s, dtype=np.float64)
+        n = self.nop
+        self.qc, self.qp, self.qold, self.res = z(n, 4), z(n, 4), z(n, 4), z(n, 4)
+        self.grad = z(n, 8)
+        self.phi, self.dumin, self.dumax = z(n, 4), z(n, 4), z(n, 4)
+        self.la, self.dt = z(n), z(n)
+        self.fla = z(self.nof)
+        s = _NCState()
+        s.nop, s.nof = self.nop, self.nof
+        for k, v in self._keep.items():
+            setattr(s, k, v.ctypes.data_as(_ip if v.dtype == np.int32 else _dp))
+        for k in ("qc", "qp", "qold", "res", "grad", "phi", "dumin", "dumax", "la", "dt", "fla"):
+            setattr(s, k, getattr(self, k).ctypes.data_as(_dp))
+        s.mach, s.aoa_deg, s.cfl_max = mach, aoa_deg, cfl_max
+        s.grad_kind = {"gg": 1, "lsqr": 2}[grad]
+        self.s = s
+        _lib.orc_nc_initialize(C.byref(s))
+
+    def cfl(self, it):
+        return _lib.orc_nc_cfl(self.s.cfl_max, it)
+
+    def save_old(self): _lib.orc_nc_save_old(C.byref(self.s))
+    def gradient(self): _lib.orc_nc_gradient(C.byref(self.s))
+    def limiter(self): _lib.orc_nc_limiter(C.byref(self.s))
+
+    def fvresidual(self, cfl):
+        self.s.cfl = cfl
+        _lib.orc_nc_fvresidual(C.byref(self.s))
+
+    def rk_update(self, k): _lib.orc_nc_rk_update(C.byref(self.s), k)
+
+    def lusgs(self, cfl):
+        self.s.cfl = cfl
+        _lib.orc_nc_lusgs(C.byref(self.s))
+
+    def residue(self, it):
+        _lib.orc_nc_residue(C.byref(self.s), it)
+        return self.s.fres
+
+    def iteration(self, it, timemode="lusgs"):
+        _lib.orc_nc_iteration(C.byref(self.s), it, 1 if timemode == "rk3" else 2)
+        return self.s.fres
