@@ -1,13 +1,20 @@
 #!/usr/bin/env python3
-"""Write the golden fixtures of the cell-centred hot path from the reference.
+"""Write the golden fixtures of the hot paths from the reference.
 
-Runs the reference solver built from its own sources (oracle/_ref/cc, no
-floating-point contraction; `make -C oracle ref`) through the harness driver
-oracle/ref_cc_driver.F90 on seeded meshes from unsflow2d_amd.meshgen, and
-stores what it dumps -- the preprocessed mesh, every routine's output for the
-first iterations, the residual history and the final state -- in one .npz
-per case under tests/golden/.  Needs /root/reference (this container only);
-the fixtures themselves are data and travel with the repository.
+Runs the reference solvers built from their own sources (oracle/_ref/cc and
+oracle/_ref/nc, no floating-point contraction; `make -C oracle ref`) through
+the harness drivers oracle/ref_cc_driver.F90 and oracle/ref_nc_driver.F90 on
+seeded meshes from unsflow2d_amd.meshgen / unsflow2d_amd.nc_mesh, and stores
+what they dump -- the preprocessed mesh, every routine's output for the first
+iterations, the residual history and the final state -- in one .npz per case
+under tests/golden/.  Needs /root/reference (this container only); the
+fixtures themselves are data and travel with the repository.
+
+The vertex-centred reference reads freshly allocated memory before writing
+it (NCFVM/geometric.f90:101-104, :906-951); its driver runs with
+MALLOC_MMAP_THRESHOLD_=0 so that memory is zero, the meaning the code
+intends.  Its preprocessing is checked against the library's restatement
+(bit for bit) in tests/test_golden_nc.py.
 
     python tests/golden/make_golden.py [case ...]
 """
@@ -26,9 +33,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from unsflow2d_amd import meshgen  # noqa: E402
+from unsflow2d_amd import meshgen, nc_mesh  # noqa: E402
 
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "cc", "ref_cc_driver")
+NC_DRIVER = os.path.join(ROOT, "oracle", "_ref", "nc", "ref_nc_driver")
 
 # name -> (mesh kwargs, deck kwargs, niter, ndump, phi0, keep_mesh, keep_final)
 TINY = dict(nx=12, ny=6, seed=7, h=0.08)
@@ -58,6 +66,28 @@ CASES = {
     "cc_c1_second_order": (dict(nx=100, ny=100, seed=1, h=0.05), dict(mach=0.5), 1000, 0, -1.0,
                            False, ("final_qc",)),
 }
+
+# vertex-centred cases: name -> (o_grid kwargs, deck kwargs, gradient, niter,
+# ndump, keep_mesh)
+NC_TINY = dict(ni=24, nj=10, seed=3, stretch=1.2)
+NC_CASES = {
+    "nc_tiny_lusgs_gg": (NC_TINY, dict(mach=0.4, cfl_max=50.0, timemode="lusgs"), "gg", 100, 2,
+                         True),
+    "nc_tiny_lusgs_lsqr": (NC_TINY, dict(mach=0.4, cfl_max=50.0, timemode="lusgs"), "lsqr", 60,
+                           2, False),
+    "nc_tiny_rk3_gg": (NC_TINY, dict(mach=0.4, cfl_max=0.8, timemode="rk3"), "gg", 60, 2, False),
+    "nc_tiny_rk3_lsqr": (dict(ni=20, nj=8, seed=5, stretch=1.25),
+                         dict(mach=0.5, aoa_deg=2.0, cfl_max=0.8, timemode="rk3"), "lsqr", 40, 2,
+                         True),
+    # the vertex-centred workload's generator at 200 x 100 points (20 000
+    # vertices), LU-SGS to CFL 50, 500 iterations
+    "nc_c3_small": (dict(ni=200, nj=100, seed=3, stretch=1.005 ** (999 / 99)),
+                    dict(mach=0.4, cfl_max=50.0, timemode="lusgs"), "gg", 500, 0, False),
+}
+
+NC_MESH_I = ["fc_p1", "fc_p2", "fc_in", "fc_out", "fc_bc", "pt_nv2v", "pt_v2v", "pt_nv2f",
+             "pt_v2f", "counts"]
+NC_MESH_F = ["fc_nx", "fc_ny", "fc_area", "pt_x", "pt_y", "pt_cv", "pt_wxy"]
 
 MESH_I = ["fc_p1", "fc_p2", "fc_in", "fc_out", "fc_bc", "cell_nc2c", "cell_nc2f", "cell_c2f",
           "pt_nv2c", "pt_v2c", "counts"]
@@ -108,11 +138,52 @@ def make(name: str) -> str:
     return path
 
 
+def nc_raw_sha(m: nc_mesh.NCRawMesh) -> str:
+    h = hashlib.sha256()
+    for a in (m.x, m.y, m.pbc, m.cen, m.area, m.edge, m.ebc):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def make_nc(name: str) -> str:
+    mkw, dkw, grad, niter, ndump, keep_mesh = NC_CASES[name]
+    mkw = dict(mkw)
+    m = nc_mesh.o_grid(mkw.pop("ni"), mkw.pop("nj"), mkw.pop("seed"), **mkw)
+    deck = nc_mesh.NCDeck(**dkw)
+    out = {}
+    env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+    with tempfile.TemporaryDirectory() as tmp:
+        nc_mesh.write_geometry_inp(m, os.path.join(tmp, "geometry.inp"))
+        with open(os.path.join(tmp, "flo.inp"), "w") as f:
+            f.write(deck.text())
+        os.makedirs(os.path.join(tmp, "d"))
+        r = subprocess.run([NC_DRIVER, "flo.inp", "d", str(niter), str(ndump), grad], cwd=tmp,
+                           capture_output=True, text=True, timeout=3600, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: reference failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        ddir = os.path.join(tmp, "d")
+        for fn in sorted(os.listdir(ddir)):
+            base, ext = os.path.splitext(fn)
+            dt = np.int32 if ext == ".i32" else np.float64
+            a = np.fromfile(os.path.join(ddir, fn), dtype=dt)
+            if (base in NC_MESH_I or base in NC_MESH_F) and not keep_mesh:
+                out["sha_" + base] = np.frombuffer(hashlib.sha256(a.tobytes()).digest(), np.uint8)
+                continue
+            out[base] = a
+    meta = dict(mesh=NC_CASES[name][0], deck=dkw, grad=grad, niter=niter, ndump=ndump,
+                raw_sha=nc_raw_sha(m), reference="NCFVM, amdflang -O2 -ffp-contract=off",
+                driver="oracle/ref_nc_driver.F90", env="MALLOC_MMAP_THRESHOLD_=0")
+    out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
 def main(argv):
-    if not os.path.exists(DRIVER):
+    if not os.path.exists(DRIVER) or not os.path.exists(NC_DRIVER):
         raise SystemExit("build the reference first: make -C oracle ref")
-    for name in (argv or list(CASES)):
-        p = make(name)
+    for name in (argv or list(CASES) + list(NC_CASES)):
+        p = make_nc(name) if name in NC_CASES else make(name)
         print(f"{name}: {os.path.getsize(p) / 1024:.0f} KiB", flush=True)
 
 

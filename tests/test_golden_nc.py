@@ -1,0 +1,111 @@
+"""The vertex-centred preprocessing and oracle against the reference's own
+outputs (fixtures written by tests/golden/make_golden.py from the reference
+NCFVM solver).
+
+Both are required to agree bit for bit: the preprocessing restatement in the
+library (breadth-first numbering, dual normals and volumes, neighbour and
+edge lists, least-squares weights) and the C oracle's every routine output
+and residual history, for GG and LSQ gradients, LU-SGS and RK3.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+CASES = ["nc_tiny_lusgs_gg", "nc_tiny_lusgs_lsqr", "nc_tiny_rk3_gg", "nc_tiny_rk3_lsqr",
+         "nc_c3_small"]
+TINY = CASES[:4]
+
+I32 = {"fc_p1", "fc_p2", "fc_in", "fc_out", "fc_bc", "pt_nv2v", "pt_v2v", "pt_nv2f", "pt_v2f",
+       "counts"}
+
+
+def load(case):
+    z = np.load(os.path.join(GOLD, case + ".npz"))
+    meta = json.loads(bytes(z["meta"]).decode())
+    return z, meta
+
+
+def raw_of(meta):
+    from unsflow2d_amd import nc_mesh
+    mkw = dict(meta["mesh"])
+    return nc_mesh.o_grid(mkw.pop("ni"), mkw.pop("nj"), mkw.pop("seed"), **mkw)
+
+
+def lib_mesh(meta):
+    """The library's preprocessing of the case's mesh, in the fixture's names."""
+    from unsflow2d_amd import nc
+    m = nc.NCMesh.from_raw(raw_of(meta))
+    a = m.arrays()
+    a["pt_wxy"] = np.concatenate([a["pt_wx"], a["pt_wy"]])
+    a["counts"] = a["counts"]
+    return a
+
+
+def oracle_mesh(a):
+    return {k: a[k] for k in ("pt_x", "pt_y", "pt_cv", "fc_p1", "fc_p2", "fc_bc", "fc_nx", "fc_ny",
+                              "fc_area", "pt_nv2v", "pt_v2v", "pt_nv2f", "pt_v2f")} | {
+        "pt_wx": a["pt_wxy"][: a["pt_wxy"].size // 2], "pt_wy": a["pt_wxy"][a["pt_wxy"].size // 2:]}
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_preprocessing_matches_reference(case):
+    z, meta = load(case)
+    a = lib_mesh(meta)
+    names = sorted(I32 | {"fc_nx", "fc_ny", "fc_area", "pt_x", "pt_y", "pt_cv", "pt_wxy"})
+    for n in names:
+        got = np.ascontiguousarray(a[n], dtype=np.int32 if n in I32 else np.float64)
+        if n in z.files:
+            assert np.array_equal(got, z[n]), n
+        else:
+            assert hashlib.sha256(got.tobytes()).digest() == bytes(z["sha_" + n]), n
+
+
+def _eq(name, got, want):
+    got, want = np.asarray(got).ravel(), np.asarray(want).ravel()
+    assert got.shape == want.shape, name
+    ok = np.array_equal(got, want) or np.array_equal(np.isnan(got), np.isnan(want)) and \
+        np.array_equal(got[~np.isnan(got)], want[~np.isnan(want)])
+    assert ok, f"{name}: max |d| {np.nanmax(np.abs(got - want)):.3e}"
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_matches_reference(case):
+    from oracle.oracle import NCOracle
+    z, meta = load(case)
+    a = {k: z[k] for k in z.files} if "pt_x" in z.files else lib_mesh(meta)
+    d = meta["deck"]
+    o = NCOracle(oracle_mesh(a), mach=d["mach"], aoa_deg=d.get("aoa_deg", 0.0),
+                 cfl_max=d["cfl_max"], grad=meta["grad"])
+    tm = d["timemode"]
+    hist = z["hist"].reshape(-1, 8)
+    for it in range(1, meta["niter"] + 1):
+        t = lambda n, k: z[f"{n}_it{it:04d}_{k}"]
+        if it <= meta["ndump"]:
+            cfl = o.cfl(it)
+            o.save_old()
+            _eq(f"qp {it}", o.qp, t("qp", 0))
+            for k in ((1, 2, 3) if tm == "rk3" else (1,)):
+                if tm == "rk3":
+                    o.fvresidual(cfl)
+                else:
+                    o.lusgs(cfl)
+                _eq(f"grad {it}.{k}", o.grad, t("grad", k))
+                _eq(f"phi {it}.{k}", o.phi, t("phi", k).reshape(-1, 12)[:, :4])
+                _eq(f"res {it}.{k}", o.res, t("res", k))
+                _eq(f"la {it}.{k}", o.la, t("la", k))
+                _eq(f"dt {it}.{k}", o.dt, t("dt", k))
+                _eq(f"fla {it}.{k}", o.fla, t("fcla", k))
+                if tm == "rk3":
+                    o.rk_update(k)
+                _eq(f"qc {it}.{k}", o.qc, t("qc", k))
+            o.residue(it)
+        else:
+            o.iteration(it, tm)
+        _eq(f"fres {it}", [o.s.fres, o.s.fresi, o.s.dtglobal], hist[it - 1, [1, 4, 6]])
+    _eq("final qc", o.qc, z["final_qc"])
