@@ -247,6 +247,55 @@ int ufx_cc_sync(ufx_cc* ctx);
 /* The context's HIP stream (hipStream_t), for timing with HIP events. */
 void* ufx_cc_stream(ufx_cc* ctx);
 
+/* ============================================ vertex-centred (median dual)
+ * The NCFVM solver's hot path (NCFVM/main.f90:51-88).  State per vertex, in
+ * the reference's breadth-first vertex order, variables of one vertex
+ * contiguous. FP64. */
+typedef struct ufx_nc ufx_nc;
+
+/* Run parameters (NCFVM/read_input.f90:23-40; the deck's Flux and Limiter
+ * entries are ignored by the reference, which always runs van Leer with the
+ * Venkatakrishnan limiter). */
+typedef struct {
+  double mach;       /* Mach */
+  double aoa_deg;    /* AOA */
+  double cfl_max;    /* CFL: the target of the tanh ramp, NCFVM/main.f90:54 */
+  int grad;          /* UFX_GRAD_GG or UFX_GRAD_LSQR (the two sides of the
+                        merge conflict at NCFVM/fvresidual.f90:11-17) */
+  int timemode;      /* UFX_TIME_RK3 (needs cfl_max < 1) or UFX_TIME_LUSGS */
+  int precision;     /* UFX_PREC_FP64 */
+} ufx_nc_params;
+
+/* Context on the current device in the state `initialize` leaves
+ * (NCFVM/common.f90:21-58): free stream everywhere, limiter value 1e20. */
+int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* params, ufx_nc** ctx);
+int ufx_nc_destroy(ufx_nc* ctx);
+/* save_old, NCFVM/common.f90:88-104 (primitive refresh and qold = qc). */
+int ufx_nc_save_old(ufx_nc* ctx);
+/* fvresidual, NCFVM/fvresidual.f90:4-85: gradient, limiter, edge fluxes
+ * with wall and farfield closure, then time_step2 with `cfl`. */
+int ufx_nc_fvresidual(ufx_nc* ctx, double cfl);
+/* fvresidual and the RK update of NCFVM/main.f90:59-70 for stage 1..3. */
+int ufx_nc_rk_stage(ufx_nc* ctx, int stage, double cfl);
+/* lusgs, NCFVM/lusgs.f90:4-131 (calls fvresidual itself). */
+int ufx_nc_lusgs(ufx_nc* ctx, double cfl);
+/* residue, NCFVM/common.f90:108-154; synchronising.  out[5] = {L2 / L2 of
+ * iteration 1, L2, Linf, its vertex (1-based), global dt}. */
+int ufx_nc_residue(ufx_nc* ctx, int iter, double* out);
+/* Iterations iter0+1 .. iter0+niter of NCFVM/main.f90:51-88 (tanh CFL
+ * ramp); hist (5 x niter, as ufx_nc_residue's out) may be NULL. */
+int ufx_nc_iterate(ufx_nc* ctx, int iter0, int niter, double* hist);
+/* "qc","qold" (4 x nop) writable, "phi" (4 x nop) writable; readable also
+ * "qp" (4), "grad" (8: d/dx, d/dy per variable), "res" (4), "la", "dt"
+ * (nop), "fc_la" (nof). */
+int ufx_nc_set(ufx_nc* ctx, const char* name, const double* src);
+int ufx_nc_get(ufx_nc* ctx, const char* name, double* dst);
+int ufx_nc_sync(ufx_nc* ctx);
+/* Kernel timing: categories save_old, gradient+limiter, residual, norm,
+ * LU-SGS (5 entries). */
+int ufx_nc_profile(ufx_nc* ctx, int enable);
+int ufx_nc_profile_read(ufx_nc* ctx, double* ms, int* count);
+
 #ifdef __cplusplus
 }
 #endif

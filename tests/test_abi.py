@@ -18,13 +18,14 @@ def declared():
 
 
 def test_header_symbols_exported():
-    from unsflow2d_amd import cc
+    from unsflow2d_amd import cc, nc
     names = declared()
-    assert len(names) >= 25
+    assert len(names) >= 40
     missing = [n for n in names if not hasattr(cc.lib, n)]
     assert not missing, missing
+    exported = set(cc.EXPORTED) | set(nc.NC_EXPORTED)
     assert sorted(cc.EXPORTED) == sorted(n for n in names if n in cc.EXPORTED)
-    assert set(names) <= set(cc.EXPORTED) | {"ufx_mesh_size"}
+    assert set(names) <= exported | {"ufx_mesh_size"}
 
 
 def test_mesh_errors_map_to_reference_stops():

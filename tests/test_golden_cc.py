@@ -17,7 +17,7 @@ import pytest
 from conftest import ROOT
 
 GOLD = os.path.join(ROOT, "tests", "golden")
-CASES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLD, "*.npz")))
+CASES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLD, "cc_*.npz")))
 TINY = [c for c in CASES if not c.startswith("cc_c1")]
 
 

@@ -1,0 +1,121 @@
+"""Device parity of the vertex-centred (median-dual) hot path.
+
+Against the C oracle (bit-exact with the reference, see test_golden_nc.py) on
+seeded O-grids, per routine and over iterations, and against the reference's
+own fixtures.  FP64: 1e-12 normwise per routine, looser over long runs where
+the limiter ratchet and LU-SGS amplify last-bit differences (stated per test).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, relerr
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _check(name, got, want, tol):
+    e = relerr(got, want)
+    print(f"  {name:14s} rel {e:.2e}", flush=True)
+    assert e <= tol, f"{name}: {e:.3e} > {tol}"
+
+
+def _setup(ni, nj, seed, stretch, grad, timemode, cfl_max, mach=0.4, aoa=0.0):
+    from unsflow2d_amd import nc, nc_mesh
+    from oracle.oracle import NCOracle
+    raw = nc_mesh.o_grid(ni, nj, seed, stretch=stretch)
+    m = nc.NCMesh.from_raw(raw)
+    dev = nc.NCSolver(m, mach=mach, aoa_deg=aoa, cfl_max=cfl_max, grad=grad, timemode=timemode)
+    orc = NCOracle(m.arrays(), mach=mach, aoa_deg=aoa, cfl_max=cfl_max, grad=grad)
+    return dev, orc
+
+
+@pytest.mark.parametrize("grad,timemode", [("gg", "lusgs"), ("lsqr", "lusgs"), ("gg", "rk3"),
+                                           ("lsqr", "rk3")])
+def test_nc_routines_match_oracle(grad, timemode):
+    print(f"case: nc routines {grad} {timemode} o-grid 30x12 seed 4", flush=True)
+    cfl_max = 50.0 if timemode == "lusgs" else 0.8
+    dev, orc = _setup(30, 12, 4, 1.2, grad, timemode, cfl_max)
+    for it in (1, 2, 3):
+        cfl = orc.cfl(it)
+        dev.save_old(); orc.save_old()
+        _check("qp", dev.get("qp"), orc.qp, 1e-12)
+        for k in ((1, 2, 3) if timemode == "rk3" else (1,)):
+            if timemode == "rk3":
+                dev.rk_stage(k, cfl)
+                orc.fvresidual(cfl)
+            else:
+                dev.lusgs(cfl)
+                orc.lusgs(cfl)
+            _check("grad", dev.get("grad"), orc.grad, 1e-12)
+            _check("phi", dev.get("phi"), orc.phi, 1e-12)
+            _check("res", dev.get("res"), orc.res, 1e-12)
+            _check("la", dev.get("la"), orc.la, 1e-12)
+            _check("dt", dev.get("dt"), orc.dt, 1e-12)
+            _check("fc_la", dev.get("fc_la"), orc.fla, 1e-12)
+            if timemode == "rk3":
+                orc.rk_update(k)
+            _check("qc", dev.get("qc"), orc.qc, 1e-12)
+        h = dev.residue(it)
+        orc.residue(it)
+        _check("fres", h[[0, 2]], [orc.s.fres, orc.s.fresi], 1e-12)
+        assert int(h[3]) == orc.s.iresi
+
+
+@pytest.mark.parametrize("grad,timemode", [("gg", "lusgs"), ("lsqr", "rk3")])
+def test_nc_iterations_match_oracle(grad, timemode):
+    n = 80
+    print(f"case: nc iterate {grad} {timemode} o-grid 40x16 seed 6 n={n}", flush=True)
+    cfl_max = 50.0 if timemode == "lusgs" else 0.8
+    dev, orc = _setup(40, 16, 6, 1.15, grad, timemode, cfl_max, mach=0.5, aoa=1.5)
+    h = dev.iterate(n)
+    ref = np.array([orc.iteration(it, timemode) for it in range(1, n + 1)])
+    _check("history", h[:, 0], ref, 1e-10)
+    _check("final qc", dev.get("qc"), orc.qc, 1e-10)
+
+
+def _load(case):
+    z = np.load(os.path.join(GOLD, case + ".npz"))
+    return z, json.loads(bytes(z["meta"]).decode())
+
+
+@pytest.mark.parametrize("case,tol", [("nc_tiny_lusgs_gg", 1e-10), ("nc_tiny_lusgs_lsqr", 1e-10),
+                                      ("nc_tiny_rk3_gg", 1e-10), ("nc_tiny_rk3_lsqr", 1e-10),
+                                      ("nc_c3_small", 1e-8)])
+def test_nc_device_matches_reference_fixture(case, tol):
+    """Per-routine outputs of the first iterations and the whole residual
+    history and final state against the reference's dumps."""
+    from unsflow2d_amd import nc, nc_mesh
+    print(f"case: {case}", flush=True)
+    z, meta = _load(case)
+    mkw = dict(meta["mesh"])
+    raw = nc_mesh.o_grid(mkw.pop("ni"), mkw.pop("nj"), mkw.pop("seed"), **mkw)
+    d = meta["deck"]
+    dev = nc.NCSolver(nc.NCMesh.from_raw(raw), mach=d["mach"], aoa_deg=d.get("aoa_deg", 0.0),
+                      cfl_max=d["cfl_max"], grad=meta["grad"], timemode=d["timemode"])
+    tm = d["timemode"]
+    hist = z["hist"].reshape(-1, 8)
+    nd = meta["ndump"]
+    for it in range(1, nd + 1):
+        t = lambda n, k: z[f"{n}_it{it:04d}_{k}"]
+        cfl = dev.cfl(it)
+        dev.save_old()
+        _check("qp", dev.get("qp").ravel(), t("qp", 0), 1e-12)
+        for k in ((1, 2, 3) if tm == "rk3" else (1,)):
+            if tm == "rk3":
+                dev.rk_stage(k, cfl)
+            else:
+                dev.lusgs(cfl)
+            _check("res", dev.get("res").ravel(), t("res", k), 1e-12)
+            _check("phi", dev.get("phi"), t("phi", k).reshape(-1, 12)[:, :4], 1e-12)
+            _check("qc", dev.get("qc").ravel(), t("qc", k), 1e-12)
+        h = dev.residue(it)
+        _check("fres", h[0], hist[it - 1, 1], 1e-12)
+    dev.iter = nd
+    h = dev.iterate(meta["niter"] - nd)
+    _check("history", h[:, 0], hist[nd:, 1], tol)
+    _check("final qc", dev.get("qc").ravel(), z["final_qc"], tol)

@@ -53,7 +53,7 @@ def build_lib() -> str:
     os.makedirs(bdir, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include")]
     objs = []
-    for src in ("cc_solver.hip",):
+    for src in ("cc_solver.hip", "nc_solver.hip"):
         obj = os.path.join(bdir, src + ".o")
         _run([HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
         tg = _gpu_targets(obj)

@@ -100,6 +100,79 @@ struct DevT {
   double* upper;
 };
 
+// ---------------------------------------------------------------- loads
+template <typename R, int N>
+__device__ __forceinline__ void ldv(const R* __restrict__ p, R* out) {
+  constexpr int B = N * (int)sizeof(R);
+  if constexpr (B % 16 == 0) {
+    if constexpr (sizeof(R) == 8) {
+      const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+      for (int i = 0; i < N / 2; ++i) { double2 v = q[i]; out[2 * i] = v.x; out[2 * i + 1] = v.y; }
+    } else {
+      const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+      for (int i = 0; i < N / 4; ++i) {
+        float4 v = q[i];
+        out[4 * i] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = v.w;
+      }
+    }
+  } else {
+    static_assert(sizeof(R) == 4 && N % 2 == 0, "8-byte vector loads");
+    const float2* q = reinterpret_cast<const float2*>(p);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) { float2 v = q[i]; out[2 * i] = v.x; out[2 * i + 1] = v.y; }
+  }
+}
+
+template <typename R, int N>
+__device__ __forceinline__ void stv(R* __restrict__ p, const R* v) {
+  constexpr int B = N * (int)sizeof(R);
+  if constexpr (B % 16 == 0) {
+    if constexpr (sizeof(R) == 8) {
+      double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+      for (int i = 0; i < N / 2; ++i) q[i] = make_double2(v[2 * i], v[2 * i + 1]);
+    } else {
+      float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+      for (int i = 0; i < N / 4; ++i) q[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    }
+  } else {
+    float2* q = reinterpret_cast<float2*>(p);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) q[i] = make_float2(v[2 * i], v[2 * i + 1]);
+  }
+}
+
+__device__ __forceinline__ void ld4d(const double* p, double* o) {
+  const double2* q = reinterpret_cast<const double2*>(p);
+  double2 a = q[0], b = q[1];
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4d(double* p, const double* v) {
+  double2* q = reinterpret_cast<double2*>(p);
+  q[0] = make_double2(v[0], v[1]);
+  q[1] = make_double2(v[2], v[3]);
+}
+
+// Streaming (non-temporal) forms for data a kernel touches exactly once: they
+// keep such data from displacing the neighbour records in L2.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ld4d_nt(const double* p, double* o) {
+  const nt_d2* q = reinterpret_cast<const nt_d2*>(p);
+  nt_d2 a = __builtin_nontemporal_load(q), b = __builtin_nontemporal_load(q + 1);
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ void st4d_nt(double* p, const double* v) {
+  nt_d2* q = reinterpret_cast<nt_d2*>(p);
+  nt_d2 a = {v[0], v[1]}, b = {v[2], v[3]};
+  __builtin_nontemporal_store(a, q);
+  __builtin_nontemporal_store(b, q + 1);
+}
+template <typename T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+
 // Block index remapped so that each XCD (blocks are dealt to the 8 dies
 // round-robin) works on one contiguous slice of the cells: neighbour data
 // then mostly sits in the die's own L2.

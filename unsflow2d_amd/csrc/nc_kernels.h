@@ -1,0 +1,497 @@
+// Kernels of the vertex-centred (median-dual) hot path, FP64 (gfx950).
+// Included once, by nc_solver.hip.  Each kernel cites the reference routine
+// it performs.  The reference scatters edge contributions into both end
+// points in edge order; here every vertex gathers over its edges in the
+// same order (its v2f list is in ascending edge order), so each per-vertex
+// sum keeps the reference's order.
+//
+// HBM layout (vertices in the reference's breadth-first order):
+//   vrec [nop][18] f64  primitives rho,u,v,p | gradient (d/dx, d/dy per
+//                       variable) | limiter values | x, y
+//   qc, qold, res [nop][4] f64 (conserved order E, rho, rho u, rho v)
+//   la, dt, cv [nop] f64;  fla [nof] f64
+//   erec [nof][4] f64   dual-face nx, ny, length, pad;  ep [nof] int2 (p1, p2)
+//   v2f [..] int        edges of a vertex, bit-inverted when it is p2
+//   v2v [..] int, wxy [..][2] f64   least-squares neighbours and weights
+#pragma once
+
+#include "cc_device.h"
+#include "cc_physics.h"
+
+namespace ufx {
+
+constexpr int NW = 18;  // vertex record width (same offsets as the cell record)
+constexpr double NC_EPS_SP = 1.1920928955078125e-07;  // epsilon(1.0), NCFVM/All_modules.f90:9
+
+struct NCFree {
+  double gamma, gamma1, r_inf, u_inf, v_inf, p_inf, a_inf;
+};
+
+struct DevNC {
+  int nop, nof;
+  double* vrec;
+  double *qc, *qold, *res, *la, *dt, *dd, *dqc;
+  const double* cv;
+  const double* erec;
+  const int2* ep;
+  const int* ebc;
+  double* fla;
+  const int *v2f_ptr, *v2f;
+  const int *v2v_ptr, *v2v;
+  const double* wxy;
+  double* part;  // reduction partials [nblk][4]
+  double* scal;  // [0] fres1
+  int* err;
+};
+
+__device__ __forceinline__ void ld2d(const double* p, double& a, double& b) {
+  double2 v = *reinterpret_cast<const double2*>(p);
+  a = v.x;
+  b = v.y;
+}
+
+// save_old, NCFVM/common.f90:88-104: primitives of every vertex (the only
+// primitive refresh of an iteration) and qold = qc.
+__global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
+  int p = cell_index();
+  if (p >= d.nop) return;
+  double q[4];
+  ld4d(d.qc + (size_t)p * 4, q);
+  st4d(d.qold + (size_t)p * 4, q);
+  double rho = q[1], u = q[2] / q[1], v = q[3] / q[1], et = q[0];
+  double qq = u * u + v * v;
+  double pr = (g.gamma - 1.0) * (et - 0.5 * rho * qq);
+  double o[4] = {rho, u, v, pr};
+  st4d(d.vrec + (size_t)p * NW + R_QP, o);
+}
+
+// Gradient and limiter of one vertex:
+//   GRAD 1  Gradient_GG, NCFVM/gradient.f90:35-75 (edge midpoint values on
+//           the dual faces, no boundary closure), / dual volume;
+//   GRAD 2  Gradient_LSQR, NCFVM/gradient.f90:2-30;
+// then limiter_VKN, NCFVM/limiter.f90:15-98: neighbour difference range
+// (starting at -+eps of single precision), the Venkatakrishnan function
+// with eps^2 = (kappa sqrt(V))^3, kappa = 0.01 in single precision, per edge
+// end, ratcheted into the stored limiter value.
+template <int GRAD>
+__global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
+  int p = cell_index();
+  if (p >= d.nop) return;
+  const double* own = d.vrec + (size_t)p * NW;
+  double q[4], x, y;
+  ld4d(own + R_QP, q);
+  ld2d(own + R_CEN, x, y);
+  double gr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int j0 = d.v2f_ptr[p], j1 = d.v2f_ptr[p + 1];
+  if (GRAD == 1) {
+    for (int j = j0; j < j1; ++j) {
+      int e = d.v2f[j];
+      bool first = e >= 0;
+      e = first ? e : ~e;
+      int2 pp = d.ep[e];
+      int o = first ? pp.y : pp.x;
+      double qo[4];
+      ld4d(d.vrec + (size_t)o * NW + R_QP, qo);
+      double nx, ny, ds, pad;
+      ld2d(d.erec + (size_t)e * 4, nx, ny);
+      ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
+      double dx = nx * ds, dy = ny * ds;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double var = first ? 0.5 * (q[k] + qo[k]) : 0.5 * (qo[k] + q[k]);
+        if (first) {
+          gr[2 * k] = gr[2 * k] + var * dx;
+          gr[2 * k + 1] = gr[2 * k + 1] + var * dy;
+        } else {
+          gr[2 * k] = gr[2 * k] - var * dx;
+          gr[2 * k + 1] = gr[2 * k + 1] - var * dy;
+        }
+      }
+    }
+    double v = d.cv[p];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gr[k] = gr[k] / v;
+  } else {
+    for (int j = d.v2v_ptr[p]; j < d.v2v_ptr[p + 1]; ++j) {
+      int c = d.v2v[j];
+      double wx, wy, qo[4];
+      ld2d(d.wxy + (size_t)j * 2, wx, wy);
+      ld4d(d.vrec + (size_t)c * NW + R_QP, qo);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        gr[2 * k] = gr[2 * k] + (qo[k] - q[k]) * wx;
+        gr[2 * k + 1] = gr[2 * k + 1] + (qo[k] - q[k]) * wy;
+      }
+    }
+  }
+  // difference range over the edges
+  double dumax[4], dumin[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { dumin[k] = NC_EPS_SP; dumax[k] = -NC_EPS_SP; }
+  for (int j = j0; j < j1; ++j) {
+    int e = d.v2f[j];
+    bool first = e >= 0;
+    e = first ? e : ~e;
+    int2 pp = d.ep[e];
+    int o = first ? pp.y : pp.x;
+    double qo[4];
+    ld4d(d.vrec + (size_t)o * NW + R_QP, qo);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double du = first ? qo[k] - q[k] : -(q[k] - qo[k]);
+      dumax[k] = dumax[k] > du ? dumax[k] : du;
+      dumin[k] = dumin[k] < du ? dumin[k] : du;
+    }
+  }
+  double phi[4];
+  ld4d(own + R_PHI, phi);
+  const double kappa = (double)0.01f;
+  double t = kappa * sqrt(d.cv[p]);
+  const double tol = t * t * t;
+  for (int j = j0; j < j1; ++j) {
+    int e = d.v2f[j];
+    bool first = e >= 0;
+    e = first ? e : ~e;
+    int2 pp = d.ep[e];
+    int o = first ? pp.y : pp.x;
+    double xo, yo;
+    ld2d(d.vrec + (size_t)o * NW + R_CEN, xo, yo);
+    double d0 = first ? xo - x : x - xo, d1 = first ? yo - y : y - yo;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double sum = (0.0 + gr[2 * k] * d0) + gr[2 * k + 1] * d1;
+      double dl = first ? 0.5 * sum : -0.5 * sum;
+      double alfa = dl > 0.0 ? dumax[k] : dumin[k];
+      if (fabs(alfa) < NC_EPS_SP) alfa = 0.0;
+      double nr = alfa * alfa + 2.0 * dl * alfa + tol;
+      double dr = alfa * alfa + 2.0 * dl * dl + dl * alfa + tol;
+      double ph = nr / dr;
+      ph = 1.0 < ph ? 1.0 : ph;
+      phi[k] = phi[k] < ph ? phi[k] : ph;
+    }
+  }
+  double* w = d.vrec + (size_t)p * NW;
+  st4d(w + R_G, gr);
+  st4d(w + R_G + 4, gr + 4);
+  st4d(w + R_PHI, phi);
+}
+
+// van Leer flux of NCFVM/vanleer_flux.f90:2-118 on one dual face between
+// the reconstructed states ql, qr (primitives): F (per unit length) and the
+// Roe-averaged spectral radius times the face length.
+__device__ __forceinline__ void nc_vanleer(const NCFree& g, const double* ql, const double* qr,
+                                           double nx, double ny, double area, double* F,
+                                           double& ll) {
+  const double gm = g.gamma, gm1 = g.gamma1;
+  double rl = ql[0], ul = ql[1], vl = ql[2], pl = ql[3];
+  double unl = ul * nx + vl * ny, ql2 = ul * ul + vl * vl;
+  double al2 = gm * pl / rl, hl = al2 / gm1 + 0.5 * ql2, cl = sqrt(al2), ml = unl / cl;
+  double rr = qr[0], ur = qr[1], vr = qr[2], pr = qr[3];
+  double unr = ur * nx + vr * ny, qr2 = ur * ur + vr * vr;
+  double ar2 = gm * pr / rr, hr = ar2 / gm1 + 0.5 * qr2, cr = sqrt(ar2), mr = unr / cr;
+  double plp = p5p<double>(ml, 0.0), prm = p5m<double>(mr, 0.0);
+  double mlp = m4p<double>(ml, 0.0), mlm = m4m<double>(ml, 0.0);
+  double mrp = m4p<double>(mr, 0.0), mrm = m4m<double>(mr, 0.0);
+  double p12 = plp * pl + prm * pr;
+  double fmp = rl * cl * mlp, fmm = rr * cr * mrm;
+  double fm12 = fmp + fmm;
+  double delta = -(gm * (cr * mrp * mrm * pr - cl * mlp * mlm * pl) / (gm - 1.0));
+  double dissi = fmp - fmm;
+  double fl[4] = {hl, 1.0, ul, vl}, fr[4] = {hr, 1.0, ur, vr};
+  double fp[4] = {delta, 0.0, nx * p12, ny * p12};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) F[i] = 0.5 * fm12 * (fl[i] + fr[i]) - 0.5 * dissi * (fr[i] - fl[i]) + fp[i];
+  double rt = sqrt(rr / rl);
+  double u = (ul + rt * ur) / (1.0 + rt), v = (vl + rt * vr) / (1.0 + rt);
+  double h = (hl + rt * hr) / (1.0 + rt);
+  double a = sqrt(gm1 * (h - 0.5 * (u * u + v * v)));
+  ll = (fabs(u * nx + v * ny) + a) * area;
+}
+
+// Residual of one vertex, NCFVM/fvresidual.f90:34-64 as a gather over its
+// edges in edge order: reconstruction of both ends (:40-54), van Leer on the
+// dual face (+ into p1, - into p2, spectral radius into both), then for a
+// wall edge solid_flux (NCFVM/solid_flux.f90:1-43) and for a farfield edge
+// farfield_flux1 (NCFVM/farfield_flux.f90:1-95); then time_step2
+// (NCFVM/timestep.f90:5-37).
+//   mode 0: store res, la, dt;  mode 1: also the RK stage update of
+//   NCFVM/main.f90:59-70 in place (no vertex reads another's qc in a stage).
+__global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, double cfl, int mode,
+                                                     double ak, double bk) {
+  int p = cell_index();
+  if (p >= d.nop) return;
+  double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0;
+  const int j0 = d.v2f_ptr[p], j1 = d.v2f_ptr[p + 1];
+  for (int j = j0; j < j1; ++j) {
+    int e = d.v2f[j];
+    bool first = e >= 0;
+    e = first ? e : ~e;
+    int2 pp = d.ep[e];
+    double a[NW], b[NW];
+    ldv<double, NW>(d.vrec + (size_t)pp.x * NW, a);
+    ldv<double, NW>(d.vrec + (size_t)pp.y * NW, b);
+    double nx, ny, ds, pad;
+    ld2d(d.erec + (size_t)e * 4, nx, ny);
+    ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
+    double d0 = b[R_CEN] - a[R_CEN], d1 = b[R_CEN + 1] - a[R_CEN + 1];
+    double ql[4], qr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double s = (0.0 + a[R_G + 2 * k] * d0) + a[R_G + 2 * k + 1] * d1;
+      ql[k] = a[R_QP + k] + 0.5 * s * a[R_PHI + k];
+      s = (0.0 + b[R_G + 2 * k] * d0) + b[R_G + 2 * k + 1] * d1;
+      qr[k] = b[R_QP + k] - 0.5 * s * b[R_PHI + k];
+    }
+    double F[4], ll;
+    nc_vanleer(g, ql, qr, nx, ny, ds, F, ll);
+    if (first) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i] * ds;
+      d.fla[e] = ll;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = r[i] - F[i] * ds;
+    }
+    la = la + ll;
+    int bc = d.ebc[e];
+    if (bc == WALL) {
+      // solid_flux: 3/4-1/4 end-point pressures on the half-edge right normals
+      double p1w = 0.75 * a[R_QP + 3] + 0.25 * b[R_QP + 3];
+      double p2w = 0.25 * a[R_QP + 3] + 0.75 * b[R_QP + 3];
+      double wnx = 0.5 * (b[R_CEN + 1] - a[R_CEN + 1]);
+      double wny = -0.5 * (b[R_CEN] - a[R_CEN]);
+      double area = sqrt(wnx * wnx + wny * wny);
+      const double* me = first ? a : b;
+      la = la + sqrt(g.gamma * me[R_QP + 3] / me[R_QP]) * area;
+      double pw = first ? p1w : p2w;
+      r[2] = r[2] + pw * wnx;
+      r[3] = r[3] + pw * wny;
+    } else if (bc == FARF) {
+      // farfield_flux1: Riemann invariants on the half-edge left normal from
+      // the edge-averaged primitives, the same flux into both end points
+      double fnx = -0.5 * (b[R_CEN + 1] - a[R_CEN + 1]);
+      double fny = 0.5 * (b[R_CEN] - a[R_CEN]);
+      double dr = sqrt(fnx * fnx + fny * fny);
+      fnx = fnx / dr;
+      fny = fny / dr;
+      double un_inf = g.u_inf * fnx + g.v_inf * fny;
+      double rho = 0.5 * (a[R_QP] + b[R_QP]), u = 0.5 * (a[R_QP + 1] + b[R_QP + 1]);
+      double v = 0.5 * (a[R_QP + 2] + b[R_QP + 2]), pr = 0.5 * (a[R_QP + 3] + b[R_QP + 3]);
+      double as = sqrt(g.gamma * pr / rho);
+      double un = u * fnx + v * fny;
+      double rp = un + 2.0 * as / g.gamma1;
+      double rm = un_inf - 2.0 * g.a_inf / g.gamma1;
+      double unf = 0.5 * (rp + rm);
+      double af = 0.25 * (rp - rm) * g.gamma1;
+      double uf, vf, sf;
+      if (unf > 0.0) {
+        uf = u + fnx * (unf - un);
+        vf = v + fny * (unf - un);
+        sf = pr / pow(rho, g.gamma);
+      } else {
+        uf = g.u_inf + fnx * (unf - un_inf);
+        vf = g.v_inf + fny * (unf - un_inf);
+        sf = g.p_inf / pow(g.r_inf, g.gamma);
+      }
+      double rhof = pow(af * af / g.gamma / sf, 1.0 / g.gamma1);
+      double pf = rhof * af * af / g.gamma;
+      double ef = pf / g.gamma1 + 0.5 * rhof * (uf * uf + vf * vf);
+      double fl[4];
+      fl[0] = (ef + pf) * unf;
+      fl[1] = rhof * unf;
+      fl[2] = fl[1] * uf + pf * fnx;
+      fl[3] = fl[1] * vf + pf * fny;
+      double area = sqrt(fnx * fnx + fny * fny);
+      const double* me = first ? a : b;
+      la = la + (fabs(unf) + sqrt(g.gamma * me[R_QP + 3] / me[R_QP])) * area;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = r[i] + dr * fl[i];
+    }
+  }
+  double cvp = d.cv[p];
+  double dt = cfl * cvp / la;
+  d.la[p] = la;
+  d.dt[p] = dt;
+  st4d(d.res + (size_t)p * 4, r);
+  if (mode == 1) {
+    double dtbyarea = dt / cvp, q[4], o[4];
+    ld4d(d.qc + (size_t)p * 4, q);
+    ld4d(d.qold + (size_t)p * 4, o);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dtbyarea * r[i]);
+    st4d(d.qc + (size_t)p * 4, q);
+  }
+}
+
+// residue, NCFVM/common.f90:108-154: block partials of sum |res|^2, the
+// largest per-vertex |res| with its (first) vertex, and min dt.
+__global__ __launch_bounds__(BLK) void k_nc_norm(DevNC d) {
+  __shared__ double sh[2][BLK];
+  __shared__ int si[BLK];
+  int p = blockIdx.x * BLK + threadIdx.x;
+  double s = 0.0, mx = 0.0, dmn = 1.0e20;
+  int imx = 0x7fffffff;
+  if (p < d.nop) {
+    double r[4];
+    ld4d(d.res + (size_t)p * 4, r);
+    double fr = 0.0;
+    for (int j = 0; j < 4; ++j) fr = fr + r[j] * r[j];
+    s = fr;
+    mx = sqrt(fr);
+    imx = p;
+    dmn = d.dt[p];
+  }
+  double v[1] = {s};
+  block_reduce<1>(sh, v, OpAdd());
+  // argmax, first index on ties (the reference's strict '>' scan)
+  __syncthreads();
+  sh[0][threadIdx.x] = mx;
+  si[threadIdx.x] = imx;
+  sh[1][threadIdx.x] = dmn;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double m2 = sh[0][threadIdx.x + w];
+      int i2 = si[threadIdx.x + w];
+      if (m2 > sh[0][threadIdx.x] || (m2 == sh[0][threadIdx.x] && i2 < si[threadIdx.x])) {
+        sh[0][threadIdx.x] = m2;
+        si[threadIdx.x] = i2;
+      }
+      double d2 = sh[1][threadIdx.x + w];
+      sh[1][threadIdx.x] = d2 < sh[1][threadIdx.x] ? d2 : sh[1][threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.part[blockIdx.x * 4 + 0] = v[0];
+    d.part[blockIdx.x * 4 + 1] = sh[0][0];
+    d.part[blockIdx.x * 4 + 2] = (double)si[0];
+    d.part[blockIdx.x * 4 + 3] = sh[1][0];
+  }
+}
+
+// Final reduction: hist[k] = {fres (normalised by the iteration-1 value),
+// fres raw, fresi, iresi (1-based), dtglobal}.
+__global__ __launch_bounds__(BLK) void k_nc_norm_final(DevNC d, int nblk, int first,
+                                                       double* hist, int k) {
+  __shared__ double sh[2][BLK];
+  __shared__ int si[BLK];
+  double s = 0.0, mx = 0.0, dmn = 1.0e20;
+  int imx = 0x7fffffff;
+  for (int b = threadIdx.x; b < nblk; b += BLK) {
+    s = s + d.part[b * 4];
+    double m2 = d.part[b * 4 + 1];
+    int i2 = (int)d.part[b * 4 + 2];
+    if (m2 > mx || (m2 == mx && i2 < imx)) { mx = m2; imx = i2; }
+    double d2 = d.part[b * 4 + 3];
+    dmn = d2 < dmn ? d2 : dmn;
+  }
+  double v[1] = {s};
+  block_reduce<1>(sh, v, OpAdd());
+  __syncthreads();
+  sh[0][threadIdx.x] = mx;
+  si[threadIdx.x] = imx;
+  sh[1][threadIdx.x] = dmn;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double m2 = sh[0][threadIdx.x + w];
+      int i2 = si[threadIdx.x + w];
+      if (m2 > sh[0][threadIdx.x] || (m2 == sh[0][threadIdx.x] && i2 < si[threadIdx.x])) {
+        sh[0][threadIdx.x] = m2;
+        si[threadIdx.x] = i2;
+      }
+      double d2 = sh[1][threadIdx.x + w];
+      sh[1][threadIdx.x] = d2 < sh[1][threadIdx.x] ? d2 : sh[1][threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double fres = sqrt(v[0] / (double)d.nop);
+    if (first) d.scal[0] = fres;
+    double f1 = d.scal[0];
+    double* h = hist + (size_t)k * 5;
+    h[0] = f1 != 0.0 ? fres / f1 : fres;
+    h[1] = fres;
+    h[2] = sh[0][0];
+    h[3] = sh[0][0] > 0.0 ? (double)(si[0] + 1) : 0.0;
+    h[4] = sh[1][0];
+  }
+}
+
+// LU-SGS of NCFVM/lusgs.f90:4-131 (omega = 1.5): diagonal D = V/dt +
+// omega/2 lambda and dq = 0.
+__global__ __launch_bounds__(BLK) void k_nc_lusgs_diag(DevNC d) {
+  int p = cell_index();
+  if (p >= d.nop) return;
+  d.dd[p] = d.cv[p] / d.dt[p] + 0.5 * 1.5 * d.la[p];
+  double z[4] = {0.0, 0.0, 0.0, 0.0};
+  st4d(d.dqc + (size_t)p * 4, z);
+}
+
+// normalflux of NCFVM/lusgs.f90:135-155 (energy as (E/rho) rho).
+__device__ __forceinline__ void nc_normalflux(const NCFree& g, const double* q, double nx,
+                                              double ny, double ds, double* f) {
+  double rho = q[1], u = q[2] / q[1], v = q[3] / q[1], et = q[0];
+  double e = et / rho;
+  double qq = u * u + v * v;
+  double p = (g.gamma - 1.0) * (et - 0.5 * rho * qq);
+  double ee = e * rho;
+  double un = u * nx + v * ny;
+  f[0] = (ee + p) * un;
+  f[1] = rho * un;
+  f[2] = f[1] * u + p * nx;
+  f[3] = f[1] * v + p * ny;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) f[i] = f[i] * ds;
+}
+
+// One level of the forward (dir 0) or backward (dir 1) sweep: the vertices
+// of a level depend only on vertices of earlier levels, so the level's
+// vertices update concurrently and the sweep equals the reference's
+// sequential one.
+__global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const int* verts,
+                                                        int n, int dir) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= n) return;
+  int it = verts[t];
+  double cres[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = d.v2f_ptr[it]; j < d.v2f_ptr[it + 1]; ++j) {
+    int e = d.v2f[j];
+    bool first = e >= 0;
+    e = first ? e : ~e;
+    int2 pp = d.ep[e];
+    int o = first ? pp.y : pp.x;
+    double sign = first ? 1.0 : -1.0;
+    if (dir == 0 ? o < it : o > it) {
+      double nx, ny, ds, pad, f1[4], f2[4], qo[4], qn[4], dq[4];
+      ld2d(d.erec + (size_t)e * 4, nx, ny);
+      ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
+      ld4d(d.qold + (size_t)o * 4, qo);
+      ld4d(d.qc + (size_t)o * 4, qn);
+      ld4d(d.dqc + (size_t)o * 4, dq);
+      nc_normalflux(g, qo, nx, ny, ds, f1);
+      nc_normalflux(g, qn, nx, ny, ds, f2);
+      double lam = d.fla[e];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cres[i] = cres[i] + 0.5 * (sign * (f2[i] - f1[i]) - 1.5 * lam * dq[i]);
+    }
+  }
+  double dd = d.dd[it], res[4], dq[4], qo[4], q[4];
+  ld4d(d.res + (size_t)it * 4, res);
+  ld4d(d.qold + (size_t)it * 4, qo);
+  if (dir == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dq[i] = (-res[i] - cres[i]) / dd;
+  } else {
+    ld4d(d.dqc + (size_t)it * 4, dq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dq[i] = dq[i] - cres[i] / dd;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q[i] = qo[i] + dq[i];
+  st4d(d.dqc + (size_t)it * 4, dq);
+  st4d(d.qc + (size_t)it * 4, q);
+}
+
+}  // namespace ufx

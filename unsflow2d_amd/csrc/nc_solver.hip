@@ -1,0 +1,462 @@
+// Host side of the vertex-centred (median-dual) hot path and its C-ABI
+// (include/ufx.h, ufx_nc_*): context creation from a preprocessed mesh, one
+// entry per reference routine, whole iterations of NCFVM/main.f90:51-88, and
+// state injection / read-back in the reference's per-vertex layout.  FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nc_kernels.h"
+#include "ufx_internal.h"
+
+using namespace ufx;
+
+#define NCCHK(x)                                                                 \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) {                                                      \
+      set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " +    \
+                __FILE__ + ":" + std::to_string(__LINE__));                      \
+      return UFX_ERR_HIP;                                                        \
+    }                                                                            \
+  } while (0)
+
+struct ufx_nc {
+  ufx_nc_params prm{};
+  NCFree g{};
+  DevNC d{};
+  int nop = 0, nof = 0, nblk = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  double* d_hist = nullptr;
+  int hist_cap = 0;
+  int* lv_verts = nullptr;                 // forward levels then backward levels
+  std::vector<int> lv_fwd_off, lv_bwd_off;  // offsets into lv_verts
+  double cfl = 0.0;
+  bool prof = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_cat;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+int nblk_of(int n) { return std::max(1, (n + BLK - 1) / BLK); }
+
+template <typename T>
+int nc_alloc(ufx_nc* x, T** p, size_t n) {
+  if (n == 0) n = 1;
+  NCCHK(hipMalloc((void**)p, n * sizeof(T)));
+  x->allocs.push_back((void*)*p);
+  NCCHK(hipMemsetAsync(*p, 0, n * sizeof(T), x->stream));
+  return UFX_OK;
+}
+
+template <typename T>
+int nc_upload(ufx_nc* x, T** dst, const std::vector<T>& src) {
+  T* p = nullptr;
+  if (int rc = nc_alloc(x, &p, src.size())) return rc;
+  if (!src.empty())
+    NCCHK(hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, x->stream));
+  *dst = p;
+  return UFX_OK;
+}
+
+template <typename T>
+int nc_upload(ufx_nc* x, const T** dst, const std::vector<T>& src) {
+  T* p = nullptr;
+  int rc = nc_upload(x, &p, src);
+  *dst = p;
+  return rc;
+}
+
+int launch_ok() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("kernel launch failed: ") + hipGetErrorString(e));
+    return UFX_ERR_HIP;
+  }
+  return UFX_OK;
+}
+
+// kernel timing categories (ufx_nc_profile_read)
+enum { NK_SAVE = 0, NK_GRAD = 1, NK_RES = 2, NK_NORM = 3, NK_LUSGS = 4, NK_NCAT = 5 };
+
+void prof_mark(ufx_nc* x, int cat) {
+  if (!x->prof) return;
+  if (x->ev_used == x->ev.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    x->ev.push_back(e);
+    x->ev_cat.push_back(-1);
+  }
+  x->ev_cat[x->ev_used] = cat;
+  (void)hipEventRecord(x->ev[x->ev_used++], x->stream);
+}
+
+#define NLAUNCH(x, cat, kern, grid, ...)                    \
+  do {                                                      \
+    prof_mark((x), (cat));                                  \
+    kern<<<(grid), BLK, 0, (x)->stream>>>(__VA_ARGS__);     \
+    prof_mark((x), -1);                                     \
+  } while (0)
+
+int run_save_old(ufx_nc* x) {
+  NLAUNCH(x, NK_SAVE, k_nc_save_old, x->nblk, x->d, x->g);
+  return launch_ok();
+}
+
+int run_grad_lim(ufx_nc* x) {
+  if (x->prm.grad == UFX_GRAD_LSQR) NLAUNCH(x, NK_GRAD, k_nc_grad_lim<2>, x->nblk, x->d);
+  else NLAUNCH(x, NK_GRAD, k_nc_grad_lim<1>, x->nblk, x->d);
+  return launch_ok();
+}
+
+// fvresidual (gradient, limiter, edge fluxes, time step); mode 1 fuses the
+// RK stage update with coefficients (ak, bk)
+int run_fvresidual(ufx_nc* x, double cfl, int mode, double ak, double bk) {
+  if (int rc = run_grad_lim(x)) return rc;
+  NLAUNCH(x, NK_RES, k_nc_residual, x->nblk, x->d, x->g, cfl, mode, ak, bk);
+  return launch_ok();
+}
+
+int run_lusgs(ufx_nc* x, double cfl) {
+  if (int rc = run_fvresidual(x, cfl, 0, 0.0, 0.0)) return rc;
+  NLAUNCH(x, NK_LUSGS, k_nc_lusgs_diag, x->nblk, x->d);
+  for (int dir = 0; dir < 2; ++dir) {
+    const std::vector<int>& off = dir == 0 ? x->lv_fwd_off : x->lv_bwd_off;
+    for (size_t l = 0; l + 1 < off.size(); ++l) {
+      int n = off[l + 1] - off[l];
+      if (n <= 0) continue;
+      NLAUNCH(x, NK_LUSGS, k_nc_lusgs_level, nblk_of(n), x->d, x->g, x->lv_verts + off[l], n, dir);
+    }
+  }
+  return launch_ok();
+}
+
+int ensure_hist(ufx_nc* x, int n) {
+  if (n <= x->hist_cap) return UFX_OK;
+  if (x->d_hist) (void)hipFree(x->d_hist);
+  x->d_hist = nullptr;
+  NCCHK(hipMalloc((void**)&x->d_hist, sizeof(double) * 5 * (size_t)n));
+  x->hist_cap = n;
+  return UFX_OK;
+}
+
+int run_norm(ufx_nc* x, int iter, int k) {
+  NLAUNCH(x, NK_NORM, k_nc_norm, x->nblk, x->d);
+  NLAUNCH(x, NK_NORM, k_nc_norm_final, 1, x->d, x->nblk, iter == 1 ? 1 : 0, x->d_hist, k);
+  return launch_ok();
+}
+
+// NCFVM/main.f90:54
+double nc_cfl(double cfl_max, int iter) {
+  return 0.5 * cfl_max * (1.0 + std::tanh((double)iter * 7.0 / 500.0 - 3.5));
+}
+
+int check(ufx_nc* x) {
+  if (!x) { set_error("null context"); return UFX_ERR_ARG; }
+  return UFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ufx_nc_destroy(ufx_nc* x) {
+  if (!x) return UFX_OK;
+  if (x->stream) (void)hipStreamSynchronize(x->stream);
+  for (void* p : x->allocs) (void)hipFree(p);
+  if (x->d_hist) (void)hipFree(x->d_hist);
+  for (hipEvent_t e : x->ev) (void)hipEventDestroy(e);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+  delete x;
+  return UFX_OK;
+}
+
+int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) {
+  if (!mesh || !prm || !out) { set_error("ufx_nc_create: null argument"); return UFX_ERR_ARG; }
+  const Mesh& m = mesh->m;
+  if (m.kind != UFX_KIND_NC) { set_error("ufx_nc_create: not a vertex-centred mesh"); return UFX_ERR_ARG; }
+  if (prm->grad != UFX_GRAD_GG && prm->grad != UFX_GRAD_LSQR) {
+    set_error("ufx_nc_create: the vertex-centred path has the GG and LSQR gradients");
+    return UFX_ERR_ARG;
+  }
+  if (prm->timemode != UFX_TIME_RK3 && prm->timemode != UFX_TIME_LUSGS) {
+    set_error("Unknown time integration scheme");
+    return UFX_ERR_ARG;
+  }
+  if (prm->precision != UFX_PREC_FP64) {
+    set_error("ufx_nc_create: the vertex-centred path is FP64 only");
+    return UFX_ERR_UNSUPPORTED;
+  }
+  if (prm->timemode == UFX_TIME_RK3 && !(prm->cfl_max < 1.0)) {
+    // NCFVM/main.f90:31-44 sets the RK coefficients only for cfl_max < 1
+    set_error("CFL should be less than 1 for rk3 !!!");
+    return UFX_ERR_ARG;
+  }
+  ufx_nc* x = new ufx_nc();
+  x->prm = *prm;
+  const int nop = m.nop, nof = m.nof();
+  x->nop = nop;
+  x->nof = nof;
+  x->nblk = nblk_of(nop);
+  if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    delete x;
+    return UFX_ERR_HIP;
+  }
+  // free stream, NCFVM/common.f90:21-58
+  const double gamma = 1.4, pi = 4.0 * std::atan(1.0);
+  double aoa = prm->aoa_deg * pi / 180.0;
+  double qinf[4];
+  qinf[1] = 1.0;
+  qinf[2] = std::cos(aoa) * 1.0;
+  qinf[3] = std::sin(aoa) * 1.0;
+  double p_inf = 1.0 / (gamma * prm->mach * prm->mach);
+  qinf[0] = p_inf / (gamma - 1.0) + 0.5;
+  x->g.gamma = gamma;
+  x->g.gamma1 = gamma - 1.0;
+  x->g.r_inf = qinf[1];
+  x->g.p_inf = p_inf;
+  x->g.a_inf = std::sqrt(gamma * p_inf / qinf[1]);
+  x->g.u_inf = qinf[2] / qinf[1];
+  x->g.v_inf = qinf[3] / qinf[1];
+  // device arrays
+  std::vector<double> vrec((size_t)nop * NW, 0.0);
+  const double phi0 = (double)1e20f;  // default limiter value, NCFVM/All_modules.f90:200
+  for (int p = 0; p < nop; ++p) {
+    for (int k = 0; k < 4; ++k) vrec[(size_t)p * NW + R_PHI + k] = phi0;
+    vrec[(size_t)p * NW + R_CEN] = m.px[p];
+    vrec[(size_t)p * NW + R_CEN + 1] = m.py[p];
+  }
+  std::vector<double> qc((size_t)nop * 4), erec((size_t)nof * 4, 0.0);
+  for (int p = 0; p < nop; ++p)
+    for (int k = 0; k < 4; ++k) qc[(size_t)p * 4 + k] = qinf[k];
+  std::vector<int2> ep(nof);
+  for (int e = 0; e < nof; ++e) {
+    erec[(size_t)e * 4] = m.fnx[e];
+    erec[(size_t)e * 4 + 1] = m.fny[e];
+    erec[(size_t)e * 4 + 2] = m.farea[e];
+    ep[e] = make_int2(m.fp1[e], m.fp2[e]);
+  }
+  std::vector<int> v2f(m.v2f.size());
+  for (int p = 0; p < nop; ++p)
+    for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
+      int e = m.v2f[j];
+      v2f[j] = m.fp1[e] == p ? e : ~e;
+    }
+  std::vector<double> wxy(2 * m.v2v.size());
+  for (size_t j = 0; j < m.v2v.size(); ++j) {
+    wxy[2 * j] = m.wx[j];
+    wxy[2 * j + 1] = m.wy[j];
+  }
+  // LU-SGS level schedule of the sequential sweeps in vertex order
+  {
+    std::vector<int> lf(nop, 0), lb(nop, 0);
+    int nlf = 0, nlb = 0;
+    for (int p = 0; p < nop; ++p) {
+      int l = 0;
+      for (int j = m.v2v_ptr[p]; j < m.v2v_ptr[p + 1]; ++j)
+        if (m.v2v[j] < p) l = std::max(l, lf[m.v2v[j]] + 1);
+      lf[p] = l;
+      nlf = std::max(nlf, l + 1);
+    }
+    for (int p = nop - 1; p >= 0; --p) {
+      int l = 0;
+      for (int j = m.v2v_ptr[p]; j < m.v2v_ptr[p + 1]; ++j)
+        if (m.v2v[j] > p) l = std::max(l, lb[m.v2v[j]] + 1);
+      lb[p] = l;
+      nlb = std::max(nlb, l + 1);
+    }
+    std::vector<int> verts(2 * (size_t)nop);
+    std::vector<int> cf(nlf + 1, 0), cb(nlb + 1, 0);
+    for (int p = 0; p < nop; ++p) { cf[lf[p] + 1]++; cb[lb[p] + 1]++; }
+    for (int l = 0; l < nlf; ++l) cf[l + 1] += cf[l];
+    for (int l = 0; l < nlb; ++l) cb[l + 1] += cb[l];
+    for (int l = 0; l <= nlb; ++l) cb[l] += nop;
+    x->lv_fwd_off = cf;
+    x->lv_bwd_off = cb;
+    std::vector<int> pf(cf.begin(), cf.end() - 1), pb(cb.begin(), cb.end() - 1);
+    for (int p = 0; p < nop; ++p) verts[pf[lf[p]]++] = p;
+    for (int p = nop - 1; p >= 0; --p) verts[pb[lb[p]]++] = p;
+    int rc = nc_upload(x, &x->lv_verts, verts);
+    if (rc) { ufx_nc_destroy(x); return rc; }
+  }
+  DevNC& d = x->d;
+  d.nop = nop;
+  d.nof = nof;
+  int rc = 0;
+#define NUP(dst, src) if (!rc) rc = nc_upload(x, &d.dst, src)
+#define NAL(dst, n) if (!rc) rc = nc_alloc(x, &d.dst, (size_t)(n))
+  NUP(vrec, vrec); NUP(qc, qc); NUP(cv, m.pcv); NUP(erec, erec); NUP(ebc, m.fbc);
+  NUP(v2f_ptr, m.v2f_ptr); NUP(v2f, v2f); NUP(v2v_ptr, m.v2v_ptr); NUP(v2v, m.v2v); NUP(wxy, wxy);
+  if (!rc) {
+    int2* p = nullptr;
+    rc = nc_upload(x, &p, ep);
+    d.ep = p;
+  }
+  NAL(qold, 4 * (size_t)nop); NAL(res, 4 * (size_t)nop); NAL(la, nop); NAL(dt, nop);
+  NAL(dd, nop); NAL(dqc, 4 * (size_t)nop); NAL(fla, nof); NAL(part, 4 * (size_t)x->nblk);
+  NAL(scal, 4); NAL(err, 4);
+#undef NUP
+#undef NAL
+  if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) {
+    set_error("upload failed");
+    rc = UFX_ERR_HIP;
+  }
+  if (rc) { ufx_nc_destroy(x); return rc; }
+  *out = x;
+  return UFX_OK;
+}
+
+int ufx_nc_save_old(ufx_nc* x) {
+  if (int rc = check(x)) return rc;
+  return run_save_old(x);
+}
+
+int ufx_nc_fvresidual(ufx_nc* x, double cfl) {
+  if (int rc = check(x)) return rc;
+  x->cfl = cfl;
+  return run_fvresidual(x, cfl, 0, 0.0, 0.0);
+}
+
+int ufx_nc_rk_stage(ufx_nc* x, int stage, double cfl) {
+  if (int rc = check(x)) return rc;
+  static const double airk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+  static const double birk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+  if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
+  x->cfl = cfl;
+  return run_fvresidual(x, cfl, 1, airk[stage - 1], birk[stage - 1]);
+}
+
+int ufx_nc_lusgs(ufx_nc* x, double cfl) {
+  if (int rc = check(x)) return rc;
+  x->cfl = cfl;
+  return run_lusgs(x, cfl);
+}
+
+int ufx_nc_residue(ufx_nc* x, int iter, double* out) {
+  if (int rc = check(x)) return rc;
+  if (int rc = ensure_hist(x, 1)) return rc;
+  if (int rc = run_norm(x, iter, 0)) return rc;
+  double h[5];
+  NCCHK(hipMemcpyAsync(h, x->d_hist, sizeof(h), hipMemcpyDeviceToHost, x->stream));
+  NCCHK(hipStreamSynchronize(x->stream));
+  if (out) std::memcpy(out, h, sizeof(h));
+  return UFX_OK;
+}
+
+int ufx_nc_iterate(ufx_nc* x, int iter0, int niter, double* hist) {
+  if (int rc = check(x)) return rc;
+  if (niter <= 0) return UFX_OK;
+  if (int rc = ensure_hist(x, niter)) return rc;
+  for (int k = 0; k < niter; ++k) {
+    int iter = iter0 + k + 1;
+    double cfl = nc_cfl(x->prm.cfl_max, iter);
+    x->cfl = cfl;
+    if (int rc = run_save_old(x)) return rc;
+    if (x->prm.timemode == UFX_TIME_RK3) {
+      for (int s = 1; s <= 3; ++s)
+        if (int rc = ufx_nc_rk_stage(x, s, cfl)) return rc;
+    } else {
+      if (int rc = run_lusgs(x, cfl)) return rc;
+    }
+    if (int rc = run_norm(x, iter, k)) return rc;
+  }
+  if (hist) {
+    NCCHK(hipMemcpyAsync(hist, x->d_hist, sizeof(double) * 5 * (size_t)niter,
+                         hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+  }
+  return UFX_OK;
+}
+
+int ufx_nc_set(ufx_nc* x, const char* name, const double* src) {
+  if (int rc = check(x)) return rc;
+  if (!name || !src) { set_error("ufx_nc_set: null argument"); return UFX_ERR_ARG; }
+  std::string n(name);
+  const int nop = x->nop;
+  if (n == "qc" || n == "qold") {
+    NCCHK(hipMemcpyAsync(n == "qc" ? x->d.qc : x->d.qold, src, sizeof(double) * 4 * nop,
+                         hipMemcpyHostToDevice, x->stream));
+  } else if (n == "phi") {
+    std::vector<double> v((size_t)nop * NW);
+    NCCHK(hipMemcpyAsync(v.data(), x->d.vrec, v.size() * 8, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+    for (int p = 0; p < nop; ++p)
+      for (int k = 0; k < 4; ++k) v[(size_t)p * NW + R_PHI + k] = src[p * 4 + k];
+    NCCHK(hipMemcpyAsync(x->d.vrec, v.data(), v.size() * 8, hipMemcpyHostToDevice, x->stream));
+  } else {
+    set_error("ufx_nc_set: unknown or read-only array " + n);
+    return UFX_ERR_ARG;
+  }
+  NCCHK(hipStreamSynchronize(x->stream));
+  return UFX_OK;
+}
+
+int ufx_nc_get(ufx_nc* x, const char* name, double* dst) {
+  if (int rc = check(x)) return rc;
+  if (!name || !dst) { set_error("ufx_nc_get: null argument"); return UFX_ERR_ARG; }
+  std::string n(name);
+  const int nop = x->nop;
+  auto copy = [&](const double* src, size_t cnt) -> int {
+    NCCHK(hipMemcpyAsync(dst, src, cnt * 8, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  };
+  if (n == "qc") return copy(x->d.qc, 4 * (size_t)nop);
+  if (n == "qold") return copy(x->d.qold, 4 * (size_t)nop);
+  if (n == "res") return copy(x->d.res, 4 * (size_t)nop);
+  if (n == "la") return copy(x->d.la, nop);
+  if (n == "dt") return copy(x->d.dt, nop);
+  if (n == "fc_la") return copy(x->d.fla, x->nof);
+  if (n == "qp" || n == "grad" || n == "phi") {
+    std::vector<double> v((size_t)nop * NW);
+    NCCHK(hipMemcpyAsync(v.data(), x->d.vrec, v.size() * 8, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+    int off = n == "qp" ? R_QP : n == "grad" ? R_G : R_PHI, w = n == "grad" ? 8 : 4;
+    for (int p = 0; p < nop; ++p)
+      for (int k = 0; k < w; ++k) dst[(size_t)p * w + k] = v[(size_t)p * NW + off + k];
+    return UFX_OK;
+  }
+  set_error("ufx_nc_get: unknown array " + n);
+  return UFX_ERR_ARG;
+}
+
+int ufx_nc_sync(ufx_nc* x) {
+  if (int rc = check(x)) return rc;
+  NCCHK(hipStreamSynchronize(x->stream));
+  return UFX_OK;
+}
+
+int ufx_nc_profile(ufx_nc* x, int enable) {
+  if (int rc = check(x)) return rc;
+  NCCHK(hipStreamSynchronize(x->stream));
+  x->prof = enable != 0;
+  x->ev_used = 0;
+  return UFX_OK;
+}
+
+int ufx_nc_profile_read(ufx_nc* x, double* ms, int* count) {
+  if (int rc = check(x)) return rc;
+  NCCHK(hipStreamSynchronize(x->stream));
+  for (int c = 0; c < NK_NCAT; ++c) { ms[c] = 0.0; count[c] = 0; }
+  for (size_t i = 0; i + 1 < x->ev_used; ++i) {
+    int c = x->ev_cat[i];
+    if (c < 0) continue;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, x->ev[i], x->ev[i + 1]) == hipSuccess) {
+      ms[c] += t;
+      count[c] += 1;
+    }
+  }
+  x->ev_used = 0;
+  return UFX_OK;
+}
+
+}  // extern "C"
