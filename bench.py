@@ -55,6 +55,26 @@ def iteration_bytes(noc: int, nof: int, nop: int, nv2c: int, w: int, d: int = 8,
     return p1 + p2 + p3 + p4 + 3 * stage_bytes(noc, nof, w, d, i)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_bump2M_hbm_traffic.md")
+
+
+def measured_traffic(precision: str) -> int | None:
+    """HBM bytes per launch of the stage kernel, from the committed rocprofv3
+    --pmc summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
+    same counter reads for a plain copy of known size): lines
+    ``traffic <precision> <kernel> <bytes>``.  None when not measured."""
+    import re
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                m = re.match(r"^traffic (\S+) (\S+) (\d+)\s*$", line.strip())
+                if m and m.group(1) == precision:
+                    return int(m.group(3))
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(budget_s: float = 20.0) -> dict | None:
     """Time the reference solver (oracle/_ref, built from the reference's own
     sources) on a bounded sample of the workload: the same generator, flow
@@ -201,7 +221,8 @@ def main() -> int:
                                f"RK3, CFL 0.8 local time step, {args.precision}",
                    "cells": noc, "faces": nof, "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.precision),
                      "kernel": "k_stage", "bytes_per_launch": int(sb),
                      "avg_launch_ms": round(st_avg, 5)},
         "cpu_baseline": cpu,
