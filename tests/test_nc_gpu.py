@@ -63,7 +63,12 @@ def test_nc_routines_match_oracle(grad, timemode):
         h = dev.residue(it)
         orc.residue(it)
         _check("fres", h[[0, 2]], [orc.s.fres, orc.s.fresi], 1e-12)
-        assert int(h[3]) == orc.s.iresi
+        # the vertex of the largest |res| (mirror-symmetric vertices can tie
+        # in exact arithmetic, so last-bit differences may pick the other
+        # one): it must attain the maximum
+        r = dev.get("res")
+        i = int(h[3]) - 1
+        assert abs(np.sqrt(np.sum(r[i] ** 2)) - orc.s.fresi) <= 1e-12 * orc.s.fresi
 
 
 @pytest.mark.parametrize("grad,timemode", [("gg", "lusgs"), ("lsqr", "rk3")])

@@ -142,9 +142,20 @@ __global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
   if (p >= d.nop || d.err[0]) return;
   double w = 0.0, con[6] = {0, 0, 0, 0, 0, 0};
   int j0 = d.v2c_ptr[p], j1 = d.v2c_ptr[p + 1];
-  for (int j = j0; j < j1; ++j) {
-    int c = ld_nt(d.v2c + j);
-    double cw = ld_nt(d.wt + j);
+  // the cell list first (independent loads), then the gathers, so the
+  // latency of the list is paid once; a tail loop takes vertices with more
+  // than AVG_UNR cells
+  constexpr int AVG_UNR = 8;
+  int cs[AVG_UNR];
+  double ws[AVG_UNR];
+  const int n = j1 - j0;
+#pragma unroll
+  for (int k = 0; k < AVG_UNR; ++k)
+    if (k < n) {
+      cs[k] = ld_nt(d.v2c + j0 + k);
+      ws[k] = ld_nt(d.wt + j0 + k);
+    }
+  auto add = [&](int c, double cw) {
     R q[4], t, mu = R(0);
     unsigned l = (unsigned)(c - w0);
     if (l < (unsigned)wn) {
@@ -161,7 +172,11 @@ __global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
     for (int k = 0; k < 4; ++k) con[k] = con[k] + (double)q[k] * cw;
     con[4] = con[4] + (double)t * cw;
     if (VISC) con[5] = con[5] + (double)mu * cw;
-  }
+  };
+#pragma unroll
+  for (int k = 0; k < AVG_UNR; ++k)
+    if (k < n) add(cs[k], ws[k]);
+  for (int j = j0 + AVG_UNR; j < j1; ++j) add(ld_nt(d.v2c + j), ld_nt(d.wt + j));
   R o[PW];
 #pragma unroll
   for (int k = 0; k < 5; ++k) o[k] = (R)(con[k] / w);
@@ -228,11 +243,19 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
   double la = 0.0;
   R r11 = 0, r12 = 0, r22 = 0;
   if (GRAD == G_LSQR) { r11 = (R)d.r11[c]; r12 = (R)d.r12[c]; r22 = (R)d.r22[c]; }
+  // slot lists first (independent loads), then the gathers
+  int sf[MAXF], sn[MAXF];
+#pragma unroll
+  for (int s = 0; s < MAXF; ++s)
+    if (s < nf) {
+      sf[s] = ld_nt(d.sl_face + s * noc + c);
+      sn[s] = ld_nt(d.sl_nbr + s * noc + c);
+    }
 #pragma unroll
   for (int s = 0; s < MAXF; ++s) {
     if (s >= nf) break;
-    int f = ld_nt(d.sl_face + s * noc + c);
-    int n = ld_nt(d.sl_nbr + s * noc + c);
+    int f = sf[s];
+    int n = sn[s];
     bool in_side = n >= 0;
     n = in_side ? n : ~n;
     bool bnd = n >= noc;
@@ -587,11 +610,18 @@ __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, d
     }
     // interior faces, CCFVM/fvresidual.F90:133-171
     const R zhi = R(1) / R(3);
+    int sf[MAXF], sn[MAXF];
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s)
+      if (s < nf - nb) {
+        sf[s] = ld_nt(d.sl_face + s * noc + c);
+        sn[s] = ld_nt(d.sl_nbr + s * noc + c);
+      }
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
       if (s >= nf - nb) break;
-      int f = ld_nt(d.sl_face + s * noc + c);
-      int n = ld_nt(d.sl_nbr + s * noc + c);
+      int f = sf[s];
+      int n = sn[s];
       bool in_side = n >= 0;
       n = in_side ? n : ~n;
       R nr[RW], fr[FW];
@@ -717,10 +747,14 @@ __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mod
       ld4d(d.bres + (size_t)k * 4, F);
       for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i];
     }
+    int sl[MAXF];
+#pragma unroll
+    for (int s = 0; s < MAXF; ++s)
+      if (s < nf - nb) sl[s] = ld_nt(d.sl_loc + s * noc + c);
 #pragma unroll
     for (int s = 0; s < MAXF; ++s) {
       if (s >= nf - nb) break;
-      int l = ld_nt(d.sl_loc + s * noc + c);
+      int l = sl[s];
       bool in_side = l >= 0;
       l = in_side ? l : ~l;
       if (in_side) {
