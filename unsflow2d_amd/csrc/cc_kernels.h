@@ -41,7 +41,7 @@ __global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, d
   double a0 = sqrt(g.gamma * p0 / r0);
   double un = u0 * (double)fr[F_NX] + v0 * (double)fr[F_NY];
   double ll = (double)fr[F_AREA] * (fabs(un) + a0);
-  ll = ll + vll * d.fmu[f] / r0;
+  if (vll != 0.0) ll = ll + vll * d.fmu[f] / r0;
   d.fla[f] = ll;
 }
 
@@ -86,12 +86,31 @@ __global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_p
     for (int k = 0; k < 4; ++k) mx[k] = mn[k] = (double)q4[k];
   }
   if (!do_range) return;
-  block_reduce<4>(sh, mx, OpMax());
-  block_reduce<4>(sh, mn, OpMin());
+  // wave-level max/min by lane shuffles, then across the block's waves
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double a = __shfl_xor(mx[k], off), b = __shfl_xor(mn[k], off);
+      mx[k] = a > mx[k] ? a : mx[k];
+      mn[k] = b < mn[k] ? b : mn[k];
+    }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) {
+      sh[k][wave] = mx[k];
+      sh[k][BLK / 64 + wave] = mn[k];
+    }
+  __syncthreads();
   if (threadIdx.x == 0)
     for (int k = 0; k < 4; ++k) {
-      d.part[(size_t)blockIdx.x * 8 + k] = mx[k];
-      d.part[(size_t)blockIdx.x * 8 + 4 + k] = mn[k];
+      double a = sh[k][0], b = sh[k][BLK / 64];
+      for (int w = 1; w < BLK / 64; ++w) {
+        a = sh[k][w] > a ? sh[k][w] : a;
+        b = sh[k][BLK / 64 + w] < b ? sh[k][BLK / 64 + w] : b;
+      }
+      d.part[(size_t)blockIdx.x * 8 + k] = a;
+      d.part[(size_t)blockIdx.x * 8 + 4 + k] = b;
     }
 }
 

@@ -945,9 +945,10 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   p.precision = prm->precision; p.cfl_max = prm->cfl_max;
   p.vll = 0.0;
   if (p.flow != UFX_FLOW_INVISCID) p.vll = 2.0 * p.gamma / (0.72 * p.rey * 0.25);
-  x->g = Gas<double>{p.gamma, p.gamma1, p.znd, p.r_inf, p.u_inf, p.v_inf, p.p_inf, p.mref};
+  x->g = Gas<double>{p.gamma, p.gamma1, p.znd, p.r_inf, p.u_inf, p.v_inf, p.p_inf, p.mref,
+                     1.0 / p.gamma1};
   x->gf = Gas<float>{(float)p.gamma, (float)p.gamma1, (float)p.znd, (float)p.r_inf,
-                     (float)p.u_inf, (float)p.v_inf, (float)p.p_inf, (float)p.mref};
+                     (float)p.u_inf, (float)p.v_inf, (float)p.p_inf, (float)p.mref, (float)(1.0 / p.gamma1)};
   // residual stage form (measured on MI355X at 2M cells): FP64, the
   // per-cell gather with the block's records staged in LDS; mixed, the
   // shared-flux kernel reading records through L2 (its 80-byte records cache
