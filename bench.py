@@ -136,6 +136,101 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def nc_cpu_baseline(budget_s: float = 20.0) -> dict | None:
+    """The reference vertex-centred solver (oracle/_ref/nc_fast) on a bounded
+    sample of the vertex-centred workload: the same generator at 200 x 100
+    points (20 000 vertices), LU-SGS, GG, single thread."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "nc_fast", "ref_nc_driver")
+    if not os.path.exists(exe):
+        return None
+    from unsflow2d_amd import nc_mesh
+    m = nc_mesh.o_grid(200, 100, 3, stretch=1.005 ** (999 / 99))
+    env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+    with tempfile.TemporaryDirectory() as tmp:
+        nc_mesh.write_geometry_inp(m, os.path.join(tmp, "geometry.inp"))
+        with open(os.path.join(tmp, "flo.inp"), "w") as f:
+            f.write(nc_mesh.NCDeck(mach=0.4, cfl_max=50.0, timemode="lusgs").text())
+        warm = 3
+        out = subprocess.run([exe, "flo.inp", ".", str(warm + 10), "0", "gg", str(warm)], cwd=tmp,
+                             capture_output=True, text=True, timeout=600, env=env)
+        ms = float(out.stdout.split("ms_per_iteration")[-1].split()[0])
+        nit = max(10, int(budget_s * 1000.0 / ms))
+        out = subprocess.run([exe, "flo.inp", ".", str(warm + nit), "0", "gg", str(warm)], cwd=tmp,
+                             capture_output=True, text=True, timeout=1200, env=env)
+        ms = float(out.stdout.split("ms_per_iteration")[-1].split()[0])
+    value = m.nop / (ms / 1000.0) / 1e6
+    return {"value": round(value, 4), "unit": "M vertex-iterations/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"reference NCFVM (amdflang -O3 -march=x86-64-v3), O-grid 200x100 points "
+                      f"({m.nop} vertices), M=0.4, GG + limiter, van Leer, LU-SGS, {nit} timed "
+                      f"iterations after {warm} warm-up, {ms:.2f} ms/iteration, 1 thread"}
+
+
+def nc_bytes(nop: int, nof: int, w: int = 8, d: int = 8, i: int = 4) -> float:
+    """Algorithmic bytes of one residual pass of the vertex-centred solver:
+    edge connectivity and geometry once per edge, each vertex's record
+    (primitives, gradient, limiter, coordinates), dual volume, and the
+    written residual, spectral radius and time step."""
+    return nof * (2 * i + 3 * w + w) + nop * (4 * w + 8 * w + 4 * w + 2 * w + w + 4 * d + 2 * d)
+
+
+def main_nc(args) -> int:
+    from unsflow2d_amd import nc, nc_mesh
+    t0 = time.time()
+    raw = nc_mesh.config3_mesh()
+    mesh = nc.NCMesh.from_raw(raw)
+    cnt = mesh.counts
+    solver = nc.NCSolver(mesh, mach=0.4, cfl_max=50.0, grad="gg", timemode="lusgs")
+    print(f"# setup {time.time() - t0:.1f}s: {cnt['nop']} vertices, {cnt['nof']} edges",
+          file=sys.stderr, flush=True)
+    solver.iterate(args.warmup, hist=False)
+    solver.sync()
+    t1 = time.perf_counter()
+    solver.iterate(args.steps, hist=False)
+    solver.sync()
+    wall = time.perf_counter() - t1
+    # per-kernel times from one more iteration with HIP events around every
+    # launch (kept out of the timed region: the LU-SGS sweeps are thousands
+    # of small launches)
+    solver.profile(True)
+    h = solver.iterate(1)
+    prof = solver.profile_read()
+    solver.profile(False)
+    prof = {k: (v[0] * args.steps, v[1] * args.steps) for k, v in prof.items()}
+    if not np.all(np.isfinite(h)):
+        print("# residual is not finite", file=sys.stderr)
+        return 1
+    nop, nof = cnt["nop"], cnt["nof"]
+    ms_step = wall * 1000.0 / args.steps
+    kern = {k: round(v[0] / max(1, v[1]), 4) for k, v in prof.items() if v[1]}
+    print(f"# kernel avg ms: {kern}; LU-SGS total per iteration "
+          f"{prof['lusgs'][0] / args.steps:.3f} ms; iteration {ms_step:.3f} ms",
+          file=sys.stderr, flush=True)
+    r_ms, r_n = prof["residual"]
+    r_avg = r_ms / max(1, r_n)
+    rb = nc_bytes(nop, nof)
+    achieved = rb / (r_avg / 1000.0) / 1e9
+    cpu = None if args.no_cpu else nc_cpu_baseline(args.cpu_budget)
+    line = {
+        "metric": "M vertex-iterations/s @2M vertices, NCFVM LU-SGS",
+        "value": round(nop * args.steps / wall / 1e6, 3),
+        "unit": "M vertex-iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"NCFVM median dual, seeded O-grid 2000x1000 points around the "
+                               f"unit circle ({nop} vertices, {nof} edges), M=0.4, GG + "
+                               f"Venkatakrishnan, van Leer, LU-SGS (level-scheduled, reference "
+                               f"order), tanh CFL ramp to 50", "vertices": nop, "edges": nof},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_nc_residual", "bytes_per_launch": int(rb),
+                     "avg_launch_ms": round(r_avg, 5)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,9 +242,16 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
+    ap.add_argument("--workload", choices=["cc", "nc"], default="cc",
+                    help="cc: the cell-centred headline workload; nc: the vertex-centred one")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.workload == "nc":
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            return 2
+        return main_nc(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
               "shards by mesh partition, which this build does not implement)", file=sys.stderr)

@@ -83,20 +83,18 @@ def o_grid(ni: int, nj: int, seed: int, *, stretch: float = 1.005, outer: float 
     # exact circle and square on the boundary rings
     xs[:ni], ys[:ni] = c, s
     xs[(nj - 1) * ni:], ys[(nj - 1) * ni:] = sq * c, sq * s
-    pid = lambda jj, ii: jj * ni + (ii % ni)
-    tris = []
     diag = rng.integers(0, 2, size=(nj - 1, ni))
-    for jj in range(nj - 1):
-        for ii in range(ni):
-            a, b = pid(jj, ii), pid(jj, ii + 1)
-            cc, d = pid(jj + 1, ii + 1), pid(jj + 1, ii)
-            # (a, b, cc, d) runs clockwise (b follows a counter-clockwise
-            # around the ring, d lies outward of a): triangles reversed
-            if diag[jj, ii]:
-                tris.append((a, cc, b)); tris.append((a, d, cc))
-            else:
-                tris.append((a, d, b)); tris.append((b, d, cc))
-    tri = np.array(tris, dtype=np.int64)
+    jj, ii = np.meshgrid(np.arange(nj - 1), np.arange(ni), indexing="ij")
+    a = jj * ni + ii
+    b = jj * ni + (ii + 1) % ni
+    cc = (jj + 1) * ni + (ii + 1) % ni
+    d = (jj + 1) * ni + ii
+    # (a, b, cc, d) runs clockwise (b follows a counter-clockwise around the
+    # ring, d lies outward of a): triangles reversed; quads in ring-major
+    # order, each split along its seeded diagonal
+    t1 = np.where(diag[..., None] == 1, np.stack([a, cc, b], -1), np.stack([a, d, b], -1))
+    t2 = np.where(diag[..., None] == 1, np.stack([a, d, cc], -1), np.stack([b, d, cc], -1))
+    tri = np.stack([t1, t2], axis=2).reshape(-1, 3).astype(np.int64)
     pbc = np.zeros(nj * ni, dtype=np.int32)
     pbc[:ni] = WALL
     pbc[(nj - 1) * ni:] = FARFIELD
