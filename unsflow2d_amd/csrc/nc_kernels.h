@@ -12,6 +12,7 @@
 //   la, dt, cv [nop] f64;  fla [nof] f64
 //   erec [nof][4] f64   dual-face nx, ny, length, pad;  ep [nof] int2 (p1, p2)
 //   v2f [..] int        edges of a vertex, bit-inverted when it is p2
+//   v2fo [..] int2      the same with the vertex at the other end
 //   v2v [..] int, wxy [..][2] f64   least-squares neighbours and weights
 #pragma once
 
@@ -37,6 +38,7 @@ struct DevNC {
   const int* ebc;
   double* fla;
   const int *v2f_ptr, *v2f;
+  const int2* v2fo;          // per v2f entry: (edge, bit-inverted when the vertex is p2; other end)
   const int *v2v_ptr, *v2v;
   const double* wxy;
   double* part;  // reduction partials [nblk][4]
@@ -73,6 +75,17 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
 // (starting at -+eps of single precision), the Venkatakrishnan function
 // with eps^2 = (kappa sqrt(V))^3, kappa = 0.01 in single precision, per edge
 // end, ratcheted into the stored limiter value.
+// Up to NCU edges of a vertex are fetched ahead of their gathers (the rest,
+// rare, in a tail loop): the gathers then do not wait on the index chain.
+constexpr int NCU = 8;
+template <typename F>
+__device__ __forceinline__ void for_edges(const DevNC& d, int j0, int n, const int2* nb, F&& f) {
+#pragma unroll
+  for (int k = 0; k < NCU; ++k)
+    if (k < n) f(nb[k]);
+  for (int k = NCU; k < n; ++k) f(d.v2fo[j0 + k]);
+}
+
 template <int GRAD>
 __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
   int p = cell_index();
@@ -82,14 +95,15 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
   ld4d(own + R_QP, q);
   ld2d(own + R_CEN, x, y);
   double gr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int j0 = d.v2f_ptr[p], j1 = d.v2f_ptr[p + 1];
+  const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
+  int2 nb[NCU];
+#pragma unroll
+  for (int k = 0; k < NCU; ++k)
+    if (k < n) nb[k] = d.v2fo[j0 + k];
   if (GRAD == 1) {
-    for (int j = j0; j < j1; ++j) {
-      int e = d.v2f[j];
-      bool first = e >= 0;
-      e = first ? e : ~e;
-      int2 pp = d.ep[e];
-      int o = first ? pp.y : pp.x;
+    for_edges(d, j0, n, nb, [&](int2 eo) {
+      bool first = eo.x >= 0;
+      int e = first ? eo.x : ~eo.x, o = eo.y;
       double qo[4];
       ld4d(d.vrec + (size_t)o * NW + R_QP, qo);
       double nx, ny, ds, pad;
@@ -107,7 +121,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
           gr[2 * k + 1] = gr[2 * k + 1] - var * dy;
         }
       }
-    }
+    });
     double v = d.cv[p];
 #pragma unroll
     for (int k = 0; k < 8; ++k) gr[k] = gr[k] / v;
@@ -128,34 +142,26 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
   double dumax[4], dumin[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) { dumin[k] = NC_EPS_SP; dumax[k] = -NC_EPS_SP; }
-  for (int j = j0; j < j1; ++j) {
-    int e = d.v2f[j];
-    bool first = e >= 0;
-    e = first ? e : ~e;
-    int2 pp = d.ep[e];
-    int o = first ? pp.y : pp.x;
+  for_edges(d, j0, n, nb, [&](int2 eo) {
+    bool first = eo.x >= 0;
     double qo[4];
-    ld4d(d.vrec + (size_t)o * NW + R_QP, qo);
+    ld4d(d.vrec + (size_t)eo.y * NW + R_QP, qo);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       double du = first ? qo[k] - q[k] : -(q[k] - qo[k]);
       dumax[k] = dumax[k] > du ? dumax[k] : du;
       dumin[k] = dumin[k] < du ? dumin[k] : du;
     }
-  }
+  });
   double phi[4];
   ld4d(own + R_PHI, phi);
   const double kappa = (double)0.01f;
   double t = kappa * sqrt(d.cv[p]);
   const double tol = t * t * t;
-  for (int j = j0; j < j1; ++j) {
-    int e = d.v2f[j];
-    bool first = e >= 0;
-    e = first ? e : ~e;
-    int2 pp = d.ep[e];
-    int o = first ? pp.y : pp.x;
+  for_edges(d, j0, n, nb, [&](int2 eo) {
+    bool first = eo.x >= 0;
     double xo, yo;
-    ld2d(d.vrec + (size_t)o * NW + R_CEN, xo, yo);
+    ld2d(d.vrec + (size_t)eo.y * NW + R_CEN, xo, yo);
     double d0 = first ? xo - x : x - xo, d1 = first ? yo - y : y - yo;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
       ph = 1.0 < ph ? 1.0 : ph;
       phi[k] = phi[k] < ph ? phi[k] : ph;
     }
-  }
+  });
   double* w = d.vrec + (size_t)p * NW;
   st4d(w + R_G, gr);
   st4d(w + R_G + 4, gr + 4);
@@ -221,15 +227,23 @@ __global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, double c
   int p = cell_index();
   if (p >= d.nop) return;
   double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0;
-  const int j0 = d.v2f_ptr[p], j1 = d.v2f_ptr[p + 1];
-  for (int j = j0; j < j1; ++j) {
-    int e = d.v2f[j];
-    bool first = e >= 0;
-    e = first ? e : ~e;
-    int2 pp = d.ep[e];
-    double a[NW], b[NW];
-    ldv<double, NW>(d.vrec + (size_t)pp.x * NW, a);
-    ldv<double, NW>(d.vrec + (size_t)pp.y * NW, b);
+  const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
+  int2 nb[NCU];
+#pragma unroll
+  for (int k = 0; k < NCU; ++k)
+    if (k < n) nb[k] = d.v2fo[j0 + k];
+  double me[NW];
+  ldv<double, NW>(d.vrec + (size_t)p * NW, me);
+  for_edges(d, j0, n, nb, [&](int2 eo) {
+    bool first = eo.x >= 0;
+    int e = first ? eo.x : ~eo.x;
+    double ot[NW], a[NW], b[NW];
+    ldv<double, NW>(d.vrec + (size_t)eo.y * NW, ot);
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      a[i] = first ? me[i] : ot[i];
+      b[i] = first ? ot[i] : me[i];
+    }
     double nx, ny, ds, pad;
     ld2d(d.erec + (size_t)e * 4, nx, ny);
     ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
@@ -307,7 +321,7 @@ __global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, double c
 #pragma unroll
       for (int i = 0; i < 4; ++i) r[i] = r[i] + dr * fl[i];
     }
-  }
+  });
   double cvp = d.cv[p];
   double dt = cfl * cvp / la;
   d.la[p] = la;
@@ -446,22 +460,16 @@ __device__ __forceinline__ void nc_normalflux(const NCFree& g, const double* q, 
   for (int i = 0; i < 4; ++i) f[i] = f[i] * ds;
 }
 
-// One level of the forward (dir 0) or backward (dir 1) sweep: the vertices
-// of a level depend only on vertices of earlier levels, so the level's
-// vertices update concurrently and the sweep equals the reference's
-// sequential one.
-__global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const int* verts,
-                                                        int n, int dir) {
-  int t = blockIdx.x * BLK + threadIdx.x;
-  if (t >= n) return;
-  int it = verts[t];
+// The update of one vertex in the forward (dir 0) or backward (dir 1)
+// sweep, NCFVM/lusgs.f90:46-122: the off-diagonal sum over the neighbours
+// already swept (lower-numbered forward, higher-numbered backward), then
+// dq and qc = qold + dq.
+__device__ __forceinline__ void nc_lusgs_vertex(const DevNC& d, const NCFree& g, int it, int dir) {
   double cres[4] = {0.0, 0.0, 0.0, 0.0};
   for (int j = d.v2f_ptr[it]; j < d.v2f_ptr[it + 1]; ++j) {
-    int e = d.v2f[j];
-    bool first = e >= 0;
-    e = first ? e : ~e;
-    int2 pp = d.ep[e];
-    int o = first ? pp.y : pp.x;
+    int2 eo = d.v2fo[j];
+    bool first = eo.x >= 0;
+    int e = first ? eo.x : ~eo.x, o = eo.y;
     double sign = first ? 1.0 : -1.0;
     if (dir == 0 ? o < it : o > it) {
       double nx, ny, ds, pad, f1[4], f2[4], qo[4], qn[4], dq[4];
@@ -492,6 +500,31 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
   for (int i = 0; i < 4; ++i) q[i] = qo[i] + dq[i];
   st4d(d.dqc + (size_t)it * 4, dq);
   st4d(d.qc + (size_t)it * 4, q);
+}
+
+// One level of a sweep as its own launch: the vertices of a level depend
+// only on vertices of earlier levels, so they update concurrently and the
+// sweep equals the reference's sequential one.
+__global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const int* verts,
+                                                        int n, int dir) {
+  int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= n) return;
+  nc_lusgs_vertex(d, g, verts[t], dir);
+}
+
+// A whole sweep in one workgroup: the levels in order, a workgroup barrier
+// between consecutive levels (which also orders the global-memory updates
+// among the workgroup's waves).  Every loop is bounded by the level lists.
+constexpr int SWEEP_THREADS = 1024;
+__global__ __launch_bounds__(SWEEP_THREADS) void k_nc_lusgs_sweep(DevNC d, NCFree g,
+                                                                  const int* verts,
+                                                                  const int* off, int nlev,
+                                                                  int dir) {
+  for (int l = 0; l < nlev; ++l) {
+    const int b = off[l], n = off[l + 1] - b;
+    for (int t = threadIdx.x; t < n; t += SWEEP_THREADS) nc_lusgs_vertex(d, g, verts[b + t], dir);
+    __syncthreads();
+  }
 }
 
 }  // namespace ufx

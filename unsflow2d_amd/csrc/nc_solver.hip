@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -36,6 +37,8 @@ struct ufx_nc {
   int hist_cap = 0;
   int* lv_verts = nullptr;                 // forward levels then backward levels
   std::vector<int> lv_fwd_off, lv_bwd_off;  // offsets into lv_verts
+  int *d_fwd_off = nullptr, *d_bwd_off = nullptr;  // the same on the device
+  bool sweep_single = true;                 // whole sweeps in one workgroup
   double cfl = 0.0;
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -129,6 +132,15 @@ int run_lusgs(ufx_nc* x, double cfl) {
   NLAUNCH(x, NK_LUSGS, k_nc_lusgs_diag, x->nblk, x->d);
   for (int dir = 0; dir < 2; ++dir) {
     const std::vector<int>& off = dir == 0 ? x->lv_fwd_off : x->lv_bwd_off;
+    if (x->sweep_single) {
+      // one workgroup walks all levels (level offsets relative to lv_verts)
+      const int* doff = dir == 0 ? x->d_fwd_off : x->d_bwd_off;
+      prof_mark(x, NK_LUSGS);
+      k_nc_lusgs_sweep<<<1, SWEEP_THREADS, 0, x->stream>>>(x->d, x->g, x->lv_verts, doff,
+                                                          (int)off.size() - 1, dir);
+      prof_mark(x, -1);
+      continue;
+    }
     for (size_t l = 0; l + 1 < off.size(); ++l) {
       int n = off[l + 1] - off[l];
       if (n <= 0) continue;
@@ -245,10 +257,13 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     ep[e] = make_int2(m.fp1[e], m.fp2[e]);
   }
   std::vector<int> v2f(m.v2f.size());
+  std::vector<int2> v2fo(m.v2f.size());
   for (int p = 0; p < nop; ++p)
     for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
       int e = m.v2f[j];
-      v2f[j] = m.fp1[e] == p ? e : ~e;
+      bool first = m.fp1[e] == p;
+      v2f[j] = first ? e : ~e;
+      v2fo[j] = make_int2(v2f[j], first ? m.fp2[e] : m.fp1[e]);
     }
   std::vector<double> wxy(2 * m.v2v.size());
   for (size_t j = 0; j < m.v2v.size(); ++j) {
@@ -285,7 +300,12 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     for (int p = 0; p < nop; ++p) verts[pf[lf[p]]++] = p;
     for (int p = nop - 1; p >= 0; --p) verts[pb[lb[p]]++] = p;
     int rc = nc_upload(x, &x->lv_verts, verts);
+    if (!rc) rc = nc_upload(x, &x->d_fwd_off, x->lv_fwd_off);
+    if (!rc) rc = nc_upload(x, &x->d_bwd_off, x->lv_bwd_off);
     if (rc) { ufx_nc_destroy(x); return rc; }
+    // UFX_NC_SWEEP=levels launches one kernel per level instead
+    const char* env = std::getenv("UFX_NC_SWEEP");
+    x->sweep_single = !(env && std::strcmp(env, "levels") == 0);
   }
   DevNC& d = x->d;
   d.nop = nop;
@@ -299,6 +319,11 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     int2* p = nullptr;
     rc = nc_upload(x, &p, ep);
     d.ep = p;
+  }
+  if (!rc) {
+    int2* p = nullptr;
+    rc = nc_upload(x, &p, v2fo);
+    d.v2fo = p;
   }
   NAL(qold, 4 * (size_t)nop); NAL(res, 4 * (size_t)nop); NAL(la, nop); NAL(dt, nop);
   NAL(dd, nop); NAL(dqc, 4 * (size_t)nop); NAL(fla, nof); NAL(part, 4 * (size_t)x->nblk);
