@@ -466,8 +466,12 @@ __device__ __forceinline__ void nc_normalflux(const NCFree& g, const double* q, 
 // dq and qc = qold + dq.
 __device__ __forceinline__ void nc_lusgs_vertex(const DevNC& d, const NCFree& g, int it, int dir) {
   double cres[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = d.v2f_ptr[it]; j < d.v2f_ptr[it + 1]; ++j) {
-    int2 eo = d.v2fo[j];
+  const int j0 = d.v2f_ptr[it], n = d.v2f_ptr[it + 1] - j0;
+  int2 nb[NCU];
+#pragma unroll
+  for (int k = 0; k < NCU; ++k)
+    if (k < n) nb[k] = d.v2fo[j0 + k];
+  for_edges(d, j0, n, nb, [&](int2 eo) {
     bool first = eo.x >= 0;
     int e = first ? eo.x : ~eo.x, o = eo.y;
     double sign = first ? 1.0 : -1.0;
@@ -484,7 +488,7 @@ __device__ __forceinline__ void nc_lusgs_vertex(const DevNC& d, const NCFree& g,
 #pragma unroll
       for (int i = 0; i < 4; ++i) cres[i] = cres[i] + 0.5 * (sign * (f2[i] - f1[i]) - 1.5 * lam * dq[i]);
     }
-  }
+  });
   double dd = d.dd[it], res[4], dq[4], qo[4], q[4];
   ld4d(d.res + (size_t)it * 4, res);
   ld4d(d.qold + (size_t)it * 4, qo);
