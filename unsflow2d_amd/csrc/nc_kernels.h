@@ -519,14 +519,77 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
 // A whole sweep in one workgroup: the levels in order, a workgroup barrier
 // between consecutive levels (which also orders the global-memory updates
 // among the workgroup's waves).  Every loop is bounded by the level lists.
-constexpr int SWEEP_THREADS = 1024;
+// Each level slot is packed: its vertex, the count of neighbours already
+// swept and up to SWQ (edge, neighbour) pairs in edge order, so a level costs
+// one dependent load; the next level's slots are fetched before the barrier.
+constexpr int SWEEP_THREADS = 512;
+constexpr int SWQ = 8;
+struct SweepSlot {
+  int vert, cnt;
+  int2 eo[SWQ];
+};
+
+__device__ __forceinline__ void nc_sweep_slot(const DevNC& d, const NCFree& g, const SweepSlot& sl,
+                                              int dir) {
+  const int it = sl.vert;
+  double cres[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < SWQ; ++k) {
+    if (k < sl.cnt) {
+      int2 eo = sl.eo[k];
+      bool first = eo.x >= 0;
+      int e = first ? eo.x : ~eo.x, o = eo.y;
+      double sign = first ? 1.0 : -1.0;
+      double nx, ny, ds, pad, f1[4], f2[4], qo[4], qn[4], dq[4];
+      ld2d(d.erec + (size_t)e * 4, nx, ny);
+      ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
+      ld4d(d.qold + (size_t)o * 4, qo);
+      ld4d(d.qc + (size_t)o * 4, qn);
+      ld4d(d.dqc + (size_t)o * 4, dq);
+      nc_normalflux(g, qo, nx, ny, ds, f1);
+      nc_normalflux(g, qn, nx, ny, ds, f2);
+      double lam = d.fla[e];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cres[i] = cres[i] + 0.5 * (sign * (f2[i] - f1[i]) - 1.5 * lam * dq[i]);
+    }
+  }
+  double dd = d.dd[it], res[4], dq[4], qo[4], q[4];
+  ld4d(d.res + (size_t)it * 4, res);
+  ld4d(d.qold + (size_t)it * 4, qo);
+  if (dir == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dq[i] = (-res[i] - cres[i]) / dd;
+  } else {
+    ld4d(d.dqc + (size_t)it * 4, dq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dq[i] = dq[i] - cres[i] / dd;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q[i] = qo[i] + dq[i];
+  st4d(d.dqc + (size_t)it * 4, dq);
+  st4d(d.qc + (size_t)it * 4, q);
+}
+
 __global__ __launch_bounds__(SWEEP_THREADS) void k_nc_lusgs_sweep(DevNC d, NCFree g,
-                                                                  const int* verts,
+                                                                  const SweepSlot* slots,
                                                                   const int* off, int nlev,
                                                                   int dir) {
+  // levels wider than the workgroup take several passes; the first pass of
+  // the next level is prefetched
+  SweepSlot cur;
+  int b = off[0], n = off[1] - b;
+  if ((int)threadIdx.x < n) cur = slots[b + threadIdx.x];
   for (int l = 0; l < nlev; ++l) {
-    const int b = off[l], n = off[l + 1] - b;
-    for (int t = threadIdx.x; t < n; t += SWEEP_THREADS) nc_lusgs_vertex(d, g, verts[b + t], dir);
+    for (int t = threadIdx.x + SWEEP_THREADS; t < n; t += SWEEP_THREADS) {
+      SweepSlot sl = slots[b + t];
+      nc_sweep_slot(d, g, sl, dir);
+    }
+    if ((int)threadIdx.x < n) nc_sweep_slot(d, g, cur, dir);
+    if (l + 1 < nlev) {
+      b = off[l + 1];
+      n = off[l + 2] - b;
+      if ((int)threadIdx.x < n) cur = slots[b + threadIdx.x];
+    }
     __syncthreads();
   }
 }

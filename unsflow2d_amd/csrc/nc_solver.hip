@@ -38,6 +38,7 @@ struct ufx_nc {
   int* lv_verts = nullptr;                 // forward levels then backward levels
   std::vector<int> lv_fwd_off, lv_bwd_off;  // offsets into lv_verts
   int *d_fwd_off = nullptr, *d_bwd_off = nullptr;  // the same on the device
+  SweepSlot* d_slots = nullptr;             // packed sweep slots, forward then backward
   bool sweep_single = true;                 // whole sweeps in one workgroup
   double cfl = 0.0;
   bool prof = false;
@@ -136,7 +137,7 @@ int run_lusgs(ufx_nc* x, double cfl) {
       // one workgroup walks all levels (level offsets relative to lv_verts)
       const int* doff = dir == 0 ? x->d_fwd_off : x->d_bwd_off;
       prof_mark(x, NK_LUSGS);
-      k_nc_lusgs_sweep<<<1, SWEEP_THREADS, 0, x->stream>>>(x->d, x->g, x->lv_verts, doff,
+      k_nc_lusgs_sweep<<<1, SWEEP_THREADS, 0, x->stream>>>(x->d, x->g, x->d_slots, doff,
                                                           (int)off.size() - 1, dir);
       prof_mark(x, -1);
       continue;
@@ -302,10 +303,31 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     int rc = nc_upload(x, &x->lv_verts, verts);
     if (!rc) rc = nc_upload(x, &x->d_fwd_off, x->lv_fwd_off);
     if (!rc) rc = nc_upload(x, &x->d_bwd_off, x->lv_bwd_off);
+    // packed slots of the one-workgroup sweeps: per level position its
+    // vertex and the (edge, neighbour) pairs already swept, in edge order
+    bool fits = true;
+    std::vector<SweepSlot> slots(2 * (size_t)nop);
+    for (size_t s = 0; s < slots.size(); ++s) {
+      int p = verts[s], dir = s < (size_t)nop ? 0 : 1, cnt = 0;
+      SweepSlot& sl = slots[s];
+      sl.vert = p;
+      for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
+        int e = m.v2f[j];
+        bool first = m.fp1[e] == p;
+        int o = first ? m.fp2[e] : m.fp1[e];
+        if (dir == 0 ? o < p : o > p) {
+          if (cnt == SWQ) { fits = false; break; }
+          sl.eo[cnt++] = make_int2(first ? e : ~e, o);
+        }
+      }
+      sl.cnt = cnt;
+    }
+    if (!rc && fits) rc = nc_upload(x, &x->d_slots, slots);
     if (rc) { ufx_nc_destroy(x); return rc; }
-    // UFX_NC_SWEEP=levels launches one kernel per level instead
+    // UFX_NC_SWEEP=levels launches one kernel per level instead (also the
+    // form for meshes with more than SWQ swept neighbours at a vertex)
     const char* env = std::getenv("UFX_NC_SWEEP");
-    x->sweep_single = !(env && std::strcmp(env, "levels") == 0);
+    x->sweep_single = fits && !(env && std::strcmp(env, "levels") == 0);
   }
   DevNC& d = x->d;
   d.nop = nop;
