@@ -148,3 +148,30 @@ def test_nc_device_matches_reference_fixture(case, tol):
     h = dev.iterate(meta["niter"] - nd)
     _check("history", h[:, 0], hist[nd:, 1], tol)
     _check("final qc", dev.get("qc").ravel(), z["final_qc"], tol)
+
+
+def test_nc_sweep_forms_agree(monkeypatch):
+    """The three forms of the LU-SGS sweeps -- one launch per level, the same
+    launches captured once in a graph and replayed every iteration (with the
+    residual and norm kernels launched eagerly between replays), and one
+    workgroup walking all levels -- give the same iterates, every iteration."""
+    from unsflow2d_amd import nc, nc_mesh
+    print("case: nc sweep forms o-grid 40x16 seed 6", flush=True)
+    raw = nc_mesh.o_grid(40, 16, 6, stretch=1.15)
+    m = nc.NCMesh.from_raw(raw)
+    runs = {}
+    for mode in ("levels", "graph", "single"):
+        monkeypatch.setenv("UFX_NC_SWEEP", mode)
+        s = nc.NCSolver(m, mach=0.5, aoa_deg=1.5, cfl_max=50.0, grad="gg", timemode="lusgs")
+        hs, qs = [], []
+        for _ in range(6):
+            hs.append(s.iterate(1)[0, 0])
+            qs.append(s.get("qc"))
+        runs[mode] = (np.array(hs), qs)
+    monkeypatch.delenv("UFX_NC_SWEEP")
+    h0, q0 = runs["levels"]
+    for mode in ("graph", "single"):
+        h, q = runs[mode]
+        for k in range(6):
+            _check(f"{mode} qc it{k + 1}", q[k], q0[k], 1e-13)
+        _check(f"{mode} history", h, h0, 1e-13)

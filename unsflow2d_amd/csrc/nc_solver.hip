@@ -39,7 +39,9 @@ struct ufx_nc {
   std::vector<int> lv_fwd_off, lv_bwd_off;  // offsets into lv_verts
   int *d_fwd_off = nullptr, *d_bwd_off = nullptr;  // the same on the device
   SweepSlot* d_slots = nullptr;             // packed sweep slots, forward then backward
-  bool sweep_single = true;                 // whole sweeps in one workgroup
+  int sweep_mode = 0;                       // 0 graph of level launches, 1 level launches, 2 one workgroup
+  hipGraph_t sweep_graph = nullptr;         // diagonal + both sweeps, captured once
+  hipGraphExec_t sweep_exec = nullptr;
   double cfl = 0.0;
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -128,12 +130,52 @@ int run_fvresidual(ufx_nc* x, double cfl, int mode, double ak, double bk) {
   return launch_ok();
 }
 
+// The diagonal and both sweeps as level launches (each level one kernel).
+int launch_sweeps_levels(ufx_nc* x) {
+  k_nc_lusgs_diag<<<x->nblk, BLK, 0, x->stream>>>(x->d);
+  for (int dir = 0; dir < 2; ++dir) {
+    const std::vector<int>& off = dir == 0 ? x->lv_fwd_off : x->lv_bwd_off;
+    for (size_t l = 0; l + 1 < off.size(); ++l) {
+      int n = off[l + 1] - off[l];
+      if (n <= 0) continue;
+      k_nc_lusgs_level<<<nblk_of(n), BLK, 0, x->stream>>>(x->d, x->g, x->lv_verts + off[l], n, dir);
+    }
+  }
+  return launch_ok();
+}
+
+// Capture the level launches once into a graph (their arguments never
+// change: the sweeps read the residual, time steps and states through the
+// context's fixed device arrays).  The graph holds kernel nodes only.
+int build_sweep_graph(ufx_nc* x) {
+  NCCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launch_sweeps_levels(x);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(x->stream, &gr);
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  NCCHK(e);
+  x->sweep_graph = gr;
+  NCCHK(hipGraphInstantiate(&x->sweep_exec, gr, nullptr, nullptr, 0));
+  return UFX_OK;
+}
+
 int run_lusgs(ufx_nc* x, double cfl) {
   if (int rc = run_fvresidual(x, cfl, 0, 0.0, 0.0)) return rc;
+  if (x->sweep_mode == 0) {
+    if (!x->sweep_exec)
+      if (int rc = build_sweep_graph(x)) return rc;
+    prof_mark(x, NK_LUSGS);
+    NCCHK(hipGraphLaunch(x->sweep_exec, x->stream));
+    prof_mark(x, -1);
+    return UFX_OK;
+  }
   NLAUNCH(x, NK_LUSGS, k_nc_lusgs_diag, x->nblk, x->d);
   for (int dir = 0; dir < 2; ++dir) {
     const std::vector<int>& off = dir == 0 ? x->lv_fwd_off : x->lv_bwd_off;
-    if (x->sweep_single) {
+    if (x->sweep_mode == 2) {
       // one workgroup walks all levels (level offsets relative to lv_verts)
       const int* doff = dir == 0 ? x->d_fwd_off : x->d_bwd_off;
       prof_mark(x, NK_LUSGS);
@@ -186,6 +228,8 @@ int ufx_nc_destroy(ufx_nc* x) {
   for (void* p : x->allocs) (void)hipFree(p);
   if (x->d_hist) (void)hipFree(x->d_hist);
   for (hipEvent_t e : x->ev) (void)hipEventDestroy(e);
+  if (x->sweep_exec) (void)hipGraphExecDestroy(x->sweep_exec);
+  if (x->sweep_graph) (void)hipGraphDestroy(x->sweep_graph);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
   return UFX_OK;
@@ -324,10 +368,14 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     }
     if (!rc && fits) rc = nc_upload(x, &x->d_slots, slots);
     if (rc) { ufx_nc_destroy(x); return rc; }
-    // UFX_NC_SWEEP=levels launches one kernel per level instead (also the
-    // form for meshes with more than SWQ swept neighbours at a vertex)
+    // sweep form: a replayed graph of the level launches (default), the
+    // level launches themselves (UFX_NC_SWEEP=levels), or one workgroup
+    // walking the packed slots (UFX_NC_SWEEP=single; needs at most SWQ swept
+    // neighbours per vertex)
     const char* env = std::getenv("UFX_NC_SWEEP");
-    x->sweep_single = fits && !(env && std::strcmp(env, "levels") == 0);
+    x->sweep_mode = 0;
+    if (env && std::strcmp(env, "levels") == 0) x->sweep_mode = 1;
+    if (env && std::strcmp(env, "single") == 0 && fits) x->sweep_mode = 2;
   }
   DevNC& d = x->d;
   d.nop = nop;
