@@ -9,6 +9,7 @@
 //   qc   [ncell][4] f64  conserved state [E, rho, rho u, rho v] (ghosts last)
 //   qold [noc][4] f64, res [noc][4] f64, dt/dta/la [noc] f64
 //   frec [nof][FW] R     face record nx, ny, area, centre x, y
+//   fcen [nof][2] R      face centre alone (for the gradient/limiter pass)
 //   fpt  [nof] int2      face endpoints (storage numbering of points)
 //   prec [nop][PW] R     vertex record: averaged primitives rho,u,v,p,T, mu
 //   sl_face/sl_nbr [MAXF][noc] int  the faces of each cell in ascending
@@ -53,6 +54,7 @@ struct DevT {
   int nop, noc, nfi, nbf, nof, ncell;
   // mesh
   const R* frec;
+  const R* fcen;             // [nof][2] face centres (the limiter's only face data)
   const int2* fpt;
   const int* fbc;            // bc code per boundary ordinal (nbf)
   const int *bface, *bcell;  // storage face and inner cell per boundary ordinal

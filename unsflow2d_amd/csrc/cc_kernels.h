@@ -237,7 +237,13 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
   R own[RW];
   R tc = R(0);
   if (c < d.noc) {
-    ldv<R, RW>(d.rec + (size_t)c * RW, own);
+    // the fields this kernel reads: primitives, limiter values, centroid,
+    // and the old gradient only for a limiter-only call
+    const R* rc = d.rec + (size_t)c * RW;
+    ldv<R, 4>(rc + R_QP, own + R_QP);
+    ldv<R, 4>(rc + R_PHI, own + R_PHI);
+    ldv<R, 2>(rc + R_CEN, own + R_CEN);
+    if (GRAD == G_NONE) ldv<R, 8>(rc + R_G, own + R_G);
     tc = d.qT[c];
   }
   if (SL) {
@@ -279,9 +285,16 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
     n = in_side ? n : ~n;
     bool bnd = n >= noc;
     R fr[FW];
-    face_rec(d, f, fr);
-    fcx[s] = fr[F_CX];
-    fcy[s] = fr[F_CY];
+    if (GRAD == G_GG) {
+      face_rec(d, f, fr);
+      fcx[s] = fr[F_CX];
+      fcy[s] = fr[F_CY];
+    } else {
+      R fc[2];
+      ldv<R, 2>(d.fcen + (size_t)f * 2, fc);
+      fcx[s] = fc[0];
+      fcy[s] = fc[1];
+    }
     R fq[5];
     if (GRAD == G_GG || (LIMIT && bnd)) {
       if (GRAD == G_GG) face_prims<R, 5>(d, f, fq);
