@@ -264,7 +264,15 @@ typedef struct {
                         merge conflict at NCFVM/fvresidual.f90:11-17) */
   int timemode;      /* UFX_TIME_RK3 (needs cfl_max < 1) or UFX_TIME_LUSGS */
   int precision;     /* UFX_PREC_FP64 */
+  int flux;          /* UFX_FLUX_VANLEER (the reference's) or UFX_FLUX_AUSM (build-
+                        defined: the cell-centred AUSM+-up on the dual faces) */
+  int limiter;       /* UFX_NC_LIM_VKN (limiter_VKN, NCFVM/limiter.f90:15-98) or
+                        UFX_NC_LIM_BJ (limiter_min_max, :192-260, dead code in the
+                        reference, with its +1/2 at :241) */
 } ufx_nc_params;
+
+#define UFX_NC_LIM_VKN 0
+#define UFX_NC_LIM_BJ 1
 
 /* Context on the current device in the state `initialize` leaves
  * (NCFVM/common.f90:21-58): free stream everywhere, limiter value 1e20. */

@@ -142,14 +142,23 @@ void orc_nc_limiter(orc_nc* s) {
       for (int side = 0; side < 2; ++side) {
         int p = side == 0 ? p1 : p2;
         double sum = (0.0 + s->grad[p * 8 + 2 * j] * d0) + s->grad[p * 8 + 2 * j + 1] * d1;
-        double d_l = side == 0 ? 0.5 * sum : -0.5 * sum;
-        double t = kappa * sqrt(s->pcv[p]);
-        double tol = t * t * t;
-        double alfa = d_l > 0.0 ? s->dumax[p * 4 + j] : s->dumin[p * 4 + j];
-        if (fabs(alfa) < EPS_SP) alfa = 0.0;
-        double nr = alfa * alfa + 2.0 * d_l * alfa + tol;
-        double dr = alfa * alfa + 2.0 * d_l * d_l + d_l * alfa + tol;
-        double phi = dmin(1.0, nr / dr);
+        double phi;
+        if (s->lim_kind == 1) {
+          /* limiter_min_max, NCFVM/limiter.f90:221-257: +1/2 at both ends */
+          double d_l = 0.5 * sum;
+          if (d_l > 0.0) phi = dmin(1.0, s->dumax[p * 4 + j] / d_l);
+          else if (d_l < 0.0) phi = dmin(1.0, s->dumin[p * 4 + j] / d_l);
+          else phi = 1.0;
+        } else {
+          double d_l = side == 0 ? 0.5 * sum : -0.5 * sum;
+          double t = kappa * sqrt(s->pcv[p]);
+          double tol = t * t * t;
+          double alfa = d_l > 0.0 ? s->dumax[p * 4 + j] : s->dumin[p * 4 + j];
+          if (fabs(alfa) < EPS_SP) alfa = 0.0;
+          double nr = alfa * alfa + 2.0 * d_l * alfa + tol;
+          double dr = alfa * alfa + 2.0 * d_l * d_l + d_l * alfa + tol;
+          phi = dmin(1.0, nr / dr);
+        }
         s->phi[p * 4 + j] = dmin(s->phi[p * 4 + j], phi);
       }
     }
@@ -218,6 +227,34 @@ static void vanleer(orc_nc* s, int ie, int c1, int c2, const double* ql, const d
   double a = sqrt(GAMMA1 * (h - 0.5 * (u * u + v * v)));
   double un = fabs(u * nx + v * ny);
   double ll = (un + a) * area;
+  s->la[c1] = s->la[c1] + ll;
+  s->la[c2] = s->la[c2] + ll;
+  s->fla[ie] = ll;
+}
+
+/* AUSM+-up on a dual face (build-defined): the cell-centred routine
+ * (orc_flux kind 2: CCFVM/ausm_up_flux.F90:2-162, unit normal) times the
+ * face length, with van Leer's Roe-averaged spectral radius */
+int orc_flux(int kind, double mach, double* ql, double* qr, double nx, double ny, double area,
+             double* flux);
+static void ausm_edge(orc_nc* s, int ie, int c1, int c2, const double* ql, const double* qr) {
+  double nx = s->fnx[ie], ny = s->fny[ie], area = s->farea[ie];
+  double a[4] = {ql[0], ql[1], ql[2], ql[3]}, b[4] = {qr[0], qr[1], qr[2], qr[3]}, F[4];
+  orc_flux(2, s->mach, a, b, nx, ny, 1.0, F);
+  for (int i = 0; i < 4; ++i) {
+    s->res[c1 * 4 + i] = s->res[c1 * 4 + i] + F[i] * area;
+    s->res[c2 * 4 + i] = s->res[c2 * 4 + i] - F[i] * area;
+  }
+  double rl = ql[0], ul = ql[1], vl = ql[2], pl = ql[3];
+  double rr = qr[0], ur = qr[1], vr = qr[2], pr = qr[3];
+  double hl = GAMMA * pl / rl / GAMMA1 + 0.5 * (ul * ul + vl * vl);
+  double hr = GAMMA * pr / rr / GAMMA1 + 0.5 * (ur * ur + vr * vr);
+  double rt = sqrt(rr / rl);
+  double u = (ul + rt * ur) / (1.0 + rt);
+  double v = (vl + rt * vr) / (1.0 + rt);
+  double h = (hl + rt * hr) / (1.0 + rt);
+  double aa = sqrt(GAMMA1 * (h - 0.5 * (u * u + v * v)));
+  double ll = (fabs(u * nx + v * ny) + aa) * area;
   s->la[c1] = s->la[c1] + ll;
   s->la[c2] = s->la[c2] + ll;
   s->fla[ie] = ll;
@@ -305,7 +342,8 @@ void orc_nc_fvresidual(orc_nc* s) {
       g = (0.0 + s->grad[p2 * 8 + 2 * k] * d0) + s->grad[p2 * 8 + 2 * k + 1] * d1;
       right[k] = s->qp[p2 * 4 + k] - 0.5 * g * s->phi[p2 * 4 + k];
     }
-    vanleer(s, ie, p1, p2, left, right);
+    if (s->flux_kind == 2) ausm_edge(s, ie, p1, p2, left, right);
+    else vanleer(s, ie, p1, p2, left, right);
     if (s->fbc[ie] == 1001) solid(s, p1, p2);
     if (s->fbc[ie] == 2001) farfield(s, &f, p1, p2);
   }

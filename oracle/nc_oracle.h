@@ -38,6 +38,10 @@ typedef struct {
   /* outputs */
   double dtglobal, fres, fres1, fresi;
   int iresi;
+  /* options: flux 3 van Leer (the reference's), 2 AUSM+-up (build-defined:
+   * the cell-centred routine on the dual face); limiter 0 limiter_VKN,
+   * 1 limiter_min_max (Barth-Jespersen, dead code in the reference) */
+  int flux_kind, lim_kind;
 } orc_nc;
 
 void orc_nc_initialize(orc_nc* s);

@@ -193,7 +193,7 @@ class _NCState(C.Structure):
         + [(n, _dp) for n in ("qc", "qp", "qold", "res", "grad", "phi", "dumin", "dumax", "la",
                               "dt", "fla")]
         + [(n, C.c_double) for n in ("dtglobal", "fres", "fres1", "fresi")]
-        + [("iresi", C.c_int)]
+        + [("iresi", C.c_int), ("flux_kind", C.c_int), ("lim_kind", C.c_int)]
     )
 
 
@@ -214,7 +214,8 @@ class NCOracle:
     fc_ny, fc_area, pt_nv2v, pt_v2v, pt_wx, pt_wy, pt_nv2f, pt_v2f.
     """
 
-    def __init__(self, mesh: dict, *, mach=0.4, aoa_deg=0.0, cfl_max=50.0, grad="gg"):
+    def __init__(self, mesh: dict, *, mach=0.4, aoa_deg=0.0, cfl_max=50.0, grad="gg",
+                 flux="vanleer", limiter="vkn"):
         i0 = lambda n: np.ascontiguousarray(mesh[n], dtype=np.int32) - 1
         i1 = lambda n: np.ascontiguousarray(mesh[n], dtype=np.int32)
         f = lambda n: np.ascontiguousarray(mesh[n], dtype=np.float64)
@@ -240,6 +241,8 @@ class NCOracle:
             setattr(s, k, getattr(self, k).ctypes.data_as(_dp))
         s.mach, s.aoa_deg, s.cfl_max = mach, aoa_deg, cfl_max
         s.grad_kind = {"gg": 1, "lsqr": 2}[grad]
+        s.flux_kind = {"vanleer": 3, "ausm": 2}[flux]
+        s.lim_kind = {"vkn": 0, "bj": 1}[limiter]
         self.s = s
         _lib.orc_nc_initialize(C.byref(s))
 

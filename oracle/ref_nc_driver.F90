@@ -21,7 +21,9 @@
 ! meaning (zero and not associated).
 !
 ! Usage (in the directory holding geometry.inp):
-!   ref_nc_driver DECK OUTDIR NITER DUMPITERS GRAD [TIMEWARM]
+!   ref_nc_driver DECK OUTDIR NITER DUMPITERS GRAD[:bj] [TIMEWARM]
+! (":bj" calls the reference's limiter_min_max, NCFVM/limiter.f90:192-260,
+! which its own fvresidual never calls, in place of LinearReconLimiter)
 !------------------------------------------------------------------------------
 module nc_harness
 use data_type
@@ -29,6 +31,7 @@ use grid
 use param, only: iter
 implicit none
 integer :: grad_kind = 1          ! 1 = Gradient_GG, 2 = Gradient_LSQR
+integer :: lim_kind = 0           ! 0 = LinearReconLimiter (limiter_VKN), 1 = limiter_min_max
 integer :: ndump = 0, stage = 0
 character(len=512) :: outdir = '.'
 contains
@@ -116,7 +119,11 @@ enddo
 pt(:)%la = 0.d0
 fc(:)%la = 0.d0
 
-call LinearReconLimiter
+if (lim_kind == 1) then
+   call limiter_min_max
+else
+   call LinearReconLimiter
+endif
 if (iter <= ndump) call dump_phi()
 
 do ie = 1, nof
@@ -168,12 +175,16 @@ call get_command_argument(2, outdir)
 call get_command_argument(3, arg); read(arg,*) niter
 call get_command_argument(4, arg); read(arg,*) ndump
 call get_command_argument(5, arg)
+if (index(arg, ':bj') > 0) then
+   lim_kind = 1
+   arg = arg(1:index(arg, ':bj') - 1)
+endif
 if (trim(arg) == 'lsqr') then
    grad_kind = 2
 elseif (trim(arg) == 'gg') then
    grad_kind = 1
 else
-   stop 'GRAD must be gg or lsqr'
+   stop 'GRAD must be gg or lsqr, optionally followed by :bj'
 endif
 timing = .false.
 twarm = 0

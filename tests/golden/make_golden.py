@@ -79,6 +79,14 @@ NC_CASES = {
     "nc_tiny_rk3_lsqr": (dict(ni=20, nj=8, seed=5, stretch=1.25),
                          dict(mach=0.5, aoa_deg=2.0, cfl_max=0.8, timemode="rk3"), "lsqr", 40, 2,
                          True),
+    # limiter_min_max (Barth-Jespersen) in place of limiter_VKN; under LU-SGS
+    # at CFL 50 the reference's own solution turns NaN at iteration 9 (its
+    # phi can go negative: dumax starts at -eps), so the case stops at 8
+    "nc_tiny_lusgs_gg_bj": (NC_TINY, dict(mach=0.4, cfl_max=50.0, timemode="lusgs"), "gg:bj",
+                            8, 2, False),
+    "nc_tiny_rk3_lsqr_bj": (dict(ni=20, nj=8, seed=5, stretch=1.25),
+                            dict(mach=0.5, aoa_deg=2.0, cfl_max=0.8, timemode="rk3"), "lsqr:bj",
+                            30, 2, False),
     # the vertex-centred workload's generator at 200 x 100 points (20 000
     # vertices), LU-SGS to CFL 50, 500 iterations
     "nc_c3_small": (dict(ni=200, nj=100, seed=3, stretch=1.005 ** (999 / 99)),

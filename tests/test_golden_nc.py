@@ -5,7 +5,8 @@ NCFVM solver).
 Both are required to agree bit for bit: the preprocessing restatement in the
 library (breadth-first numbering, dual normals and volumes, neighbour and
 edge lists, least-squares weights) and the C oracle's every routine output
-and residual history, for GG and LSQ gradients, LU-SGS and RK3.
+and residual history, for GG and LSQ gradients, LU-SGS and RK3, with the
+reference's limiter_VKN and with its (uncalled) limiter_min_max.
 """
 import hashlib
 import json
@@ -18,8 +19,8 @@ from conftest import ROOT
 
 GOLD = os.path.join(ROOT, "tests", "golden")
 CASES = ["nc_tiny_lusgs_gg", "nc_tiny_lusgs_lsqr", "nc_tiny_rk3_gg", "nc_tiny_rk3_lsqr",
-         "nc_c3_small"]
-TINY = CASES[:4]
+         "nc_c3_small", "nc_tiny_lusgs_gg_bj", "nc_tiny_rk3_lsqr_bj"]
+TINY = [c for c in CASES if "tiny" in c]
 
 I32 = {"fc_p1", "fc_p2", "fc_in", "fc_out", "fc_bc", "pt_nv2v", "pt_v2v", "pt_nv2f", "pt_v2f",
        "counts"}
@@ -45,6 +46,12 @@ def lib_mesh(meta):
     a["pt_wxy"] = np.concatenate([a["pt_wx"], a["pt_wy"]])
     a["counts"] = a["counts"]
     return a
+
+
+def grad_limiter(meta):
+    """The fixture's gradient and limiter: "gg", "lsqr", "gg:bj", "lsqr:bj"."""
+    g, _, lim = meta["grad"].partition(":")
+    return g, (lim or "vkn")
 
 
 def oracle_mesh(a):
@@ -80,8 +87,9 @@ def test_oracle_matches_reference(case):
     z, meta = load(case)
     a = {k: z[k] for k in z.files} if "pt_x" in z.files else lib_mesh(meta)
     d = meta["deck"]
+    g, lim = grad_limiter(meta)
     o = NCOracle(oracle_mesh(a), mach=d["mach"], aoa_deg=d.get("aoa_deg", 0.0),
-                 cfl_max=d["cfl_max"], grad=meta["grad"])
+                 cfl_max=d["cfl_max"], grad=g, limiter=lim)
     tm = d["timemode"]
     hist = z["hist"].reshape(-1, 8)
     for it in range(1, meta["niter"] + 1):
@@ -109,3 +117,52 @@ def test_oracle_matches_reference(case):
             o.iteration(it, tm)
         _eq(f"fres {it}", [o.s.fres, o.s.fresi, o.s.dtglobal], hist[it - 1, [1, 4, 6]])
     _eq("final qc", o.qc, z["final_qc"])
+
+
+def _o_grid_oracle(**kw):
+    from oracle.oracle import NCOracle
+    from unsflow2d_amd import nc, nc_mesh
+    m = nc.NCMesh.from_raw(nc_mesh.o_grid(30, 12, 4, stretch=1.2))
+    return m.arrays(), (lambda **k: NCOracle(m.arrays(), mach=0.5, aoa_deg=2.0, cfl_max=0.8,
+                                             grad="lsqr", **(kw | k)))
+
+
+def test_ausm_matches_van_leer_at_free_stream():
+    """The build-defined AUSM+-up option on the dual faces.  At the free
+    stream the least-squares gradient is exactly zero, so every edge sees
+    equal states; both splittings are consistent, so both residuals are the
+    exact Euler flux summed over the dual boundary and must agree at every
+    vertex (to rounding), interior vertices giving zero.  This pins the
+    edge assembly (sign, face length, both ends) against the reference's
+    van Leer path."""
+    a, make = _o_grid_oracle()
+    res = {}
+    for flux in ("vanleer", "ausm"):
+        o = make(flux=flux)
+        o.save_old()
+        o.fvresidual(0.8)
+        res[flux] = (o.res.copy(), o.la.copy())
+    scale = np.abs(res["vanleer"][0]).max()
+    assert np.abs(res["ausm"][0] - res["vanleer"][0]).max() <= 1e-12 * scale
+    interior = a["pt_bc"] == 0
+    assert np.abs(res["ausm"][0][interior]).max() <= 1e-12 * scale
+    assert np.array_equal(res["ausm"][1], res["vanleer"][1])
+
+
+def test_ausm_spectral_radius_is_van_leers():
+    """Away from the free stream the AUSM option keeps van Leer's
+    Roe-averaged spectral radius, so the local time steps agree exactly."""
+    a, make = _o_grid_oracle()
+    o = make(flux="vanleer")
+    for it in range(1, 6):
+        o.iteration(it, "rk3")
+    q, phi = o.qc.copy(), o.phi.copy()
+    o2 = make(flux="ausm")
+    o2.qc[:] = q
+    o2.phi[:] = phi
+    for s in (o, o2):
+        s.save_old()
+        s.fvresidual(0.8)
+    assert np.array_equal(o.la, o2.la) and np.array_equal(o.dt, o2.dt)
+    assert not np.allclose(o.res, o2.res, rtol=1e-6, atol=0)
+

@@ -24,27 +24,33 @@ def _check(name, got, want, tol):
     assert e <= tol, f"{name}: {e:.3e} > {tol}"
 
 
-def _setup(ni, nj, seed, stretch, grad, timemode, cfl_max, mach=0.4, aoa=0.0):
+def _setup(ni, nj, seed, stretch, grad, timemode, cfl_max, mach=0.4, aoa=0.0, flux="vanleer",
+           limiter="vkn"):
     from unsflow2d_amd import nc, nc_mesh
     from oracle.oracle import NCOracle
     raw = nc_mesh.o_grid(ni, nj, seed, stretch=stretch)
     m = nc.NCMesh.from_raw(raw)
-    dev = nc.NCSolver(m, mach=mach, aoa_deg=aoa, cfl_max=cfl_max, grad=grad, timemode=timemode)
-    orc = NCOracle(m.arrays(), mach=mach, aoa_deg=aoa, cfl_max=cfl_max, grad=grad)
+    dev = nc.NCSolver(m, mach=mach, aoa_deg=aoa, cfl_max=cfl_max, grad=grad, timemode=timemode,
+                      flux=flux, limiter=limiter)
+    orc = NCOracle(m.arrays(), mach=mach, aoa_deg=aoa, cfl_max=cfl_max, grad=grad, flux=flux,
+                   limiter=limiter)
     return dev, orc
 
 
-@pytest.mark.parametrize("grad,timemode", [("gg", "lusgs"), ("lsqr", "lusgs"), ("gg", "rk3"),
-                                           ("lsqr", "rk3")])
-def test_nc_routines_match_oracle(grad, timemode):
+@pytest.mark.parametrize("grad,timemode,flux,limiter", [
+    ("gg", "lusgs", "vanleer", "vkn"), ("lsqr", "lusgs", "vanleer", "vkn"),
+    ("gg", "rk3", "vanleer", "vkn"), ("lsqr", "rk3", "vanleer", "vkn"),
+    ("gg", "rk3", "vanleer", "bj"), ("lsqr", "lusgs", "ausm", "vkn"),
+    ("gg", "rk3", "ausm", "bj")])
+def test_nc_routines_match_oracle(grad, timemode, flux, limiter):
     """Every routine from the same developed state: the oracle runs 20
     iterations, its state (qc and the limiter ratchet) is injected into the
     device, and both run two more iterations routine by routine.  (From a
     free-stream start the gradients are differences of nearly equal values,
     whose relative rounding is amplified by the conditioning.)"""
-    print(f"case: nc routines {grad} {timemode} o-grid 30x12 seed 4", flush=True)
+    print(f"case: nc routines {grad} {timemode} {flux} {limiter} o-grid 30x12 seed 4", flush=True)
     cfl_max = 50.0 if timemode == "lusgs" else 0.8
-    dev, orc = _setup(30, 12, 4, 1.2, grad, timemode, cfl_max)
+    dev, orc = _setup(30, 12, 4, 1.2, grad, timemode, cfl_max, flux=flux, limiter=limiter)
     for it in range(1, 21):
         orc.iteration(it, timemode)
     fres1 = orc.s.fres1
@@ -95,12 +101,13 @@ def test_nc_first_iteration_from_free_stream():
     _check("fres", h[[0, 1, 2]], [orc.s.fres, orc.s.fres1, orc.s.fresi], 1e-12)
 
 
-@pytest.mark.parametrize("grad,timemode", [("gg", "lusgs"), ("lsqr", "rk3")])
-def test_nc_iterations_match_oracle(grad, timemode):
+@pytest.mark.parametrize("grad,timemode,flux", [("gg", "lusgs", "vanleer"), ("lsqr", "rk3", "vanleer"),
+                                                ("gg", "lusgs", "ausm")])
+def test_nc_iterations_match_oracle(grad, timemode, flux):
     n = 80
-    print(f"case: nc iterate {grad} {timemode} o-grid 40x16 seed 6 n={n}", flush=True)
+    print(f"case: nc iterate {grad} {timemode} {flux} o-grid 40x16 seed 6 n={n}", flush=True)
     cfl_max = 50.0 if timemode == "lusgs" else 0.8
-    dev, orc = _setup(40, 16, 6, 1.15, grad, timemode, cfl_max, mach=0.5, aoa=1.5)
+    dev, orc = _setup(40, 16, 6, 1.15, grad, timemode, cfl_max, mach=0.5, aoa=1.5, flux=flux)
     h = dev.iterate(n)
     ref = np.array([orc.iteration(it, timemode) for it in range(1, n + 1)])
     _check("history", h[:, 0], ref, 1e-10)
@@ -114,7 +121,8 @@ def _load(case):
 
 @pytest.mark.parametrize("case,tol", [("nc_tiny_lusgs_gg", 1e-10), ("nc_tiny_lusgs_lsqr", 1e-10),
                                       ("nc_tiny_rk3_gg", 1e-10), ("nc_tiny_rk3_lsqr", 1e-10),
-                                      ("nc_c3_small", 1e-8)])
+                                      ("nc_tiny_lusgs_gg_bj", 1e-10),
+                                      ("nc_tiny_rk3_lsqr_bj", 1e-10), ("nc_c3_small", 1e-8)])
 def test_nc_device_matches_reference_fixture(case, tol):
     """Per-routine outputs of the first iterations and the whole residual
     history and final state against the reference's dumps."""
@@ -124,8 +132,10 @@ def test_nc_device_matches_reference_fixture(case, tol):
     mkw = dict(meta["mesh"])
     raw = nc_mesh.o_grid(mkw.pop("ni"), mkw.pop("nj"), mkw.pop("seed"), **mkw)
     d = meta["deck"]
+    grad, _, lim = meta["grad"].partition(":")
     dev = nc.NCSolver(nc.NCMesh.from_raw(raw), mach=d["mach"], aoa_deg=d.get("aoa_deg", 0.0),
-                      cfl_max=d["cfl_max"], grad=meta["grad"], timemode=d["timemode"])
+                      cfl_max=d["cfl_max"], grad=grad, timemode=d["timemode"],
+                      limiter=lim or "vkn")
     tm = d["timemode"]
     hist = z["hist"].reshape(-1, 8)
     nd = meta["ndump"]

@@ -86,7 +86,7 @@ __device__ __forceinline__ void for_edges(const DevNC& d, int j0, int n, const i
   for (int k = NCU; k < n; ++k) f(d.v2fo[j0 + k]);
 }
 
-template <int GRAD>
+template <int GRAD, int LIM>
 __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
   int p = cell_index();
   if (p >= d.nop) return;
@@ -166,13 +166,29 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       double sum = (0.0 + gr[2 * k] * d0) + gr[2 * k + 1] * d1;
-      double dl = first ? 0.5 * sum : -0.5 * sum;
-      double alfa = dl > 0.0 ? dumax[k] : dumin[k];
-      if (fabs(alfa) < NC_EPS_SP) alfa = 0.0;
-      double nr = alfa * alfa + 2.0 * dl * alfa + tol;
-      double dr = alfa * alfa + 2.0 * dl * dl + dl * alfa + tol;
-      double ph = nr / dr;
-      ph = 1.0 < ph ? 1.0 : ph;
+      double ph;
+      if (LIM == 0) {
+        double dl = first ? 0.5 * sum : -0.5 * sum;
+        double alfa = dl > 0.0 ? dumax[k] : dumin[k];
+        if (fabs(alfa) < NC_EPS_SP) alfa = 0.0;
+        double nr = alfa * alfa + 2.0 * dl * alfa + tol;
+        double dr = alfa * alfa + 2.0 * dl * dl + dl * alfa + tol;
+        ph = nr / dr;
+        ph = 1.0 < ph ? 1.0 : ph;
+      } else {
+        // limiter_min_max (Barth-Jespersen), NCFVM/limiter.f90:192-260:
+        // both edge ends take +1/2 of the projected gradient (:227, :241)
+        double dl = 0.5 * sum;
+        if (dl > 0.0) {
+          ph = dumax[k] / dl;
+          ph = 1.0 < ph ? 1.0 : ph;
+        } else if (dl < 0.0) {
+          ph = dumin[k] / dl;
+          ph = 1.0 < ph ? 1.0 : ph;
+        } else {
+          ph = 1.0;
+        }
+      }
       phi[k] = phi[k] < ph ? phi[k] : ph;
     }
   });
@@ -180,6 +196,32 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
   st4d(w + R_G, gr);
   st4d(w + R_G + 4, gr + 4);
   st4d(w + R_PHI, phi);
+}
+
+// Spectral radius of a dual face from the Roe average of its two states,
+// NCFVM/vanleer_flux.f90:103-115, times the face length.
+__device__ __forceinline__ double nc_roe_lambda(const NCFree& g, double rl, double ul, double vl,
+                                                double hl, double rr, double ur, double vr,
+                                                double hr, double nx, double ny, double area) {
+  double rt = sqrt(rr / rl);
+  double u = (ul + rt * ur) / (1.0 + rt), v = (vl + rt * vr) / (1.0 + rt);
+  double h = (hl + rt * hr) / (1.0 + rt);
+  double a = sqrt(g.gamma1 * (h - 0.5 * (u * u + v * v)));
+  return (fabs(u * nx + v * ny) + a) * area;
+}
+
+// AUSM+-up on a dual face (build-defined: the vertex-centred reference has
+// no compilable AUSM): the cell-centred routine's flux per unit length
+// (CCFVM/ausm_up_flux.F90:2-162, unit normal), with the van Leer routine's
+// Roe-averaged spectral radius.
+__device__ __forceinline__ void nc_ausm(const NCFree& g, const Gas<double>& ga, const double* ql,
+                                        const double* qr, double nx, double ny, double area,
+                                        double* F, double& ll) {
+  double ql_[4] = {ql[0], ql[1], ql[2], ql[3]}, qr_[4] = {qr[0], qr[1], qr[2], qr[3]};
+  ausm_flux<double>(ga, ql_, qr_, nx, ny, F);
+  double hl = (g.gamma * ql[3] / ql[0]) / g.gamma1 + 0.5 * (ql[1] * ql[1] + ql[2] * ql[2]);
+  double hr = (g.gamma * qr[3] / qr[0]) / g.gamma1 + 0.5 * (qr[1] * qr[1] + qr[2] * qr[2]);
+  ll = nc_roe_lambda(g, ql[0], ql[1], ql[2], hl, qr[0], qr[1], qr[2], hr, nx, ny, area);
 }
 
 // van Leer flux of NCFVM/vanleer_flux.f90:2-118 on one dual face between
@@ -207,11 +249,7 @@ __device__ __forceinline__ void nc_vanleer(const NCFree& g, const double* ql, co
   double fp[4] = {delta, 0.0, nx * p12, ny * p12};
 #pragma unroll
   for (int i = 0; i < 4; ++i) F[i] = 0.5 * fm12 * (fl[i] + fr[i]) - 0.5 * dissi * (fr[i] - fl[i]) + fp[i];
-  double rt = sqrt(rr / rl);
-  double u = (ul + rt * ur) / (1.0 + rt), v = (vl + rt * vr) / (1.0 + rt);
-  double h = (hl + rt * hr) / (1.0 + rt);
-  double a = sqrt(gm1 * (h - 0.5 * (u * u + v * v)));
-  ll = (fabs(u * nx + v * ny) + a) * area;
+  ll = nc_roe_lambda(g, rl, ul, vl, hl, rr, ur, vr, hr, nx, ny, area);
 }
 
 // Residual of one vertex, NCFVM/fvresidual.f90:34-64 as a gather over its
@@ -222,8 +260,9 @@ __device__ __forceinline__ void nc_vanleer(const NCFree& g, const double* ql, co
 // (NCFVM/timestep.f90:5-37).
 //   mode 0: store res, la, dt;  mode 1: also the RK stage update of
 //   NCFVM/main.f90:59-70 in place (no vertex reads another's qc in a stage).
-__global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, double cfl, int mode,
-                                                     double ak, double bk) {
+template <int FLUX>
+__global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, Gas<double> ga, double cfl,
+                                                     int mode, double ak, double bk) {
   int p = cell_index();
   if (p >= d.nop) return;
   double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0;
@@ -257,7 +296,8 @@ __global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, double c
       qr[k] = b[R_QP + k] - 0.5 * s * b[R_PHI + k];
     }
     double F[4], ll;
-    nc_vanleer(g, ql, qr, nx, ny, ds, F, ll);
+    if (FLUX == 2) nc_ausm(g, ga, ql, qr, nx, ny, ds, F, ll);
+    else nc_vanleer(g, ql, qr, nx, ny, ds, F, ll);
     if (first) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i] * ds;

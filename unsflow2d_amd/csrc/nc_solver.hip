@@ -29,6 +29,7 @@ using namespace ufx;
 struct ufx_nc {
   ufx_nc_params prm{};
   NCFree g{};
+  Gas<double> ga{};  // for the AUSM+-up option (mref as CCFVM/ausm_up_flux.F90:79)
   DevNC d{};
   int nop = 0, nof = 0, nblk = 0;
   hipStream_t stream = nullptr;
@@ -117,8 +118,11 @@ int run_save_old(ufx_nc* x) {
 }
 
 int run_grad_lim(ufx_nc* x) {
-  if (x->prm.grad == UFX_GRAD_LSQR) NLAUNCH(x, NK_GRAD, k_nc_grad_lim<2>, x->nblk, x->d);
-  else NLAUNCH(x, NK_GRAD, k_nc_grad_lim<1>, x->nblk, x->d);
+  bool lsq = x->prm.grad == UFX_GRAD_LSQR, bj = x->prm.limiter == UFX_NC_LIM_BJ;
+  if (lsq && bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<2, 1>), x->nblk, x->d);
+  else if (lsq) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<2, 0>), x->nblk, x->d);
+  else if (bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<1, 1>), x->nblk, x->d);
+  else NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<1, 0>), x->nblk, x->d);
   return launch_ok();
 }
 
@@ -126,7 +130,10 @@ int run_grad_lim(ufx_nc* x) {
 // RK stage update with coefficients (ak, bk)
 int run_fvresidual(ufx_nc* x, double cfl, int mode, double ak, double bk) {
   if (int rc = run_grad_lim(x)) return rc;
-  NLAUNCH(x, NK_RES, k_nc_residual, x->nblk, x->d, x->g, cfl, mode, ak, bk);
+  if (x->prm.flux == UFX_FLUX_AUSM)
+    NLAUNCH(x, NK_RES, k_nc_residual<2>, x->nblk, x->d, x->g, x->ga, cfl, mode, ak, bk);
+  else
+    NLAUNCH(x, NK_RES, k_nc_residual<3>, x->nblk, x->d, x->g, x->ga, cfl, mode, ak, bk);
   return launch_ok();
 }
 
@@ -251,6 +258,14 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     set_error("ufx_nc_create: the vertex-centred path is FP64 only");
     return UFX_ERR_UNSUPPORTED;
   }
+  if (prm->flux != UFX_FLUX_VANLEER && prm->flux != UFX_FLUX_AUSM) {
+    set_error("ufx_nc_create: the vertex-centred path has the van Leer and AUSM+-up fluxes");
+    return UFX_ERR_ARG;
+  }
+  if (prm->limiter != UFX_NC_LIM_VKN && prm->limiter != UFX_NC_LIM_BJ) {
+    set_error("ufx_nc_create: unknown limiter");
+    return UFX_ERR_ARG;
+  }
   if (prm->timemode == UFX_TIME_RK3 && !(prm->cfl_max < 1.0)) {
     // NCFVM/main.f90:31-44 sets the RK coefficients only for cfl_max < 1
     set_error("CFL should be less than 1 for rk3 !!!");
@@ -283,6 +298,9 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   x->g.a_inf = std::sqrt(gamma * p_inf / qinf[1]);
   x->g.u_inf = qinf[2] / qinf[1];
   x->g.v_inf = qinf[3] / qinf[1];
+  x->ga = Gas<double>{gamma, gamma - 1.0, 1.0 / (gamma * prm->mach * prm->mach), qinf[1],
+                      x->g.u_inf, x->g.v_inf, p_inf, std::max((double)0.32f, 0.5 * prm->mach),
+                      1.0 / (gamma - 1.0)};
   // device arrays
   std::vector<double> vrec((size_t)nop * NW, 0.0);
   const double phi0 = (double)1e20f;  // default limiter value, NCFVM/All_modules.f90:200

@@ -26,7 +26,8 @@ _sig("ufx_nc_geometric", [C.c_int] * 3 + [_dp, _dp, _ip, _dp, _dp, _dp] + [_ip] 
 
 class NCParams(C.Structure):
     _fields_ = [("mach", C.c_double), ("aoa_deg", C.c_double), ("cfl_max", C.c_double),
-                ("grad", C.c_int), ("timemode", C.c_int), ("precision", C.c_int)]
+                ("grad", C.c_int), ("timemode", C.c_int), ("precision", C.c_int),
+                ("flux", C.c_int), ("limiter", C.c_int)]
 
 
 _sig("ufx_nc_create", [_vp, C.POINTER(NCParams), C.POINTER(_vp)])
@@ -111,13 +112,14 @@ class NCSolver:
     SHAPES = {"qc": 4, "qold": 4, "qp": 4, "res": 4, "phi": 4, "grad": 8, "la": 1, "dt": 1}
 
     def __init__(self, mesh: NCMesh, *, mach=0.4, aoa_deg=0.0, cfl_max=50.0, grad="gg",
-                 timemode="lusgs"):
+                 timemode="lusgs", flux="vanleer", limiter="vkn"):
         self.mesh = mesh
         self.counts = mesh.counts
         self.timemode = timemode
         self.cfl_max = cfl_max
         prm = NCParams(mach, aoa_deg, cfl_max, {"lsqr": 1, "gg": 2}[grad],
-                       {"rk3": 1, "lusgs": 2}[timemode], 0)
+                       {"rk3": 1, "lusgs": 2}[timemode], 0, {"ausm": 2, "vanleer": 3}[flux],
+                       {"vkn": 0, "bj": 1}[limiter])
         self._h = _vp()
         _check(lib.ufx_nc_create(mesh._h, C.byref(prm), C.byref(self._h)))
         self.iter = 0
