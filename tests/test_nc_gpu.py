@@ -1,9 +1,16 @@
 """Device parity of the vertex-centred (median-dual) hot path.
 
-Against the C oracle (bit-exact with the reference, see test_golden_nc.py) on
-seeded O-grids, per routine and over iterations, and against the reference's
-own fixtures.  FP64: 1e-12 normwise per routine, looser over long runs where
-the limiter ratchet and LU-SGS amplify last-bit differences (stated per test).
+The device kernels gather per vertex over its edges in ascending edge order,
+which is the order in which the reference's edge loops scatter into that
+vertex, and are compiled without a*b+c contraction, so every per-vertex
+quantity (primitives, gradients, limiter values, residuals, spectral radii,
+time steps, states) must equal the C oracle's -- and through it the
+reference's (test_golden_nc.py) -- bit for bit.  Only the residual norm is a
+parallel reduction in another order: 1e-12.  One operation is not exactly
+reproducible: pow in farfield_flux1, where the device's and the host C
+library's pow (neither correctly rounded in every case) can differ in the
+last bit; on the 20 000-vertex case the first such difference comes at
+iteration 221, after which the states agree to about 1e-13.
 """
 import json
 import os
@@ -19,9 +26,15 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 
 
 def _check(name, got, want, tol):
+    """tol 0: bit for bit."""
     e = relerr(got, want)
     print(f"  {name:14s} rel {e:.2e}", flush=True)
-    assert e <= tol, f"{name}: {e:.3e} > {tol}"
+    if tol == 0:
+        g, w = np.asarray(got, np.float64).ravel(), np.asarray(want, np.float64).ravel()
+        nd = int(np.sum(g.view(np.int64) != w.view(np.int64)))
+        assert nd == 0, f"{name}: {nd} of {g.size} values differ (max rel {e:.3e})"
+    else:
+        assert e <= tol, f"{name}: {e:.3e} > {tol}"
 
 
 def _setup(ni, nj, seed, stretch, grad, timemode, cfl_max, mach=0.4, aoa=0.0, flux="vanleer",
@@ -59,7 +72,7 @@ def test_nc_routines_match_oracle(grad, timemode, flux, limiter):
     for it in (21, 22):
         cfl = orc.cfl(it)
         dev.save_old(); orc.save_old()
-        _check("qp", dev.get("qp"), orc.qp, 1e-12)
+        _check("qp", dev.get("qp"), orc.qp, 0)
         for k in ((1, 2, 3) if timemode == "rk3" else (1,)):
             if timemode == "rk3":
                 dev.rk_stage(k, cfl)
@@ -67,24 +80,19 @@ def test_nc_routines_match_oracle(grad, timemode, flux, limiter):
             else:
                 dev.lusgs(cfl)
                 orc.lusgs(cfl)
-            _check("grad", dev.get("grad"), orc.grad, 1e-12)
-            _check("phi", dev.get("phi"), orc.phi, 1e-12)
-            _check("res", dev.get("res"), orc.res, 1e-12)
-            _check("la", dev.get("la"), orc.la, 1e-12)
-            _check("dt", dev.get("dt"), orc.dt, 1e-12)
-            _check("fc_la", dev.get("fc_la"), orc.fla, 1e-12)
+            _check("grad", dev.get("grad"), orc.grad, 0)
+            _check("phi", dev.get("phi"), orc.phi, 0)
+            _check("res", dev.get("res"), orc.res, 0)
+            _check("la", dev.get("la"), orc.la, 0)
+            _check("dt", dev.get("dt"), orc.dt, 0)
+            _check("fc_la", dev.get("fc_la"), orc.fla, 0)
             if timemode == "rk3":
                 orc.rk_update(k)
-            _check("qc", dev.get("qc"), orc.qc, 1e-12)
+            _check("qc", dev.get("qc"), orc.qc, 0)
         h = dev.residue(it)
         orc.residue(it)
         _check("fres", h[[1, 2]], [orc.s.fres * fres1, orc.s.fresi], 1e-12)
-        # the vertex of the largest |res| (mirror-symmetric vertices can tie
-        # in exact arithmetic, so last-bit differences may pick the other
-        # one): it must attain the maximum
-        r = dev.get("res")
-        i = int(h[3]) - 1
-        assert abs(np.sqrt(np.sum(r[i] ** 2)) - orc.s.fresi) <= 1e-12 * orc.s.fresi
+        assert int(h[3]) == orc.s.iresi
 
 
 def test_nc_first_iteration_from_free_stream():
@@ -94,8 +102,8 @@ def test_nc_first_iteration_from_free_stream():
     cfl = orc.cfl(1)
     dev.save_old(); orc.save_old()
     dev.lusgs(cfl); orc.lusgs(cfl)
-    _check("res", dev.get("res"), orc.res, 1e-12)
-    _check("qc", dev.get("qc"), orc.qc, 1e-12)
+    _check("res", dev.get("res"), orc.res, 0)
+    _check("qc", dev.get("qc"), orc.qc, 0)
     h = dev.residue(1)
     orc.residue(1)
     _check("fres", h[[0, 1, 2]], [orc.s.fres, orc.s.fres1, orc.s.fresi], 1e-12)
@@ -110,8 +118,8 @@ def test_nc_iterations_match_oracle(grad, timemode, flux):
     dev, orc = _setup(40, 16, 6, 1.15, grad, timemode, cfl_max, mach=0.5, aoa=1.5, flux=flux)
     h = dev.iterate(n)
     ref = np.array([orc.iteration(it, timemode) for it in range(1, n + 1)])
-    _check("history", h[:, 0], ref, 1e-10)
-    _check("final qc", dev.get("qc"), orc.qc, 1e-10)
+    _check("history", h[:, 0], ref, 1e-12)
+    _check("final qc", dev.get("qc"), orc.qc, 0)
 
 
 def _load(case):
@@ -119,13 +127,15 @@ def _load(case):
     return z, json.loads(bytes(z["meta"]).decode())
 
 
-@pytest.mark.parametrize("case,tol", [("nc_tiny_lusgs_gg", 1e-10), ("nc_tiny_lusgs_lsqr", 1e-10),
-                                      ("nc_tiny_rk3_gg", 1e-10), ("nc_tiny_rk3_lsqr", 1e-10),
-                                      ("nc_tiny_lusgs_gg_bj", 1e-10),
-                                      ("nc_tiny_rk3_lsqr_bj", 1e-10), ("nc_c3_small", 1e-8)])
+@pytest.mark.parametrize("case,tol", [("nc_tiny_lusgs_gg", 0), ("nc_tiny_lusgs_lsqr", 0),
+                                      ("nc_tiny_rk3_gg", 0), ("nc_tiny_rk3_lsqr", 0),
+                                      ("nc_tiny_lusgs_gg_bj", 0), ("nc_tiny_rk3_lsqr_bj", 0),
+                                      ("nc_c3_small", 1e-12)])
 def test_nc_device_matches_reference_fixture(case, tol):
-    """Per-routine outputs of the first iterations and the whole residual
-    history and final state against the reference's dumps."""
+    """Per-routine outputs of the first iterations bit for bit against the
+    reference's dumps, and the final state (bit for bit on the short cases;
+    on the 500-iteration case the farfield pow differs in the last bit from
+    iteration 221 on, so 1e-12); the residual history (a norm) to 1e-12."""
     from unsflow2d_amd import nc, nc_mesh
     print(f"case: {case}", flush=True)
     z, meta = _load(case)
@@ -143,20 +153,24 @@ def test_nc_device_matches_reference_fixture(case, tol):
         t = lambda n, k: z[f"{n}_it{it:04d}_{k}"]
         cfl = dev.cfl(it)
         dev.save_old()
-        _check("qp", dev.get("qp").ravel(), t("qp", 0), 1e-12)
+        _check("qp", dev.get("qp").ravel(), t("qp", 0), 0)
         for k in ((1, 2, 3) if tm == "rk3" else (1,)):
             if tm == "rk3":
                 dev.rk_stage(k, cfl)
             else:
                 dev.lusgs(cfl)
-            _check("res", dev.get("res").ravel(), t("res", k), 1e-12)
-            _check("phi", dev.get("phi"), t("phi", k).reshape(-1, 12)[:, :4], 1e-12)
-            _check("qc", dev.get("qc").ravel(), t("qc", k), 1e-12)
+            _check("grad", dev.get("grad").ravel(), t("grad", k), 0)
+            _check("res", dev.get("res").ravel(), t("res", k), 0)
+            _check("phi", dev.get("phi"), t("phi", k).reshape(-1, 12)[:, :4], 0)
+            _check("la", dev.get("la"), t("la", k), 0)
+            _check("dt", dev.get("dt"), t("dt", k), 0)
+            _check("fc_la", dev.get("fc_la"), t("fcla", k), 0)
+            _check("qc", dev.get("qc").ravel(), t("qc", k), 0)
         h = dev.residue(it)
         _check("fres", h[0], hist[it - 1, 1], 1e-12)
     dev.iter = nd
     h = dev.iterate(meta["niter"] - nd)
-    _check("history", h[:, 0], hist[nd:, 1], tol)
+    _check("history", h[:, 0], hist[nd:, 1], 1e-12)
     _check("final qc", dev.get("qc").ravel(), z["final_qc"], tol)
 
 
@@ -183,5 +197,5 @@ def test_nc_sweep_forms_agree(monkeypatch):
     for mode in ("graph", "single"):
         h, q = runs[mode]
         for k in range(6):
-            _check(f"{mode} qc it{k + 1}", q[k], q0[k], 1e-13)
-        _check(f"{mode} history", h, h0, 1e-13)
+            _check(f"{mode} qc it{k + 1}", q[k], q0[k], 0)
+        _check(f"{mode} history", h, h0, 0)

@@ -11,6 +11,7 @@ cell-centred path; there is no fallback.
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -125,7 +126,7 @@ class NCSolver:
         self.iter = 0
 
     def cfl(self, it):
-        return 0.5 * self.cfl_max * (1.0 + np.tanh(it * 7.0 / 500.0 - 3.5))
+        return 0.5 * self.cfl_max * (1.0 + math.tanh(it * 7.0 / 500.0 - 3.5))
 
     def save_old(self): _check(lib.ufx_nc_save_old(self._h))
     def fvresidual(self, cfl): _check(lib.ufx_nc_fvresidual(self._h, cfl))
