@@ -80,6 +80,7 @@ struct DevT {
   const int *lu_face, *lu_cnt;   // [MAXF][noc] reference c2f order
   // state
   R* rec;
+  R* qpc;                    // [noc][4] the interior cells' record primitives, compact
   R* qT;
   R* gT;
   double* qc;

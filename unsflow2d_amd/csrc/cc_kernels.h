@@ -71,6 +71,7 @@ __global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_p
 #pragma unroll
         for (int k = 0; k < 4; ++k) q4[k] = (R)p[k];
         stv<R, 4>(d.rec + (size_t)c * RW + R_QP, q4);
+        stv<R, 4>(d.qpc + (size_t)c * 4, q4);
         d.qT[c] = (R)p[4];
         if (do_old && (p[0] <= 0.0 || p[3] <= 0.0)) {
           atomicMin(&d.err[2], d.cref[c] + 1);
@@ -140,6 +141,9 @@ __global__ __launch_bounds__(BLK) void k_range_final(DevT<R> d, int nblk) {
 // around a block of vertices (numbered by first touch in cell order) lie
 // mostly in a window of AVG_WIN cells starting at the block's lowest cell
 // (avg_w0); their primitives are staged in LDS, the rest read from HBM.
+// Interior primitives come from the compact copy (32 B per cell in FP64,
+// where the same field inside the 144 B record would fetch whole lines);
+// ghost primitives from the records.
 constexpr int AVG_WIN = 768;
 template <typename R, bool VISC>
 __global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
   const int w0 = d.avg_w0[vb], wn = min(AVG_WIN, d.noc - w0);
   for (int i = threadIdx.x; i < wn; i += BLK) {
     R q[4];
-    ldv<R, 4>(d.rec + (size_t)(w0 + i) * RW + R_QP, q);
+    ldv<R, 4>(d.qpc + (size_t)(w0 + i) * 4, q);
     stv<R, 4>(s_q + i * 4, q);
     s_t[i] = d.qT[w0 + i];
     if (VISC) s_mu[i] = d.cmul[w0 + i];
@@ -182,7 +186,8 @@ __global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
       t = s_t[l];
       if (VISC) mu = s_mu[l];
     } else {
-      ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q);
+      if (c < d.noc) ldv<R, 4>(d.qpc + (size_t)c * 4, q);
+      else ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q);
       t = d.qT[c];
       if (VISC) mu = d.cmul[c];
     }

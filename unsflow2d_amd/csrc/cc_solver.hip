@@ -605,6 +605,10 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
     HIPCHK(up(d.rec, rec.data(), rec.size() * sizeof(R)));
     HIPCHK(up(d.qT, t.data(), t.size() * sizeof(R)));
     HIPCHK(up(d.gT, gt.data(), gt.size() * sizeof(R)));
+    std::vector<R> qpc((size_t)noc * 4);
+    for (int s = 0; s < noc; ++s)
+      for (int k = 0; k < 4; ++k) qpc[(size_t)s * 4 + k] = rec[(size_t)s * RW + R_QP + k];
+    HIPCHK(up(d.qpc, qpc.data(), qpc.size() * sizeof(R)));
     HIPCHK(hipStreamSynchronize(x->stream));
     return UFX_OK;
   }
@@ -721,6 +725,10 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py); UP(avg_w0, avg_w0);
   UP(fin, fin); UP(fout, fout); UP(lu_face, lu_face); UP(lu_cnt, lu_cnt);
   UP(rec, rec); UP(qc, qc);
+  {
+    std::vector<R> qpc((size_t)noc * 4, R(0));
+    UP(qpc, qpc);
+  }
   AL(fw, nof); AL(qT, ncell); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc);
   AL(res, 4 * (size_t)noc); AL(dt, noc); AL(dta, noc); AL(la, noc); AL(fla, nof); AL(fmu, nof);
   AL(cmul, ncell); AL(prec, (size_t)nop * PW); AL(fgrad, (size_t)nof * 10);
