@@ -126,7 +126,9 @@ int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count);
 
 /* Copy a named mesh array out.  int32 arrays: "fc_p1","fc_p2","fc_in",
  * "fc_out","fc_bc","cell_nc2c","cell_nc2f","cell_c2f","pt_nv2c","pt_v2c",
- * "oldnum","counts" (nop,noc,nof,nbf); float64 arrays: "fc_nx","fc_ny",
+ * "cell_nc2v","cell_c2v" (the vertex walk round each cell of
+ * CCFVM/geometric.F90:546-613, closed with its first vertex; empty for an
+ * imported mesh), "oldnum","counts" (nop,noc,nof,nbf); float64 arrays: "fc_nx","fc_ny",
  * "fc_area","fc_cenx","fc_ceny","fc_cov","cell_cenx","cell_ceny","cell_cv",
  * "cell_cov","cell_r11","cell_r12","cell_r22","pt_x","pt_y","pt_wt".
  * Indices are returned 1-based. */
@@ -168,7 +170,7 @@ int ufx_cc_destroy(ufx_cc* ctx);
  *   "qp"   5 x ncell  primitives [rho, u, v, p, T]
  *   "qold" 4 x noc    "res" 4 x noc   "phi" 4 x noc
  *   "grad" 10 x noc   (d/dx, d/dy) per variable, variable-major
- *   "dt","la" noc;  "fc_la" nof;  "pt_prim" 5 x nop;  "fc_qp" 5 x nof
+ *   "dt","la" noc;  "fc_la" nof;  "pt_prim" 5 x nop;  "pt_mu" nop (get only);  "fc_qp" 5 x nof
  *   "fc_mu" nof;  "cell_mul" ncell;  "fc_grad" 10 x nof (viscous)
  * Not every name is writable ("fc_qp" is derived). */
 int ufx_cc_set(ufx_cc* ctx, const char* name, const double* src);
@@ -293,6 +295,16 @@ int ufx_nc_residue(ufx_nc* ctx, int iter, double* out);
 /* Iterations iter0+1 .. iter0+niter of NCFVM/main.f90:51-88 (tanh CFL
  * ramp); hist (5 x niter, as ufx_nc_residue's out) may be NULL. */
 int ufx_nc_iterate(ufx_nc* ctx, int iter0, int niter, double* hist);
+/* Wall forces of the current state.  NCFVM has no force routine (its
+ * find_cl_cd counterpart, CCFVM/main.F90:263-355, exists only in the
+ * cell-centred solver); this one is consistent with the wall closure of
+ * NCFVM/solid_flux.f90:20-40: per wall edge (bc 1001, ascending edge order)
+ * the end-point pressures weighted 3/4-1/4 on the half-edge right normal,
+ * summed in edge order.  out[0..3] = cl, cd, fx, fy with the dynamic
+ * pressure 0.5 gamma p_inf M^2 and the angle of attack as
+ * CCFVM/main.F90:347-348; cp (may be NULL) receives -Cp at the first end
+ * point of each wall edge, one value per wall edge in edge order. */
+int ufx_nc_forces(ufx_nc* ctx, double* out, double* cp);
 /* "qc","qold" (4 x nop) writable, "phi" (4 x nop) writable; readable also
  * "qp" (4), "grad" (8: d/dx, d/dy per variable), "res" (4), "la", "dt"
  * (nop), "fc_la" (nof). */

@@ -456,3 +456,32 @@ void orc_nc_iteration(orc_nc* s, int iter, int timemode) {
   }
   orc_nc_residue(s, iter);
 }
+
+/* Wall forces (build-defined; the reference's NCFVM has no force routine):
+ * the wall closure of solid_flux, NCFVM/solid_flux.f90:20-40, per wall edge
+ * in edge order, from the current conserved state; cl/cd as
+ * CCFVM/main.F90:347-348 with dyna = 0.5 gamma p_inf M^2. */
+void orc_nc_forces(orc_nc* s, double* out, double* cp) {
+  freestream f = fs_of(s);
+  double dyna = 0.5 * GAMMA * f.p_inf * s->mach * s->mach;
+  double fx = 0.0, fy = 0.0;
+  int k = 0;
+  for (int ie = 0; ie < s->nof; ++ie) {
+    if (s->fbc[ie] != 1001) continue;
+    int p1 = s->fp1[ie], p2 = s->fp2[ie];
+    double a[4], b[4];
+    con2prim(s->qc + p1 * 4, a, NULL);
+    con2prim(s->qc + p2 * 4, b, NULL);
+    double p1w = 0.75 * a[3] + 0.25 * b[3], p2w = 0.25 * a[3] + 0.75 * b[3];
+    double nx = 0.5 * (s->py[p2] - s->py[p1]), ny = -0.5 * (s->px[p2] - s->px[p1]);
+    fx = fx + (p1w * nx + p2w * nx);
+    fy = fy + (p1w * ny + p2w * ny);
+    if (cp) cp[k] = -((a[3] - f.p_inf) / dyna);
+    ++k;
+  }
+  double pi = 4.0 * atan(1.0), aoa = s->aoa_deg * pi / 180.0;
+  out[0] = (fy * cos(aoa) - fx * sin(aoa)) / dyna;
+  out[1] = (fy * sin(aoa) + fx * cos(aoa)) / dyna;
+  out[2] = fx;
+  out[3] = fy;
+}

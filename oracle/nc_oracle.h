@@ -55,5 +55,8 @@ void orc_nc_residue(orc_nc* s, int iter);
 double orc_nc_cfl(double cfl_max, int iter);
 /* One iteration of NCFVM/main.f90:51-88 (timemode 1: rk3, 2: lusgs). */
 void orc_nc_iteration(orc_nc* s, int iter, int timemode);
+/* Wall forces, build-defined (see nc_oracle.c): out = cl, cd, fx, fy; cp
+ * (may be NULL) -Cp at the first end point of each wall edge. */
+void orc_nc_forces(orc_nc* s, double* out, double* cp);
 
 #endif

@@ -204,6 +204,7 @@ _lib.orc_nc_residue.argtypes = [C.POINTER(_NCState), C.c_int]
 _lib.orc_nc_iteration.argtypes = [C.POINTER(_NCState), C.c_int, C.c_int]
 _lib.orc_nc_cfl.argtypes = [C.c_double, C.c_int]
 _lib.orc_nc_cfl.restype = C.c_double
+_lib.orc_nc_forces.argtypes = [C.POINTER(_NCState), _dp, _dp]
 
 
 class NCOracle:
@@ -270,3 +271,11 @@ class NCOracle:
     def iteration(self, it, timemode="lusgs"):
         _lib.orc_nc_iteration(C.byref(self.s), it, 1 if timemode == "rk3" else 2)
         return self.s.fres
+
+    def forces(self):
+        """(cl, cd, fx, fy) and -Cp per wall edge (build-defined wall force)."""
+        nw = int(np.sum(self._keep["fbc"] == 1001))
+        out = np.zeros(4)
+        cp = np.zeros(max(nw, 1))
+        _lib.orc_nc_forces(C.byref(self.s), out.ctypes.data_as(_dp), cp.ctypes.data_as(_dp))
+        return out, cp[:nw]

@@ -166,3 +166,29 @@ def test_ausm_spectral_radius_is_van_leers():
     assert np.array_equal(o.la, o2.la) and np.array_equal(o.dt, o2.dt)
     assert not np.allclose(o.res, o2.res, rtol=1e-6, atol=0)
 
+
+
+def test_wall_force_closed_forms():
+    """The build-defined wall force (solid_flux's 3/4-1/4 pressures on the
+    half-edge normals, summed over the wall edges).  The two weights of an
+    edge add up to the trapezoid rule, which is exact for a pressure linear
+    in x and y, so by the divergence theorem a state with p = p0 + b x must
+    give fx = -b A (A the area enclosed by the wall polygon; the half-edge
+    right normals point into the body) and fy = 0; a uniform state gives no
+    force.  A dropped weight, a flipped normal or a wrong end point fails."""
+    a, make = _o_grid_oracle()
+    o = make()
+    f, cp = o.forces()
+    assert np.abs(f).max() < 1e-13 * o.s.mach ** -2
+    w = np.nonzero(a["fc_bc"] == 1001)[0]
+    p1, p2 = a["fc_p1"][w] - 1, a["fc_p2"][w] - 1
+    x1, y1, x2, y2 = a["pt_x"][p1], a["pt_y"][p1], a["pt_x"][p2], a["pt_y"][p2]
+    area = abs(0.5 * np.sum(x1 * y2 - x2 * y1))
+    b, p0 = 0.7, 3.0
+    p = p0 + b * a["pt_x"]
+    o.qc[:] = np.stack([p / 0.4, np.ones_like(p), 0 * p, 0 * p], axis=1)
+    f, cp = o.forces()
+    assert abs(f[2] - (-b * area)) <= 1e-12 * b * area
+    assert abs(f[3]) <= 1e-12 * b * area
+    dyna = 0.5
+    assert np.allclose(cp, -(p[p1] - 1.0 / (1.4 * 0.5 ** 2)) / dyna, rtol=1e-12)

@@ -199,3 +199,23 @@ def test_nc_sweep_forms_agree(monkeypatch):
         for k in range(6):
             _check(f"{mode} qc it{k + 1}", q[k], q0[k], 0)
         _check(f"{mode} history", h, h0, 0)
+
+
+@pytest.mark.parametrize("flux", ["vanleer", "ausm"])
+def test_nc_wall_forces_match_oracle(flux):
+    """The wall force (the wall closure of solid_flux summed in edge order)
+    and -Cp per wall edge of a developed state, bit for bit with the
+    oracle's sequential sum."""
+    print(f"case: nc forces {flux} o-grid 40x16 seed 6", flush=True)
+    dev, orc = _setup(40, 16, 6, 1.15, "gg", "lusgs", 50.0, mach=0.5, aoa=1.5, flux=flux)
+    f0, _ = dev.forces()
+    g0, _ = orc.forces()
+    _check("free-stream forces", f0, g0, 0)
+    dev.iterate(60)
+    for it in range(1, 61):
+        orc.iteration(it, "lusgs")
+    f, cp = dev.forces()
+    g, cq = orc.forces()
+    print(f"  cl {f[0]:.6f} cd {f[1]:.6f}", flush=True)
+    _check("cl cd fx fy", f, g, 0)
+    _check("-cp", cp, cq, 0)

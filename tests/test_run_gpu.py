@@ -1,0 +1,102 @@
+"""Whole runs through the reference-format drivers (unsflow2d_amd.run): deck
+and geometry.inp in, FLO.RES / XYCpCf.dat / tecplt.dat / FLO.DAT out, the
+MinRes stop, and the FLO.DAT restart; each compared with the same run made
+directly through the solver API (and, vertex-centred, the oracle)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _nc_case(tmp, maxiter, start=1, minres=0.0):
+    from unsflow2d_amd import nc_mesh
+    m = nc_mesh.o_grid(30, 12, 4, stretch=1.2)
+    nc_mesh.write_geometry_inp(m, os.path.join(tmp, "geometry.inp"))
+    d = nc_mesh.NCDeck(mach=0.5, aoa_deg=1.5, cfl_max=50.0, timemode="lusgs", maxiter=maxiter,
+                       minres=minres)
+    with open(os.path.join(tmp, "flo.inp"), "w") as f:
+        f.write(d.text().replace("Start     1", f"Start     {start}"))
+    return m, os.path.join(tmp, "flo.inp")
+
+
+def test_run_nc_matches_direct_run_and_oracle(tmp_path):
+    from unsflow2d_amd import flo_io, nc, run
+    from oracle.oracle import NCOracle
+    print("case: run nc o-grid 30x12 seed 4, 40 LU-SGS iterations", flush=True)
+    raw, deck = _nc_case(str(tmp_path), 40)
+    r = run.run_nc(deck, out=str(tmp_path))
+    assert r["iter"] == 40
+    m = nc.NCMesh.from_raw(raw)
+    s = nc.NCSolver(m, mach=0.5, aoa_deg=1.5, cfl_max=50.0, grad="gg", timemode="lusgs")
+    h = s.iterate(40)
+    want = flo_io.flo_res_lines(range(1, 41), np.log10(h[:, 0]), [s.cfl(i) for i in range(1, 41)],
+                                np.log10(h[:, 2]))
+    assert open(tmp_path / "FLO.RES").read() == want
+    o = NCOracle(m.arrays(), mach=0.5, aoa_deg=1.5, cfl_max=50.0, grad="gg")
+    for it in range(1, 41):
+        o.iteration(it, "lusgs")
+    f, mcp = s.forces()
+    g, gcp = o.forces()
+    assert (r["cl"], r["cd"]) == (f[0], f[1]) == (g[0], g[1])
+    xy = np.loadtxt(tmp_path / "XYCpCf.dat")
+    assert np.array_equal(xy[:, 2], mcp)
+    q = flo_io.read_flow(str(tmp_path / "FLO.DAT"), raw.nop)
+    assert relerr(q, o.qc) < 5e-10
+    head = open(tmp_path / "tecplt.dat").read().splitlines()[3].split()
+    assert head == ["N=", str(raw.nop), ",E=", str(raw.noc)]
+
+
+def test_run_nc_restart_and_minres(tmp_path):
+    """Start 2 continues from FLO.DAT: one more iteration equals one
+    iteration of a solver given the same (10-digit) state; MinRes stops the
+    loop at the first iteration at or below it."""
+    from unsflow2d_amd import flo_io, nc, run
+    print("case: run nc restart o-grid 30x12 seed 4", flush=True)
+    raw, deck = _nc_case(str(tmp_path), 30)
+    run.run_nc(deck, out=str(tmp_path))
+    q0 = flo_io.read_flow(str(tmp_path / "FLO.DAT"), raw.nop)
+    _, deck2 = _nc_case(str(tmp_path), 1, start=2)
+    run.run_nc(deck2, out=str(tmp_path))
+    s = nc.NCSolver(nc.NCMesh.from_raw(raw), mach=0.5, aoa_deg=1.5, cfl_max=50.0)
+    s.set("qc", q0)
+    s.iterate(1)
+    q1 = flo_io.read_flow(str(tmp_path / "FLO.DAT"), raw.nop)
+    assert relerr(q1, s.get("qc")) < 5e-10
+    s2 = nc.NCSolver(nc.NCMesh.from_raw(raw), mach=0.5, aoa_deg=1.5, cfl_max=50.0)
+    h = s2.iterate(60)[:, 0]
+    k = 25
+    _, deck3 = _nc_case(str(tmp_path), 60, minres=h[k - 1])
+    r = run.run_nc(deck3, out=str(tmp_path))
+    first = int(np.argmax(h <= h[k - 1])) + 1
+    assert r["iter"] == first
+    assert len(open(tmp_path / "FLO.RES").read().splitlines()) == first
+
+
+def test_run_cc_matches_direct_run(tmp_path):
+    from unsflow2d_amd import cc, flo_io, meshgen, run
+    print("case: run cc bump 24x12 seed 3, 30 RK3 iterations", flush=True)
+    raw = meshgen.bump_channel(24, 12, 3, h=0.05)
+    meshgen.write_geometry_inp(raw, str(tmp_path / "geometry.inp"))
+    d = meshgen.Deck(mach=0.6, cfl_max=0.8, cfl_min=0.8, maxiter=30, grad_type="lsqr",
+                     flux_type="roe")
+    open(tmp_path / "flo.inp", "w").write(d.text())
+    r = run.run_cc(str(tmp_path / "flo.inp"), out=str(tmp_path))
+    m = cc.Mesh.from_raw(raw)
+    s = cc.Solver(m, cc.Config(mach=0.6, cfl_max=0.8, cfl_min=0.8, grad="lsqr", flux="roe"))
+    h = s.iterate(30)
+    lines = open(tmp_path / "FLO.RES").read().splitlines()
+    assert lines == flo_io.flo_res_lines(range(1, 31), np.log10(h), [0.8] * 30).splitlines()
+    fo = s.forces(wall=True)
+    assert (r["cl"], r["cd"]) == (fo["cl"], fo["cd"])
+    xy = np.loadtxt(tmp_path / "XYCpCf.dat")
+    assert np.array_equal(xy[:, 2], fo["mcp"]) and np.array_equal(xy[:, 3], fo["cf"])
+    q = flo_io.read_flow(str(tmp_path / "FLO.DAT"), m.counts["noc"])
+    assert relerr(q, s.get("qc")[:m.counts["noc"]]) < 5e-10
+    t = open(tmp_path / "tecplt.dat").read().splitlines()
+    assert t[3].split() == ["N=", str(raw.nop), ",E=", str(raw.noc)]
+    assert len(t) == 4 + 2 * raw.nop + raw.noc

@@ -120,6 +120,8 @@ def _i(a: np.ndarray):
 
 MESH_I32 = ["fc_p1", "fc_p2", "fc_in", "fc_out", "fc_bc", "cell_nc2c", "cell_nc2f", "cell_c2f",
             "pt_nv2c", "pt_v2c", "oldnum", "counts"]
+# the vertex walk round each cell (output only; empty for an imported mesh)
+MESH_I32_WALK = ["cell_nc2v", "cell_c2v"]
 MESH_F64 = ["fc_nx", "fc_ny", "fc_area", "fc_cenx", "fc_ceny", "fc_cov", "cell_cenx",
             "cell_ceny", "cell_cv", "cell_cov", "cell_r11", "cell_r12", "cell_r22", "pt_x",
             "pt_y", "pt_wt"]
@@ -182,7 +184,7 @@ class Mesh:
     def get(self, name: str) -> np.ndarray:
         n = C.c_int64()
         _check(lib.ufx_mesh_size(self._h, name.encode(), C.byref(n)))
-        dt = np.int32 if name in MESH_I32 else np.float64
+        dt = np.int32 if name in MESH_I32 or name in MESH_I32_WALK else np.float64
         out = np.empty(n.value, dtype=dt)
         _check(lib.ufx_mesh_get(self._h, name.encode(), out.ctypes.data_as(_vp)))
         return out
@@ -262,7 +264,8 @@ class Solver:
         self.cfl = float("nan")
 
     NVAR = {"qc": 4, "qp": 5, "qold": 4, "res": 4, "phi": 4, "grad": 10, "dt": 1, "la": 1,
-            "fc_la": 1, "fc_mu": 1, "cell_mul": 1, "pt_prim": 5, "fc_grad": 10}
+            "fc_la": 1, "fc_mu": 1, "cell_mul": 1, "pt_prim": 5, "pt_mu": 1,
+            "fc_grad": 10}
 
     def _count(self, name):
         c = self.counts
@@ -310,11 +313,21 @@ class Solver:
             _check(rc)
         return bad.value
 
-    def forces(self):
+    def forces(self, wall: bool = False):
+        """find_cl_cd (CCFVM/main.F90:263-355).  With ``wall`` also -Cp and Cf
+        per wall face (bc 1001) in boundary order, under "mcp" and "cf"."""
         out = np.zeros(8)
-        _check(lib.ufx_cc_forces(self._h, _d(out), None, None))
+        mcp = cf = None
+        if wall:
+            nw = int(np.sum(self.mesh.get("fc_bc") == 1001))
+            mcp, cf = np.zeros(max(nw, 1)), np.zeros(max(nw, 1))
+        _check(lib.ufx_cc_forces(self._h, _d(out), _d(mcp) if wall else None,
+                                 _d(cf) if wall else None))
         keys = ["cl", "cd", "cl1", "cd1", "cm", "fx1", "fy1", "dtglobal"]
-        return dict(zip(keys, out.tolist()))
+        res = dict(zip(keys, out.tolist()))
+        if wall:
+            res["mcp"], res["cf"] = mcp[:nw], cf[:nw]
+        return res
 
     def iterate(self, niter: int, hist: bool = True):
         """Run ``niter`` iterations of the reference's main loop on the device."""

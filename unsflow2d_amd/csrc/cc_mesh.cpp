@@ -223,7 +223,10 @@ int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
   }
 
   // ---- cell_2_vertex (CCFVM/geometric.F90:533-616): its walk round each
-  // cell reorders the cell's face list, which fixes the LU-SGS sweep order.
+  // cell reorders the cell's face list, which fixes the LU-SGS sweep order;
+  // the walk, closed with its first vertex (:611-613), is kept for output.
+  m.c2v_ptr.assign(noc + 1, 0);
+  m.c2v.clear();
   for (int i = 0; i < noc; ++i) {
     std::vector<int>& f = c2f[i];
     int nf = (int)f.size();
@@ -244,6 +247,9 @@ int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
         }
       }
     }
+    c2v.push_back(c2v.front());
+    m.c2v.insert(m.c2v.end(), c2v.begin(), c2v.end());
+    m.c2v_ptr[i + 1] = (int)m.c2v.size();
   }
   m.c2f_ptr.assign(noc + 1, 0);
   for (int i = 0; i < noc; ++i) m.c2f_ptr[i + 1] = m.c2f_ptr[i] + (int)c2f[i].size();

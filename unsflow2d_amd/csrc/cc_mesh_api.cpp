@@ -142,6 +142,8 @@ int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count) {
   if (n == "cell_cv" || n == "cell_r11" || n == "cell_r12" || n == "cell_r22" ||
       n == "cell_nc2c" || n == "cell_nc2f" || n == "oldnum") v = m.noc;
   if (n == "cell_c2f") v = (int64_t)m.c2f.size();
+  if (n == "cell_nc2v") v = m.c2v_ptr.empty() ? 0 : m.noc;
+  if (n == "cell_c2v") v = (int64_t)m.c2v.size();
   if (n == "pt_x" || n == "pt_y" || n == "pt_nv2c") v = m.nop;
   if (n == "pt_v2c" || n == "pt_wt") v = (int64_t)m.v2c.size();
   if (n == "counts") v = 4;
@@ -220,6 +222,8 @@ int ufx_mesh_get(const ufx_mesh* mesh, const char* name, void* dst) {
   else if (n == "cell_nc2c") put0(m.nc2c);
   else if (n == "cell_nc2f") { for (int i = 0; i < m.noc; ++i) di[i] = m.c2f_ptr[i + 1] - m.c2f_ptr[i]; }
   else if (n == "cell_c2f") put1(m.c2f);
+  else if (n == "cell_nc2v") { for (int i = 0; i < m.noc; ++i) di[i] = m.c2v_ptr[i + 1] - m.c2v_ptr[i]; }
+  else if (n == "cell_c2v") put1(m.c2v);
   else if (n == "pt_x") putd(m.px);
   else if (n == "pt_y") putd(m.py);
   else if (n == "pt_nv2c") { for (int i = 0; i < m.nop; ++i) di[i] = m.v2c_ptr[i + 1] - m.v2c_ptr[i]; }

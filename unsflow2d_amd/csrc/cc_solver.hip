@@ -535,6 +535,13 @@ int get_field(ufx_cc* x, DevT<R>& d, const std::string& n, double* dst) {
       for (int k = 0; k < 5; ++k) dst[(size_t)i * 5 + k] = t[(size_t)x->pt_of_ref[i] * PW + k];
     return UFX_OK;
   }
+  if (n == "pt_mu") {
+    std::vector<R> t((size_t)nop * PW);
+    HIPCHK(copy(t.data(), d.prec, t.size() * sizeof(R)));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int i = 0; i < nop; ++i) dst[i] = t[(size_t)x->pt_of_ref[i] * PW + P_MU];
+    return UFX_OK;
+  }
   if (n == "cell_mul") {
     std::vector<R> t((size_t)ncell);
     HIPCHK(copy(t.data(), d.cmul, t.size() * sizeof(R)));

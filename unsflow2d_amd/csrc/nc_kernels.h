@@ -634,4 +634,60 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_nc_lusgs_sweep(DevNC d, NCFre
   }
 }
 
+// Wall forces of the vertex-centred path.  The reference has no force
+// routine for it; here the force is the wall closure of solid_flux
+// (NCFVM/solid_flux.f90:20-40) summed over the wall edges: per wall edge the
+// end-point pressures weighted 3/4-1/4 on the half-edge right normal, i.e.
+// the pressure terms the wall adds to the momentum residuals of its two end
+// points.  Pressures come from the current conserved state (the expressions
+// of save_old).
+//   k_nc_wall_terms: per wall edge (edge order) its force contribution and
+//                    -Cp at its first end point;
+//   k_nc_forces_sum: one wave adds the contributions in edge order.
+__device__ __forceinline__ double nc_pressure(const NCFree& g, const double* q) {
+  double rho = q[1], u = q[2] / q[1], v = q[3] / q[1], et = q[0];
+  double qq = u * u + v * v;
+  return (g.gamma - 1.0) * (et - 0.5 * rho * qq);
+}
+
+__global__ __launch_bounds__(BLK) void k_nc_wall_terms(DevNC d, NCFree g, const int* wedge,
+                                                       int nw, double dyna, double* fe,
+                                                       double* cp) {
+  int k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= nw) return;
+  int2 pp = d.ep[wedge[k]];
+  double q1[4], q2[4], x1, y1, x2, y2;
+  ld4d(d.qc + (size_t)pp.x * 4, q1);
+  ld4d(d.qc + (size_t)pp.y * 4, q2);
+  ld2d(d.vrec + (size_t)pp.x * NW + R_CEN, x1, y1);
+  ld2d(d.vrec + (size_t)pp.y * NW + R_CEN, x2, y2);
+  double pr1 = nc_pressure(g, q1), pr2 = nc_pressure(g, q2);
+  double p1w = 0.75 * pr1 + 0.25 * pr2, p2w = 0.25 * pr1 + 0.75 * pr2;
+  double nx = 0.5 * (y2 - y1), ny = -0.5 * (x2 - x1);
+  fe[2 * (size_t)k] = p1w * nx + p2w * nx;
+  fe[2 * (size_t)k + 1] = p1w * ny + p2w * ny;
+  cp[k] = -((pr1 - g.p_inf) / dyna);
+}
+
+__global__ __launch_bounds__(64) void k_nc_forces_sum(const double* fe, int nw, double* out) {
+  double sx = 0.0, sy = 0.0;
+  for (int b = 0; b < nw; b += 64) {
+    int k = b + (int)threadIdx.x;
+    double ex = 0.0, ey = 0.0;
+    if (k < nw) {
+      ex = fe[2 * (size_t)k];
+      ey = fe[2 * (size_t)k + 1];
+    }
+    const int m = min(64, nw - b);
+    for (int j = 0; j < m; ++j) {
+      sx = sx + __shfl(ex, j);
+      sy = sy + __shfl(ey, j);
+    }
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sx;
+    out[1] = sy;
+  }
+}
+
 }  // namespace ufx

@@ -40,6 +40,9 @@ struct ufx_nc {
   std::vector<int> lv_fwd_off, lv_bwd_off;  // offsets into lv_verts
   int *d_fwd_off = nullptr, *d_bwd_off = nullptr;  // the same on the device
   SweepSlot* d_slots = nullptr;             // packed sweep slots, forward then backward
+  int nwall = 0;                            // wall edges (bc 1001)
+  int* d_wedge = nullptr;                   // their edge numbers, ascending
+  double *d_fe = nullptr, *d_cp = nullptr;  // [nwall][2] force terms, [nwall] -Cp
   int sweep_mode = 0;                       // 0 graph of level launches, 1 level launches, 2 one workgroup
   hipGraph_t sweep_graph = nullptr;         // diagonal + both sweeps, captured once
   hipGraphExec_t sweep_exec = nullptr;
@@ -416,6 +419,15 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   NAL(qold, 4 * (size_t)nop); NAL(res, 4 * (size_t)nop); NAL(la, nop); NAL(dt, nop);
   NAL(dd, nop); NAL(dqc, 4 * (size_t)nop); NAL(fla, nof); NAL(part, 4 * (size_t)x->nblk);
   NAL(scal, 4); NAL(err, 4);
+  {
+    std::vector<int> wedge;
+    for (int e = 0; e < nof; ++e)
+      if (m.fbc[e] == 1001) wedge.push_back(e);
+    x->nwall = (int)wedge.size();
+    if (!rc) rc = nc_upload(x, &x->d_wedge, wedge);
+    if (!rc) rc = nc_alloc(x, &x->d_fe, 2 * (size_t)x->nwall + 2);
+    if (!rc) rc = nc_alloc(x, &x->d_cp, (size_t)x->nwall);
+  }
 #undef NUP
 #undef NAL
   if (!rc && hipStreamSynchronize(x->stream) != hipSuccess) {
@@ -461,6 +473,33 @@ int ufx_nc_residue(ufx_nc* x, int iter, double* out) {
   NCCHK(hipMemcpyAsync(h, x->d_hist, sizeof(h), hipMemcpyDeviceToHost, x->stream));
   NCCHK(hipStreamSynchronize(x->stream));
   if (out) std::memcpy(out, h, sizeof(h));
+  return UFX_OK;
+}
+
+int ufx_nc_forces(ufx_nc* x, double* out, double* cp) {
+  if (int rc = check(x)) return rc;
+  if (!out) { set_error("ufx_nc_forces: out is NULL"); return UFX_ERR_ARG; }
+  const NCFree& g = x->g;
+  const double dyna = 0.5 * g.gamma * g.p_inf * x->prm.mach * x->prm.mach;
+  if (x->nwall > 0) {
+    k_nc_wall_terms<<<nblk_of(x->nwall), BLK, 0, x->stream>>>(x->d, g, x->d_wedge, x->nwall, dyna,
+                                                              x->d_fe, x->d_cp);
+    k_nc_forces_sum<<<1, 64, 0, x->stream>>>(x->d_fe, x->nwall, x->d_fe + 2 * (size_t)x->nwall);
+    NCCHK(hipGetLastError());
+  }
+  double f[2] = {0.0, 0.0};
+  if (x->nwall > 0)
+    NCCHK(hipMemcpyAsync(f, x->d_fe + 2 * (size_t)x->nwall, sizeof(f), hipMemcpyDeviceToHost,
+                         x->stream));
+  if (cp && x->nwall > 0)
+    NCCHK(hipMemcpyAsync(cp, x->d_cp, sizeof(double) * x->nwall, hipMemcpyDeviceToHost,
+                         x->stream));
+  NCCHK(hipStreamSynchronize(x->stream));
+  const double pi = 4.0 * std::atan(1.0), aoa = x->prm.aoa_deg * pi / 180.0;
+  out[0] = (f[1] * std::cos(aoa) - f[0] * std::sin(aoa)) / dyna;
+  out[1] = (f[1] * std::sin(aoa) + f[0] * std::cos(aoa)) / dyna;
+  out[2] = f[0];
+  out[3] = f[1];
   return UFX_OK;
 }
 

@@ -34,6 +34,7 @@ struct Mesh {
   std::vector<double> ccv, r11, r12, r22;  // noc
   std::vector<int> nc2c;                // noc
   std::vector<int> c2f_ptr, c2f;        // LU-SGS face order per cell
+  std::vector<int> c2v_ptr, c2v;        // vertex walk round each cell, closed (first repeated)
   std::vector<int> v2c_ptr, v2c;        // vertex -> cells (ghosts included)
   std::vector<double> wt;               // pseudo-Laplacian weights per v2c entry
   std::vector<int> oldnum;              // new cell -> input cell (0-based)

@@ -44,11 +44,12 @@ _sig("ufx_nc_get", [_vp, C.c_char_p, _dp])
 _sig("ufx_nc_sync", [_vp])
 _sig("ufx_nc_profile", [_vp, C.c_int])
 _sig("ufx_nc_profile_read", [_vp, _dp, _ip])
+_sig("ufx_nc_forces", [_vp, _dp, _dp])
 
 NC_EXPORTED = ["ufx_nc_geometric", "ufx_nc_create", "ufx_nc_destroy", "ufx_nc_save_old",
                "ufx_nc_fvresidual", "ufx_nc_rk_stage", "ufx_nc_lusgs", "ufx_nc_residue",
                "ufx_nc_iterate", "ufx_nc_set", "ufx_nc_get", "ufx_nc_sync", "ufx_nc_profile",
-               "ufx_nc_profile_read"]
+               "ufx_nc_profile_read", "ufx_nc_forces"]
 NKCAT = ["save_old", "gradient", "residual", "norm", "lusgs"]
 
 NC_MESH_I32 = ["fc_p1", "fc_p2", "fc_in", "fc_out", "fc_bc", "pt_bc", "pt_nv2v", "pt_v2v",
@@ -144,6 +145,17 @@ class NCSolver:
         _check(lib.ufx_nc_iterate(self._h, self.iter, n, _d(h) if hist else None))
         self.iter += n
         return h
+
+    def forces(self):
+        """Wall force of the current state (the wall closure of solid_flux
+        summed in edge order; the reference's vertex-centred solver has no
+        force routine): (cl, cd, fx, fy) and -Cp at the first end point of
+        each wall edge, in edge order."""
+        nw = int(np.sum(self.mesh.get("fc_bc") == 1001))
+        out = np.zeros(4)
+        cp = np.zeros(max(nw, 1))
+        _check(lib.ufx_nc_forces(self._h, _d(out), _d(cp)))
+        return out, cp[:nw]
 
     def get(self, name):
         n = self.counts["nof"] if name == "fc_la" else self.counts["nop"]

@@ -183,6 +183,21 @@ def write_geometry_inp(mesh: NCRawMesh, path: str) -> None:
                            mesh.ebc]))
 
 
+def read_geometry_inp(path: str) -> NCRawMesh:
+    """Parse a vertex-centred ``geometry.inp`` (NCFVM/read_input.f90:101-126)
+    back into an :class:`NCRawMesh` (the triangles' vertex lists are not part
+    of the file and are left zero)."""
+    with open(path) as f:
+        nop, noc, nof = (int(t) for t in f.readline().split()[:3])
+        pts = np.loadtxt(f, max_rows=nop, ndmin=2)
+        cel = np.loadtxt(f, max_rows=noc, ndmin=2)
+        edg = np.loadtxt(f, max_rows=nof, ndmin=2)
+    return NCRawMesh(x=pts[:, 1].copy(), y=pts[:, 2].copy(), pbc=pts[:, 3].astype(np.int32),
+                     cen=cel[:, 1:3].copy(), area=cel[:, 3].copy(),
+                     edge=edg[:, 1:5].astype(np.int32), ebc=edg[:, 7].astype(np.int32),
+                     tri=np.zeros((noc, 3), dtype=np.int32))
+
+
 @dataclasses.dataclass
 class NCDeck:
     """The vertex-centred input deck (NCFVM/read_input.f90:23-40)."""
