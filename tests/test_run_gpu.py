@@ -100,3 +100,27 @@ def test_run_cc_matches_direct_run(tmp_path):
     t = open(tmp_path / "tecplt.dat").read().splitlines()
     assert t[3].split() == ["N=", str(raw.nop), ",E=", str(raw.noc)]
     assert len(t) == 4 + 2 * raw.nop + raw.noc
+
+
+def test_fortran_nc_host_matches_python_driver(tmp_path):
+    """nc2d_gpu (Fortran over the ISO_C_BINDING interface, the reference's
+    NCFVM main restated) writes the same FLO.RES and FLO.DAT, character for
+    character, as the Python driver: the same device results, formatted by
+    the Fortran runtime on one side and by flo_io on the other."""
+    import subprocess
+    from conftest import ROOT
+    from unsflow2d_amd import run
+    exe = os.path.join(ROOT, "unsflow2d_amd", "fortran", "nc2d_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (amdflang absent at build time)")
+    print("case: fortran nc host o-grid 30x12 seed 4, 25 iterations", flush=True)
+    a, b = tmp_path / "f", tmp_path / "p"
+    a.mkdir()
+    b.mkdir()
+    _, deck = _nc_case(str(a), 25)
+    r = subprocess.run([exe, "flo.inp"], cwd=a, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, deck_b = _nc_case(str(b), 25)
+    run.run_nc(deck_b, out=str(b))
+    assert (a / "FLO.RES").read_text() == (b / "FLO.RES").read_text()
+    assert (a / "FLO.DAT").read_text() == (b / "FLO.DAT").read_text()

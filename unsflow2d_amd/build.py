@@ -81,16 +81,19 @@ def build_fortran() -> str | None:
         print("amdflang not found: Fortran host not built", flush=True)
         return None
     fdir = os.path.join(PKG, "fortran")
-    srcs = ["ufx_api.f90", "ufx_cc.f90", "flo2d_gpu.f90"]
+    srcs = ["ufx_api.f90", "ufx_cc.f90", "flo2d_gpu.f90", "ufx_nc.f90", "nc2d_gpu.f90"]
     if not all(os.path.exists(os.path.join(fdir, s)) for s in srcs):
         return None
-    out = os.path.join(fdir, "flo2d_gpu")
     for s in srcs:
         _run([fc, "-O2", "-ffree-form", "-module-dir", fdir, "-c", os.path.join(fdir, s),
               "-o", os.path.join(fdir, s.replace(".f90", ".o"))])
-    objs = [os.path.join(fdir, s.replace(".f90", ".o")) for s in srcs]
-    _run([fc, "-o", out, *objs, "-L", PKG, "-lufx", f"-Wl,-rpath,{PKG}",
-          "-Wl,-rpath,/opt/rocm/lib"])
+    obj = lambda s: os.path.join(fdir, s.replace(".f90", ".o"))
+    link = ["-L", PKG, "-lufx", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,/opt/rocm/lib"]
+    # the cell-centred host (the reference's CCFVM main) and the vertex-centred one (NCFVM main)
+    _run([fc, "-o", os.path.join(fdir, "flo2d_gpu"), obj("ufx_api.f90"), obj("ufx_cc.f90"),
+          obj("flo2d_gpu.f90"), *link])
+    out = os.path.join(fdir, "nc2d_gpu")
+    _run([fc, "-o", out, obj("ufx_api.f90"), obj("ufx_nc.f90"), obj("nc2d_gpu.f90"), *link])
     return out
 
 

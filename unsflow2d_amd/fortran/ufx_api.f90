@@ -18,6 +18,14 @@ integer(c_int), parameter :: UFX_FLOW_INVISCID = 1, UFX_FLOW_LAMINAR = 2
 integer(c_int), parameter :: UFX_TIME_RK3 = 1, UFX_TIME_LUSGS = 2
 integer(c_int), parameter :: UFX_PREC_FP64 = 0, UFX_PREC_MIXED = 1
 
+integer(c_int), parameter :: UFX_NC_LIM_VKN = 0, UFX_NC_LIM_BJ = 1
+
+! ufx_nc_params
+type, bind(C) :: ufx_nc_params
+   real(c_double) :: mach, aoa_deg, cfl_max
+   integer(c_int) :: grad, timemode, precision, flux, limiter
+end type ufx_nc_params
+
 ! ufx_cc_params
 type, bind(C) :: ufx_cc_params
    real(c_double) :: mach, aoa_deg, reynolds, t_wall
@@ -144,6 +152,94 @@ interface
       real(c_double), intent(out) :: hist(*)
    end function
    integer(c_int) function ufx_cc_sync(ctx) bind(C, name="ufx_cc_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+   end function
+
+   ! ---------------------------------------------- vertex-centred (NCFVM)
+   integer(c_int) function ufx_nc_geometric(nop, noc, nof, px, py, pbc, ccx, ccy, cv, &
+         e_p1, e_p2, e_in, e_out, e_bc, mesh) bind(C, name="ufx_nc_geometric")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nop, noc, nof
+      real(c_double), intent(in) :: px(*), py(*)
+      integer(c_int), intent(in) :: pbc(*)
+      real(c_double), intent(in) :: ccx(*), ccy(*), cv(*)
+      integer(c_int), intent(in) :: e_p1(*), e_p2(*), e_in(*), e_out(*), e_bc(*)
+      type(c_ptr) :: mesh
+   end function
+
+   integer(c_int) function ufx_nc_create(mesh, prm, ctx) bind(C, name="ufx_nc_create")
+      import :: c_int, c_ptr, ufx_nc_params
+      type(c_ptr), value :: mesh
+      type(ufx_nc_params), intent(in) :: prm
+      type(c_ptr) :: ctx
+   end function
+
+   integer(c_int) function ufx_nc_destroy(ctx) bind(C, name="ufx_nc_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+   end function
+
+   integer(c_int) function ufx_nc_save_old(ctx) bind(C, name="ufx_nc_save_old")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+   end function
+
+   integer(c_int) function ufx_nc_fvresidual(ctx, cfl) bind(C, name="ufx_nc_fvresidual")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: cfl
+   end function
+
+   integer(c_int) function ufx_nc_rk_stage(ctx, stage, cfl) bind(C, name="ufx_nc_rk_stage")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: stage
+      real(c_double), value :: cfl
+   end function
+
+   integer(c_int) function ufx_nc_lusgs(ctx, cfl) bind(C, name="ufx_nc_lusgs")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: cfl
+   end function
+
+   integer(c_int) function ufx_nc_residue(ctx, iter, out) bind(C, name="ufx_nc_residue")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iter
+      real(c_double), intent(out) :: out(5)
+   end function
+
+   integer(c_int) function ufx_nc_iterate(ctx, iter0, niter, hist) bind(C, name="ufx_nc_iterate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iter0, niter
+      type(c_ptr), value :: hist
+   end function
+
+   integer(c_int) function ufx_nc_set(ctx, name, src) bind(C, name="ufx_nc_set")
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), intent(in) :: src(*)
+   end function
+
+   integer(c_int) function ufx_nc_get(ctx, name, dst) bind(C, name="ufx_nc_get")
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: name(*)
+      real(c_double), intent(out) :: dst(*)
+   end function
+
+   integer(c_int) function ufx_nc_forces(ctx, out, cp) bind(C, name="ufx_nc_forces")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(4)
+      type(c_ptr), value :: cp
+   end function
+
+   integer(c_int) function ufx_nc_sync(ctx) bind(C, name="ufx_nc_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
    end function
