@@ -376,6 +376,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       int p = verts[s], dir = s < (size_t)nop ? 0 : 1, cnt = 0;
       SweepSlot& sl = slots[s];
       sl.vert = p;
+      for (int k = 0; k < SWQ; ++k) sl.eo[k] = make_int2(0, 0);  // unused slots: in bounds
       for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
         int e = m.v2f[j];
         bool first = m.fp1[e] == p;
