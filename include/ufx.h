@@ -225,6 +225,25 @@ int ufx_cc_forces(ufx_cc* ctx, double* out, double* cp, double* cf);
  * call synchronising.  Without hist no host synchronisation happens. */
 int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double* hist);
 
+/* The same iterations replayed from a HIP graph (no reference counterpart;
+ * for batched runs and serving loops, where the launch cost of small meshes
+ * dominates).  Iterations iter0+1 .. iter0+niter with one constant CFL, from
+ * iteration 2 on (iter0 >= 1: iteration 1 fixes the residual normalisation;
+ * run it with ufx_cc_iterate).  The chunk is captured on the first call and
+ * replayed while niter and cfl stay the same.  hist as ufx_cc_iterate. */
+int ufx_cc_iterate_graph(ufx_cc* ctx, int iter0, int niter, double cfl, double* hist);
+
+/* Keep the residual of the last stage of every whole iteration
+ * (ufx_cc_iterate, ufx_cc_iterate_graph) in the context, for
+ * ufx_cc_get("res") and ufx_cc_residual_norm.  No reference counterpart:
+ * the fused stages otherwise reduce the residual without storing it.  Costs
+ * one more write of the residual per iteration. */
+int ufx_cc_keep_residual(ufx_cc* ctx, int on);
+/* sqrt(sum_i res_i(var)^2 / noc) of the stored residual for one conserved
+ * variable (0 E, 1 rho, 2 rho u, 3 rho v), or with var = -1 of all four (the
+ * reference's raw norm, CCFVM/solution.F90:159-166); synchronising. */
+int ufx_cc_residual_norm(ufx_cc* ctx, int var, double* out);
+
 /* Kernel timing (no reference counterpart; instrumentation for benchmarks).
  * When enabled, a pair of HIP events is recorded on the context's stream
  * around every kernel launch; ufx_cc_profile_read (synchronising) returns,

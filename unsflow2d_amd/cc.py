@@ -89,6 +89,9 @@ _sig("ufx_cc_residue", [_vp, C.c_int, _dp, _dp])
 _sig("ufx_cc_check_positivity", [_vp, _ip])
 _sig("ufx_cc_forces", [_vp, _dp, _dp, _dp])
 _sig("ufx_cc_iterate", [_vp, C.c_int, C.c_int, _dp, _dp])
+_sig("ufx_cc_iterate_graph", [_vp, C.c_int, C.c_int, C.c_double, _dp])
+_sig("ufx_cc_keep_residual", [_vp, C.c_int])
+_sig("ufx_cc_residual_norm", [_vp, C.c_int, _dp])
 _sig("ufx_cc_stream", [_vp], C.c_void_p)
 _sig("ufx_cc_profile", [_vp, C.c_int])
 _sig("ufx_cc_profile_read", [_vp, _dp, _ip])
@@ -101,6 +104,7 @@ EXPORTED = [
     "ufx_cc_prim", "ufx_cc_avg_c2v", "ufx_cc_time_step", "ufx_cc_save_old", "ufx_cc_gradient",
     "ufx_cc_limit", "ufx_cc_fvresidual", "ufx_cc_rk_update", "ufx_cc_rk_stage", "ufx_cc_lusgs",
     "ufx_cc_residue", "ufx_cc_check_positivity", "ufx_cc_forces", "ufx_cc_iterate",
+    "ufx_cc_iterate_graph", "ufx_cc_keep_residual", "ufx_cc_residual_norm",
     "ufx_cc_sync", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
 ]
 
@@ -341,6 +345,35 @@ class Solver:
         self.iter += niter
         self.cfl = c
         return h
+
+    def iterate_graph(self, niter: int, hist: bool = True):
+        """``niter`` iterations replayed from a captured HIP graph (from
+        iteration 2 on, while the CFL stays constant); the same results as
+        :meth:`iterate`."""
+        if self.iter < 1:
+            raise ValueError("iterate_graph starts at iteration 2: run iterate(1) first")
+        c = self.cfl
+        for k in range(niter):
+            c2 = self.cfg.cfl(self.iter + 1 + k, c)
+            if k and c2 != c:
+                raise ValueError("iterate_graph needs a constant CFL over the chunk")
+            c = c2
+        h = np.empty(niter) if hist else None
+        _check(lib.ufx_cc_iterate_graph(self._h, self.iter, niter, c, _d(h) if hist else None))
+        self.iter += niter
+        self.cfl = c
+        return h
+
+    def keep_residual(self, on: bool = True) -> None:
+        """Whole iterations also store the last stage's residual."""
+        _check(lib.ufx_cc_keep_residual(self._h, 1 if on else 0))
+
+    def residual_norm(self, var: int = -1) -> float:
+        """L2 norm of the stored residual: one variable (0 E, 1 rho, 2 rho u,
+        3 rho v) or all four (-1)."""
+        out = C.c_double()
+        _check(lib.ufx_cc_residual_norm(self._h, var, C.byref(out)))
+        return out.value
 
     def profile(self, enable: bool = True) -> None:
         """Start (or stop) per-kernel HIP-event timing on the context's stream."""

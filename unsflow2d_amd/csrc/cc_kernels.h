@@ -941,6 +941,34 @@ __global__ __launch_bounds__(BLK) void k_sumsq(DevT<R> d) {
   if (threadIdx.x == 0) d.part[blockIdx.x] = sq;
 }
 
+// Block partials of the squares of one stored residual variable (var 0..3)
+// or of all four (var -1).
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_sumsq_var(DevT<R> d, int var) {
+  __shared__ double sh[1][BLK];
+  int c = cell_index();
+  double sq = 0.0;
+  if (c < d.noc) {
+    double r[4];
+    ld4d(d.res + (size_t)c * 4, r);
+    if (var < 0)
+      for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];
+    else
+      sq = r[var] * r[var];
+  }
+  block_reduce<1>(sh, &sq, OpAdd());
+  if (threadIdx.x == 0) d.part[blockIdx.x] = sq;
+}
+
+template <typename R>
+__global__ void k_norm_var_final(DevT<R> d, int nblk, int var, double* out) {
+  __shared__ double sh[1][BLK];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
+  block_reduce<1>(sh, &s, OpAdd());
+  if (threadIdx.x == 0) out[0] = var < 0 ? sqrt(s / d.noc / 4) : sqrt(s / d.noc);
+}
+
 // L2 norm (CCFVM/solution.F90:159-166): raw norm, iteration-1 normalisation,
 // normalised history entry.
 template <typename R>

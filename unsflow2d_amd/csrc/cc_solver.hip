@@ -58,6 +58,12 @@ struct ufx_cc {
   size_t stage_lds = 0;          // LDS bytes per block of the stage kernel
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
+  // a captured chunk of whole iterations (ufx_cc_iterate_graph)
+  hipGraphExec_t chunk_exec = nullptr;
+  int chunk_n = 0;
+  double chunk_cfl = 0.0;
+  bool keep_res = false;           // whole iterations also store the last residual
+  double* d_out = nullptr;         // small device results (residual norms)
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
   double* d_hist = nullptr;
   int hist_cap = 0;
@@ -429,6 +435,10 @@ int run_lusgs(ufx_cc* x, DevT<R>& d) {
 
 int ensure_hist(ufx_cc* x, int n) {
   if (n <= x->hist_cap) return UFX_OK;
+  // a captured chunk writes the old buffer: it goes with it
+  if (x->stream) HIPCHK(hipStreamSynchronize(x->stream));
+  if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
+  x->chunk_exec = nullptr;
   if (x->d_hist) HIPCHK(hipFree(x->d_hist));
   x->d_hist = nullptr;
   HIPCHK(hipMalloc((void**)&x->d_hist, (size_t)n * sizeof(double)));
@@ -452,7 +462,7 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
   if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
   if (x->p.timemode == UFX_TIME_RK3) {
     for (int s = 1; s <= 3; ++s)
-      if (int rc = run_stage(x, d, s, s == 3, false)) return rc;
+      if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res)) return rc;
   } else {
     if (int rc = run_lusgs(x, d)) return rc;
     LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
@@ -761,6 +771,8 @@ int ufx_cc_destroy(ufx_cc* x) {
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   for (void* p : x->allocs) (void)hipFree(p);
   if (x->d_hist) (void)hipFree(x->d_hist);
+  if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
+  if (x->d_out) (void)hipFree(x->d_out);
   for (hipEvent_t e : x->ev_pool) (void)hipEventDestroy(e);
   if (x->stream) (void)hipStreamDestroy(x->stream);
   delete x;
@@ -1236,6 +1248,77 @@ int ufx_cc_iterate(ufx_cc* x, int iter0, int niter, const double* cfl, double* h
     return device_error(x);
   }
   return UFX_OK;
+}
+
+int ufx_cc_iterate_graph(ufx_cc* x, int iter0, int niter, double cfl, double* hist) {
+  if (int rc = check_ctx(x)) return rc;
+  if (niter <= 0 || iter0 < 1) {
+    set_error("ufx_cc_iterate_graph: needs niter >= 1 and iter0 >= 1 (iteration 1 sets the "
+              "residual normalisation)");
+    return UFX_ERR_ARG;
+  }
+  if (x->prof) { set_error("ufx_cc_iterate_graph: not while profiling"); return UFX_ERR_ARG; }
+  if (int rc = ensure_hist(x, niter)) return rc;
+  // the launches of a chunk depend only on the CFL and the chunk length:
+  // capture once, replay while both stay the same
+  if (!x->chunk_exec || x->chunk_n != niter || x->chunk_cfl != cfl) {
+    if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
+    x->chunk_exec = nullptr;
+    HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < niter && !rc; ++k) {
+      x->cfl = cfl;
+      rc = with_dev(x, [&](auto& d) { return run_iteration(x, d, 2, k); });
+    }
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(x->stream, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    HIPCHK(e);
+    hipError_t ie = hipGraphInstantiate(&x->chunk_exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIPCHK(ie);
+    x->chunk_n = niter;
+    x->chunk_cfl = cfl;
+  }
+  (void)iter0;
+  x->cfl = cfl;
+  HIPCHK(hipGraphLaunch(x->chunk_exec, x->stream));
+  if (hist) {
+    HIPCHK(hipMemcpyAsync(hist, x->d_hist, (size_t)niter * 8, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return device_error(x);
+  }
+  return UFX_OK;
+}
+
+int ufx_cc_keep_residual(ufx_cc* x, int on) {
+  if (int rc = check_ctx(x)) return rc;
+  if (x->keep_res != (on != 0)) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
+    x->chunk_exec = nullptr;
+  }
+  x->keep_res = on != 0;
+  return UFX_OK;
+}
+
+int ufx_cc_residual_norm(ufx_cc* x, int var, double* out) {
+  if (int rc = check_ctx(x)) return rc;
+  if (var < -1 || var > 3 || !out) { set_error("ufx_cc_residual_norm: var must be -1..3"); return UFX_ERR_ARG; }
+  if (!x->d_out) HIPCHK(hipMalloc((void**)&x->d_out, 8 * sizeof(double)));
+  int rc = with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    LAUNCH(x, UFX_K_NORM, k_sumsq_var<R>, x->nblk_cell, d, var);
+    LAUNCH(x, UFX_K_NORM, k_norm_var_final<R>, 1, d, x->nblk_cell, var, x->d_out);
+    return launch_check();
+  });
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(out, x->d_out, 8, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return device_error(x);
 }
 
 int ufx_cc_profile(ufx_cc* x, int enable) {
