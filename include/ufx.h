@@ -260,6 +260,10 @@ int ufx_cc_residual_norm(ufx_cc* ctx, int var, double* out);
 #define UFX_K_BFACE 8     /* boundary faces of a residual */
 #define UFX_K_NCAT 9
 int ufx_cc_profile(ufx_cc* ctx, int enable);
+/* Time only the categories whose bit is set in mask (1 << UFX_K_STAGE: the
+ * residual stages alone); each timed launch adds an event pair to the
+ * stream, so a narrow mask disturbs the timed work least.  0 stops. */
+int ufx_cc_profile_mask(ufx_cc* ctx, unsigned mask);
 int ufx_cc_profile_read(ufx_cc* ctx, double* ms, int* count);
 
 /* Block until all queued work has finished; returns the first device-detected

@@ -94,6 +94,7 @@ _sig("ufx_cc_keep_residual", [_vp, C.c_int])
 _sig("ufx_cc_residual_norm", [_vp, C.c_int, _dp])
 _sig("ufx_cc_stream", [_vp], C.c_void_p)
 _sig("ufx_cc_profile", [_vp, C.c_int])
+_sig("ufx_cc_profile_mask", [_vp, C.c_uint])
 _sig("ufx_cc_profile_read", [_vp, _dp, _ip])
 
 KCAT = ["prim", "range", "avg_c2v", "time_step", "gradient", "stage", "norm", "other", "bface"]
@@ -105,6 +106,7 @@ EXPORTED = [
     "ufx_cc_limit", "ufx_cc_fvresidual", "ufx_cc_rk_update", "ufx_cc_rk_stage", "ufx_cc_lusgs",
     "ufx_cc_residue", "ufx_cc_check_positivity", "ufx_cc_forces", "ufx_cc_iterate",
     "ufx_cc_iterate_graph", "ufx_cc_keep_residual", "ufx_cc_residual_norm",
+    "ufx_cc_profile_mask",
     "ufx_cc_sync", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
 ]
 
@@ -378,6 +380,13 @@ class Solver:
     def profile(self, enable: bool = True) -> None:
         """Start (or stop) per-kernel HIP-event timing on the context's stream."""
         _check(lib.ufx_cc_profile(self._h, 1 if enable else 0))
+
+    def profile_only(self, *categories: str) -> None:
+        """Time only the named kernel categories (see KCAT); none stops."""
+        mask = 0
+        for c in categories:
+            mask |= 1 << KCAT.index(c)
+        _check(lib.ufx_cc_profile_mask(self._h, mask))
 
     def profile_read(self) -> dict:
         """{category: (milliseconds, launches)} since the last read."""

@@ -63,6 +63,7 @@ struct ufx_cc {
   int chunk_n = 0;
   double chunk_cfl = 0.0;
   bool keep_res = false;           // whole iterations also store the last residual
+  unsigned prof_mask = ~0u;        // kernel categories timed while profiling
   double* d_out = nullptr;         // small device results (residual norms)
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
   double* d_hist = nullptr;
@@ -100,7 +101,7 @@ hipEvent_t prof_event(ufx_cc* x) {
 }
 
 int prof_begin(ufx_cc* x, int cat) {
-  if (!x->prof) return -1;
+  if (!x->prof || !((x->prof_mask >> cat) & 1u)) return -1;
   size_t i = x->ev_used;
   hipEvent_t e = prof_event(x);
   if (!e || hipEventRecord(e, x->stream) != hipSuccess) return -1;
@@ -1321,10 +1322,21 @@ int ufx_cc_residual_norm(ufx_cc* x, int var, double* out) {
   return device_error(x);
 }
 
+int ufx_cc_profile_mask(ufx_cc* x, unsigned mask) {
+  if (int rc = check_ctx(x)) return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->prof = mask != 0;
+  x->prof_mask = mask;
+  x->recs.clear();
+  x->ev_used = 0;
+  return UFX_OK;
+}
+
 int ufx_cc_profile(ufx_cc* x, int enable) {
   if (int rc = check_ctx(x)) return rc;
   HIPCHK(hipStreamSynchronize(x->stream));
   x->prof = enable != 0;
+  x->prof_mask = ~0u;
   x->recs.clear();
   x->ev_used = 0;
   return UFX_OK;
