@@ -382,7 +382,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.precision),
-                     "kernel": "k_stage", "bytes_per_launch": int(sb),
+                     "kernel": "k_stage" if args.precision == "fp64" else "k_stage_blk",
+                     "bytes_per_launch": int(sb),
                      "avg_launch_ms": round(st_avg, 5)},
         "cpu_baseline": cpu,
     }
