@@ -272,6 +272,89 @@ def main_batch(args) -> int:
     return 0
 
 
+def viscous_solver(precision: str, mesh=None):
+    """The laminar channel: 800 x 400 quads, first wall layer 1e-3 growing
+    by 1.12 (capped), streamwise-only jitter (640 000 triangles); M = 0.5,
+    Re = 5000 per unit length, Sutherland viscosity, Pr 0.72, adiabatic
+    no-slip bump wall, GG-FC gradients with the limiter, Roe, RK3 with
+    residual smoothing at CFL 0.8.  (With a first layer of 1e-5 or 1e-4 the
+    reference scheme itself stops at iteration 1: its ghost primitives are
+    still zero then, CCFVM/solution.F90:78-81, and across so thin a wall
+    layer the reconstruction makes the Roe sonic speed negative.)"""
+    from unsflow2d_amd import cc, meshgen
+    if mesh is None:
+        mesh = cc.Mesh.from_raw(meshgen.bump_channel(800, 400, 4, h=0.05, first=1e-3,
+                                                     growth=1.12, jitter_y=False))
+    cfg = cc.Config(mach=0.5, flow_type="laminar", reynolds=5000.0, iwall=1, flux="roe",
+                    grad="ggfc", ilimit=1, irs=1, precision=precision, cfl_type="ramp",
+                    cfl_min=0.8, cfl_max=0.8, cfl_ramp_steps=1)
+    return mesh, cc.Solver(mesh, cfg)
+
+
+def main_viscous(args) -> int:
+    t0 = time.time()
+    mesh, solver = viscous_solver(args.precision)
+    counts = mesh.counts
+    print(f"# setup {time.time() - t0:.1f}s: {counts['noc']} cells", file=sys.stderr, flush=True)
+    solver.iterate(args.warmup, hist=True)
+    solver.sync()
+    t1 = time.perf_counter()
+    solver.iterate(args.steps, hist=False)
+    solver.sync()
+    wall = time.perf_counter() - t1
+    # a device error (the reference's STOPs) would have cut the timed work
+    # short: the next whole iteration reports it
+    solver.iterate(1, hist=True)
+    solver.profile(True)
+    solver.iterate(min(args.steps, 10), hist=False)
+    prof = solver.profile_read()
+    solver.profile(False)
+    n = min(args.steps, 10)
+    kern = {k: round(v[0] / n, 4) for k, v in prof.items() if v[1]}
+    print(f"# kernel ms per iteration: {kern}", file=sys.stderr, flush=True)
+    cf_err = None
+    if args.cf_check:
+        # wall skin friction of both precisions after the same iterations
+        # from one developed FP64 state (100 FP64 iterations; the state,
+        # primitives with the ghosts', and limiter values handed over): from
+        # the free stream the mixed form cannot start, since the reference's
+        # iteration-1 vertex averages (ghost primitives still zero) hold
+        # negative densities and pressures at wall vertices, and FP32
+        # rounding flips the sign of p/rho on some faces
+        _, s64 = viscous_solver("fp64", mesh)
+        s64.iterate(100, hist=False)
+        _, s32 = viscous_solver("mixed", mesh)
+        for name in ("qc", "qp", "phi"):
+            s32.set(name, s64.get(name))
+        s64.iterate(args.cf_check, hist=False)
+        s32.iterate(args.cf_check, hist=False)
+        ref = s64.forces(wall=True)["cf"]
+        cf32 = s32.forces(wall=True)["cf"]
+        cf_err = float(np.max(np.abs(cf32 - ref)) / np.max(np.abs(ref)))
+        s64.close()
+        s32.close()
+    noc = counts["noc"]
+    line = {
+        "metric": "M cell-RK3-stages/s @640k cells, laminar Navier-Stokes channel",
+        "value": round(3.0 * noc * args.steps / wall / 1e6, 2), "unit": "M cell-RK3-stages/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1000.0 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": "CCFVM laminar NS, bump channel 800x400 quads (640000 triangles), "
+                               "first wall layer 1e-3 x1.12 capped, M=0.5, Re=5000, Sutherland, "
+                               "Pr 0.72, adiabatic no-slip, GG-FC + limiter, Roe, RK3 + residual "
+                               f"smoothing, CFL 0.8, {args.precision}",
+                   "cells": noc, "precision": args.precision},
+        "kernel_ms_per_iteration": kern,
+        "cf_max_rel_error_mixed_vs_fp64": cf_err,
+        "cf_check_iterations": args.cf_check or None,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,15 +366,23 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
-    ap.add_argument("--workload", choices=["cc", "nc", "batch"], default="cc",
+    ap.add_argument("--workload", choices=["cc", "nc", "batch", "viscous"], default="cc",
                     help="cc: the cell-centred headline workload; nc: the vertex-centred one; "
-                         "batch: many independent cell-centred cases")
+                         "batch: many independent cell-centred cases; viscous: the laminar "
+                         "channel")
+    ap.add_argument("--cf-check", type=int, default=0, metavar="N",
+                    help="viscous: also compare wall Cf of mixed vs FP64 after N iterations")
     ap.add_argument("--cases", type=int, default=256, help="batch: number of cases")
     ap.add_argument("--maxiter", type=int, default=2000, help="batch: MaxIter per case")
     ap.add_argument("--chunk", type=int, default=50, help="batch: iterations per graph replay")
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.workload == "viscous":
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            return 2
+        return main_viscous(args)
     if args.workload == "batch":
         if args.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
