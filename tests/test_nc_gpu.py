@@ -175,16 +175,19 @@ def test_nc_device_matches_reference_fixture(case, tol):
 
 
 def test_nc_sweep_forms_agree(monkeypatch):
-    """The three forms of the LU-SGS sweeps -- one launch per level, the same
+    """The four forms of the LU-SGS sweeps -- one launch per level, the same
     launches captured once in a graph and replayed every iteration (with the
-    residual and norm kernels launched eagerly between replays), and one
-    workgroup walking all levels -- give the same iterates, every iteration."""
+    residual and norm kernels launched eagerly between replays), one
+    workgroup walking all levels, and one cooperative launch of several
+    workgroups with a grid barrier between levels, and one workgroup over
+    level-ordered data -- give the same iterates, bit for bit, every
+    iteration."""
     from unsflow2d_amd import nc, nc_mesh
     print("case: nc sweep forms o-grid 40x16 seed 6", flush=True)
     raw = nc_mesh.o_grid(40, 16, 6, stretch=1.15)
     m = nc.NCMesh.from_raw(raw)
     runs = {}
-    for mode in ("levels", "graph", "single"):
+    for mode in ("levels", "graph", "single", "grid", "stream"):
         monkeypatch.setenv("UFX_NC_SWEEP", mode)
         s = nc.NCSolver(m, mach=0.5, aoa_deg=1.5, cfl_max=50.0, grad="gg", timemode="lusgs")
         hs, qs = [], []
@@ -194,7 +197,7 @@ def test_nc_sweep_forms_agree(monkeypatch):
         runs[mode] = (np.array(hs), qs)
     monkeypatch.delenv("UFX_NC_SWEEP")
     h0, q0 = runs["levels"]
-    for mode in ("graph", "single"):
+    for mode in ("graph", "single", "grid", "stream"):
         h, q = runs[mode]
         for k in range(6):
             _check(f"{mode} qc it{k + 1}", q[k], q0[k], 0)

@@ -16,6 +16,8 @@
 //   v2v [..] int, wxy [..][2] f64   least-squares neighbours and weights
 #pragma once
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "cc_device.h"
 #include "cc_physics.h"
 
@@ -44,6 +46,14 @@ struct DevNC {
   double* part;  // reduction partials [nblk][4]
   double* scal;  // [0] fres1
   int* err;
+  // level-ordered sweep data (positions: forward levels, then backward)
+  const int* sw_vert;   // [2 nop] vertex of each position
+  const int* sw_ptr;    // [2 nop + 1] swept-neighbour entries of each position
+  const int2* sw_oe;    // [nent] (edge, bit-inverted when the vertex is p2; neighbour)
+  double* sw_ent;       // [nent][8] nx, ny, ds, la_e, F(qold_o) (4)
+  double* sw_own;       // [2 nop][10] res (4), qold (4), D, pad
+  double* sw_state;     // [nop][8] swept qc (4), dq (4)
+  int sw_nent;
 };
 
 __device__ __forceinline__ void ld2d(const double* p, double& a, double& b) {
@@ -554,6 +564,149 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= n) return;
   nc_lusgs_vertex(d, g, verts[t], dir);
+}
+
+// ---- level-ordered one-workgroup sweeps
+// A sweep level touches a few hundred vertices scattered over the mesh; in
+// one workgroup everything it reads enters through one CU, so the data that
+// does not depend on the sweep is first laid out in level order by a
+// parallel pass, and inside the sweep only the swept neighbours' state and
+// dq (one 64-byte record per vertex) are read scattered.
+constexpr int SWE = 8, SWO = 10;
+
+// Per position: the vertex's residual, old state and diagonal; per swept
+// entry: the edge's spectral radius and normalflux of the neighbour's old
+// state (NCFVM/lusgs.f90:62-72).  The geometry of the entries (nx, ny, ds)
+// is written once at set-up.
+__global__ __launch_bounds__(BLK) void k_nc_sweep_prep(DevNC d, NCFree g) {
+  int s = blockIdx.x * BLK + threadIdx.x;
+  if (s >= 2 * d.nop) return;
+  const int v = d.sw_vert[s];
+  double a[4], b[4];
+  ld4d(d.res + (size_t)v * 4, a);
+  ld4d(d.qold + (size_t)v * 4, b);
+  double* own = d.sw_own + (size_t)s * SWO;
+  st4d(own, a);
+  st4d(own + 4, b);
+  own[8] = d.dd[v];
+  for (int j = d.sw_ptr[s]; j < d.sw_ptr[s + 1]; ++j) {
+    int2 oe = d.sw_oe[j];
+    int e = oe.x >= 0 ? oe.x : ~oe.x;
+    double* en = d.sw_ent + (size_t)j * SWE;
+    double q[4], f1[4];
+    ld4d(d.qold + (size_t)oe.y * 4, q);
+    nc_normalflux(g, q, en[0], en[1], en[2], f1);
+    en[3] = d.fla[e];
+    st4d(en + 4, f1);
+  }
+}
+
+// One sweep in one workgroup over the level-ordered data.  The loads of the
+// first SWP entries of a position are issued together (entries past the
+// position's own belong to the next positions or, past the last, are
+// clamped to the last entry: in bounds, and their terms are not added).
+constexpr int SWP = 4;
+constexpr int STREAM_THREADS = 512;
+
+__device__ __forceinline__ void nc_stream_term(const NCFree& g, const double* en, int2 oe,
+                                               const double* st, double* cres) {
+  double f2[4];
+  nc_normalflux(g, st, en[0], en[1], en[2], f2);
+  double sign = oe.x >= 0 ? 1.0 : -1.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    cres[i] = cres[i] + 0.5 * (sign * (f2[i] - en[4 + i]) - 1.5 * en[3] * st[4 + i]);
+}
+
+__device__ __forceinline__ void nc_stream_pos(const DevNC& d, const NCFree& g, int s, int dir) {
+  const int v = d.sw_vert[s];
+  const int j0 = d.sw_ptr[s], cnt = d.sw_ptr[s + 1] - j0;
+  const double* own = d.sw_own + (size_t)s * SWO;
+  double res[4], qo[4], dd = own[8], dq0[4];
+  ld4d(own, res);
+  ld4d(own + 4, qo);
+  if (dir == 1) ld4d(d.sw_state + (size_t)v * SWE + 4, dq0);
+  double en[SWP][SWE], st[SWP][SWE];
+  int2 oe[SWP];
+#pragma unroll
+  for (int k = 0; k < SWP; ++k) {
+    const int j = min(j0 + k, d.sw_nent - 1);
+    oe[k] = d.sw_oe[j];
+    const double* pe = d.sw_ent + (size_t)j * SWE;
+    ld4d(pe, en[k]);
+    ld4d(pe + 4, en[k] + 4);
+    const double* ps = d.sw_state + (size_t)oe[k].y * SWE;
+    ld4d(ps, st[k]);
+    ld4d(ps + 4, st[k] + 4);
+  }
+  double cres[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < SWP; ++k)
+    if (k < cnt) nc_stream_term(g, en[k], oe[k], st[k], cres);
+  for (int k = SWP; k < cnt; ++k) {
+    const int j = j0 + k;
+    const int2 o2 = d.sw_oe[j];
+    double e2[SWE], s2[SWE];
+    ld4d(d.sw_ent + (size_t)j * SWE, e2);
+    ld4d(d.sw_ent + (size_t)j * SWE + 4, e2 + 4);
+    ld4d(d.sw_state + (size_t)o2.y * SWE, s2);
+    ld4d(d.sw_state + (size_t)o2.y * SWE + 4, s2 + 4);
+    nc_stream_term(g, e2, o2, s2, cres);
+  }
+  double out[8];
+  if (dir == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[4 + i] = (-res[i] - cres[i]) / dd;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[4 + i] = dq0[i] - cres[i] / dd;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) out[i] = qo[i] + out[4 + i];
+  double* w = d.sw_state + (size_t)v * SWE;
+  st4d(w, out);
+  st4d(w + 4, out + 4);
+}
+
+__global__ __launch_bounds__(STREAM_THREADS) void k_nc_lusgs_stream(DevNC d, NCFree g,
+                                                                  const int* off, int nlev,
+                                                                  int dir) {
+  for (int l = 0; l < nlev; ++l) {
+    const int b = off[l], n = off[l + 1] - b;
+    for (int t = threadIdx.x; t < n; t += STREAM_THREADS) nc_stream_pos(d, g, b + t, dir);
+    __syncthreads();
+  }
+}
+
+// The swept states back into qc (the sweeps' result).
+__global__ __launch_bounds__(BLK) void k_nc_sweep_final(DevNC d) {
+  int p = cell_index();
+  if (p >= d.nop) return;
+  double q[4];
+  ld4d(d.sw_state + (size_t)p * SWE, q);
+  st4d(d.qc + (size_t)p * 4, q);
+}
+
+// A whole sweep in one cooperative launch of a few workgroups: the vertices
+// of a level are spread over all of them (so a level's scattered records
+// come in through several CUs), and a grid barrier (HIP cooperative groups:
+// the launch fails unless every workgroup is resident, and the barrier
+// makes each level's stores visible device-wide) separates the levels.
+// Every loop is bounded by the level lists.
+constexpr int GRID_SWEEP_THREADS = 256;
+
+__global__ __launch_bounds__(GRID_SWEEP_THREADS) void k_nc_lusgs_sweep_grid(DevNC d, NCFree g,
+                                                                          const int* verts,
+                                                                          const int* off,
+                                                                          int nlev, int dir) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int nthr = gridDim.x * GRID_SWEEP_THREADS;
+  const int t0 = blockIdx.x * GRID_SWEEP_THREADS + threadIdx.x;
+  for (int l = 0; l < nlev; ++l) {
+    const int b = off[l], n = off[l + 1] - b;
+    for (int t = t0; t < n; t += nthr) nc_lusgs_vertex(d, g, verts[b + t], dir);
+    grid.sync();
+  }
 }
 
 // A whole sweep in one workgroup: the levels in order, a workgroup barrier

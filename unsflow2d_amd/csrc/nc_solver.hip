@@ -43,7 +43,8 @@ struct ufx_nc {
   int nwall = 0;                            // wall edges (bc 1001)
   int* d_wedge = nullptr;                   // their edge numbers, ascending
   double *d_fe = nullptr, *d_cp = nullptr;  // [nwall][2] force terms, [nwall] -Cp
-  int sweep_mode = 0;                       // 0 graph of level launches, 1 level launches, 2 one workgroup
+  int sweep_mode = 4;                       // 0 graph of level launches, 1 level launches, 2 one workgroup, 3 cooperative grid, 4 level-ordered one workgroup
+  int grid_wgs = 8;                         // workgroups of the cooperative sweep
   hipGraph_t sweep_graph = nullptr;         // diagonal + both sweeps, captured once
   hipGraphExec_t sweep_exec = nullptr;
   double cfl = 0.0;
@@ -183,8 +184,35 @@ int run_lusgs(ufx_nc* x, double cfl) {
     return UFX_OK;
   }
   NLAUNCH(x, NK_LUSGS, k_nc_lusgs_diag, x->nblk, x->d);
+  if (x->sweep_mode == 4) {
+    NLAUNCH(x, NK_LUSGS, k_nc_sweep_prep, nblk_of(2 * x->nop), x->d, x->g);
+    for (int dir = 0; dir < 2; ++dir) {
+      const std::vector<int>& off = dir == 0 ? x->lv_fwd_off : x->lv_bwd_off;
+      const int* doff = dir == 0 ? x->d_fwd_off : x->d_bwd_off;
+      prof_mark(x, NK_LUSGS);
+      k_nc_lusgs_stream<<<1, STREAM_THREADS, 0, x->stream>>>(x->d, x->g, doff,
+                                                            (int)off.size() - 1, dir);
+      prof_mark(x, -1);
+    }
+    NLAUNCH(x, NK_LUSGS, k_nc_sweep_final, x->nblk, x->d);
+    return launch_ok();
+  }
   for (int dir = 0; dir < 2; ++dir) {
     const std::vector<int>& off = dir == 0 ? x->lv_fwd_off : x->lv_bwd_off;
+    if (x->sweep_mode == 3) {
+      // one cooperative launch per sweep (offsets relative to lv_verts)
+      const int* doff = dir == 0 ? x->d_fwd_off : x->d_bwd_off;
+      int nlev = (int)off.size() - 1;
+      DevNC dd = x->d;
+      NCFree gg = x->g;
+      const int* vv = x->lv_verts;
+      void* args[] = {&dd, &gg, &vv, &doff, &nlev, &dir};
+      prof_mark(x, NK_LUSGS);
+      NCCHK(hipLaunchCooperativeKernel((const void*)k_nc_lusgs_sweep_grid, dim3(x->grid_wgs),
+                                       dim3(GRID_SWEEP_THREADS), args, 0, x->stream));
+      prof_mark(x, -1);
+      continue;
+    }
     if (x->sweep_mode == 2) {
       // one workgroup walks all levels (level offsets relative to lv_verts)
       const int* doff = dir == 0 ? x->d_fwd_off : x->d_bwd_off;
@@ -389,15 +417,56 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       sl.cnt = cnt;
     }
     if (!rc && fits) rc = nc_upload(x, &x->d_slots, slots);
+    // level-ordered entries of the streamed one-workgroup sweeps: per
+    // position its swept neighbours in edge order, with the edge geometry
+    {
+      std::vector<int> sw_ptr(2 * (size_t)nop + 1, 0);
+      std::vector<int2> sw_oe;
+      std::vector<double> sw_ent;
+      for (size_t s = 0; s < verts.size(); ++s) {
+        int p = verts[s], dir = s < (size_t)nop ? 0 : 1;
+        for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
+          int e = m.v2f[j];
+          bool first = m.fp1[e] == p;
+          int o = first ? m.fp2[e] : m.fp1[e];
+          if (dir == 0 ? o < p : o > p) {
+            sw_oe.push_back(make_int2(first ? e : ~e, o));
+            const double en[SWE] = {m.fnx[e], m.fny[e], m.farea[e], 0, 0, 0, 0, 0};
+            sw_ent.insert(sw_ent.end(), en, en + SWE);
+          }
+        }
+        sw_ptr[s + 1] = (int)sw_oe.size();
+      }
+      if (sw_oe.empty()) {  // keeps the clamped loads in bounds
+        sw_oe.push_back(make_int2(0, 0));
+        sw_ent.assign(SWE, 0.0);
+      }
+      x->d.sw_nent = (int)sw_oe.size();
+      x->d.sw_vert = x->lv_verts;
+      if (!rc) rc = nc_upload(x, &x->d.sw_ptr, sw_ptr);
+      if (!rc) {
+        int2* q = nullptr;
+        rc = nc_upload(x, &q, sw_oe);
+        x->d.sw_oe = q;
+      }
+      if (!rc) rc = nc_upload(x, &x->d.sw_ent, sw_ent);
+      if (!rc) rc = nc_alloc(x, &x->d.sw_own, 2 * (size_t)nop * SWO);
+      if (!rc) rc = nc_alloc(x, &x->d.sw_state, (size_t)nop * SWE);
+    }
     if (rc) { ufx_nc_destroy(x); return rc; }
-    // sweep form: a replayed graph of the level launches (default), the
-    // level launches themselves (UFX_NC_SWEEP=levels), or one workgroup
-    // walking the packed slots (UFX_NC_SWEEP=single; needs at most SWQ swept
-    // neighbours per vertex)
+    // sweep form: one workgroup over the level-ordered data (default,
+    // UFX_NC_SWEEP=stream), a replayed graph of the level launches (graph),
+    // the level launches themselves (levels), one workgroup walking the
+    // packed slots (single; needs at most SWQ swept neighbours per vertex),
+    // or one cooperative launch of several workgroups (grid)
     const char* env = std::getenv("UFX_NC_SWEEP");
-    x->sweep_mode = 0;
+    x->sweep_mode = 4;
+    if (env && std::strcmp(env, "graph") == 0) x->sweep_mode = 0;
     if (env && std::strcmp(env, "levels") == 0) x->sweep_mode = 1;
     if (env && std::strcmp(env, "single") == 0 && fits) x->sweep_mode = 2;
+    if (env && std::strcmp(env, "grid") == 0) x->sweep_mode = 3;
+    if (env && std::strcmp(env, "stream") == 0) x->sweep_mode = 4;
+    if (const char* w = std::getenv("UFX_NC_SWEEP_WGS")) x->grid_wgs = std::max(1, std::min(64, std::atoi(w)));
   }
   DevNC& d = x->d;
   d.nop = nop;
