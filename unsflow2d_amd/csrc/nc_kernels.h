@@ -47,13 +47,20 @@ struct DevNC {
   double* scal;  // [0] fres1
   int* err;
   // level-ordered sweep data (positions: forward levels, then backward)
+  // Streamed sweeps: position s (0 .. 2 nop - 1) is the s-th vertex of the
+  // forward levels then of the backward levels; every array below is
+  // position-major (a wave's lanes read consecutive addresses), with sw_q
+  // entry slots per position for its swept neighbours in edge order.
   const int* sw_vert;   // [2 nop] vertex of each position
-  const int* sw_ptr;    // [2 nop + 1] swept-neighbour entries of each position
-  const int2* sw_oe;    // [nent] (edge, bit-inverted when the vertex is p2; neighbour)
-  double* sw_ent;       // [nent][8] nx, ny, ds, la_e, F(qold_o) (4)
-  double* sw_own;       // [2 nop][10] res (4), qold (4), D, pad
-  double* sw_state;     // [nop][8] swept qc (4), dq (4)
-  int sw_nent;
+  const int* sw_cnt;    // [2 nop] swept neighbours of each position
+  const int* sw_nb;     // [sw_q][2 nop] neighbour's position, bit-inverted when the vertex is p2
+  const int2* sw_pe;    // [sw_q][2 nop] (edge, neighbour vertex)
+  const int* sw_d0;     // [2 nop] forward position of the vertex (read by backward positions)
+  double2* sw_ent;      // [sw_q][6][2 nop] (nx,ny) (ds,la_e) F(qold_o) (2) qold_o (2)
+  double2* sw_own;      // [4][2 nop] res (2), qold (2)
+  double* sw_dd;        // [2 nop] diagonal
+  double* sw_dq;        // [2 nop][4] dq of each position
+  int sw_q;             // entry slots per position
 };
 
 __device__ __forceinline__ void ld2d(const double* p, double& a, double& b) {
@@ -567,105 +574,115 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
 }
 
 // ---- level-ordered one-workgroup sweeps
-// A sweep level touches a few hundred vertices scattered over the mesh; in
-// one workgroup everything it reads enters through one CU, so the data that
-// does not depend on the sweep is first laid out in level order by a
-// parallel pass, and inside the sweep only the swept neighbours' state and
-// dq (one 64-byte record per vertex) are read scattered.
-constexpr int SWE = 8, SWO = 10;
+// A sweep level touches a few hundred vertices scattered over the mesh, and
+// in one workgroup everything it reads enters through one CU, whose address
+// unit handles about one distinct cache line per lane-access of a scattered
+// load.  So the data that does not depend on the sweep is first written in
+// level order by a parallel pass, position-major, and inside the sweep only
+// the swept neighbours' dq (one 32-byte record per position) is gathered;
+// their state is rebuilt as qold_o + dq_o, the same sum that produced it.
+constexpr int SWE = 6;
 
 // Per position: the vertex's residual, old state and diagonal; per swept
-// entry: the edge's spectral radius and normalflux of the neighbour's old
-// state (NCFVM/lusgs.f90:62-72).  The geometry of the entries (nx, ny, ds)
-// is written once at set-up.
+// entry: the edge's spectral radius, normalflux of the neighbour's old state
+// (NCFVM/lusgs.f90:62-72) and that old state.  The geometry of the entries
+// (nx, ny, ds) is written once at set-up.
 __global__ __launch_bounds__(BLK) void k_nc_sweep_prep(DevNC d, NCFree g) {
+  const int np = 2 * d.nop;
   int s = blockIdx.x * BLK + threadIdx.x;
-  if (s >= 2 * d.nop) return;
+  if (s >= np) return;
   const int v = d.sw_vert[s];
-  double a[4], b[4];
-  ld4d(d.res + (size_t)v * 4, a);
-  ld4d(d.qold + (size_t)v * 4, b);
-  double* own = d.sw_own + (size_t)s * SWO;
-  st4d(own, a);
-  st4d(own + 4, b);
-  own[8] = d.dd[v];
-  for (int j = d.sw_ptr[s]; j < d.sw_ptr[s + 1]; ++j) {
-    int2 oe = d.sw_oe[j];
-    int e = oe.x >= 0 ? oe.x : ~oe.x;
-    double* en = d.sw_ent + (size_t)j * SWE;
+  const double2* r = reinterpret_cast<const double2*>(d.res + (size_t)v * 4);
+  const double2* qo = reinterpret_cast<const double2*>(d.qold + (size_t)v * 4);
+  d.sw_own[s] = r[0];
+  d.sw_own[(size_t)np + s] = r[1];
+  d.sw_own[2 * (size_t)np + s] = qo[0];
+  d.sw_own[3 * (size_t)np + s] = qo[1];
+  d.sw_dd[s] = d.dd[v];
+  const int cnt = d.sw_cnt[s];
+  for (int k = 0; k < cnt; ++k) {
+    const int2 pe = d.sw_pe[(size_t)k * np + s];
+    double2* en = d.sw_ent + (size_t)k * SWE * np + s;
+    const double2 nxy = en[0];
+    double2 dsl = en[np];
     double q[4], f1[4];
-    ld4d(d.qold + (size_t)oe.y * 4, q);
-    nc_normalflux(g, q, en[0], en[1], en[2], f1);
-    en[3] = d.fla[e];
-    st4d(en + 4, f1);
+    ld4d(d.qold + (size_t)pe.y * 4, q);
+    nc_normalflux(g, q, nxy.x, nxy.y, dsl.x, f1);
+    dsl.y = d.fla[pe.x];
+    en[np] = dsl;
+    en[2 * (size_t)np] = make_double2(f1[0], f1[1]);
+    en[3 * (size_t)np] = make_double2(f1[2], f1[3]);
+    en[4 * (size_t)np] = make_double2(q[0], q[1]);
+    en[5 * (size_t)np] = make_double2(q[2], q[3]);
   }
 }
 
 // One sweep in one workgroup over the level-ordered data.  The loads of the
-// first SWP entries of a position are issued together (entries past the
-// position's own belong to the next positions or, past the last, are
-// clamped to the last entry: in bounds, and their terms are not added).
+// first SWP entry slots of a position are issued together, each under its
+// slot's own k < cnt test.
 constexpr int SWP = 4;
 constexpr int STREAM_THREADS = 512;
 
-__device__ __forceinline__ void nc_stream_term(const NCFree& g, const double* en, int2 oe,
-                                               const double* st, double* cres) {
+struct SwSlot {
+  double2 nxy, dsl, f01, f23, q01, q23;
+  int nb;
+  double dq[4];
+};
+
+__device__ __forceinline__ void nc_stream_load(const DevNC& d, int np, int s, int k, SwSlot& e) {
+  e.nb = d.sw_nb[(size_t)k * np + s];
+  const double2* en = d.sw_ent + (size_t)k * SWE * np + s;
+  e.nxy = en[0];
+  e.dsl = en[np];
+  e.f01 = en[2 * (size_t)np];
+  e.f23 = en[3 * (size_t)np];
+  e.q01 = en[4 * (size_t)np];
+  e.q23 = en[5 * (size_t)np];
+  const int o = e.nb >= 0 ? e.nb : ~e.nb;
+  ld4d(d.sw_dq + (size_t)o * 4, e.dq);
+}
+
+__device__ __forceinline__ void nc_stream_term(const NCFree& g, const SwSlot& e, double* cres) {
+  const double q[4] = {e.q01.x + e.dq[0], e.q01.y + e.dq[1], e.q23.x + e.dq[2], e.q23.y + e.dq[3]};
+  const double f1[4] = {e.f01.x, e.f01.y, e.f23.x, e.f23.y};
   double f2[4];
-  nc_normalflux(g, st, en[0], en[1], en[2], f2);
-  double sign = oe.x >= 0 ? 1.0 : -1.0;
+  nc_normalflux(g, q, e.nxy.x, e.nxy.y, e.dsl.x, f2);
+  double sign = e.nb >= 0 ? 1.0 : -1.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    cres[i] = cres[i] + 0.5 * (sign * (f2[i] - en[4 + i]) - 1.5 * en[3] * st[4 + i]);
+    cres[i] = cres[i] + 0.5 * (sign * (f2[i] - f1[i]) - 1.5 * e.dsl.y * e.dq[i]);
 }
 
 __device__ __forceinline__ void nc_stream_pos(const DevNC& d, const NCFree& g, int s, int dir) {
-  const int v = d.sw_vert[s];
-  const int j0 = d.sw_ptr[s], cnt = d.sw_ptr[s + 1] - j0;
-  const double* own = d.sw_own + (size_t)s * SWO;
-  double res[4], qo[4], dd = own[8], dq0[4];
-  ld4d(own, res);
-  ld4d(own + 4, qo);
-  if (dir == 1) ld4d(d.sw_state + (size_t)v * SWE + 4, dq0);
-  double en[SWP][SWE], st[SWP][SWE];
-  int2 oe[SWP];
+  const int np = 2 * d.nop;
+  const int cnt = d.sw_cnt[s];
+  const double2 r01 = d.sw_own[s], r23 = d.sw_own[(size_t)np + s];
+  const double dd = d.sw_dd[s];
+  double dq0[4];
+  if (dir == 1) ld4d(d.sw_dq + (size_t)d.sw_d0[s] * 4, dq0);
+  SwSlot e[SWP];
 #pragma unroll
-  for (int k = 0; k < SWP; ++k) {
-    const int j = min(j0 + k, d.sw_nent - 1);
-    oe[k] = d.sw_oe[j];
-    const double* pe = d.sw_ent + (size_t)j * SWE;
-    ld4d(pe, en[k]);
-    ld4d(pe + 4, en[k] + 4);
-    const double* ps = d.sw_state + (size_t)oe[k].y * SWE;
-    ld4d(ps, st[k]);
-    ld4d(ps + 4, st[k] + 4);
-  }
+  for (int k = 0; k < SWP; ++k)
+    if (k < cnt) nc_stream_load(d, np, s, k, e[k]);
   double cres[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < SWP; ++k)
-    if (k < cnt) nc_stream_term(g, en[k], oe[k], st[k], cres);
+    if (k < cnt) nc_stream_term(g, e[k], cres);
   for (int k = SWP; k < cnt; ++k) {
-    const int j = j0 + k;
-    const int2 o2 = d.sw_oe[j];
-    double e2[SWE], s2[SWE];
-    ld4d(d.sw_ent + (size_t)j * SWE, e2);
-    ld4d(d.sw_ent + (size_t)j * SWE + 4, e2 + 4);
-    ld4d(d.sw_state + (size_t)o2.y * SWE, s2);
-    ld4d(d.sw_state + (size_t)o2.y * SWE + 4, s2 + 4);
-    nc_stream_term(g, e2, o2, s2, cres);
+    SwSlot e2;
+    nc_stream_load(d, np, s, k, e2);
+    nc_stream_term(g, e2, cres);
   }
-  double out[8];
+  const double res[4] = {r01.x, r01.y, r23.x, r23.y};
+  double out[4];
   if (dir == 0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[4 + i] = (-res[i] - cres[i]) / dd;
+    for (int i = 0; i < 4; ++i) out[i] = (-res[i] - cres[i]) / dd;
   } else {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) out[4 + i] = dq0[i] - cres[i] / dd;
+    for (int i = 0; i < 4; ++i) out[i] = dq0[i] - cres[i] / dd;
   }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) out[i] = qo[i] + out[4 + i];
-  double* w = d.sw_state + (size_t)v * SWE;
-  st4d(w, out);
-  st4d(w + 4, out + 4);
+  st4d(d.sw_dq + (size_t)s * 4, out);
 }
 
 __global__ __launch_bounds__(STREAM_THREADS) void k_nc_lusgs_stream(DevNC d, NCFree g,
@@ -678,13 +695,19 @@ __global__ __launch_bounds__(STREAM_THREADS) void k_nc_lusgs_stream(DevNC d, NCF
   }
 }
 
-// The swept states back into qc (the sweeps' result).
+// The sweeps' result: qc = qold + dq of the backward sweep, per backward
+// position (the same sum the sweep used for the neighbours' state).
 __global__ __launch_bounds__(BLK) void k_nc_sweep_final(DevNC d) {
-  int p = cell_index();
-  if (p >= d.nop) return;
-  double q[4];
-  ld4d(d.sw_state + (size_t)p * SWE, q);
-  st4d(d.qc + (size_t)p * 4, q);
+  const int np = 2 * d.nop;
+  int t = cell_index();
+  if (t >= d.nop) return;
+  const int s = d.nop + t;
+  const int v = d.sw_vert[s];
+  const double2 q01 = d.sw_own[2 * (size_t)np + s], q23 = d.sw_own[3 * (size_t)np + s];
+  double dq[4];
+  ld4d(d.sw_dq + (size_t)s * 4, dq);
+  const double q[4] = {q01.x + dq[0], q01.y + dq[1], q23.x + dq[2], q23.y + dq[3]};
+  st4d(d.qc + (size_t)v * 4, q);
 }
 
 // A whole sweep in one cooperative launch of a few workgroups: the vertices

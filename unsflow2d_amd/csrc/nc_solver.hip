@@ -417,41 +417,59 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       sl.cnt = cnt;
     }
     if (!rc && fits) rc = nc_upload(x, &x->d_slots, slots);
-    // level-ordered entries of the streamed one-workgroup sweeps: per
-    // position its swept neighbours in edge order, with the edge geometry
+    // position-major data of the streamed one-workgroup sweeps: per
+    // position sw_q slots (the largest swept-neighbour count) holding its
+    // swept neighbours in edge order, with the edge geometry (unused
+    // slots: zeros, never read)
     {
-      std::vector<int> sw_ptr(2 * (size_t)nop + 1, 0);
-      std::vector<int2> sw_oe;
-      std::vector<double> sw_ent;
-      for (size_t s = 0; s < verts.size(); ++s) {
-        int p = verts[s], dir = s < (size_t)nop ? 0 : 1;
+      const size_t np = verts.size();
+      std::vector<int> pos(np);  // position of each vertex in each sweep
+      for (size_t s = 0; s < np; ++s) pos[(s < (size_t)nop ? 0 : nop) + verts[s]] = (int)s;
+      std::vector<int> sw_cnt(np, 0), sw_d0(np, 0);
+      std::vector<int2> eo_all;
+      std::vector<size_t> eo_ptr(np + 1, 0);
+      int q = 1;
+      for (size_t s = 0; s < np; ++s) {
+        int p = verts[s], dir = s < (size_t)nop ? 0 : 1, cnt = 0;
         for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
           int e = m.v2f[j];
           bool first = m.fp1[e] == p;
           int o = first ? m.fp2[e] : m.fp1[e];
           if (dir == 0 ? o < p : o > p) {
-            sw_oe.push_back(make_int2(first ? e : ~e, o));
-            const double en[SWE] = {m.fnx[e], m.fny[e], m.farea[e], 0, 0, 0, 0, 0};
-            sw_ent.insert(sw_ent.end(), en, en + SWE);
+            eo_all.push_back(make_int2(first ? e : ~e, o));
+            ++cnt;
           }
         }
-        sw_ptr[s + 1] = (int)sw_oe.size();
+        sw_cnt[s] = cnt;
+        sw_d0[s] = pos[p];
+        eo_ptr[s + 1] = eo_all.size();
+        q = std::max(q, cnt);
       }
-      if (sw_oe.empty()) {  // keeps the clamped loads in bounds
-        sw_oe.push_back(make_int2(0, 0));
-        sw_ent.assign(SWE, 0.0);
-      }
-      x->d.sw_nent = (int)sw_oe.size();
+      const size_t nq = (size_t)q;
+      std::vector<int> sw_nb(nq * np, 0);
+      std::vector<int2> sw_pe(nq * np, make_int2(0, 0));
+      std::vector<double2> sw_ent(nq * SWE * np, make_double2(0.0, 0.0));
+      for (size_t s = 0; s < np; ++s)
+        for (size_t j = eo_ptr[s]; j < eo_ptr[s + 1]; ++j) {
+          const size_t k = j - eo_ptr[s];
+          const int2 eo = eo_all[j];
+          const int e = eo.x >= 0 ? eo.x : ~eo.x;
+          const int on = pos[(s < (size_t)nop ? 0 : nop) + eo.y];
+          sw_nb[k * np + s] = eo.x >= 0 ? on : ~on;
+          sw_pe[k * np + s] = make_int2(e, eo.y);
+          sw_ent[k * SWE * np + s] = make_double2(m.fnx[e], m.fny[e]);
+          sw_ent[(k * SWE + 1) * np + s] = make_double2(m.farea[e], 0.0);
+        }
+      x->d.sw_q = q;
       x->d.sw_vert = x->lv_verts;
-      if (!rc) rc = nc_upload(x, &x->d.sw_ptr, sw_ptr);
-      if (!rc) {
-        int2* q = nullptr;
-        rc = nc_upload(x, &q, sw_oe);
-        x->d.sw_oe = q;
-      }
+      if (!rc) rc = nc_upload(x, &x->d.sw_cnt, sw_cnt);
+      if (!rc) rc = nc_upload(x, &x->d.sw_nb, sw_nb);
+      if (!rc) rc = nc_upload(x, &x->d.sw_d0, sw_d0);
+      if (!rc) rc = nc_upload(x, &x->d.sw_pe, sw_pe);
       if (!rc) rc = nc_upload(x, &x->d.sw_ent, sw_ent);
-      if (!rc) rc = nc_alloc(x, &x->d.sw_own, 2 * (size_t)nop * SWO);
-      if (!rc) rc = nc_alloc(x, &x->d.sw_state, (size_t)nop * SWE);
+      if (!rc) rc = nc_alloc(x, &x->d.sw_own, 4 * np);
+      if (!rc) rc = nc_alloc(x, &x->d.sw_dd, np);
+      if (!rc) rc = nc_alloc(x, &x->d.sw_dq, 4 * np);
     }
     if (rc) { ufx_nc_destroy(x); return rc; }
     // sweep form: one workgroup over the level-ordered data (default,
