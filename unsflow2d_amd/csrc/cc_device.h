@@ -15,7 +15,8 @@
 //   sl_face/sl_nbr [MAXF][noc] int  the faces of each cell in ascending
 //                        reference face order, and the cell across each
 //                        (bit-inverted when the cell is the face's `out`)
-//   bf_ptr/bf_face       per block of BLK cells, the interior faces it touches
+//   bf_ptr/bf_ab/bf_rec  per block of BLK cells, the interior faces it touches
+//                        (in/out cells and face record, in list order)
 //   sl_loc [MAXF][noc]   per cell slot, the face's position in its block's list
 // R is double for the FP64 form and float for the bulk data of the mixed
 // form; states, residuals and their sums stay double in both.  Face and
@@ -65,7 +66,9 @@ struct DevT {
   // shared-flux residual: the interior faces touched by each block of BLK
   // cells (ascending storage order) and, per cell slot, the position of its
   // face in that list (bit-inverted when the cell is the face's `out`)
-  const int *bf_ptr, *bf_face, *sl_loc;
+  const int *bf_ptr, *sl_loc;
+  const int2* bf_ab;  // per block-list entry: the face's in and out cell
+  const R* bf_rec;    // per block-list entry: the face record
   int bf_max;
   const double *r11, *r12, *r22, *cv, *cov;  // noc (cov: ncell)
   const int* nc2c;

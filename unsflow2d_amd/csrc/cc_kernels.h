@@ -761,12 +761,12 @@ __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mod
   const bool run = s_err == 0;  // uniform over the block
   const int j0 = d.bf_ptr[b], nj = run ? d.bf_ptr[b + 1] - j0 : 0;
   for (int j = threadIdx.x; j < nj; j += BLK) {
-    int f = d.bf_face[j0 + j];
-    int ca = d.fin[f], cb = d.fout[f];
+    const int2 ab = d.bf_ab[j0 + j];
+    const int ca = ab.x, cb = ab.y;
     R ra[RW], rb[RW], fr[FW], F[4];
+    ldv<R, FW>(d.bf_rec + (size_t)(j0 + j) * FW, fr);
     load_rec<R, SL>(d, srec, base, ca, ra);
     load_rec<R, SL>(d, srec, base, cb, rb);
-    face_rec(d, f, fr);
     if (!face_flux_ab<R, FLUX>(gr, ra, rb, fr, F)) set_err(d, DERR_ROE, d.cref[ca] + 1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) fl[j * 4 + i] = F[i];

@@ -736,8 +736,23 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   }
   UP(fbc, fbc); UP(bface, bface); UP(bcell, bcell);
   UP(sl_face, sl_face); UP(sl_nbr, sl_nbr); UP(cnf, cnf);
-  UP(bf_ptr, sl.bf_ptr); UP(bf_face, sl.bf_face); UP(sl_loc, sl.sl_loc);
+  UP(bf_ptr, sl.bf_ptr); UP(sl_loc, sl.sl_loc);
   d.bf_max = sl.bf_max;
+  {
+    // the block face lists' cells and face records, in list order
+    const size_t nbl = sl.bf_face.size();
+    std::vector<int2> bf_ab(std::max<size_t>(1, nbl), make_int2(0, 0));
+    std::vector<R> bf_rec(std::max<size_t>(1, nbl) * FW, R(0));
+    for (size_t j = 0; j < nbl; ++j) {
+      const int f = sl.bf_face[j];
+      bf_ab[j] = make_int2(fin[f], fout[f]);
+      for (int k = 0; k < FW; ++k) bf_rec[j * FW + k] = frec[(size_t)f * FW + k];
+    }
+    int2* p = nullptr;
+    if ((rc = dupload(x, &p, bf_ab))) return rc;
+    d.bf_ab = p;
+    UP(bf_rec, bf_rec);
+  }
   UP(r11, r11); UP(r12, r12); UP(r22, r22); UP(cv, cv); UP(cov, cov);
   UP(nc2c, nc2c); UP(cref, x->ref_of_sto);
   UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py); UP(avg_w0, avg_w0);

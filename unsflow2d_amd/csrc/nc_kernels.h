@@ -620,7 +620,7 @@ __global__ __launch_bounds__(BLK) void k_nc_sweep_prep(DevNC d, NCFree g) {
 // One sweep in one workgroup over the level-ordered data.  The loads of the
 // first SWP entry slots of a position are issued together, each under its
 // slot's own k < cnt test.
-constexpr int SWP = 4;
+constexpr int SWP = 5;
 constexpr int STREAM_THREADS = 512;
 
 struct SwSlot {

@@ -393,6 +393,22 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     std::vector<int> pf(cf.begin(), cf.end() - 1), pb(cb.begin(), cb.end() - 1);
     for (int p = 0; p < nop; ++p) verts[pf[lf[p]]++] = p;
     for (int p = nop - 1; p >= 0; --p) verts[pb[lb[p]]++] = p;
+    // inside a level the vertices are independent, so any order gives the
+    // same bits; ordering them by swept-neighbour count (most first) lets a
+    // wave's lanes share their count and skip the unused entry slots
+    {
+      std::vector<int> nlo(nop, 0), nhi(nop, 0);
+      for (int p = 0; p < nop; ++p)
+        for (int j = m.v2v_ptr[p]; j < m.v2v_ptr[p + 1]; ++j)
+          ++(m.v2v[j] < p ? nlo[p] : nhi[p]);
+      auto sort_levels = [&](const std::vector<int>& off, const std::vector<int>& cnt) {
+        for (size_t l = 0; l + 1 < off.size(); ++l)
+          std::stable_sort(verts.begin() + off[l], verts.begin() + off[l + 1],
+                           [&](int a, int b) { return cnt[a] > cnt[b]; });
+      };
+      sort_levels(cf, nlo);
+      sort_levels(cb, nhi);
+    }
     int rc = nc_upload(x, &x->lv_verts, verts);
     if (!rc) rc = nc_upload(x, &x->d_fwd_off, x->lv_fwd_off);
     if (!rc) rc = nc_upload(x, &x->d_bwd_off, x->lv_bwd_off);
