@@ -2,8 +2,10 @@
 //
 // HBM layout (one context):
 //   rec  [ncell][RW] R   per-cell record gathered by neighbours: primitives
-//                        rho,u,v,p | gradient d/dx,d/dy of each | limiter
-//                        values | centroid (interior cells then ghosts)
+//                        rho,u,v,p | limiter values | gradient d/dx,d/dy of
+//                        each | centroid (interior cells then ghosts); the
+//                        gradient/limiter pass reads the first 8 values and
+//                        the centroid and writes values 4..15 in one piece
 //   qT   [ncell] R       temperature (fifth primitive)
 //   gT   [noc][2] R      its gradient (only when a consumer needs it)
 //   qc   [ncell][4] f64  conserved state [E, rho, rho u, rho v] (ghosts last)
@@ -39,7 +41,7 @@ template <typename R> struct Layout;
 template <> struct Layout<double> { static constexpr int RW = 18, FW = 6, PW = 6; };
 template <> struct Layout<float> { static constexpr int RW = 20, FW = 6, PW = 6; };
 // record offsets
-constexpr int R_QP = 0, R_G = 4, R_PHI = 12, R_CEN = 16;
+constexpr int R_QP = 0, R_PHI = 4, R_G = 8, R_CEN = 16;
 constexpr int F_NX = 0, F_NY = 1, F_AREA = 2, F_CX = 3, F_CY = 4;
 constexpr int P_MU = 5;
 

@@ -245,8 +245,8 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
     // the fields this kernel reads: primitives, limiter values, centroid,
     // and the old gradient only for a limiter-only call
     const R* rc = d.rec + (size_t)c * RW;
-    ldv<R, 4>(rc + R_QP, own + R_QP);
-    ldv<R, 4>(rc + R_PHI, own + R_PHI);
+    static_assert(R_PHI == R_QP + 4, "primitives and limiter values are adjacent");
+    ldv<R, 8>(rc + R_QP, own + R_QP);
     ldv<R, 2>(rc + R_CEN, own + R_CEN);
     if (GRAD == G_NONE) ldv<R, 8>(rc + R_G, own + R_G);
     tc = d.qT[c];
@@ -416,7 +416,8 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
       }
     }
   }
-  if (GRAD != G_NONE || LIMIT) stv<R, 12>(d.rec + (size_t)c * RW + R_G, own + R_G);
+  static_assert(R_G == R_PHI + 4, "limiter values and gradient are adjacent");
+  if (GRAD != G_NONE || LIMIT) stv<R, 12>(d.rec + (size_t)c * RW + R_PHI, own + R_PHI);
   if (TSTEP) {
     d.la[c] = la;
     double dt = cfl * d.cv[c] / la;
