@@ -122,6 +122,51 @@ def cpu_baseline(budget_s: float = 20.0) -> dict | None:
                       f"{ms:.2f} ms/iteration, 1 thread"}
 
 
+def batch_cpu_baseline(cs, maxiter: int, budget_s: float = 20.0) -> dict | None:
+    """The reference (oracle/_ref/cc_fast) on a bounded sample of the batched
+    workload: P evenly spaced cases (P = min(16, host cores)), each in its own
+    single-threaded process, all P at once, each timed over a window sized to
+    about `budget_s`.  The sample's cost per cell-iteration, times the whole
+    batch's cells x MaxIter spread over P cores, gives the host's solves/s."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "cc_fast", "ref_cc_driver")
+    if not os.path.exists(exe):
+        return None
+    from unsflow2d_amd import meshgen
+    ncore = max(1, min(16, os.cpu_count() or 1))
+    sample = [cs[i * len(cs) // ncore] for i in range(min(ncore, len(cs)))]
+    warm = 2
+    with tempfile.TemporaryDirectory() as tmp:
+        procs = []
+        for c in sample:
+            d = os.path.join(tmp, f"case{c.k}")
+            os.makedirs(d)
+            meshgen.write_geometry_inp(meshgen.bump_channel(c.nx, c.ny, c.seed, h=c.h),
+                                       os.path.join(d, "geometry.inp"))
+            with open(os.path.join(d, "flo.inp"), "w") as f:
+                f.write(meshgen.Deck(mach=c.mach, aoa_deg=c.aoa_deg, maxiter=10).text())
+            # window sized from a nominal 3 M cell-stages/s per core
+            nit = int(min(20000, max(5, budget_s * 1000.0 / (c.cells * 3.0 / 3.0e6 * 1000.0))))
+            procs.append((c, nit, subprocess.Popen(
+                [exe, "flo.inp", ".", str(warm + nit), "0", "-1", str(warm)], cwd=d,
+                stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        ms_sum = cells_sum = 0.0
+        for c, nit, pr in procs:
+            out, _ = pr.communicate(timeout=1200)
+            ms_sum += float(out.split("ms_per_iteration")[-1].split()[0])
+            cells_sum += c.cells
+    ms_per_cell_it = ms_sum / cells_sum
+    total_s = sum(c.cells for c in cs) * maxiter * ms_per_cell_it / 1000.0 / ncore
+    return {"value": round(len(cs) / total_s, 4), "unit": "solves/s", "cores": ncore,
+            "kind": "reference",
+            "sample": f"reference CCFVM (amdflang -O3 -march=x86-64-v3, FP64) on {len(sample)} of "
+                      f"the {len(cs)} cases (k = 0, {sample[1].k if len(sample) > 1 else 0}, ...; "
+                      f"{int(cells_sum)} cells), one single-threaded process per case, "
+                      f"{len(sample)} at once, each timed over a window of about {budget_s:.0f} s "
+                      f"after {warm} warm-up iterations; {ms_per_cell_it * 1e6:.1f} ns per "
+                      f"cell-iteration scaled to all {len(cs)} cases x {maxiter} iterations "
+                      f"over {ncore} cores"}
+
+
 def copy_probe(mib: int) -> int:
     """A plain device-to-device copy of known size, five times: the reference
     point that scales the profiler's HBM read/write counters."""
@@ -266,7 +311,8 @@ def main_batch(args) -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "basis": "model bytes per cell-stage x aggregate cell-stages/s"},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else batch_cpu_baseline(cs, args.maxiter,
+                                                                     args.cpu_budget),
     }
     print(json.dumps(line), flush=True)
     return 0

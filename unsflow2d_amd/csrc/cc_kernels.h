@@ -249,7 +249,7 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
     ldv<R, 8>(rc + R_QP, own + R_QP);
     ldv<R, 2>(rc + R_CEN, own + R_CEN);
     if (GRAD == G_NONE) ldv<R, 8>(rc + R_G, own + R_G);
-    tc = d.qT[c];
+    if (WITH_T) tc = d.qT[c];
   }
   if (SL) {
     if (c < d.noc) {
