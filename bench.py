@@ -406,7 +406,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="default: mixed for the batched cases (their scheme), else fp64")
     ap.add_argument("--quads", type=int, default=1000, help="quads per side of the channel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -422,6 +423,8 @@ def main() -> int:
     ap.add_argument("--maxiter", type=int, default=2000, help="batch: MaxIter per case")
     ap.add_argument("--chunk", type=int, default=50, help="batch: iterations per graph replay")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "mixed" if args.workload == "batch" else "fp64"
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.workload == "viscous":
