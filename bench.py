@@ -292,7 +292,8 @@ def main_batch(args) -> int:
           f"{args.precision}, MaxIter {args.maxiter}, chunk {args.chunk}", file=sys.stderr,
           flush=True)
     r = batch.run(cs, precision=args.precision, maxiter=args.maxiter, minres=1e-8,
-                  chunk=args.chunk, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+                  chunk=args.chunk, fused=not args.per_case,
+                  log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     achieved = r["cell_stages_per_s"] * MODEL_BYTES_PER_CELL_STAGE[args.precision] / 1e9
     line = {
         "metric": "batched solves/s (bump channels, 5k-200k cells each)",
@@ -305,7 +306,9 @@ def main_batch(args) -> int:
                                "deg), LSQ+Venkatakrishnan, Roe, RK3, CFL 0.8, each to a density "
                                "residual drop of 1e-8 or MaxIter",
                    "cases": len(cs), "cells": sum(c.cells for c in cs), "max_iter": args.maxiter,
-                   "precision": args.precision},
+                   "precision": args.precision,
+                   "launch": "one context per case, each from its own graph" if args.per_case
+                             else "one launch per kernel for all live cases"},
         "cell_stages_per_s": round(r["cell_stages_per_s"]),
         "converged": r["converged"], "wall_s": round(r["wall"], 3), "setup_s": round(r["setup"], 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -422,6 +425,8 @@ def main() -> int:
     ap.add_argument("--cases", type=int, default=256, help="batch: number of cases")
     ap.add_argument("--maxiter", type=int, default=2000, help="batch: MaxIter per case")
     ap.add_argument("--chunk", type=int, default=50, help="batch: iterations per graph replay")
+    ap.add_argument("--per-case", action="store_true",
+                    help="batch: launch every case's kernels separately (one graph per case)")
     args = ap.parse_args()
     if args.precision is None:
         args.precision = "mixed" if args.workload == "batch" else "fp64"

@@ -244,6 +244,28 @@ int ufx_cc_keep_residual(ufx_cc* ctx, int on);
  * reference's raw norm, CCFVM/solution.F90:159-166); synchronising. */
 int ufx_cc_residual_norm(ufx_cc* ctx, int var, double* out);
 
+/* Many independent cases per launch.  No reference counterpart: the
+ * reference solves one mesh per process; this is the scheduler of the
+ * batched-cases workload (many small meshes on one GPU), where one launch
+ * per kernel and case would leave the GPU mostly idle.  The cases are
+ * existing contexts (inviscid, RK3, no residual smoothing, LSQ or
+ * Green-Gauss gradient; equal precision, flux, gradient and limiter
+ * options), each already past iteration 1.  ufx_cc_batch_iterate runs niter
+ * whole iterations (constant CFL) of every active case with one launch per
+ * kernel for all of them, replayed from a HIP graph captured on the first
+ * call; per cell it computes exactly what ufx_cc_iterate_graph computes for
+ * each case alone.  It synchronises, and reports the first case whose
+ * device error word is set ("case i: ...").  A case's own history buffer
+ * receives its norms as with ufx_cc_iterate_graph.  Retire cases with
+ * ufx_cc_batch_set_active (active[i] = 0; a retired case does not return);
+ * a retired case may then be destroyed.  Destroying the batch leaves the
+ * cases alone. */
+typedef struct ufx_cc_batch ufx_cc_batch;
+int ufx_cc_batch_create(ufx_cc* const* cases, int n, ufx_cc_batch** out);
+int ufx_cc_batch_set_active(ufx_cc_batch* batch, const int* active);
+int ufx_cc_batch_iterate(ufx_cc_batch* batch, int niter, double cfl);
+int ufx_cc_batch_destroy(ufx_cc_batch* batch);
+
 /* Kernel timing (no reference counterpart; instrumentation for benchmarks).
  * When enabled, a pair of HIP events is recorded on the context's stream
  * around every kernel launch; ufx_cc_profile_read (synchronising) returns,

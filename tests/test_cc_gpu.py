@@ -349,3 +349,71 @@ def test_kept_residual_and_its_norms():
         for v in range(4):
             want = np.sqrt(np.sum(r[:, v] ** 2) / r.shape[0])
             assert abs(s.residual_norm(v) - want) <= 1e-13 * want
+
+
+def _batch_cases(precision):
+    from unsflow2d_amd import cc, meshgen
+    specs = [(30, 10, 11, 0.5, 0.0, 0.05), (45, 15, 12, 0.7, 1.5, 0.08),
+             (21, 7, 13, 0.6, -2.0, 0.03)]
+    out = []
+    for nx, ny, seed, mach, aoa, h in specs:
+        m = cc.Mesh.from_raw(meshgen.bump_channel(nx, ny, seed, h=h))
+        cfg = cc.Config(mach=mach, aoa_deg=aoa, flux="roe", grad="lsqr", ilimit=1,
+                        precision=precision, cfl_type="ramp", cfl_min=0.8, cfl_max=0.8,
+                        cfl_ramp_steps=1)
+        s = cc.Solver(m, cfg)
+        s.keep_residual(True)
+        s.iterate(1, hist=False)
+        out.append((m, s))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_each_case_alone(precision):
+    """One launch per kernel for several cases (different sizes, Mach
+    numbers, angles of attack and bump heights) computes, per cell, exactly
+    what each case's own graph-replayed iterations compute: state,
+    primitives, gradients, limiter values, kept residual and its norm agree
+    bit for bit, over two chunks, and after one case is retired (it stops,
+    the others go on)."""
+    from unsflow2d_amd import cc
+    print(f"case: batch of 3 bump channels {precision}", flush=True)
+    a, b = _batch_cases(precision), _batch_cases(precision)
+    fb = cc.Batch([s for _, s in b])
+    for n in (7, 7):
+        for _, s in a:
+            s.iterate_graph(n, hist=False)
+        fb.iterate(n)
+        for i, ((_, sa), (_, sb)) in enumerate(zip(a, b)):
+            assert sa.iter == sb.iter
+            for f in ("qc", "qp", "grad", "phi", "res"):
+                assert np.array_equal(sa.get(f), sb.get(f)), f"case {i}: {f}"
+            assert sa.residual_norm(1) == sb.residual_norm(1)
+    fb.retire(1)
+    q1 = b[1][1].get("qc")
+    fb.iterate(5)
+    for i in (0, 2):
+        a[i][1].iterate_graph(5, hist=False)
+        assert np.array_equal(a[i][1].get("qc"), b[i][1].get("qc")), f"case {i} after retire"
+    assert np.array_equal(b[1][1].get("qc"), q1)
+    b[1][1].close()  # a retired case may go while the batch lives
+    fb.iterate(3)
+    for i in (0, 2):
+        a[i][1].iterate_graph(3, hist=False)
+        assert np.array_equal(a[i][1].get("qc"), b[i][1].get("qc")), f"case {i} after close"
+    fb.close()
+    for m, s in a + b:
+        s.close()
+        m.close()
+
+
+def test_batch_rejects_mixed_options():
+    """Cases of different precision cannot share a launch."""
+    from unsflow2d_amd import cc
+    print("case: batch of fp64 + mixed", flush=True)
+    x = _batch_cases("fp64")[:1] + _batch_cases("mixed")[:1]
+    with pytest.raises(cc.UfxError, match="differ"):
+        cc.Batch([s for _, s in x])
+    for m, s in x:
+        s.close()
+        m.close()

@@ -6,13 +6,15 @@ gradient, Venkatakrishnan limiter, Roe flux, RK3 at CFL 0.8) until its
 density residual has dropped by ``minres`` or it has run ``maxiter``
 iterations (the reference's own loop condition, CCFVM/main.F90:57).
 
-Every case has its own device context and HIP stream.  Iteration 1 runs
-eagerly (it fixes the residual normalisation); after it, each case advances
-in chunks of whole iterations replayed from a HIP graph captured once per
-case (``ufx_cc_iterate_graph``), launched round-robin over the cases so that
-the GPU overlaps small cases' kernels from several streams.  Between chunks
-the density-only residual is read back per case; a case that reaches the
-tolerance or the iteration cap is retired and its context freed.
+Every case has its own device context.  Iteration 1 runs eagerly (it fixes
+the residual normalisation); after it, all cases advance together in chunks
+of whole iterations with one launch per kernel for every live case
+(``ufx_cc_batch_iterate``, replayed from a HIP graph), or, with
+``fused=False``, each case from its own HIP graph on its own stream
+(``ufx_cc_iterate_graph``), launched round-robin so that the GPU overlaps
+small cases' kernels.  Between chunks the density-only residual is read back
+per case; a case that reaches the tolerance or the iteration cap is retired
+and its context freed.
 
 Case k draws its parameters from ``numpy.random.default_rng(seed0 + k)``:
 cells C = 6 ny^2 with ny = round(sqrt(C / 6)), C log-uniform in
@@ -68,7 +70,7 @@ def density_norm(solver: cc.Solver) -> float:
 
 
 def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
-        minres: float = 1e-8, chunk: int = 50, log=None) -> dict:
+        minres: float = 1e-8, chunk: int = 50, fused: bool = True, log=None) -> dict:
     """Solve every case; returns per-case iterations and final density drop
     and the aggregate timing (setup excluded)."""
     t_setup = time.time()
@@ -92,11 +94,18 @@ def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
         e["solver"].iterate(1, hist=False)
     for e in live:
         e["r1"] = density_norm(e["solver"])
+    for i, e in enumerate(live):
+        e["slot"] = i
+    fb = cc.Batch([e["solver"] for e in live]) if fused else None
     rounds = 0
     while live:
-        for e in live:
-            n = min(chunk, maxiter - e["solver"].iter)
-            e["solver"].iterate_graph(n, hist=False)
+        if fb is not None:
+            # every live case is at the same iteration
+            fb.iterate(min(chunk, maxiter - live[0]["solver"].iter))
+        else:
+            for e in live:
+                n = min(chunk, maxiter - e["solver"].iter)
+                e["solver"].iterate_graph(n, hist=False)
         rounds += 1
         still = []
         for e in live:
@@ -105,6 +114,8 @@ def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
             if e["drop"] <= minres or s.iter >= maxiter:
                 done.append(dict(k=e["case"].k, cells=e["case"].cells, iters=s.iter,
                                  drop=e["drop"]))
+                if fb is not None:
+                    fb.retire(e["slot"])
                 s.close()
                 e["mesh"].close()
             else:
@@ -113,6 +124,8 @@ def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
         if log:
             log(f"round {rounds}: {len(live)} cases running, {len(done)} done, "
                 f"{time.time() - t0:.1f}s")
+    if fb is not None:
+        fb.close()
     wall = time.time() - t0
     iters = sum(d["iters"] * d["cells"] for d in done)
     return dict(wall=wall, setup=t_setup, solves=len(done), solves_per_s=len(done) / wall,

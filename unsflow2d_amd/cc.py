@@ -96,6 +96,10 @@ _sig("ufx_cc_stream", [_vp], C.c_void_p)
 _sig("ufx_cc_profile", [_vp, C.c_int])
 _sig("ufx_cc_profile_mask", [_vp, C.c_uint])
 _sig("ufx_cc_profile_read", [_vp, _dp, _ip])
+_sig("ufx_cc_batch_create", [C.POINTER(_vp), C.c_int, C.POINTER(_vp)])
+_sig("ufx_cc_batch_set_active", [_vp, _ip])
+_sig("ufx_cc_batch_iterate", [_vp, C.c_int, C.c_double])
+_sig("ufx_cc_batch_destroy", [_vp])
 
 KCAT = ["prim", "range", "avg_c2v", "time_step", "gradient", "stage", "norm", "other", "bface"]
 
@@ -108,6 +112,8 @@ EXPORTED = [
     "ufx_cc_iterate_graph", "ufx_cc_keep_residual", "ufx_cc_residual_norm",
     "ufx_cc_profile_mask",
     "ufx_cc_sync", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
+    "ufx_cc_batch_create", "ufx_cc_batch_set_active", "ufx_cc_batch_iterate",
+    "ufx_cc_batch_destroy",
 ]
 
 
@@ -402,6 +408,57 @@ class Solver:
     def close(self):
         if self._h:
             lib.ufx_cc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many solvers advanced together, one launch per kernel for all of them
+    (``ufx_cc_batch_*``): inviscid RK3 cases with equal scheme options, each
+    past iteration 1.  Per cell the result equals each solver's own
+    :meth:`Solver.iterate_graph`."""
+
+    def __init__(self, solvers: list):
+        self.solvers = list(solvers)
+        arr = (_vp * len(self.solvers))(*[s._h for s in self.solvers])
+        self._h = _vp()
+        _check(lib.ufx_cc_batch_create(arr, len(self.solvers), C.byref(self._h)))
+        self.active = np.ones(len(self.solvers), dtype=np.int32)
+
+    def iterate(self, niter: int):
+        """``niter`` iterations of every active solver at its (constant) CFL."""
+        live = [s for s, a in zip(self.solvers, self.active) if a]
+        if not live:
+            return
+        cfls = set()
+        for s in live:
+            if s.iter < 1:
+                raise ValueError("batched iterations start at iteration 2: run iterate(1) first")
+            c = s.cfl
+            for k in range(niter):
+                c = s.cfg.cfl(s.iter + 1 + k, c)
+                cfls.add(c)
+        if len(cfls) != 1:
+            raise ValueError("batched iterations need one constant CFL for all cases")
+        c = cfls.pop()
+        _check(lib.ufx_cc_batch_iterate(self._h, niter, c))
+        for s in live:
+            s.cfl = c
+            s.iter += niter
+
+    def retire(self, k: int):
+        """Take solver k out of later iterations (it may be closed after)."""
+        self.active[k] = 0
+        _check(lib.ufx_cc_batch_set_active(self._h, _i(self.active)))
+
+    def close(self):
+        if self._h:
+            lib.ufx_cc_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

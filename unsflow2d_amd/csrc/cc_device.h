@@ -194,6 +194,15 @@ __device__ __forceinline__ int cell_index() {
   return xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
 }
 
+// The block a kernel body works on: the hardware block of a one-context
+// launch, or a (case, block) entry of a many-case launch's block table.
+// b: block index among the context's blocks of this kernel, n: their count.
+struct VBlock {
+  int b, n;
+  __device__ int cell() const { return xcd_block(b, n) * BLK + (int)threadIdx.x; }
+};
+__device__ __forceinline__ VBlock hw_block() { return VBlock{(int)blockIdx.x, (int)gridDim.x}; }
+
 // Monotone map of doubles onto unsigned integers, for atomic max/min.
 __device__ __forceinline__ unsigned long long ord_of(double x) {
   long long i = __double_as_longlong(x);

@@ -9,6 +9,16 @@
 
 namespace ufx {
 
+// The stage update of CCFVM/main.F90:92-101, q = a qold + b (q - dt/A R),
+// rounded after every operation (nothing contracted into fused
+// multiply-adds), so that the fused residual stage and the separate update
+// kernel give the same bits however the compiler inlines them.
+__device__ __forceinline__ double rk_update1(double ak, double o, double bk, double q, double dta,
+                                            double r) {
+#pragma clang fp contract(off)
+  return ak * o + bk * (q - dta * r);
+}
+
 template <typename R>
 __device__ __forceinline__ void face_rec(const DevT<R>& d, int f, R* fr) {
   ldv<R, Layout<R>::FW>(d.frec + (size_t)f * Layout<R>::FW, fr);
@@ -31,8 +41,8 @@ __device__ __forceinline__ void face_prims(const DevT<R>& d, int f, R* fq) {
 // inviscid |u.n| + a from the face-averaged primitives, times the face
 // length, plus the viscous term.
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, double vll) {
-  int f = cell_index();
+__device__ __forceinline__ void face_lambda_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll) {
+  int f = vb.cell();
   if (f >= d.nof || d.err[0]) return;
   R fq[4], fr[Layout<R>::FW];
   face_prims<R, 4>(d, f, fq);
@@ -44,6 +54,10 @@ __global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, d
   if (vll != 0.0) ll = ll + vll * d.fmu[f] / r0;
   d.fla[f] = ll;
 }
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, double vll) {
+  face_lambda_body<R>(d, hw_block(), g, vll);
+}
 
 // ------------------------------------------------- primitive refresh + range
 // CCFVM/main.F90:62-67 (con2prim of interior cells), qold = qc
@@ -51,11 +65,10 @@ __global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, d
 // global range of the four primitives over all cells including ghosts
 // (CCFVM/limiter.F90:146-149), accumulated atomically.
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_prim, int do_old,
-                                              int do_range) {
+__device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<double> g, int do_prim, int do_old, int do_range) {
   constexpr int RW = Layout<R>::RW;
   __shared__ double sh[4][BLK];
-  int c = cell_index();
+  int c = vb.cell();
   double mx[4], mn[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
@@ -110,14 +123,18 @@ __global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_p
         a = sh[k][w] > a ? sh[k][w] : a;
         b = sh[k][BLK / 64 + w] < b ? sh[k][BLK / 64 + w] : b;
       }
-      d.part[(size_t)blockIdx.x * 8 + k] = a;
-      d.part[(size_t)blockIdx.x * 8 + 4 + k] = b;
+      d.part[(size_t)vb.b * 8 + k] = a;
+      d.part[(size_t)vb.b * 8 + 4 + k] = b;
     }
+}
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_prim, int do_old, int do_range) {
+  prim_body<R>(d, hw_block(), g, do_prim, do_old, do_range);
 }
 
 // Final reduction of k_prim's block partials into the global range.
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_range_final(DevT<R> d, int nblk) {
+__device__ __forceinline__ void range_final_body(const DevT<R>& d, VBlock vb, int nblk) {
   __shared__ double sh[4][BLK];
   double mx[4], mn[4];
   for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
@@ -134,6 +151,10 @@ __global__ __launch_bounds__(BLK) void k_range_final(DevT<R> d, int nblk) {
     d.range[4 + threadIdx.x] = ord_of(mn[threadIdx.x]);
   }
 }
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_range_final(DevT<R> d, int nblk) {
+  range_final_body<R>(d, hw_block(), nblk);
+}
 
 // ------------------------------------------------------- vertex averages
 // CCFVM/average.F90:11-29: pseudo-Laplacian weighted average over the cells
@@ -146,14 +167,14 @@ __global__ __launch_bounds__(BLK) void k_range_final(DevT<R> d, int nblk) {
 // ghost primitives from the records.
 constexpr int AVG_WIN = 768;
 template <typename R, bool VISC>
-__global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
+__device__ __forceinline__ void avg_c2v_body(const DevT<R>& d, VBlock vb) {
   constexpr int RW = Layout<R>::RW, PW = Layout<R>::PW;
   __shared__ R s_q[AVG_WIN * 4];
   __shared__ R s_t[AVG_WIN];
   __shared__ R s_mu[VISC ? AVG_WIN : 1];
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  const int p = vb * BLK + threadIdx.x;
-  const int w0 = d.avg_w0[vb], wn = min(AVG_WIN, d.noc - w0);
+  const int pb = xcd_block(vb.b, vb.n);
+  const int p = pb * BLK + threadIdx.x;
+  const int w0 = d.avg_w0[pb], wn = min(AVG_WIN, d.noc - w0);
   for (int i = threadIdx.x; i < wn; i += BLK) {
     R q[4];
     ldv<R, 4>(d.qpc + (size_t)(w0 + i) * 4, q);
@@ -209,6 +230,10 @@ __global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
   for (int k = 6; k < PW; ++k) o[k] = R(0);
   stv<R, PW>(d.prec + (size_t)p * PW, o);
 }
+template <typename R, bool VISC>
+__global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
+  avg_c2v_body<R, VISC>(d, hw_block());
+}
 
 template <typename R>
 __device__ __forceinline__ R vk_limit(R a, R b, R tol) {
@@ -228,7 +253,7 @@ enum { G_NONE = 0, G_LSQR = 1, G_GG = 2, G_GGFC = 3 };
 //   limiter   Venkatakrishnan with the ratchet, CCFVM/limiter.F90:123-296;
 //   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177.
 template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
-__global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl) {
+__device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll, double cfl) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   // the primitives, centroids (and temperatures) of the block's cells are
   // staged in LDS: most neighbours lie in the same block
@@ -236,7 +261,7 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
   __shared__ R s_qp[SL ? BLK * 4 : 1];
   __shared__ R s_cen[SL ? BLK * 2 : 1];
   __shared__ R s_t[SL && WITH_T ? BLK : 1];
-  const int base = xcd_block(blockIdx.x, gridDim.x) * BLK;
+  const int base = xcd_block(vb.b, vb.n) * BLK;
   const int c = base + threadIdx.x;
   const int cnt = min(BLK, d.noc - base);
   R own[RW];
@@ -425,6 +450,10 @@ __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double v
     d.dta[c] = dt / d.cv[c];
   }
 }
+template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
+__global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl) {
+  cell_body<R, GRAD, WITH_T, LIMIT, TSTEP>(d, hw_block(), g, vll, cfl);
+}
 
 // ------------------------------------------------ diamond-path face gradient
 // gradtrixy over the triangles (p1, out, p2) and (p1, p2, in) and their
@@ -515,10 +544,9 @@ __device__ __forceinline__ bool face_flux(const Gas<R>& g, R* ql, R* qr, R nx, R
 // for walls, the momentum flux of the force sums.  The flux is stored for
 // k_stage, which adds it first, as the reference does.
 template <typename R, int FLUX>
-__global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int viscous_wall,
-                                               int iwall, double t_wall) {
+__device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<double> g, int viscous_wall, int iwall, double t_wall) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
-  int k = blockIdx.x * BLK + threadIdx.x;
+  int k = vb.b * BLK + threadIdx.x;
   if (k >= d.nbf || d.err[0]) return;
   int bc = d.fbc[k];
   double F[4] = {0.0, 0.0, 0.0, 0.0};
@@ -585,6 +613,10 @@ __global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int vis
   }
   st4d(d.bres + (size_t)k * 4, F);
 }
+template <typename R, int FLUX>
+__global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int viscous_wall, int iwall, double t_wall) {
+  bface_body<R, FLUX>(d, hw_block(), g, viscous_wall, iwall, t_wall);
+}
 
 // Copy the records of the block's cells [base, base + cnt) into LDS with
 // coalesced 16-byte loads (records are 16-byte multiples).
@@ -620,13 +652,12 @@ __device__ __forceinline__ void load_rec(const DevT<R>& d, const R* srec, int ba
 // the block (most of them, the cells being Hilbert-ordered) are read from
 // there instead of through L2.
 template <typename R, int FLUX, bool SL>
-__global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, double ak,
-                                               double bk, int write_res) {
+__device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode, double ak, double bk, int write_res) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);  // [BLK][RW] when SL
   __shared__ double sh[1][BLK];
-  const int base = xcd_block(blockIdx.x, gridDim.x) * BLK;
+  const int base = xcd_block(vb.b, vb.n) * BLK;
   const int c = base + threadIdx.x;
   if (SL) {
     stage_block_recs(d, srec, base, min(BLK, d.noc - base));
@@ -703,7 +734,7 @@ __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, d
       ld4d_nt(d.qc + (size_t)c * 4, q);
       ld4d_nt(d.qold + (size_t)c * 4, o);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dta * r[i]);
+      for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
       st4d_nt(d.qc + (size_t)c * 4, q);
     }
     if (mode == 2)
@@ -711,8 +742,12 @@ __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, d
   }
   if (mode == 2) {
     block_reduce<1>(sh, &sq, OpAdd());
-    if (threadIdx.x == 0) d.part[blockIdx.x] = sq;
+    if (threadIdx.x == 0) d.part[vb.b] = sq;
   }
+}
+template <typename R, int FLUX, bool SL>
+__global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, double ak, double bk, int write_res) {
+  stage_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res);
 }
 
 // Flux through interior face `f` from the records of its `in` cell (a) and
@@ -747,8 +782,7 @@ __device__ __forceinline__ bool face_flux_ab(const Gas<R>& gr, const R* ra, cons
 // order), then applies the stage update as k_stage does.  SL as in k_stage.
 // LDS: [BLK][RW] records when SL, then [bf_max][4] fluxes.
 template <typename R, int FLUX, bool SL>
-__global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mode, double ak,
-                                                   double bk, int write_res) {
+__device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode, double ak, double bk, int write_res) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);
@@ -756,7 +790,7 @@ __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mod
   __shared__ double sh[1][BLK];
   __shared__ int s_err;
   if (threadIdx.x == 0) s_err = d.err[0];
-  const int b = xcd_block(blockIdx.x, gridDim.x), base = b * BLK;
+  const int b = xcd_block(vb.b, vb.n), base = b * BLK;
   if (SL) stage_block_recs(d, srec, base, min(BLK, d.noc - base));
   __syncthreads();
   const bool run = s_err == 0;  // uniform over the block
@@ -809,7 +843,7 @@ __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mod
       ld4d_nt(d.qc + (size_t)c * 4, q);
       ld4d_nt(d.qold + (size_t)c * 4, o);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dta * r[i]);
+      for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
       st4d_nt(d.qc + (size_t)c * 4, q);
     }
     if (mode == 2)
@@ -817,8 +851,12 @@ __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mod
   }
   if (mode == 2) {
     block_reduce<1>(sh, &sq, OpAdd());
-    if (threadIdx.x == 0) d.part[blockIdx.x] = sq;
+    if (threadIdx.x == 0) d.part[vb.b] = sq;
   }
+}
+template <typename R, int FLUX, bool SL>
+__global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mode, double ak, double bk, int write_res) {
+  stage_blk_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res);
 }
 
 // Sutherland viscosity of the interior cells from the current state,
@@ -923,7 +961,7 @@ __global__ __launch_bounds__(BLK) void k_rk_update(DevT<R> d, double ak, double 
   ld4d(d.qc + (size_t)c * 4, q);
   ld4d(d.qold + (size_t)c * 4, o);
   ld4d(d.res + (size_t)c * 4, r);
-  for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dta * r[i]);
+  for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
   st4d(d.qc + (size_t)c * 4, q);
 }
 
@@ -973,7 +1011,7 @@ __global__ void k_norm_var_final(DevT<R> d, int nblk, int var, double* out) {
 // L2 norm (CCFVM/solution.F90:159-166): raw norm, iteration-1 normalisation,
 // normalised history entry.
 template <typename R>
-__global__ void k_norm_final(DevT<R> d, int nblk, int first, double* hist, int k) {
+__device__ __forceinline__ void norm_final_body(const DevT<R>& d, VBlock vb, int nblk, int first, double* hist, int k) {
   __shared__ double sh[1][BLK];
   double s = 0.0;
   for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
@@ -985,6 +1023,10 @@ __global__ void k_norm_final(DevT<R> d, int nblk, int first, double* hist, int k
     double f1 = d.scal[SC_FRES1];
     if (hist) hist[k] = f1 >= 2.220446049250313e-16 ? fres / f1 : fres;
   }
+}
+template <typename R>
+__global__ void k_norm_final(DevT<R> d, int nblk, int first, double* hist, int k) {
+  norm_final_body<R>(d, hw_block(), nblk, first, hist, k);
 }
 
 template <typename R>

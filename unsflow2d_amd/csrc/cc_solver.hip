@@ -13,6 +13,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "cc_batch.h"
 #include "cc_kernels.h"
 #include "ufx_internal.h"
 
@@ -1381,5 +1382,298 @@ int ufx_cc_sync(ufx_cc* x) {
 }
 
 void* ufx_cc_stream(ufx_cc* x) { return x ? (void*)x->stream : nullptr; }
+
+}  // extern "C"
+
+// ---- many cases per launch (the batched-cases scheduler)
+
+struct ufx_cc_batch {
+  std::vector<ufx_cc*> cs;
+  std::vector<char> on;
+  std::vector<double*> hist_seen;  // each active case's history buffer in the descriptors
+  bool f64 = true;
+  hipStream_t stream = nullptr;
+  void* d_desc = nullptr;          // CaseDesc<R>[n]
+  int4* d_map = nullptr;           // the block tables, one after another
+  size_t map_cap = 0;
+  size_t off[6] = {0, 0, 0, 0, 0, 0}, cnt[6] = {0, 0, 0, 0, 0, 0};
+  int* d_ids = nullptr;            // active cases
+  int* d_err = nullptr;            // their error words
+  int nids = 0;
+  size_t stage_lds = 0;
+  hipGraphExec_t exec = nullptr;   // a captured chunk
+  int chunk_n = 0, chunk_keep = -1;
+  double chunk_cfl = 0.0;
+  bool dirty = true;               // descriptors or tables to rewrite
+};
+
+namespace {
+
+// block tables: all cells (incl. ghosts), vertices, faces, interior cells,
+// boundary faces, one block per case
+enum { T_ALL, T_PT, T_FACE, T_CELL, T_BF, T_CASE, T_N };
+
+template <typename R> DevT<R>& dev_of(ufx_cc* x);
+template <> DevT<double>& dev_of<double>(ufx_cc* x) { return x->d64; }
+template <> DevT<float>& dev_of<float>(ufx_cc* x) { return x->d32; }
+
+void batch_drop_graph(ufx_cc_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+}
+
+// Descriptors of the active cases and the block tables over them.  Inactive
+// cases are not touched (they may have been destroyed).
+template <typename R>
+int batch_write(ufx_cc_batch* b) {
+  const int n = (int)b->cs.size();
+  HIPCHK(hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
+  std::vector<CaseDesc<R>> desc(n);
+  std::memset((void*)desc.data(), 0, desc.size() * sizeof(CaseDesc<R>));
+  std::vector<int4> map;
+  std::vector<int> ids;
+  for (int i = 0; i < n; ++i) {
+    if (!b->on[i]) continue;
+    ufx_cc* x = b->cs[i];
+    CaseDesc<R>& e = desc[i];
+    e.d = dev_of<R>(x);
+    e.g = x->g;
+    e.gr = gas_of<R>(x);
+    e.vll = x->p.vll;
+    e.t_wall = x->p.t_wall;
+    e.hist = x->d_hist;
+    e.iwall = x->p.iwall;
+    e.nblk_all = x->nblk_all;
+    e.nblk_pt = x->nblk_pt;
+    e.nblk_face = x->nblk_face;
+    e.nblk_cell = x->nblk_cell;
+    e.nblk_bf = x->nbf > 0 ? nblocks(x->nbf) : 0;
+    b->hist_seen[i] = x->d_hist;
+    ids.push_back(i);
+  }
+  for (int t = 0; t < T_N; ++t) {
+    b->off[t] = map.size();
+    for (int i : ids) {
+      const CaseDesc<R>& e = desc[i];
+      const int nb = t == T_ALL ? e.nblk_all : t == T_PT ? e.nblk_pt : t == T_FACE ? e.nblk_face
+                   : t == T_CELL ? e.nblk_cell : t == T_BF ? e.nblk_bf : 1;
+      for (int k = 0; k < nb; ++k) map.push_back(make_int4(i, k, nb, 0));
+    }
+    b->cnt[t] = map.size() - b->off[t];
+  }
+  if (!b->d_desc) HIPCHK(hipMalloc(&b->d_desc, (size_t)n * sizeof(CaseDesc<R>)));
+  if (!b->d_ids) HIPCHK(hipMalloc((void**)&b->d_ids, (size_t)n * sizeof(int)));
+  if (!b->d_err) HIPCHK(hipMalloc((void**)&b->d_err, (size_t)3 * n * sizeof(int)));
+  if (map.size() > b->map_cap) {
+    if (b->d_map) HIPCHK(hipFree(b->d_map));
+    b->d_map = nullptr;
+    HIPCHK(hipMalloc((void**)&b->d_map, map.size() * sizeof(int4)));
+    b->map_cap = map.size();
+  }
+  HIPCHK(hipMemcpyAsync(b->d_desc, desc.data(), desc.size() * sizeof(CaseDesc<R>),
+                        hipMemcpyHostToDevice, b->stream));
+  if (!map.empty())
+    HIPCHK(hipMemcpyAsync(b->d_map, map.data(), map.size() * sizeof(int4), hipMemcpyHostToDevice,
+                          b->stream));
+  if (!ids.empty())
+    HIPCHK(hipMemcpyAsync(b->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice,
+                          b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->nids = (int)ids.size();
+  b->dirty = false;
+  return UFX_OK;
+}
+
+// One inviscid RK3 iteration of every active case, as run_iteration does it
+// for one context (the first active case's options hold for all).
+template <typename R>
+int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep) {
+  const CaseDesc<R>* cs = (const CaseDesc<R>*)b->d_desc;
+  const hipStream_t s = b->stream;
+  auto M = [&](int t) { return (const int4*)(b->d_map + b->off[t]); };
+  auto G = [&](int t) { return dim3((unsigned)b->cnt[t]); };
+  const bool lim = x0->p.ilimit != 0;
+  km_prim<R><<<G(T_ALL), BLK, 0, s>>>(cs, M(T_ALL), 1, 1, lim ? 1 : 0);
+  if (lim) km_range_final<R><<<G(T_CASE), BLK, 0, s>>>(cs, M(T_CASE));
+  km_avg_c2v<R, false><<<G(T_PT), BLK, 0, s>>>(cs, M(T_PT));
+  km_face_lambda<R><<<G(T_FACE), BLK, 0, s>>>(cs, M(T_FACE));
+#define KMC(GR)                                                                            \
+  do {                                                                                     \
+    if (lim) km_cell<R, GR, false, true, true><<<G(T_CELL), BLK, 0, s>>>(cs, M(T_CELL), cfl); \
+    else km_cell<R, GR, false, false, true><<<G(T_CELL), BLK, 0, s>>>(cs, M(T_CELL), cfl);   \
+  } while (0)
+  if (x0->p.grad == UFX_GRAD_GG) KMC(G_GG);
+  else KMC(G_LSQR);
+#undef KMC
+  for (int st = 1; st <= 3; ++st) {
+    const double ak = RK_A[st - 1], bk = RK_B[st - 1];
+    const int mode = st == 3 ? 2 : 1, wr = st == 3 && keep ? 1 : 0;
+    const size_t lds = b->stage_lds;
+#define KMS(F)                                                                                  \
+  do {                                                                                          \
+    if (b->cnt[T_BF]) km_bface<R, F><<<G(T_BF), BLK, 0, s>>>(cs, M(T_BF), 0);                   \
+    if (x0->stage_shared && x0->stage_local)                                                    \
+      km_stage_blk<R, F, true><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr);    \
+    else if (x0->stage_shared)                                                                  \
+      km_stage_blk<R, F, false><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr);   \
+    else if (x0->stage_local)                                                                   \
+      km_stage<R, F, true><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr);        \
+    else                                                                                        \
+      km_stage<R, F, false><<<G(T_CELL), BLK, 0, s>>>(cs, M(T_CELL), mode, ak, bk, wr);         \
+  } while (0)
+    switch (x0->p.flux) {
+      case UFX_FLUX_ROE: KMS(1); break;
+      case UFX_FLUX_AUSM: KMS(2); break;
+      case UFX_FLUX_VANLEER: KMS(3); break;
+      default: KMS(4); break;
+    }
+#undef KMS
+  }
+  km_norm_final<R><<<G(T_CASE), BLK, 0, s>>>(cs, M(T_CASE), k);
+  return launch_check();
+}
+
+template <typename R>
+int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
+  ufx_cc* x0 = nullptr;
+  int keep = -1;
+  for (size_t i = 0; i < b->cs.size(); ++i) {
+    if (!b->on[i]) continue;
+    ufx_cc* x = b->cs[i];
+    if (!x0) x0 = x;
+    if (keep < 0) keep = x->keep_res ? 1 : 0;
+    if (keep != (x->keep_res ? 1 : 0)) {
+      set_error("ufx_cc_batch_iterate: the active cases differ in ufx_cc_keep_residual");
+      return UFX_ERR_ARG;
+    }
+    if (x->prof) { set_error("ufx_cc_batch_iterate: a case is profiling"); return UFX_ERR_ARG; }
+    // the case's own queued work first
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (int rc = ensure_hist(x, niter)) return rc;
+    if (x->d_hist != b->hist_seen[i]) b->dirty = true;
+  }
+  if (!x0) return UFX_OK;
+  if (b->dirty)
+    if (int rc = batch_write<R>(b)) return rc;
+  if (!b->exec || b->chunk_n != niter || b->chunk_cfl != cfl || b->chunk_keep != keep) {
+    batch_drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int k = 0; k < niter && !rc; ++k) rc = batch_iteration<R>(b, x0, k, cfl, keep);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (rc) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    HIPCHK(e);
+    hipError_t ie = hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIPCHK(ie);
+    b->chunk_n = niter;
+    b->chunk_cfl = cfl;
+    b->chunk_keep = keep;
+  }
+  HIPCHK(hipGraphLaunch(b->exec, b->stream));
+  // the error words of every active case: the first failing case is reported
+  const CaseDesc<R>* cs = (const CaseDesc<R>*)b->d_desc;
+  km_errors<R><<<nblocks(b->nids), BLK, 0, b->stream>>>(cs, b->d_ids, b->nids, b->d_err);
+  if (int rc = launch_check()) return rc;
+  std::vector<int> err((size_t)3 * b->nids);
+  HIPCHK(hipMemcpyAsync(err.data(), b->d_err, err.size() * sizeof(int), hipMemcpyDeviceToHost,
+                        b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int j = 0;
+  for (size_t i = 0; i < b->cs.size(); ++i) {
+    if (!b->on[i]) continue;
+    b->cs[i]->cfl = cfl;
+    if (err[3 * j] != 0) {
+      int rc = device_error(b->cs[i]);
+      set_error("case " + std::to_string(i) + ": " + ufx_last_error());
+      return rc ? rc : UFX_ERR_STATE;
+    }
+    ++j;
+  }
+  return UFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ufx_cc_batch_create(ufx_cc* const* cases, int n, ufx_cc_batch** out) {
+  if (!out || !cases || n < 1) { set_error("ufx_cc_batch_create: needs n >= 1 cases"); return UFX_ERR_ARG; }
+  *out = nullptr;
+  ufx_cc* x0 = cases[0];
+  for (int i = 0; i < n; ++i) {
+    ufx_cc* x = cases[i];
+    if (int rc = check_ctx(x)) return rc;
+    for (int j = 0; j < i; ++j)
+      if (cases[j] == x) { set_error("ufx_cc_batch_create: a case is listed twice"); return UFX_ERR_ARG; }
+    const Params &p = x->p, &p0 = x0->p;
+    if (p.flow != UFX_FLOW_INVISCID || p.irs || p.timemode != UFX_TIME_RK3 ||
+        p.grad == UFX_GRAD_GGFC) {
+      set_error("ufx_cc_batch_create: batched cases are inviscid RK3 without residual smoothing, "
+                "with the LSQ or Green-Gauss gradient");
+      return UFX_ERR_ARG;
+    }
+    if (p.precision != p0.precision || p.flux != p0.flux || p.grad != p0.grad ||
+        (p.ilimit != 0) != (p0.ilimit != 0) || x->stage_shared != x0->stage_shared ||
+        x->stage_local != x0->stage_local) {
+      set_error("ufx_cc_batch_create: the cases differ in precision, flux, gradient, limiter or "
+                "stage form");
+      return UFX_ERR_ARG;
+    }
+  }
+  auto* b = new ufx_cc_batch();
+  b->cs.assign(cases, cases + n);
+  b->on.assign(n, 1);
+  b->hist_seen.assign(n, nullptr);
+  b->f64 = x0->p.precision == UFX_PREC_FP64;
+  for (int i = 0; i < n; ++i) b->stage_lds = std::max(b->stage_lds, cases[i]->stage_lds);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    set_error("ufx_cc_batch_create: stream creation failed");
+    return UFX_ERR_HIP;
+  }
+  *out = b;
+  return UFX_OK;
+}
+
+int ufx_cc_batch_set_active(ufx_cc_batch* b, const int* active) {
+  if (!b || !active) { set_error("ufx_cc_batch_set_active: null argument"); return UFX_ERR_ARG; }
+  bool changed = false;
+  for (size_t i = 0; i < b->cs.size(); ++i) {
+    char v = active[i] != 0;
+    if (v && !b->on[i]) {
+      set_error("ufx_cc_batch_set_active: a retired case cannot return");
+      return UFX_ERR_ARG;
+    }
+    changed = changed || v != b->on[i];
+    b->on[i] = v;
+  }
+  if (changed) b->dirty = true;
+  return UFX_OK;
+}
+
+int ufx_cc_batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
+  if (!b) { set_error("ufx_cc_batch_iterate: null batch"); return UFX_ERR_ARG; }
+  if (niter < 1) { set_error("ufx_cc_batch_iterate: needs niter >= 1"); return UFX_ERR_ARG; }
+  return b->f64 ? batch_iterate<double>(b, niter, cfl) : batch_iterate<float>(b, niter, cfl);
+}
+
+int ufx_cc_batch_destroy(ufx_cc_batch* b) {
+  if (!b) return UFX_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_map) (void)hipFree(b->d_map);
+  if (b->d_ids) (void)hipFree(b->d_ids);
+  if (b->d_err) (void)hipFree(b->d_err);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return UFX_OK;
+}
 
 }  // extern "C"
