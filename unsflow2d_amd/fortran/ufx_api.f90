@@ -151,6 +151,28 @@ interface
       real(c_double), intent(in) :: cfl(*)
       real(c_double), intent(out) :: hist(*)
    end function
+   ! many cases per launch (no reference counterpart: the batched-cases scheduler)
+   integer(c_int) function ufx_cc_batch_create(cases, n, batch) bind(C, name="ufx_cc_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: cases(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: batch
+   end function
+   integer(c_int) function ufx_cc_batch_set_active(batch, active) bind(C, name="ufx_cc_batch_set_active")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), intent(in) :: active(*)
+   end function
+   integer(c_int) function ufx_cc_batch_iterate(batch, niter, cfl) bind(C, name="ufx_cc_batch_iterate")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: niter
+      real(c_double), value :: cfl
+   end function
+   integer(c_int) function ufx_cc_batch_destroy(batch) bind(C, name="ufx_cc_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+   end function
    integer(c_int) function ufx_cc_sync(ctx) bind(C, name="ufx_cc_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
