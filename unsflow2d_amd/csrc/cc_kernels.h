@@ -41,7 +41,8 @@ __device__ __forceinline__ void face_prims(const DevT<R>& d, int f, R* fq) {
 // inviscid |u.n| + a from the face-averaged primitives, times the face
 // length, plus the viscous term.
 template <typename R>
-__device__ __forceinline__ void face_lambda_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll) {
+__device__ __forceinline__ void face_lambda_body(const DevT<R>& d, VBlock vb, Gas<double> g,
+                                                 double vll) {
   int f = vb.cell();
   if (f >= d.nof || d.err[0]) return;
   R fq[4], fr[Layout<R>::FW];
@@ -65,7 +66,8 @@ __global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, d
 // global range of the four primitives over all cells including ghosts
 // (CCFVM/limiter.F90:146-149), accumulated atomically.
 template <typename R>
-__device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<double> g, int do_prim, int do_old, int do_range) {
+__device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<double> g, int do_prim,
+                                          int do_old, int do_range) {
   constexpr int RW = Layout<R>::RW;
   __shared__ double sh[4][BLK];
   int c = vb.cell();
@@ -128,7 +130,8 @@ __device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<doubl
     }
 }
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_prim, int do_old, int do_range) {
+__global__ __launch_bounds__(BLK) void k_prim(DevT<R> d, Gas<double> g, int do_prim, int do_old,
+                                              int do_range) {
   prim_body<R>(d, hw_block(), g, do_prim, do_old, do_range);
 }
 
@@ -253,7 +256,8 @@ enum { G_NONE = 0, G_LSQR = 1, G_GG = 2, G_GGFC = 3 };
 //   limiter   Venkatakrishnan with the ratchet, CCFVM/limiter.F90:123-296;
 //   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177.
 template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
-__device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll, double cfl) {
+__device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll,
+                                          double cfl) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   // the primitives, centroids (and temperatures) of the block's cells are
   // staged in LDS: most neighbours lie in the same block
@@ -544,7 +548,8 @@ __device__ __forceinline__ bool face_flux(const Gas<R>& g, R* ql, R* qr, R nx, R
 // for walls, the momentum flux of the force sums.  The flux is stored for
 // k_stage, which adds it first, as the reference does.
 template <typename R, int FLUX>
-__device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<double> g, int viscous_wall, int iwall, double t_wall) {
+__device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<double> g,
+                                           int viscous_wall, int iwall, double t_wall) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   int k = vb.b * BLK + threadIdx.x;
   if (k >= d.nbf || d.err[0]) return;
@@ -614,7 +619,8 @@ __device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<doub
   st4d(d.bres + (size_t)k * 4, F);
 }
 template <typename R, int FLUX>
-__global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int viscous_wall, int iwall, double t_wall) {
+__global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int viscous_wall,
+                                               int iwall, double t_wall) {
   bface_body<R, FLUX>(d, hw_block(), g, viscous_wall, iwall, t_wall);
 }
 
@@ -652,7 +658,8 @@ __device__ __forceinline__ void load_rec(const DevT<R>& d, const R* srec, int ba
 // the block (most of them, the cells being Hilbert-ordered) are read from
 // there instead of through L2.
 template <typename R, int FLUX, bool SL>
-__device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode, double ak, double bk, int write_res) {
+__device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
+                                           double ak, double bk, int write_res) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);  // [BLK][RW] when SL
@@ -746,7 +753,8 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
   }
 }
 template <typename R, int FLUX, bool SL>
-__global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, double ak, double bk, int write_res) {
+__global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, double ak, double bk,
+                                               int write_res) {
   stage_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res);
 }
 
@@ -782,7 +790,8 @@ __device__ __forceinline__ bool face_flux_ab(const Gas<R>& gr, const R* ra, cons
 // order), then applies the stage update as k_stage does.  SL as in k_stage.
 // LDS: [BLK][RW] records when SL, then [bf_max][4] fluxes.
 template <typename R, int FLUX, bool SL>
-__device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode, double ak, double bk, int write_res) {
+__device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
+                                               double ak, double bk, int write_res) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);
@@ -855,7 +864,8 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
   }
 }
 template <typename R, int FLUX, bool SL>
-__global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mode, double ak, double bk, int write_res) {
+__global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mode, double ak,
+                                                   double bk, int write_res) {
   stage_blk_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res);
 }
 
@@ -1011,7 +1021,8 @@ __global__ void k_norm_var_final(DevT<R> d, int nblk, int var, double* out) {
 // L2 norm (CCFVM/solution.F90:159-166): raw norm, iteration-1 normalisation,
 // normalised history entry.
 template <typename R>
-__device__ __forceinline__ void norm_final_body(const DevT<R>& d, VBlock vb, int nblk, int first, double* hist, int k) {
+__device__ __forceinline__ void norm_final_body(const DevT<R>& d, VBlock vb, int nblk, int first,
+                                                double* hist, int k) {
   __shared__ double sh[1][BLK];
   double s = 0.0;
   for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
