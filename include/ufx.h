@@ -264,6 +264,9 @@ typedef struct ufx_cc_batch ufx_cc_batch;
 int ufx_cc_batch_create(ufx_cc* const* cases, int n, ufx_cc_batch** out);
 int ufx_cc_batch_set_active(ufx_cc_batch* batch, const int* active);
 int ufx_cc_batch_iterate(ufx_cc_batch* batch, int niter, double cfl);
+/* ufx_cc_residual_norm(case, var, &out[i]) of every case i in one launch
+ * (NaN for retired cases); synchronising. */
+int ufx_cc_batch_residual_norms(ufx_cc_batch* batch, int var, double* out);
 int ufx_cc_batch_destroy(ufx_cc_batch* batch);
 
 /* Kernel timing (no reference counterpart; instrumentation for benchmarks).

@@ -389,7 +389,11 @@ def test_batch_equals_each_case_alone(precision):
             for f in ("qc", "qp", "grad", "phi", "res"):
                 assert np.array_equal(sa.get(f), sb.get(f)), f"case {i}: {f}"
             assert sa.residual_norm(1) == sb.residual_norm(1)
+        rn = fb.residual_norms(1)
+        assert [sb.residual_norm(1) for _, sb in b] == list(rn), "batched norms"
     fb.retire(1)
+    rn = fb.residual_norms(-1)
+    assert np.isnan(rn[1]) and rn[0] == b[0][1].residual_norm(-1)
     q1 = b[1][1].get("qc")
     fb.iterate(5)
     for i in (0, 2):

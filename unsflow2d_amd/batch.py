@@ -107,10 +107,13 @@ def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
                 n = min(chunk, maxiter - e["solver"].iter)
                 e["solver"].iterate_graph(n, hist=False)
         rounds += 1
+        # density norms: one launch for all live cases, or one per case
+        norms = fb.residual_norms(1) if fb is not None else None
         still = []
         for e in live:
             s = e["solver"]
-            e["drop"] = density_norm(s) / e["r1"] if e["r1"] > 0 else 0.0
+            rn = norms[e["slot"]] if norms is not None else density_norm(s)
+            e["drop"] = rn / e["r1"] if e["r1"] > 0 else 0.0
             if e["drop"] <= minres or s.iter >= maxiter:
                 done.append(dict(k=e["case"].k, cells=e["case"].cells, iters=s.iter,
                                  drop=e["drop"]))

@@ -99,6 +99,7 @@ _sig("ufx_cc_profile_read", [_vp, _dp, _ip])
 _sig("ufx_cc_batch_create", [C.POINTER(_vp), C.c_int, C.POINTER(_vp)])
 _sig("ufx_cc_batch_set_active", [_vp, _ip])
 _sig("ufx_cc_batch_iterate", [_vp, C.c_int, C.c_double])
+_sig("ufx_cc_batch_residual_norms", [_vp, C.c_int, _dp])
 _sig("ufx_cc_batch_destroy", [_vp])
 
 KCAT = ["prim", "range", "avg_c2v", "time_step", "gradient", "stage", "norm", "other", "bface"]
@@ -113,7 +114,7 @@ EXPORTED = [
     "ufx_cc_profile_mask",
     "ufx_cc_sync", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
     "ufx_cc_batch_create", "ufx_cc_batch_set_active", "ufx_cc_batch_iterate",
-    "ufx_cc_batch_destroy",
+    "ufx_cc_batch_residual_norms", "ufx_cc_batch_destroy",
 ]
 
 
@@ -450,6 +451,13 @@ class Batch:
         for s in live:
             s.cfl = c
             s.iter += niter
+
+    def residual_norms(self, var: int = -1) -> np.ndarray:
+        """Each solver's :meth:`Solver.residual_norm` (``var``) from one
+        launch; NaN for retired solvers."""
+        out = np.empty(len(self.solvers))
+        _check(lib.ufx_cc_batch_residual_norms(self._h, var, _d(out)))
+        return out
 
     def retire(self, k: int):
         """Take solver k out of later iterations (it may be closed after)."""

@@ -115,6 +115,26 @@ __global__ void km_norm_final(const CaseDesc<R>* cs, const int4* map, int k) {
   norm_final_body<R>(e.d, vb, e.nblk_cell, 0, e.hist, k);
 }
 
+// Per-variable residual norm of every case (as ufx_cc_residual_norm): block
+// partials over the interior-cell table, then one block per case writing
+// out[case].
+template <typename R>
+__global__ __launch_bounds__(BLK) void km_sumsq_var(const CaseDesc<R>* cs, const int4* map,
+                                                    int var) {
+  int c;
+  const VBlock vb = case_block(map, c);
+  const CaseDesc<R> e = cs[c];
+  sumsq_var_body<R>(e.d, vb, var);
+}
+
+template <typename R>
+__global__ void km_norm_var_final(const CaseDesc<R>* cs, const int4* map, int var, double* out) {
+  int c;
+  case_block(map, c);
+  const CaseDesc<R> e = cs[c];
+  norm_var_final_body<R>(e.d, e.nblk_cell, var, out + c);
+}
+
 // The error words of the listed cases, three ints each, into out.
 template <typename R>
 __global__ void km_errors(const CaseDesc<R>* cs, const int* ids, int n, int* out) {

@@ -993,9 +993,9 @@ __global__ __launch_bounds__(BLK) void k_sumsq(DevT<R> d) {
 // Block partials of the squares of one stored residual variable (var 0..3)
 // or of all four (var -1).
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_sumsq_var(DevT<R> d, int var) {
+__device__ __forceinline__ void sumsq_var_body(const DevT<R>& d, VBlock vb, int var) {
   __shared__ double sh[1][BLK];
-  int c = cell_index();
+  int c = vb.cell();
   double sq = 0.0;
   if (c < d.noc) {
     double r[4];
@@ -1006,16 +1006,25 @@ __global__ __launch_bounds__(BLK) void k_sumsq_var(DevT<R> d, int var) {
       sq = r[var] * r[var];
   }
   block_reduce<1>(sh, &sq, OpAdd());
-  if (threadIdx.x == 0) d.part[blockIdx.x] = sq;
+  if (threadIdx.x == 0) d.part[vb.b] = sq;
+}
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_sumsq_var(DevT<R> d, int var) {
+  sumsq_var_body<R>(d, hw_block(), var);
 }
 
 template <typename R>
-__global__ void k_norm_var_final(DevT<R> d, int nblk, int var, double* out) {
+__device__ __forceinline__ void norm_var_final_body(const DevT<R>& d, int nblk, int var,
+                                                    double* out) {
   __shared__ double sh[1][BLK];
   double s = 0.0;
   for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
   block_reduce<1>(sh, &s, OpAdd());
   if (threadIdx.x == 0) out[0] = var < 0 ? sqrt(s / d.noc / 4) : sqrt(s / d.noc);
+}
+template <typename R>
+__global__ void k_norm_var_final(DevT<R> d, int nblk, int var, double* out) {
+  norm_var_final_body<R>(d, nblk, var, out);
 }
 
 // L2 norm (CCFVM/solution.F90:159-166): raw norm, iteration-1 normalisation,

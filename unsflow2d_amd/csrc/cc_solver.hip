@@ -1399,6 +1399,7 @@ struct ufx_cc_batch {
   size_t off[6] = {0, 0, 0, 0, 0, 0}, cnt[6] = {0, 0, 0, 0, 0, 0};
   int* d_ids = nullptr;            // active cases
   int* d_err = nullptr;            // their error words
+  double* d_norm = nullptr;        // per-case residual norms
   int nids = 0;
   size_t stage_lds = 0;
   hipGraphExec_t exec = nullptr;   // a captured chunk
@@ -1465,6 +1466,7 @@ int batch_write(ufx_cc_batch* b) {
   if (!b->d_desc) HIPCHK(hipMalloc(&b->d_desc, (size_t)n * sizeof(CaseDesc<R>)));
   if (!b->d_ids) HIPCHK(hipMalloc((void**)&b->d_ids, (size_t)n * sizeof(int)));
   if (!b->d_err) HIPCHK(hipMalloc((void**)&b->d_err, (size_t)3 * n * sizeof(int)));
+  if (!b->d_norm) HIPCHK(hipMalloc((void**)&b->d_norm, (size_t)n * sizeof(double)));
   if (map.size() > b->map_cap) {
     if (b->d_map) HIPCHK(hipFree(b->d_map));
     b->d_map = nullptr;
@@ -1598,9 +1600,39 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
   return UFX_OK;
 }
 
+template <typename R>
+int batch_norms(ufx_cc_batch* b, int var, double* out) {
+  const int n = (int)b->cs.size();
+  for (int i = 0; i < n; ++i) out[i] = std::nan("");
+  if (b->dirty)
+    if (int rc = batch_write<R>(b)) return rc;
+  if (b->nids == 0) return UFX_OK;
+  const CaseDesc<R>* cs = (const CaseDesc<R>*)b->d_desc;
+  km_sumsq_var<R><<<dim3((unsigned)b->cnt[T_CELL]), BLK, 0, b->stream>>>(
+      cs, (const int4*)(b->d_map + b->off[T_CELL]), var);
+  km_norm_var_final<R><<<dim3((unsigned)b->cnt[T_CASE]), BLK, 0, b->stream>>>(
+      cs, (const int4*)(b->d_map + b->off[T_CASE]), var, b->d_norm);
+  if (int rc = launch_check()) return rc;
+  std::vector<double> v(n);
+  HIPCHK(hipMemcpyAsync(v.data(), b->d_norm, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                        b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i)
+    if (b->on[i]) out[i] = v[i];
+  return UFX_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ufx_cc_batch_residual_norms(ufx_cc_batch* b, int var, double* out) {
+  if (!b || !out) { set_error("ufx_cc_batch_residual_norms: null argument"); return UFX_ERR_ARG; }
+  if (var < -1 || var > 3) { set_error("ufx_cc_batch_residual_norms: var must be -1..3"); return UFX_ERR_ARG; }
+  for (size_t i = 0; i < b->cs.size(); ++i)
+    if (b->on[i]) HIPCHK(hipStreamSynchronize(b->cs[i]->stream));
+  return b->f64 ? batch_norms<double>(b, var, out) : batch_norms<float>(b, var, out);
+}
 
 int ufx_cc_batch_create(ufx_cc* const* cases, int n, ufx_cc_batch** out) {
   if (!out || !cases || n < 1) { set_error("ufx_cc_batch_create: needs n >= 1 cases"); return UFX_ERR_ARG; }
@@ -1671,6 +1703,7 @@ int ufx_cc_batch_destroy(ufx_cc_batch* b) {
   if (b->d_map) (void)hipFree(b->d_map);
   if (b->d_ids) (void)hipFree(b->d_ids);
   if (b->d_err) (void)hipFree(b->d_err);
+  if (b->d_norm) (void)hipFree(b->d_norm);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return UFX_OK;

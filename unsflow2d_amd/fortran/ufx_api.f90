@@ -169,6 +169,13 @@ interface
       integer(c_int), value :: niter
       real(c_double), value :: cfl
    end function
+   integer(c_int) function ufx_cc_batch_residual_norms(batch, var, out) &
+         bind(C, name="ufx_cc_batch_residual_norms")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      integer(c_int), value :: var
+      real(c_double), intent(out) :: out(*)
+   end function
    integer(c_int) function ufx_cc_batch_destroy(batch) bind(C, name="ufx_cc_batch_destroy")
       import :: c_int, c_ptr
       type(c_ptr), value :: batch
