@@ -141,12 +141,20 @@ __device__ __forceinline__ void range_final_body(const DevT<R>& d, VBlock vb, in
   __shared__ double sh[4][BLK];
   double mx[4], mn[4];
   for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
-  for (int b = threadIdx.x; b < nblk; b += BLK)
+  // 64 bytes per block partial, four of them in flight per thread (max and
+  // min do not depend on the order)
+  const double2* p2 = reinterpret_cast<const double2*>(d.part);
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nblk; b += BLK) {
+    const double2 v0 = p2[(size_t)b * 4], v1 = p2[(size_t)b * 4 + 1];
+    const double2 v2 = p2[(size_t)b * 4 + 2], v3 = p2[(size_t)b * 4 + 3];
+    const double a[4] = {v0.x, v0.y, v1.x, v1.y}, m[4] = {v2.x, v2.y, v3.x, v3.y};
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
-      double a = d.part[(size_t)b * 8 + k], m = d.part[(size_t)b * 8 + 4 + k];
-      mx[k] = a > mx[k] ? a : mx[k];
-      mn[k] = m < mn[k] ? m : mn[k];
+      mx[k] = a[k] > mx[k] ? a[k] : mx[k];
+      mn[k] = m[k] < mn[k] ? m[k] : mn[k];
     }
+  }
   block_reduce<4>(sh, mx, OpMax());
   block_reduce<4>(sh, mn, OpMin());
   if (threadIdx.x < 4) {
@@ -1034,6 +1042,9 @@ __device__ __forceinline__ void norm_final_body(const DevT<R>& d, VBlock vb, int
                                                 double* hist, int k) {
   __shared__ double sh[1][BLK];
   double s = 0.0;
+  // each thread's partials in a fixed order; unrolled so that their loads
+  // are issued together
+#pragma unroll 8
   for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
   block_reduce<1>(sh, &s, OpAdd());
   if (threadIdx.x == 0) {
