@@ -629,7 +629,15 @@ struct SwSlot {
   double dq[4];
 };
 
-__device__ __forceinline__ void nc_stream_load(const DevNC& d, int np, int s, int k, SwSlot& e) {
+// The dq of the last SW_RING positions of a sweep is also kept in LDS (a
+// ring indexed by position mod SW_RING, 128 KB): a neighbour at position o
+// is read from there when o >= lim = level start + level width - SW_RING,
+// which no write of this level or an earlier one has overwritten; the rest
+// (about 5 % on the 2M-vertex O-grid) from HBM.
+constexpr int SW_RING = 4096;
+
+__device__ __forceinline__ void nc_stream_load(const DevNC& d, int np, int s, int k, SwSlot& e,
+                                               const double2* ring, int lim) {
   e.nb = d.sw_nb[(size_t)k * np + s];
   const double2* en = d.sw_ent + (size_t)k * SWE * np + s;
   e.nxy = en[0];
@@ -639,7 +647,15 @@ __device__ __forceinline__ void nc_stream_load(const DevNC& d, int np, int s, in
   e.q01 = en[4 * (size_t)np];
   e.q23 = en[5 * (size_t)np];
   const int o = e.nb >= 0 ? e.nb : ~e.nb;
-  ld4d(d.sw_dq + (size_t)o * 4, e.dq);
+  if (o >= lim) {
+    const double2 a = ring[2 * (o & (SW_RING - 1))], b = ring[2 * (o & (SW_RING - 1)) + 1];
+    e.dq[0] = a.x;
+    e.dq[1] = a.y;
+    e.dq[2] = b.x;
+    e.dq[3] = b.y;
+  } else {
+    ld4d(d.sw_dq + (size_t)o * 4, e.dq);
+  }
 }
 
 __device__ __forceinline__ void nc_stream_term(const NCFree& g, const SwSlot& e, double* cres) {
@@ -653,7 +669,8 @@ __device__ __forceinline__ void nc_stream_term(const NCFree& g, const SwSlot& e,
     cres[i] = cres[i] + 0.5 * (sign * (f2[i] - f1[i]) - 1.5 * e.dsl.y * e.dq[i]);
 }
 
-__device__ __forceinline__ void nc_stream_pos(const DevNC& d, const NCFree& g, int s, int dir) {
+__device__ __forceinline__ void nc_stream_pos(const DevNC& d, const NCFree& g, int s, int dir,
+                                              double2* ring, int lim) {
   const int np = 2 * d.nop;
   const int cnt = d.sw_cnt[s];
   const double2 r01 = d.sw_own[s], r23 = d.sw_own[(size_t)np + s];
@@ -663,14 +680,14 @@ __device__ __forceinline__ void nc_stream_pos(const DevNC& d, const NCFree& g, i
   SwSlot e[SWP];
 #pragma unroll
   for (int k = 0; k < SWP; ++k)
-    if (k < cnt) nc_stream_load(d, np, s, k, e[k]);
+    if (k < cnt) nc_stream_load(d, np, s, k, e[k], ring, lim);
   double cres[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < SWP; ++k)
     if (k < cnt) nc_stream_term(g, e[k], cres);
   for (int k = SWP; k < cnt; ++k) {
     SwSlot e2;
-    nc_stream_load(d, np, s, k, e2);
+    nc_stream_load(d, np, s, k, e2, ring, lim);
     nc_stream_term(g, e2, cres);
   }
   const double res[4] = {r01.x, r01.y, r23.x, r23.y};
@@ -683,14 +700,22 @@ __device__ __forceinline__ void nc_stream_pos(const DevNC& d, const NCFree& g, i
     for (int i = 0; i < 4; ++i) out[i] = dq0[i] - cres[i] / dd;
   }
   st4d(d.sw_dq + (size_t)s * 4, out);
+  ring[2 * (s & (SW_RING - 1))] = make_double2(out[0], out[1]);
+  ring[2 * (s & (SW_RING - 1)) + 1] = make_double2(out[2], out[3]);
 }
 
+// A level's writes go to the slots of positions [b - SW_RING, b + n -
+// SW_RING), below lim, so no read of the level meets them; a level wider
+// than the ring reads only from HBM (lim > b).
 __global__ __launch_bounds__(STREAM_THREADS) void k_nc_lusgs_stream(DevNC d, NCFree g,
                                                                   const int* off, int nlev,
                                                                   int dir) {
+  __shared__ double2 ring[2 * SW_RING];
   for (int l = 0; l < nlev; ++l) {
     const int b = off[l], n = off[l + 1] - b;
-    for (int t = threadIdx.x; t < n; t += STREAM_THREADS) nc_stream_pos(d, g, b + t, dir);
+    const int lim = b + n - SW_RING;
+    for (int t = threadIdx.x; t < n; t += STREAM_THREADS)
+      nc_stream_pos(d, g, b + t, dir, ring, lim);
     __syncthreads();
   }
 }
