@@ -84,3 +84,19 @@ def test_renumbering_is_breadth_first():
                 order.append(n)
     assert sorted(order) == list(range(noc))
     assert np.all(np.diff(level) >= 0)
+
+
+def test_batch_entry_points_reject_bad_arguments():
+    """The batch entry points check their arguments before touching the
+    device: no cases, a null batch, a bad chunk length or variable."""
+    from unsflow2d_amd import cc
+    lib = cc.lib
+    out = ctypes.c_void_p()
+    assert lib.ufx_cc_batch_create(None, 0, ctypes.byref(out)) == 1
+    assert "n >= 1" in lib.ufx_last_error().decode()
+    assert lib.ufx_cc_batch_iterate(None, 5, 0.8) == 1
+    assert "null batch" in lib.ufx_last_error().decode()
+    assert lib.ufx_cc_batch_set_active(None, None) == 1
+    norms = np.zeros(2)
+    assert lib.ufx_cc_batch_residual_norms(None, 1, norms.ctypes.data_as(cc._dp)) == 1
+    assert lib.ufx_cc_batch_destroy(None) == 0
