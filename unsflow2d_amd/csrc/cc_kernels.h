@@ -661,7 +661,9 @@ __device__ __forceinline__ void load_rec(const DevT<R>& d, const R* srec, int ba
 //   mode 0: store the residual only;
 //   mode 1: apply the stage update of CCFVM/main.F90:92-101 in place (no
 //           other cell reads this cell's qc during a stage);
-//   mode 2: mode 1 and a block partial of the squared residual.
+//   mode 2: mode 1 and a block partial of the squared residual;
+//   mode 3: mode 1 when qc equals qold bit for bit (the first stage of a
+//           whole iteration, right after save_old): qold is read once.
 // SL: the block's records are first copied to LDS, so the neighbours inside
 // the block (most of them, the cells being Hilbert-ordered) are read from
 // there instead of through L2.
@@ -746,8 +748,13 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
     if (mode == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
     if (mode >= 1) {
       double dta = ld_nt(d.dta + c), q[4], o[4];
-      ld4d_nt(d.qc + (size_t)c * 4, q);
       ld4d_nt(d.qold + (size_t)c * 4, o);
+      if (mode == 3) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = o[i];
+      } else {
+        ld4d_nt(d.qc + (size_t)c * 4, q);
+      }
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
       st4d_nt(d.qc + (size_t)c * 4, q);
@@ -857,8 +864,13 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
     if (mode == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
     if (mode >= 1) {
       double dta = ld_nt(d.dta + c), q[4], o[4];
-      ld4d_nt(d.qc + (size_t)c * 4, q);
       ld4d_nt(d.qold + (size_t)c * 4, o);
+      if (mode == 3) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) q[i] = o[i];
+      } else {
+        ld4d_nt(d.qc + (size_t)c * 4, q);
+      }
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
       st4d_nt(d.qc + (size_t)c * 4, q);

@@ -402,11 +402,12 @@ const double RK_B[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
 
 // One Runge-Kutta stage: residual and update fused in one pass when nothing
 // else contributes to the residual, else the full residual then the update.
+// same_old: qc equals qold (the first stage of a whole iteration).
 template <typename R>
-int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res) {
+int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool same_old) {
   double ak = RK_A[stage - 1], bk = RK_B[stage - 1];
   if (x->p.flow == UFX_FLOW_INVISCID && !x->p.irs)
-    return run_stage_kernel(x, d, norm ? 2 : 1, ak, bk, write_res ? 1 : 0);
+    return run_stage_kernel(x, d, norm ? 2 : (same_old ? 3 : 1), ak, bk, write_res ? 1 : 0);
   if (int rc = run_residual_full(x, d)) return rc;
   if (int rc = run_rk_update(x, d, ak, bk)) return rc;
   if (norm) {
@@ -464,7 +465,7 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
   if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
   if (x->p.timemode == UFX_TIME_RK3) {
     for (int s = 1; s <= 3; ++s)
-      if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res)) return rc;
+      if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res, s == 1)) return rc;
   } else {
     if (int rc = run_lusgs(x, d)) return rc;
     LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
@@ -1117,7 +1118,7 @@ int ufx_cc_rk_update(ufx_cc* x, int stage) {
 int ufx_cc_rk_stage(ufx_cc* x, int stage) {
   if (int rc = check_ctx(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
-  return with_dev(x, [&](auto& d) { return run_stage(x, d, stage, false, true); });
+  return with_dev(x, [&](auto& d) { return run_stage(x, d, stage, false, true, false); });
 }
 
 int ufx_cc_lusgs(ufx_cc* x) {
@@ -1510,7 +1511,7 @@ int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep) {
 #undef KMC
   for (int st = 1; st <= 3; ++st) {
     const double ak = RK_A[st - 1], bk = RK_B[st - 1];
-    const int mode = st == 3 ? 2 : 1, wr = st == 3 && keep ? 1 : 0;
+    const int mode = st == 3 ? 2 : (st == 1 ? 3 : 1), wr = st == 3 && keep ? 1 : 0;
     const size_t lds = b->stage_lds;
 #define KMS(F)                                                                                  \
   do {                                                                                          \
