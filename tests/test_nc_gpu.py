@@ -222,3 +222,24 @@ def test_nc_wall_forces_match_oracle(flux):
     print(f"  cl {f[0]:.6f} cd {f[1]:.6f}", flush=True)
     _check("cl cd fx fy", f, g, 0)
     _check("-cp", cp, cq, 0)
+
+
+def test_stream_sweep_beyond_the_lds_ring(monkeypatch):
+    """On a 180 000-vertex O-grid some swept neighbours lie more than the
+    LDS ring's 4 096 positions back (about 0.35 % of the forward reads), so
+    the streamed sweep reads those from HBM and the rest from the ring; its
+    iterates equal the level-launch form's (per-vertex records, no ring) bit
+    for bit."""
+    from unsflow2d_amd import nc, nc_mesh
+    print("case: nc stream sweep vs levels, o-grid 600x300 seed 7", flush=True)
+    raw = nc_mesh.o_grid(600, 300, 7, stretch=1.005 ** (999 / 299))
+    m = nc.NCMesh.from_raw(raw)
+    qs = {}
+    for mode in ("levels", "stream"):
+        monkeypatch.setenv("UFX_NC_SWEEP", mode)
+        s = nc.NCSolver(m, mach=0.4, cfl_max=50.0, grad="gg", timemode="lusgs")
+        s.iterate(3, hist=False)
+        qs[mode] = s.get("qc")
+        s.close()
+    monkeypatch.delenv("UFX_NC_SWEEP")
+    _check("stream vs levels qc", qs["stream"], qs["levels"], 0)
