@@ -630,11 +630,11 @@ struct SwSlot {
 };
 
 // The dq of the last SW_RING positions of a sweep is also kept in LDS (a
-// ring indexed by position mod SW_RING, 128 KB): a neighbour at position o
-// is read from there when o >= lim = level start + level width - SW_RING,
-// which no write of this level or an earlier one has overwritten; the rest
-// (about 5 % on the 2M-vertex O-grid) from HBM.
-constexpr int SW_RING = 4096;
+// ring indexed by position mod SW_RING, 160 KB, all of a CU's LDS): a
+// neighbour at position o is read from there when o >= lim = level start +
+// level width - SW_RING, which no write of this level or an earlier one has
+// overwritten; the rest (about 3 % on the 2M-vertex O-grid) from HBM.
+constexpr int SW_RING = 5120;
 
 __device__ __forceinline__ void nc_stream_load(const DevNC& d, int np, int s, int k, SwSlot& e,
                                                const double2* ring, int lim) {
@@ -648,7 +648,7 @@ __device__ __forceinline__ void nc_stream_load(const DevNC& d, int np, int s, in
   e.q23 = en[5 * (size_t)np];
   const int o = e.nb >= 0 ? e.nb : ~e.nb;
   if (o >= lim) {
-    const double2 a = ring[2 * (o & (SW_RING - 1))], b = ring[2 * (o & (SW_RING - 1)) + 1];
+    const double2 a = ring[2 * (o % SW_RING)], b = ring[2 * (o % SW_RING) + 1];
     e.dq[0] = a.x;
     e.dq[1] = a.y;
     e.dq[2] = b.x;
@@ -700,8 +700,8 @@ __device__ __forceinline__ void nc_stream_pos(const DevNC& d, const NCFree& g, i
     for (int i = 0; i < 4; ++i) out[i] = dq0[i] - cres[i] / dd;
   }
   st4d(d.sw_dq + (size_t)s * 4, out);
-  ring[2 * (s & (SW_RING - 1))] = make_double2(out[0], out[1]);
-  ring[2 * (s & (SW_RING - 1)) + 1] = make_double2(out[2], out[3]);
+  ring[2 * (s % SW_RING)] = make_double2(out[0], out[1]);
+  ring[2 * (s % SW_RING) + 1] = make_double2(out[2], out[3]);
 }
 
 // A level's writes go to the slots of positions [b - SW_RING, b + n -
