@@ -449,7 +449,7 @@ def main() -> int:
         return main_nc(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
-              "shards by mesh partition, which this build does not implement)", file=sys.stderr)
+              "shards by mesh partition, which is not implemented)", file=sys.stderr)
         return 2
 
     from unsflow2d_amd import cc, meshgen

@@ -36,7 +36,7 @@ extern "C" {
 #define UFX_ERR_MESH 2         /* mesh rejected (the reference STOPs here) */
 #define UFX_ERR_HIP 3          /* HIP runtime failure */
 #define UFX_ERR_STATE 4        /* device-detected invalid flow state */
-#define UFX_ERR_UNSUPPORTED 5  /* option outside what this build implements */
+#define UFX_ERR_UNSUPPORTED 5  /* option outside what this library implements */
 
 #define UFX_KIND_CC 1
 #define UFX_KIND_NC 2

@@ -5,7 +5,7 @@
  * A plain C restatement of the NCFVM routines in the reference's own loop
  * structure (edge loops scattering into both end points, in edge order),
  * compiled without FMA contraction.  Only the tests, the smoke check and the
- * benchmark's CPU leg use it, as the checker.  Each function cites the
+ * benchmark's CPU leg use it, to check results against.  Each function cites the
  * reference routine it restates.
  *
  * Per-vertex records keep the variables of one vertex contiguous:

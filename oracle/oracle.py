@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
 
 Only the tests, the smoke check and the benchmark's CPU-baseline leg use this
-module, as the checker.  The solver state lives in numpy arrays in the
+module, to check results against.  The solver state lives in numpy arrays in the
 reference's per-cell record layout (variables of one cell contiguous).
 """
 from __future__ import annotations
