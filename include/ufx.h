@@ -222,7 +222,11 @@ int ufx_cc_forces(ufx_cc* ctx, double* out, double* cp, double* cf);
  * reference iteration counter before the first one; cfl[k] is the CFL of
  * iteration iter0+1+k (the host evaluates the reference's ramp).  hist, if
  * not NULL, receives the normalised residual of each iteration; that makes the
- * call synchronising.  Without hist no host synchronisation happens. */
+ * call synchronising.  Without hist no host synchronisation happens.
+ * Inviscid RK3 iterations without residual smoothing keep the iteration's
+ * start state in "qc" through the stages (the stage states live in a
+ * separate buffer) instead of copying it to "qold": after them "qold" still
+ * holds what the last save_old (or LU-SGS / viscous iteration) stored. */
 int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double* hist);
 
 /* The same iterations replayed from a HIP graph (no reference counterpart;

@@ -80,11 +80,11 @@ __global__ __launch_bounds__(BLK) void km_cell(const CaseDesc<R>* cs, const int4
 
 template <typename R, int FLUX>
 __global__ __launch_bounds__(BLK) void km_bface(const CaseDesc<R>* cs, const int4* map,
-                                                int viscous_wall) {
+                                                int viscous_wall, int src) {
   int c;
   const VBlock vb = case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  bface_body<R, FLUX>(e.d, vb, e.g, viscous_wall, e.iwall, e.t_wall);
+  bface_body<R, FLUX>(e.d, vb, e.g, viscous_wall, e.iwall, e.t_wall, src);
 }
 
 template <typename R, int FLUX, bool SL>

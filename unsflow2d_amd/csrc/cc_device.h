@@ -10,6 +10,8 @@
 //   gT   [noc][2] R      its gradient (only when a consumer needs it)
 //   qc   [ncell][4] f64  conserved state [E, rho, rho u, rho v] (ghosts last)
 //   qold [noc][4] f64, res [noc][4] f64, dt/dta/la [noc] f64
+//   qs   [noc][4] f64    stage states of a whole RK3 iteration (qc keeps
+//                        the iteration's start state until the last stage)
 //   frec [nof][FW] R     face record nx, ny, area, centre x, y
 //   fcen [nof][2] R      face centre alone (for the gradient/limiter pass)
 //   fpt  [nof] int2      face endpoints (storage numbering of points)
@@ -90,6 +92,7 @@ struct DevT {
   R* gT;
   double* qc;
   double* qold;
+  double* qs;                // [noc][4] stage states of whole RK3 iterations
   double* res;
   double* dt;
   double* dta;

@@ -79,7 +79,7 @@ __device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<doubl
     if (c < d.noc && (do_prim || do_old)) {
       double q[4];
       ld4d(d.qc + (size_t)c * 4, q);
-      if (do_old) st4d(d.qold + (size_t)c * 4, q);
+      if (do_old == 1) st4d(d.qold + (size_t)c * 4, q);
       if (do_prim) {
         double p[5];
         con2prim(g, q, p);
@@ -557,7 +557,7 @@ __device__ __forceinline__ bool face_flux(const Gas<R>& g, R* ql, R* qr, R nx, R
 // k_stage, which adds it first, as the reference does.
 template <typename R, int FLUX>
 __device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<double> g,
-                                           int viscous_wall, int iwall, double t_wall) {
+                                           int viscous_wall, int iwall, double t_wall, int src) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   int k = vb.b * BLK + threadIdx.x;
   if (k >= d.nbf || d.err[0]) return;
@@ -582,7 +582,7 @@ __device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<doub
       con2prim(g, gq, gp);
     } else {
       double q[4];
-      ld4d(d.qc + (size_t)c * 4, q);
+      ld4d(stage_in(d, src) + (size_t)c * 4, q);
       if (!viscous_wall) {
         // slip_wall, CCFVM/boundCond.F90:46-81
         double rho = q[1], u = q[2] / q[1], v = q[3] / q[1];
@@ -628,8 +628,27 @@ __device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<doub
 }
 template <typename R, int FLUX>
 __global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int viscous_wall,
-                                               int iwall, double t_wall) {
-  bface_body<R, FLUX>(d, hw_block(), g, viscous_wall, iwall, t_wall);
+                                               int iwall, double t_wall, int src) {
+  bface_body<R, FLUX>(d, hw_block(), g, viscous_wall, iwall, t_wall, src);
+}
+
+// Where a stage update reads and writes its states.  src 0 (routine by
+// routine): q = a qold + b (qc - dt/A R) in qc.  Whole RK3 iterations keep
+// the iteration's start state in qc and the stage states in qs, so no copy
+// to qold is needed: src 1 (stage 1) reads qc once and writes qs; src 2
+// (stage 2) reads qc and qs and writes qs; src 3 (stage 3) reads qc and qs
+// and writes qc (each cell reads only its own qc entry).
+template <typename R>
+__device__ __forceinline__ const double* stage_old(const DevT<R>& d, int src) {
+  return src ? d.qc : d.qold;
+}
+template <typename R>
+__device__ __forceinline__ const double* stage_in(const DevT<R>& d, int src) {
+  return src >= 2 ? d.qs : d.qc;
+}
+template <typename R>
+__device__ __forceinline__ double* stage_out(const DevT<R>& d, int src) {
+  return src == 1 || src == 2 ? d.qs : d.qc;
 }
 
 // Copy the records of the block's cells [base, base + cnt) into LDS with
@@ -658,18 +677,18 @@ __device__ __forceinline__ void load_rec(const DevT<R>& d, const R* srec, int ba
 // faces: the boundary-face fluxes from k_bface first, then the interior
 // faces with recon (:190-252) on both sides, each flux evaluated from the
 // same inputs by both adjacent cells.
-//   mode 0: store the residual only;
-//   mode 1: apply the stage update of CCFVM/main.F90:92-101 in place (no
-//           other cell reads this cell's qc during a stage);
-//   mode 2: mode 1 and a block partial of the squared residual;
-//   mode 3: mode 1 when qc equals qold bit for bit (the first stage of a
-//           whole iteration, right after save_old): qold is read once.
+//   mode & 3 = 0: store the residual only;
+//              1: apply the stage update of CCFVM/main.F90:92-101 (no other
+//                 cell reads this cell's state during a stage);
+//              2: 1 and a block partial of the squared residual;
+//   mode >> 2 (src): where the update's states live, stage_old/in/out.
 // SL: the block's records are first copied to LDS, so the neighbours inside
 // the block (most of them, the cells being Hilbert-ordered) are read from
 // there instead of through L2.
 template <typename R, int FLUX, bool SL>
 __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
                                            double ak, double bk, int write_res) {
+  const int op = mode & 3, src = mode >> 2;
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);  // [BLK][RW] when SL
@@ -745,24 +764,24 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
       }
     }
     if (!ok) set_err(d, DERR_ROE, d.cref[c] + 1);
-    if (mode == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
-    if (mode >= 1) {
+    if (op == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
+    if (op >= 1) {
       double dta = ld_nt(d.dta + c), q[4], o[4];
-      ld4d_nt(d.qold + (size_t)c * 4, o);
-      if (mode == 3) {
+      ld4d_nt(stage_old(d, src) + (size_t)c * 4, o);
+      if (src == 1) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) q[i] = o[i];
       } else {
-        ld4d_nt(d.qc + (size_t)c * 4, q);
+        ld4d_nt(stage_in(d, src) + (size_t)c * 4, q);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
-      st4d_nt(d.qc + (size_t)c * 4, q);
+      st4d_nt(stage_out(d, src) + (size_t)c * 4, q);
     }
-    if (mode == 2)
+    if (op == 2)
       for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];
   }
-  if (mode == 2) {
+  if (op == 2) {
     block_reduce<1>(sh, &sq, OpAdd());
     if (threadIdx.x == 0) d.part[vb.b] = sq;
   }
@@ -807,6 +826,7 @@ __device__ __forceinline__ bool face_flux_ab(const Gas<R>& gr, const R* ra, cons
 template <typename R, int FLUX, bool SL>
 __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
                                                double ak, double bk, int write_res) {
+  const int op = mode & 3, src = mode >> 2;
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);
@@ -861,24 +881,24 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
         for (int i = 0; i < 4; ++i) r[i] = r[i] - (double)fl[l * 4 + i];
       }
     }
-    if (mode == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
-    if (mode >= 1) {
+    if (op == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
+    if (op >= 1) {
       double dta = ld_nt(d.dta + c), q[4], o[4];
-      ld4d_nt(d.qold + (size_t)c * 4, o);
-      if (mode == 3) {
+      ld4d_nt(stage_old(d, src) + (size_t)c * 4, o);
+      if (src == 1) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) q[i] = o[i];
       } else {
-        ld4d_nt(d.qc + (size_t)c * 4, q);
+        ld4d_nt(stage_in(d, src) + (size_t)c * 4, q);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
-      st4d_nt(d.qc + (size_t)c * 4, q);
+      st4d_nt(stage_out(d, src) + (size_t)c * 4, q);
     }
-    if (mode == 2)
+    if (op == 2)
       for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];
   }
-  if (mode == 2) {
+  if (op == 2) {
     block_reduce<1>(sh, &sq, OpAdd());
     if (threadIdx.x == 0) d.part[vb.b] = sq;
   }

@@ -323,13 +323,14 @@ int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, dou
 
 template <typename R>
 int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int write_res) {
+  const int src = mode >> 2;
   int visc = x->p.flow != UFX_FLOW_INVISCID;
   const Gas<R>& gr = gas_of<R>(x);
   int nbb = nblocks(x->nbf);
 #define KS(F)                                                                                   \
   do {                                                                                          \
     if (x->nbf > 0)                                                                             \
-      LAUNCH(x, UFX_K_BFACE, (k_bface<R, F>), nbb, d, x->g, visc, x->p.iwall, x->p.t_wall);    \
+      LAUNCH(x, UFX_K_BFACE, (k_bface<R, F>), nbb, d, x->g, visc, x->p.iwall, x->p.t_wall, src); \
     if (x->stage_shared && x->stage_local)                                                      \
       LAUNCH_LDS(x, UFX_K_STAGE, (k_stage_blk<R, F, true>), x->nblk_cell, x->stage_lds, d, gr,  \
                  mode, ak, bk, write_res);                                                      \
@@ -402,12 +403,14 @@ const double RK_B[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
 
 // One Runge-Kutta stage: residual and update fused in one pass when nothing
 // else contributes to the residual, else the full residual then the update.
-// same_old: qc equals qold (the first stage of a whole iteration).
+// whole: a stage of a whole iteration (its states in qc and qs, see
+// stage_old/in/out), else routine by routine (qold and qc).
 template <typename R>
-int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool same_old) {
+int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool whole) {
   double ak = RK_A[stage - 1], bk = RK_B[stage - 1];
   if (x->p.flow == UFX_FLOW_INVISCID && !x->p.irs)
-    return run_stage_kernel(x, d, norm ? 2 : (same_old ? 3 : 1), ak, bk, write_res ? 1 : 0);
+    return run_stage_kernel(x, d, (norm ? 2 : 1) | ((whole ? stage : 0) << 2), ak, bk,
+                            write_res ? 1 : 0);
   if (int rc = run_residual_full(x, d)) return rc;
   if (int rc = run_rk_update(x, d, ak, bk)) return rc;
   if (norm) {
@@ -456,7 +459,9 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
   bool visc = x->p.flow != UFX_FLOW_INVISCID;
   // prim refresh, qold (save_old commutes with avg_c2v and the time step),
   // positivity and the global range (re-armed by the previous norm)
-  if (int rc = run_prim(x, d, 1, 1, lim ? 1 : 0)) return rc;
+  // the fused RK3 stages keep the start state in qc (no qold copy)
+  const bool fused = x->p.timemode == UFX_TIME_RK3 && !visc && !x->p.irs;
+  if (int rc = run_prim(x, d, 1, fused ? 2 : 1, lim ? 1 : 0)) return rc;
   if (lim)
     if (int rc = run_range_final(x, d)) return rc;
   if (int rc = run_avg(x, d)) return rc;
@@ -465,7 +470,7 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
   if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
   if (x->p.timemode == UFX_TIME_RK3) {
     for (int s = 1; s <= 3; ++s)
-      if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res, s == 1)) return rc;
+      if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res, fused)) return rc;
   } else {
     if (int rc = run_lusgs(x, d)) return rc;
     LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
@@ -764,7 +769,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     std::vector<R> qpc((size_t)noc * 4, R(0));
     UP(qpc, qpc);
   }
-  AL(fw, nof); AL(qT, ncell); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc);
+  AL(fw, nof); AL(qT, ncell); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc); AL(qs, 4 * (size_t)noc);
   AL(res, 4 * (size_t)noc); AL(dt, noc); AL(dta, noc); AL(la, noc); AL(fla, nof); AL(fmu, nof);
   AL(cmul, ncell); AL(prec, (size_t)nop * PW); AL(fgrad, (size_t)nof * 10);
   AL(bflux, 2 * (size_t)std::max(1, nbf)); AL(bres, 4 * (size_t)std::max(1, nbf));
@@ -1497,7 +1502,7 @@ int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep) {
   auto M = [&](int t) { return (const int4*)(b->d_map + b->off[t]); };
   auto G = [&](int t) { return dim3((unsigned)b->cnt[t]); };
   const bool lim = x0->p.ilimit != 0;
-  km_prim<R><<<G(T_ALL), BLK, 0, s>>>(cs, M(T_ALL), 1, 1, lim ? 1 : 0);
+  km_prim<R><<<G(T_ALL), BLK, 0, s>>>(cs, M(T_ALL), 1, 2, lim ? 1 : 0);
   if (lim) km_range_final<R><<<G(T_CASE), BLK, 0, s>>>(cs, M(T_CASE));
   km_avg_c2v<R, false><<<G(T_PT), BLK, 0, s>>>(cs, M(T_PT));
   km_face_lambda<R><<<G(T_FACE), BLK, 0, s>>>(cs, M(T_FACE));
@@ -1511,11 +1516,11 @@ int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep) {
 #undef KMC
   for (int st = 1; st <= 3; ++st) {
     const double ak = RK_A[st - 1], bk = RK_B[st - 1];
-    const int mode = st == 3 ? 2 : (st == 1 ? 3 : 1), wr = st == 3 && keep ? 1 : 0;
+    const int mode = (st == 3 ? 2 : 1) | (st << 2), wr = st == 3 && keep ? 1 : 0;
     const size_t lds = b->stage_lds;
 #define KMS(F)                                                                                  \
   do {                                                                                          \
-    if (b->cnt[T_BF]) km_bface<R, F><<<G(T_BF), BLK, 0, s>>>(cs, M(T_BF), 0);                   \
+    if (b->cnt[T_BF]) km_bface<R, F><<<G(T_BF), BLK, 0, s>>>(cs, M(T_BF), 0, st);               \
     if (x0->stage_shared && x0->stage_local)                                                    \
       km_stage_blk<R, F, true><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr);    \
     else if (x0->stage_shared)                                                                  \
