@@ -246,6 +246,66 @@ __global__ __launch_bounds__(BLK) void k_avg_c2v(DevT<R> d) {
   avg_c2v_body<R, VISC>(d, hw_block());
 }
 
+// The global range of k_range_final with wave shuffles and 256 bytes of
+// LDS (max and min do not depend on the order), for a block riding along
+// with another kernel's launch.
+template <typename R>
+__device__ __forceinline__ void range_final_small(const DevT<R>& d, int nblk) {
+  __shared__ double shw[8][BLK / 64];
+  double mx[4], mn[4];
+  for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
+  const double2* p2 = reinterpret_cast<const double2*>(d.part);
+#pragma unroll 1
+  for (int b = threadIdx.x; b < nblk; b += BLK) {
+    const double2 v0 = p2[(size_t)b * 4], v1 = p2[(size_t)b * 4 + 1];
+    const double2 v2 = p2[(size_t)b * 4 + 2], v3 = p2[(size_t)b * 4 + 3];
+    const double a[4] = {v0.x, v0.y, v1.x, v1.y}, m[4] = {v2.x, v2.y, v3.x, v3.y};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      mx[k] = a[k] > mx[k] ? a[k] : mx[k];
+      mn[k] = m[k] < mn[k] ? m[k] : mn[k];
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double a = __shfl_xor(mx[k], off), b = __shfl_xor(mn[k], off);
+      mx[k] = a > mx[k] ? a : mx[k];
+      mn[k] = b < mn[k] ? b : mn[k];
+    }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) {
+      shw[k][wave] = mx[k];
+      shw[4 + k][wave] = mn[k];
+    }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    double a = shw[k][0], b = shw[4 + k][0];
+    for (int w = 1; w < BLK / 64; ++w) {
+      a = shw[k][w] > a ? shw[k][w] : a;
+      b = shw[4 + k][w] < b ? shw[4 + k][w] : b;
+    }
+    d.range[k] = ord_of(a);
+    d.range[4 + k] = ord_of(b);
+  }
+}
+
+// Vertex averages with the limiter's range reduction as one extra block:
+// neither needs the other, and the range no longer costs a one-block
+// launch of its own.  The range block is block 0, so it starts first and
+// its serial loop overlaps the averages.
+template <typename R, bool VISC>
+__global__ __launch_bounds__(BLK) void k_avg_c2v_range(DevT<R> d, int nblk_all) {
+  if (blockIdx.x == 0) {
+    range_final_small<R>(d, nblk_all);
+    return;
+  }
+  avg_c2v_body<R, VISC>(d, VBlock{(int)blockIdx.x - 1, (int)gridDim.x - 1});
+}
+
 template <typename R>
 __device__ __forceinline__ R vk_limit(R a, R b, R tol) {
   R nr = a * a + R(2) * a * b + tol * tol;

@@ -278,6 +278,17 @@ int run_avg(ufx_cc* x, DevT<R>& d) {
   return launch_check();
 }
 
+// avg_c2v with the limiter's range reduction riding along (k_prim's block
+// partials must be complete, as for run_range_final).
+template <typename R>
+int run_avg_range(ufx_cc* x, DevT<R>& d) {
+  if (x->p.flow != UFX_FLOW_INVISCID)
+    LAUNCH(x, UFX_K_AVG, (k_avg_c2v_range<R, true>), x->nblk_pt + 1, d, x->nblk_all);
+  else
+    LAUNCH(x, UFX_K_AVG, (k_avg_c2v_range<R, false>), x->nblk_pt + 1, d, x->nblk_all);
+  return launch_check();
+}
+
 // k_cell with the options of one call; with_t when the temperature gradient
 // has a consumer (the drop-in gradient routines always produce it).
 template <typename R, int GR>
@@ -462,9 +473,8 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
   // the fused RK3 stages keep the start state in qc (no qold copy)
   const bool fused = x->p.timemode == UFX_TIME_RK3 && !visc && !x->p.irs;
   if (int rc = run_prim(x, d, 1, fused ? 2 : 1, lim ? 1 : 0)) return rc;
-  if (lim)
-    if (int rc = run_range_final(x, d)) return rc;
-  if (int rc = run_avg(x, d)) return rc;
+  // vertex averages, with the range reduction as one more block
+  if (int rc = lim ? run_avg_range(x, d) : run_avg(x, d)) return rc;
   // gradient, limiter and time step in one pass over the cells; the
   // temperature gradient only when the viscous flux needs it
   if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
