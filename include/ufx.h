@@ -14,7 +14,8 @@
  *  - Return value: UFX_OK (0) or a positive error code; ufx_last_error() gives
  *    the message (thread-local).  Errors the reference reports with STOP are
  *    returned as codes; device-detected ones (negative Roe sound speed,
- *    non-positive density/pressure) surface at the next synchronising call.
+ *    non-positive density/pressure) surface at the next synchronising call,
+ *    and ufx_cc_error names the face or cell.
  *  - Index arrays passed in are 1-based, in the reference's numbering; arrays
  *    read back with ufx_mesh_get are 1-based as well.
  *  - All calls on a context are asynchronous on the context's own HIP stream
@@ -195,7 +196,11 @@ int ufx_cc_limit(ufx_cc* ctx);
 /* Finite-volume residual, CCFVM/fvresidual.F90:4-187 (boundary faces first,
  * then interior faces, then viscous flux and residual smoothing). */
 int ufx_cc_fvresidual(ufx_cc* ctx);
-/* Runge-Kutta stage update of CCFVM/main.F90:92-101 for stage 1..3. */
+/* Runge-Kutta stage update of CCFVM/main.F90:92-101 for stage 1..3.  Like
+ * ufx_cc_rk_stage, ufx_cc_lusgs and ufx_cc_get("qold") it reads qold, and
+ * returns UFX_ERR_STATE after whole inviscid RK3 iterations (which keep their
+ * start state elsewhere and leave qold as it was) until ufx_cc_save_old or
+ * ufx_cc_set("qold") writes it again. */
 int ufx_cc_rk_update(ufx_cc* ctx, int stage);
 /* fvresidual followed by the stage update in one pass (same results). */
 int ufx_cc_rk_stage(ufx_cc* ctx, int stage);
@@ -236,6 +241,13 @@ int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double*
  * run it with ufx_cc_iterate).  The chunk is captured on the first call and
  * replayed while niter and cfl stay the same.  hist as ufx_cc_iterate. */
 int ufx_cc_iterate_graph(ufx_cc* ctx, int iter0, int niter, double cfl, double* hist);
+
+/* The normalised residuals of the last whole-iteration call on the context
+ * (ufx_cc_iterate, ufx_cc_iterate_graph, or the context's share of
+ * ufx_cc_batch_iterate, which has no history argument): entries 0..n-1, one
+ * per iteration of that call; synchronising.  n may not exceed the longest
+ * call so far. */
+int ufx_cc_history(ufx_cc* ctx, int n, double* out);
 
 /* Keep the residual of the last stage of every whole iteration
  * (ufx_cc_iterate, ufx_cc_iterate_graph) in the context, for
@@ -298,6 +310,17 @@ int ufx_cc_profile_read(ufx_cc* ctx, double* ms, int* count);
 /* Block until all queued work has finished; returns the first device-detected
  * error, if any. */
 int ufx_cc_sync(ufx_cc* ctx);
+/* The device-detected error of the context in reference terms; synchronising.
+ * *kind: 0 none, 1 Roe's averaged sound speed squared not positive
+ * (CCFVM/roe_flux.F90:73-87), 2 non-positive density or pressure at an
+ * iteration's start (check_positivity, CCFVM/main.F90:358-414), 3
+ * non-positive diamond-path area (CCFVM/gradient.F90:297).  *face: the
+ * 1-based reference face at which the reference's sequential loop stops
+ * (kinds 1 and 3; boundary faces before interior faces, as fvresidual loops
+ * them), else 0; *cell: its `in` cell (kinds 1, 3) or the first failing cell
+ * (kind 2), 1-based.  An error ends the context's useful life, as the
+ * reference's STOP ends the run: later kernels skip their work. */
+int ufx_cc_error(ufx_cc* ctx, int* kind, int* face, int* cell);
 /* The context's HIP stream (hipStream_t), for timing with HIP events. */
 void* ufx_cc_stream(ufx_cc* ctx);
 

@@ -319,6 +319,7 @@ void orc_cc_initialize(orc_cc* s) {
     for (int k = 0; k < 4; ++k) s->phi[c * 4 + k] = 1.0;
   s->error = 0;
   s->error_cell = 0;
+  s->error_face = 0;
 }
 
 /* CCFVM/main.F90:62-67 */
@@ -565,7 +566,7 @@ static void grad_ggfc(orc_cc* s) {
       prim[1][k] = s->qp[out * 5 + k];
       prim[2][k] = s->pprim[p2 * 5 + k];
     }
-    if (gradtrixy(prim, xy, g1, &a1)) { s->error = 2; s->error_cell = f + 1; return; }
+    if (gradtrixy(prim, xy, g1, &a1)) { s->error = 2; s->error_cell = s->error_face = f + 1; return; }
     xy[0][0] = s->px[p1]; xy[0][1] = s->py[p1];
     xy[1][0] = s->px[p2]; xy[1][1] = s->py[p2];
     xy[2][0] = s->ccx[in]; xy[2][1] = s->ccy[in];
@@ -574,7 +575,7 @@ static void grad_ggfc(orc_cc* s) {
       prim[1][k] = s->pprim[p2 * 5 + k];
       prim[2][k] = s->qp[in * 5 + k];
     }
-    if (gradtrixy(prim, xy, g2, &a2)) { s->error = 2; s->error_cell = f + 1; return; }
+    if (gradtrixy(prim, xy, g2, &a2)) { s->error = 2; s->error_cell = s->error_face = f + 1; return; }
     for (int k = 0; k < 10; ++k) s->fgrad[f * 10 + k] = (a1 * g1[k] + a2 * g2[k]) / (a1 + a2);
     for (int k = 0; k < 10; ++k)
       s->grad[in * 10 + k] = s->grad[in * 10 + k] + s->fgrad[f * 10 + k] * (a1 + a2);
@@ -685,6 +686,25 @@ static void cirs(orc_cc* s) {
   free(itr);
 }
 
+/* One face flux of fvresidual.  The reference STOPs inside roe_flux when the
+ * Roe-averaged sound speed squared is not positive, printing the two states it
+ * was given (CCFVM/roe_flux.F90:73-87); the first such face in fvresidual's
+ * loop order (boundary faces, then interior faces) is recorded with those
+ * states, 1-based. */
+static void face_flux(orc_cc* s, const gas_t* g, double* qcl, double* qcr, int f, int in,
+                      double* F) {
+  double l[4], r[4];
+  memcpy(l, qcl, sizeof l);
+  memcpy(r, qcr, sizeof r);
+  if (num_flux(s, g, qcl, qcr, f, F) && !s->error) {
+    s->error = 1;
+    s->error_cell = in + 1;
+    s->error_face = f + 1;
+    memcpy(s->error_ql, l, sizeof l);
+    memcpy(s->error_qr, r, sizeof r);
+  }
+}
+
 /* fvresidual, CCFVM/fvresidual.F90:4-187 */
 void orc_cc_fvresidual(orc_cc* s) {
   gas_t g = gas_of(s);
@@ -701,7 +721,7 @@ void orc_cc_fvresidual(orc_cc* s) {
                s->phi[in * 4 + i] * (s->grad[in * 10 + 2 * i] * dx + s->grad[in * 10 + 2 * i + 1] * dy);
     if (bc == 2001) {
       farfield(&g, qcl, s->fnx[f], s->fny[f], qcr);
-      if (num_flux(s, &g, qcl, qcr, f, F) && !s->error) { s->error = 1; s->error_cell = in + 1; }
+      face_flux(s, &g, qcl, qcr, f, in, F);
       for (int i = 0; i < 4; ++i) s->res[in * 4 + i] = s->res[in * 4 + i] + F[i];
       double conv[4];
       prim2con(&g, qcr, conv);
@@ -738,7 +758,7 @@ void orc_cc_fvresidual(orc_cc* s) {
         con2prim(&g, gq, gp);
       }
       for (int i = 0; i < 4; ++i) qcr[i] = gp[i];
-      if (num_flux(s, &g, qcl, qcr, f, F) && !s->error) { s->error = 1; s->error_cell = in + 1; }
+      face_flux(s, &g, qcl, qcr, f, in, F);
       for (int i = 0; i < 4; ++i) s->res[in * 4 + i] = s->res[in * 4 + i] + F[i];
       s->fx1 = s->fx1 + F[2];
       s->fy1 = s->fy1 + F[3];
@@ -748,7 +768,7 @@ void orc_cc_fvresidual(orc_cc* s) {
     int in = s->fin[f], out = s->fout[f];
     double qcl[4], qcr[4], F[4];
     recon(s, f, in, out, qcl, qcr);
-    if (num_flux(s, &g, qcl, qcr, f, F) && !s->error) { s->error = 1; s->error_cell = in + 1; }
+    face_flux(s, &g, qcl, qcr, f, in, F);
     for (int i = 0; i < 4; ++i) {
       s->res[in * 4 + i] = s->res[in * 4 + i] + F[i];
       s->res[out * 4 + i] = s->res[out * 4 + i] - F[i];
@@ -860,6 +880,69 @@ int orc_cc_check_positivity(const orc_cc* s) {
   for (int c = 0; c < s->noc; ++c)
     if (s->qp[c * 5] <= 0.0 || s->qp[c * 5 + 3] <= 0.0) return c + 1;
   return 0;
+}
+
+/* find_cl_cd, CCFVM/main.F90:263-355 (contained in screen).  Wall faces
+ * (bc 1001) in boundary-face order: the wall pressure is the adjacent cell's
+ * qp(4) (:302); Cp = (p - p_inf)/q_inf with q_inf = 0.5 gamma p_inf M^2
+ * (:288); the force sums p n area (:316-317) and, for viscous flow, minus the
+ * face stress with mu = fc%mu/Rey and fc%grad of the last viscous pass
+ * (:319-332); Cf = tau_w area / q_inf (:334-337); Cm sums Cp n area's moment
+ * about (xref, yref) at the face midpoint (:340).  out[0..6] = cl, cd of the
+ * surface integral (:349-350, printed "Cl, Cd"), cl1, cd1 of the flux sums
+ * Fx1, Fy1 of the last residual (:352-353, what the screen block then prints
+ * as Cl, Cd), cm, fx, fy.  xw, yw, mcp, cf (any may be NULL) receive the rows
+ * of XYCpCf.dat (:343): midpoint, -Cp, Cf.  Returns the wall face count. */
+int orc_cc_find_cl_cd(const orc_cc* s, double xref, double yref, double* out, double* xw,
+                      double* yw, double* mcp, double* cf) {
+  gas_t g = gas_of(s);
+  const double p_inf = g.p_inf, m_inf = s->mach;
+  const double dyna = 0.5 * g.gamma * p_inf * m_inf * m_inf;
+  const double two = 2.0, two_third = two / 3.0;
+  const double aoa = g.aoa;
+  double fx = 0.0, fy = 0.0, cm = 0.0;
+  int n = 0;
+  for (int f = s->nfi; f < nof(s); ++f) {
+    if (s->fbc[f] != 1001) continue;
+    double pwall = s->qp[s->fin[f] * 5 + 3];
+    int p1 = s->fp1[f], p2 = s->fp2[f];
+    double xi = 0.5 * (s->px[p1] + s->px[p2]);
+    double yi = 0.5 * (s->py[p1] + s->py[p2]);
+    double xa = xi - xref, ya = yi - yref;
+    double area = s->farea[f], nx = s->fnx[f], ny = s->fny[f];
+    double cp = (pwall - p_inf) / dyna;
+    double dcx = cp * nx * area, dcy = cp * ny * area;
+    fx = fx + pwall * nx * area;
+    fy = fy + pwall * ny * area;
+    double cfv = 0.0;
+    if (s->flow_type != 1) {
+      double mu = s->fmu[f] / s->reynolds;
+      const double* fg = &s->fgrad[f * 10];
+      double ux = fg[2], uy = fg[3], vx = fg[4], vy = fg[5];
+      double tauxx = two_third * (two * ux - vy);
+      double tauxy = uy + vx;
+      double tauyy = two_third * (two * vy - ux);
+      fx = fx - mu * (tauxx * nx + tauxy * ny) * area;
+      fy = fy - mu * (tauxy * nx + tauyy * ny) * area;
+      double tx = ny, ty = -nx;
+      double tw = mu * ((ux * tx + vx * ty) * nx + (uy * tx + vy * ty) * ny);
+      cfv = tw * area / dyna;
+    }
+    cm = cm + dcx * ya - dcy * xa;
+    if (xw) xw[n] = xi;
+    if (yw) yw[n] = yi;
+    if (mcp) mcp[n] = -cp;
+    if (cf) cf[n] = cfv;
+    ++n;
+  }
+  out[0] = (fy * cos(aoa) - fx * sin(aoa)) / dyna;
+  out[1] = (fy * sin(aoa) + fx * cos(aoa)) / dyna;
+  out[2] = (s->fy1 * cos(aoa) - s->fx1 * sin(aoa)) / dyna;
+  out[3] = (s->fy1 * sin(aoa) + s->fx1 * cos(aoa)) / dyna;
+  out[4] = cm;
+  out[5] = fx;
+  out[6] = fy;
+  return n;
 }
 
 /* one iteration of CCFVM/main.F90:59-115 */

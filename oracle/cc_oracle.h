@@ -47,7 +47,9 @@ typedef struct {
   /* outputs */
   double fx1, fy1, dtglobal, fres, fres1;
   int error;       /* 1: Roe negative sound speed; 2: non-positive area (ggfc) */
-  int error_cell;
+  int error_cell;  /* 1-based: the failing face's `in` cell (Roe); the face (ggfc) */
+  int error_face;  /* 1-based reference face of the first failure */
+  double error_ql[4], error_qr[4]; /* Roe: the states roe_flux received */
 } orc_cc;
 
 void orc_cc_initialize(orc_cc* s);
@@ -62,6 +64,11 @@ void orc_cc_rk_update(orc_cc* s, int stage);
 void orc_cc_lusgs(orc_cc* s);
 void orc_cc_residue(orc_cc* s, int iter);
 int orc_cc_check_positivity(const orc_cc* s);
+/* find_cl_cd, CCFVM/main.F90:263-355: out[0..6] = cl, cd (surface integral),
+ * cl1, cd1 (flux sums), cm, fx, fy; per wall face the XYCpCf.dat row
+ * (midpoint x, y, -Cp, Cf; any may be NULL).  Returns the wall face count. */
+int orc_cc_find_cl_cd(const orc_cc* s, double xref, double yref, double* out, double* xw,
+                      double* yw, double* mcp, double* cf);
 /* One iteration of CCFVM/main.F90:59-115 with the given CFL (rk3 or lusgs
  * by s->... timemode argument: 1 rk3, 2 lusgs). */
 void orc_cc_iteration(orc_cc* s, int iter, double cfl, int timemode);

@@ -47,7 +47,8 @@ class _State(C.Structure):
         + [(n, _dp) for n in ("qc", "qp", "qold", "res", "grad", "phi", "dt", "la", "fla", "fmu",
                               "fqp", "fgrad", "cmul", "pprim", "pmu")]
         + [(n, C.c_double) for n in ("fx1", "fy1", "dtglobal", "fres", "fres1")]
-        + [("error", C.c_int), ("error_cell", C.c_int)]
+        + [("error", C.c_int), ("error_cell", C.c_int), ("error_face", C.c_int),
+           ("error_ql", C.c_double * 4), ("error_qr", C.c_double * 4)]
     )
 
 
@@ -57,6 +58,8 @@ for _n in ("initialize", "prim", "avg_c2v", "time_step", "save_old", "gradient",
 _lib.orc_cc_rk_update.argtypes = [C.POINTER(_State), C.c_int]
 _lib.orc_cc_residue.argtypes = [C.POINTER(_State), C.c_int]
 _lib.orc_cc_check_positivity.argtypes = [C.POINTER(_State)]
+_lib.orc_cc_find_cl_cd.argtypes = [C.POINTER(_State), C.c_double, C.c_double, _dp, _dp, _dp, _dp,
+                                  _dp]
 _lib.orc_cc_iteration.argtypes = [C.POINTER(_State), C.c_int, C.c_double, C.c_int]
 _lib.orc_flux.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
 _lib.orc_farfield_state.argtypes = [C.c_double, C.c_double, _dp, C.c_double, C.c_double, _dp]
@@ -161,6 +164,28 @@ class CCOracle:
     @property
     def forces(self):
         return self.s.fx1, self.s.fy1
+
+    def find_cl_cd(self, xref=0.0, yref=0.0) -> dict:
+        """find_cl_cd of the current state (CCFVM/main.F90:263-355): the
+        surface-integral cl, cd, the flux-based cl1, cd1, cm, fx, fy, and the
+        XYCpCf.dat rows (wall-face midpoint x, y, -Cp, Cf)."""
+        nw = int(np.count_nonzero(self._keep["fbc"][self.nfi:] == 1001))
+        out = np.zeros(7)
+        xw, yw, mcp, cf = (np.zeros(nw) for _ in range(4))
+        n = _lib.orc_cc_find_cl_cd(C.byref(self.s), xref, yref, out.ctypes.data_as(_dp),
+                                   xw.ctypes.data_as(_dp), yw.ctypes.data_as(_dp),
+                                   mcp.ctypes.data_as(_dp), cf.ctypes.data_as(_dp))
+        assert n == nw
+        return dict(cl=out[0], cd=out[1], cl1=out[2], cd1=out[3], cm=out[4], fx=out[5], fy=out[6],
+                    x=xw, y=yw, mcp=mcp, cf=cf)
+
+    @property
+    def error_info(self) -> dict:
+        """The first device-style error the last calls met: kind (1 Roe,
+        2 diamond-path area), 1-based face and cell, and for Roe the two
+        states roe_flux received."""
+        return dict(kind=self.s.error, face=self.s.error_face, cell=self.s.error_cell,
+                    ql=np.array(self.s.error_ql[:]), qr=np.array(self.s.error_qr[:]))
 
 
 def flux(kind: str, mach: float, ql, qr, nx: float, ny: float, area: float):

@@ -67,6 +67,34 @@ CASES = {
                            False, ("final_qc",)),
 }
 
+# runs of the reference's own main program (CCFVM/main.F90, unmodified):
+# name -> (mesh kwargs, deck kwargs).  With ScrInt = MaxIter the last
+# iteration calls screen -> find_cl_cd, which writes XYCpCf.dat (wall-face
+# midpoint, -Cp, Cf; list-directed, i.e. every double to its shortest
+# round-trip digits) and prints the force coefficients; the "stop" cases end
+# at one of the reference's STOPs (Roe's non-positive sound speed,
+# check_positivity), whose printed values are kept.
+STEEP = dict(nx=12, ny=6, seed=7, h=0.2)
+MAIN_CASES = {
+    "ccmain_forces_euler": (TINY, dict(mach=0.5, aoa_deg=1.5, xref=0.5, yref=0.1, maxiter=30,
+                                       scrint=30)),
+    "ccmain_forces_laminar": (VISC, dict(mach=0.3, aoa_deg=1.0, flow_type="laminar",
+                                         reynolds=1000.0, iwall=1, grad_type="ggfc", irs=1,
+                                         cfl_min=0.5, cfl_max=0.8, cfl_ramp_steps=5, xref=0.3,
+                                         yref=-0.05, maxiter=10, scrint=10)),
+    "ccmain_forces_isothermal": (VISC, dict(mach=0.3, flow_type="laminar", reynolds=1000.0,
+                                            iwall=0, grad_type="lsqr", cfl_min=0.5, cfl_max=0.8,
+                                            cfl_ramp_steps=5, maxiter=10, scrint=10)),
+    # M = 3 over a steep bump: a reconstructed state with negative pressure
+    # makes Roe's averaged sound speed imaginary (CCFVM/roe_flux.F90:73-87)
+    "ccmain_stop_roe": (STEEP, dict(mach=3.0, cfl_min=0.99, cfl_max=0.99, maxiter=300)),
+    # AUSM+-up under LU-SGS at CFL 50 leaves negative states after iteration 1
+    # (CCFVM/main.F90:384-411)
+    "ccmain_stop_positivity": (STEEP, dict(mach=0.8, flux_type="ausm", timemode="lusgs",
+                                           cfl_min=50.0, cfl_max=50.0, maxiter=300)),
+}
+MAIN = os.path.join(ROOT, "oracle", "_ref", "cc", "ref_main")
+
 # vertex-centred cases: name -> (o_grid kwargs, deck kwargs, gradient, niter,
 # ndump, keep_mesh)
 NC_TINY = dict(ni=24, nj=10, seed=3, stretch=1.2)
@@ -146,6 +174,62 @@ def make(name: str) -> str:
     return path
 
 
+def _floats(text: str) -> list:
+    return [float(t) for t in text.split()]
+
+
+def make_main(name: str) -> str:
+    """Run the reference's unmodified main program and keep what it writes:
+    FLO.RES (iteration, log10 of the normalised residual, CFL), XYCpCf.dat,
+    and from the screen output the first-iteration residue, the printed force
+    coefficients and, for a STOP, the iteration it happened in with the
+    values printed there."""
+    mkw, dkw = MAIN_CASES[name]
+    mkw = dict(mkw)
+    raw = meshgen.bump_channel(mkw.pop("nx"), mkw.pop("ny"), mkw.pop("seed"), **mkw)
+    deck = meshgen.Deck(**dkw)
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        meshgen.write_geometry_inp(raw, os.path.join(tmp, "geometry.inp"))
+        with open(os.path.join(tmp, "flo.inp"), "w") as f:
+            f.write(deck.text())
+        r = subprocess.run([MAIN, "flo.inp"], cwd=tmp, capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: reference failed\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        lines = r.stdout.splitlines()
+        res = open(os.path.join(tmp, "FLO.RES")).read().split()
+        out["flo_res"] = np.array([float(t) for t in res]).reshape(-1, 3)
+        for ln in lines:
+            if "Residue in first iteration" in ln:
+                out["fres1"] = np.array([float(ln.split("=")[1])])
+            elif ln.strip().startswith("Cl, Cd, Cm"):
+                out["printed_cl_cd_cm"] = np.array(_floats(ln.split("=")[1]))
+            elif ln.strip().startswith("Cl1, Cd1"):
+                out["printed_cl1_cd1"] = np.array(_floats(ln.split("=")[1]))
+            elif ln.strip().startswith("Cl, Cd"):
+                out["printed_cl_cd"] = np.array(_floats(ln.split("=")[1]))
+        xy = os.path.join(tmp, "XYCpCf.dat")
+        if os.path.exists(xy):
+            out["xycpcf"] = np.array(_floats(open(xy).read())).reshape(-1, 4)
+        text = r.stdout
+        stop_it = len(out["flo_res"]) + 1
+        if "Sonic speed is negative" in text:
+            seg = text.split("Left/right cell values")[1].split("Left/right vertex values")[0]
+            v = _floats(seg)
+            assert len(v) == 8
+            out["stop_roe"] = np.array([stop_it] + v)
+        if "Density/pressure is negative at point" in text:
+            cell = int(text.split("Density/pressure is negative at point")[1].split()[0])
+            out["stop_positivity"] = np.array([stop_it, cell], dtype=np.int64)
+    meta = dict(mesh=MAIN_CASES[name][0], deck=dkw, raw_sha=raw_sha(raw),
+                reference="CCFVM main program unmodified, amdflang -O2 -ffp-contract=off")
+    out["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
 def nc_raw_sha(m: nc_mesh.NCRawMesh) -> str:
     h = hashlib.sha256()
     for a in (m.x, m.y, m.pbc, m.cen, m.area, m.edge, m.ebc):
@@ -188,10 +272,11 @@ def make_nc(name: str) -> str:
 
 
 def main(argv):
-    if not os.path.exists(DRIVER) or not os.path.exists(NC_DRIVER):
+    if not all(os.path.exists(p) for p in (DRIVER, NC_DRIVER, MAIN)):
         raise SystemExit("build the reference first: make -C oracle ref")
-    for name in (argv or list(CASES) + list(NC_CASES)):
-        p = make_nc(name) if name in NC_CASES else make(name)
+    for name in (argv or list(CASES) + list(NC_CASES) + list(MAIN_CASES)):
+        p = (make_nc(name) if name in NC_CASES else make_main(name) if name in MAIN_CASES
+             else make(name))
         print(f"{name}: {os.path.getsize(p) / 1024:.0f} KiB", flush=True)
 
 

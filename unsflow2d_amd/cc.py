@@ -87,10 +87,12 @@ _sig("ufx_cc_rk_update", [_vp, C.c_int])
 _sig("ufx_cc_rk_stage", [_vp, C.c_int])
 _sig("ufx_cc_residue", [_vp, C.c_int, _dp, _dp])
 _sig("ufx_cc_check_positivity", [_vp, _ip])
+_sig("ufx_cc_error", [_vp, _ip, _ip, _ip])
 _sig("ufx_cc_forces", [_vp, _dp, _dp, _dp])
 _sig("ufx_cc_iterate", [_vp, C.c_int, C.c_int, _dp, _dp])
 _sig("ufx_cc_iterate_graph", [_vp, C.c_int, C.c_int, C.c_double, _dp])
 _sig("ufx_cc_keep_residual", [_vp, C.c_int])
+_sig("ufx_cc_history", [_vp, C.c_int, _dp])
 _sig("ufx_cc_residual_norm", [_vp, C.c_int, _dp])
 _sig("ufx_cc_stream", [_vp], C.c_void_p)
 _sig("ufx_cc_profile", [_vp, C.c_int])
@@ -110,9 +112,9 @@ EXPORTED = [
     "ufx_cc_prim", "ufx_cc_avg_c2v", "ufx_cc_time_step", "ufx_cc_save_old", "ufx_cc_gradient",
     "ufx_cc_limit", "ufx_cc_fvresidual", "ufx_cc_rk_update", "ufx_cc_rk_stage", "ufx_cc_lusgs",
     "ufx_cc_residue", "ufx_cc_check_positivity", "ufx_cc_forces", "ufx_cc_iterate",
-    "ufx_cc_iterate_graph", "ufx_cc_keep_residual", "ufx_cc_residual_norm",
+    "ufx_cc_iterate_graph", "ufx_cc_history", "ufx_cc_keep_residual", "ufx_cc_residual_norm",
     "ufx_cc_profile_mask",
-    "ufx_cc_sync", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
+    "ufx_cc_sync", "ufx_cc_error", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
     "ufx_cc_batch_create", "ufx_cc_batch_set_active", "ufx_cc_batch_iterate",
     "ufx_cc_batch_residual_norms", "ufx_cc_batch_destroy",
 ]
@@ -319,6 +321,16 @@ class Solver:
         _check(lib.ufx_cc_residue(self._h, it, C.byref(f), C.byref(raw)))
         return f.value, raw.value
 
+    ERROR_KIND = {0: None, 1: "roe", 2: "positivity", 3: "area"}
+
+    def error(self) -> dict:
+        """The context's device-detected error (``ufx_cc_error``): kind
+        (None, "roe", "positivity", "area") and the 1-based reference face
+        and cell (0 where they do not apply)."""
+        k, f, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib.ufx_cc_error(self._h, C.byref(k), C.byref(f), C.byref(c)))
+        return dict(kind=self.ERROR_KIND.get(k.value, k.value), face=f.value, cell=c.value)
+
     def check_positivity(self) -> int:
         bad = C.c_int()
         rc = lib.ufx_cc_check_positivity(self._h, C.byref(bad))
@@ -371,6 +383,13 @@ class Solver:
         _check(lib.ufx_cc_iterate_graph(self._h, self.iter, niter, c, _d(h) if hist else None))
         self.iter += niter
         self.cfl = c
+        return h
+
+    def history(self, n: int) -> np.ndarray:
+        """The normalised residuals of the last whole-iteration call (also a
+        batched one), one per iteration."""
+        h = np.empty(n)
+        _check(lib.ufx_cc_history(self._h, n, _d(h)))
         return h
 
     def keep_residual(self, on: bool = True) -> None:

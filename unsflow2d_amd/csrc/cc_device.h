@@ -73,6 +73,7 @@ struct DevT {
   const int *bf_ptr, *sl_loc;
   const int2* bf_ab;  // per block-list entry: the face's in and out cell
   const R* bf_rec;    // per block-list entry: the face record
+  const int* bf_fref; // per block-list entry: the reference face number (error reports)
   int bf_max;
   const double *r11, *r12, *r22, *cv, *cov;  // noc (cov: ncell)
   const int* nc2c;
@@ -82,6 +83,7 @@ struct DevT {
   const double* wt;
   const double *px, *py;     // point storage order
   const int *fin, *fout;     // face sides (storage numbering of faces)
+  const int* fref;           // storage face -> reference face number (error reports)
   double* fw;                // a1 + a2 of the diamond path per face (GG_FC)
   // LU-SGS
   const int *lu_face, *lu_cnt;   // [MAXF][noc] reference c2f order
@@ -106,7 +108,8 @@ struct DevT {
   double* part;              // reduction partials
   double* scal;              // SC_* slots
   unsigned long long* range; // [8]: max of q0..3 (ordered), min of q0..3
-  int* err;                  // [0] code [1] cell [2] first non-positive cell
+  int* err;                  // [0] code, [1] unused, [2] lowest 1-based reference cell with
+                             // rho <= 0 or p <= 0, [3] lowest failing face key (set_err)
   double* lower;             // LU-SGS work [noc][4]
   double* upper;
 };
@@ -238,9 +241,16 @@ struct OpMax { __device__ double operator()(double a, double b) const { return a
 struct OpMin { __device__ double operator()(double a, double b) const { return a < b ? a : b; } };
 struct OpAdd { __device__ double operator()(double a, double b) const { return a + b; } };
 
+// Record an error of kind `code` at face `key`.  The first kind to arrive
+// wins; among failures of that kind the lowest key is kept, so the report
+// names the face the reference's sequential loop would have stopped at:
+// Roe, key = boundary ordinal for boundary faces (looped first,
+// CCFVM/fvresidual.F90:27) and nbf + reference face for interior faces
+// (:133); diamond-path area, key = reference face.
 template <typename R>
-__device__ __forceinline__ void set_err(const DevT<R>& d, int code, int cell) {
-  if (atomicCAS(d.err, 0, code) == 0) d.err[1] = cell;
+__device__ __forceinline__ void set_err(const DevT<R>& d, int code, int key) {
+  int old = atomicCAS(d.err, 0, code);
+  if (old == 0 || old == code) atomicMin(&d.err[3], key);
 }
 
 }  // namespace ufx

@@ -88,10 +88,11 @@ __device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<doubl
         stv<R, 4>(d.rec + (size_t)c * RW + R_QP, q4);
         stv<R, 4>(d.qpc + (size_t)c * 4, q4);
         d.qT[c] = (R)p[4];
-        if (do_old && (p[0] <= 0.0 || p[3] <= 0.0)) {
-          atomicMin(&d.err[2], d.cref[c] + 1);
-          atomicCAS(d.err, 0, DERR_POS);
-        }
+        // check_positivity (CCFVM/main.F90:384-411) runs after the
+        // iteration's update on these start-of-iteration values; the
+        // failing cell is noted here and becomes the error at the
+        // iteration's end (norm_final_body), after any error of the stages
+        if (do_old && (p[0] <= 0.0 || p[3] <= 0.0)) atomicMin(&d.err[2], d.cref[c] + 1);
       } else {
         ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q4);
       }
@@ -590,7 +591,7 @@ __global__ __launch_bounds__(BLK) void k_ggfc_face(DevT<R> d) {
   for (int k = 0; k < 4; ++k) pr[2][k] = qi[k];
   pr[2][4] = d.qT[ci];
   ok = gradtri(x, y, pr, g2, a2) && ok;
-  if (!ok) { set_err(d, DERR_AREA, f + 1); return; }
+  if (!ok) { set_err(d, DERR_AREA, d.fref[f]); return; }
   R o[10];
   R A1 = (R)a1, A2 = (R)a2;
 #pragma unroll
@@ -678,7 +679,7 @@ __device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<doub
       d.bflux[k] = F[2];
       d.bflux[d.nbf + k] = F[3];
     }
-    if (!ok) set_err(d, DERR_ROE, d.cref[c] + 1);
+    if (!ok) set_err(d, DERR_ROE, k);
     st4d(d.qc + (size_t)gh * 4, gq);
     R gp4[4] = {(R)gp[0], (R)gp[1], (R)gp[2], (R)gp[3]};
     stv<R, 4>(d.rec + (size_t)gh * RW + R_QP, gp4);
@@ -766,7 +767,7 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
     R own[RW];
     load_rec<R, SL>(d, srec, base, c, own);
     double r[4] = {0.0, 0.0, 0.0, 0.0};
-    bool ok = true;
+    int ekey = INT_MAX;  // lowest failing face key (set_err)
     for (int s = nf - nb; s < nf; ++s) {
       int k = ld_nt(d.sl_nbr + s * noc + c) - noc;
       double F[4];
@@ -814,7 +815,8 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
         beta = qa - qb;
         qr[i] = qb + R(0.5) * pb * (zhi * beta + (R(1) - zhi) * alfa);
       }
-      ok = face_flux<R, FLUX>(gr, ql, qr, fr[F_NX], fr[F_NY], fr[F_AREA], F) && ok;
+      if (!face_flux<R, FLUX>(gr, ql, qr, fr[F_NX], fr[F_NY], fr[F_AREA], F))
+        ekey = min(ekey, d.nbf + d.fref[f]);
       if (in_side) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) r[i] = r[i] + (double)F[i];
@@ -823,7 +825,7 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
         for (int i = 0; i < 4; ++i) r[i] = r[i] - (double)F[i];
       }
     }
-    if (!ok) set_err(d, DERR_ROE, d.cref[c] + 1);
+    if (ekey != INT_MAX) set_err(d, DERR_ROE, ekey);
     if (op == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
     if (op >= 1) {
       double dta = ld_nt(d.dta + c), q[4], o[4];
@@ -906,7 +908,7 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
     ldv<R, FW>(d.bf_rec + (size_t)(j0 + j) * FW, fr);
     load_rec<R, SL>(d, srec, base, ca, ra);
     load_rec<R, SL>(d, srec, base, cb, rb);
-    if (!face_flux_ab<R, FLUX>(gr, ra, rb, fr, F)) set_err(d, DERR_ROE, d.cref[ca] + 1);
+    if (!face_flux_ab<R, FLUX>(gr, ra, rb, fr, F)) set_err(d, DERR_ROE, d.nbf + d.bf_fref[j0 + j]);
 #pragma unroll
     for (int i = 0; i < 4; ++i) fl[j * 4 + i] = F[i];
   }
@@ -1140,6 +1142,8 @@ __device__ __forceinline__ void norm_final_body(const DevT<R>& d, VBlock vb, int
   for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
   block_reduce<1>(sh, &s, OpAdd());
   if (threadIdx.x == 0) {
+    // the positivity check of the iteration, CCFVM/main.F90:113
+    if (d.err[0] == 0 && d.err[2] != INT_MAX) d.err[0] = DERR_POS;
     double fres = sqrt(s / d.noc / 4);
     d.scal[SC_NORM] = fres;
     if (first) d.scal[SC_FRES1] = fres;

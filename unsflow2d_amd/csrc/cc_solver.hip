@@ -64,6 +64,7 @@ struct ufx_cc {
   int chunk_n = 0;
   double chunk_cfl = 0.0;
   bool keep_res = false;           // whole iterations also store the last residual
+  bool qold_stale = false;         // fused RK3 iterations ran since qold was last written
   unsigned prof_mask = ~0u;        // kernel categories timed while profiling
   double* d_out = nullptr;         // small device results (residual norms)
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
@@ -78,6 +79,7 @@ struct ufx_cc {
   std::vector<int> pt_of_ref;    // reference vertex -> storage vertex
   std::vector<int> wall_k;       // boundary ordinals of wall faces
   std::vector<int> fin_h, fp1_h, fp2_h;  // boundary faces (reference numbering)
+  std::vector<int> fin_ref;        // `in` cell of every face (reference numbering)
   std::vector<double> fnx_h, fny_h, farea_h, px_h, py_h;
   // LU-SGS level schedule: cells grouped by level, forward then backward
   int* lv_cells = nullptr;
@@ -196,6 +198,16 @@ int check_ctx(ufx_cc* x) {
   return UFX_OK;
 }
 
+// Whole RK3 iterations of the fused form keep their start state in qc and
+// never write qold; a routine that reads qold afterwards would see the
+// state of the last save_old.  Refuse until save_old (or set("qold")).
+int check_qold(ufx_cc* x, const char* who) {
+  if (!x->qold_stale) return UFX_OK;
+  set_error(std::string(who) + ": qold is stale after whole RK3 iterations; call "
+            "ufx_cc_save_old first");
+  return UFX_ERR_STATE;
+}
+
 template <typename T>
 int dalloc(ufx_cc* x, T** p, size_t n) {
   if (n == 0) n = 1;
@@ -225,16 +237,46 @@ int dupload(ufx_cc* x, const T** dst, const std::vector<T>& src) {
 
 int* err_word(ufx_cc* x) { return x->p.precision == UFX_PREC_FP64 ? x->d64.err : x->d32.err; }
 
-int device_error(ufx_cc* x) {
-  int e[3] = {0, 0, 0};
-  HIPCHK(hipMemcpyAsync(e, err_word(x), sizeof(e), hipMemcpyDeviceToHost, x->stream));
+// The device error word in reference terms: kind (DERR_*), 1-based
+// reference face and cell (0 where they do not apply).
+void decode_error(const ufx_cc* x, const int* e, int* kind, int* face, int* cell) {
+  *kind = e[0];
+  *face = 0;
+  *cell = 0;
+  if (e[0] == DERR_ROE && e[3] >= 0 && e[3] < x->nbf + x->nfi) {
+    // boundary faces are keyed by their ordinal, interior ones after them
+    int f = e[3] < x->nbf ? x->nfi + e[3] : e[3] - x->nbf;
+    *face = f + 1;
+    *cell = x->fin_ref[f] + 1;
+  } else if (e[0] == DERR_AREA && e[3] >= 0 && e[3] < x->nof) {
+    *face = e[3] + 1;
+    *cell = x->fin_ref[e[3]] + 1;
+  } else if (e[0] == DERR_POS) {
+    *cell = e[2];
+  }
+}
+
+int read_error(ufx_cc* x, int* e4) {
+  HIPCHK(hipMemcpyAsync(e4, err_word(x), 4 * sizeof(int), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  return UFX_OK;
+}
+
+int device_error(ufx_cc* x) {
+  int e[4] = {0, 0, 0, 0};
+  if (int rc = read_error(x, e)) return rc;
   if (e[0] == 0) return UFX_OK;
-  if (e[0] == DERR_ROE) set_error("Roe scheme: sonic speed is negative (cell " + std::to_string(e[1]) + ")");
-  else if (e[0] == DERR_POS) set_error("Density/pressure is negative at point " + std::to_string(e[2]));
-  else if (e[0] == DERR_AREA)
-    set_error("ggfc,gradtrixy,  -ve area (face " +
-              std::to_string(e[1] >= 1 && e[1] <= x->nof ? x->ref_of_face[e[1] - 1] + 1 : e[1]) + ")");
+  int kind, face, cell;
+  decode_error(x, e, &kind, &face, &cell);
+  // the reference's STOP messages (CCFVM/roe_flux.F90:74-85,
+  // CCFVM/main.F90:386, CCFVM/gradient.F90:297), with where it happened
+  if (kind == DERR_ROE)
+    set_error("Roe scheme: Sonic speed is negative (face " + std::to_string(face) + ", cell " +
+              std::to_string(cell) + ")");
+  else if (kind == DERR_POS)
+    set_error("Density/pressure is negative at point " + std::to_string(cell));
+  else if (kind == DERR_AREA)
+    set_error("ggfc,gradtrixy,  -ve area (face " + std::to_string(face) + ")");
   else set_error("device error " + std::to_string(e[0]));
   return UFX_ERR_STATE;
 }
@@ -472,6 +514,7 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
   // positivity and the global range (re-armed by the previous norm)
   // the fused RK3 stages keep the start state in qc (no qold copy)
   const bool fused = x->p.timemode == UFX_TIME_RK3 && !visc && !x->p.irs;
+  x->qold_stale = fused;
   if (int rc = run_prim(x, d, 1, fused ? 2 : 1, lim ? 1 : 0)) return rc;
   // vertex averages, with the range reduction as one more block
   if (int rc = lim ? run_avg_range(x, d) : run_avg(x, d)) return rc;
@@ -769,7 +812,11 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     if ((rc = dupload(x, &p, bf_ab))) return rc;
     d.bf_ab = p;
     UP(bf_rec, bf_rec);
+    std::vector<int> bf_fref(std::max<size_t>(1, nbl), 0);
+    for (size_t j = 0; j < nbl; ++j) bf_fref[j] = x->ref_of_face[sl.bf_face[j]];
+    UP(bf_fref, bf_fref);
   }
+  UP(fref, x->ref_of_face);
   UP(r11, r11); UP(r12, r12); UP(r22, r22); UP(cv, cv); UP(cov, cov);
   UP(nc2c, nc2c); UP(cref, x->ref_of_sto);
   UP(v2c_ptr, v2c_ptr); UP(v2c, v2c); UP(wt, wt); UP(px, px); UP(py, py); UP(avg_w0, avg_w0);
@@ -787,7 +834,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   AL(scal, SC_NSCAL); AL(range, 8); AL(err, 4); AL(lower, 4 * (size_t)noc); AL(upper, 4 * (size_t)noc);
 #undef UP
 #undef AL
-  int init[4] = {0, 0, INT_MAX, 0};
+  int init[4] = {0, 0, INT_MAX, INT_MAX};
   HIPCHK(hipMemcpyAsync(d.err, init, sizeof(init), hipMemcpyHostToDevice, x->stream));
   unsigned long long rg[8] = {0, 0, 0, 0, ~0ULL, ~0ULL, ~0ULL, ~0ULL};
   HIPCHK(hipMemcpyAsync(d.range, rg, sizeof(rg), hipMemcpyHostToDevice, x->stream));
@@ -1059,6 +1106,7 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   }
   x->px_h = m.px;
   x->py_h = m.py;
+  x->fin_ref = m.fin;
   *out = x;
   return UFX_OK;
 }
@@ -1067,13 +1115,17 @@ int ufx_cc_set(ufx_cc* x, const char* name, const double* src) {
   if (int rc = check_ctx(x)) return rc;
   if (!name || !src) { set_error("ufx_cc_set: null argument"); return UFX_ERR_ARG; }
   std::string n(name);
-  return with_dev(x, [&](auto& d) { return set_field(x, d, n, src); });
+  int rc = with_dev(x, [&](auto& d) { return set_field(x, d, n, src); });
+  if (rc == UFX_OK && n == "qold") x->qold_stale = false;
+  return rc;
 }
 
 int ufx_cc_get(ufx_cc* x, const char* name, double* dst) {
   if (int rc = check_ctx(x)) return rc;
   if (!name || !dst) { set_error("ufx_cc_get: null argument"); return UFX_ERR_ARG; }
   std::string n(name);
+  if (n == "qold")
+    if (int rc = check_qold(x, "ufx_cc_get(\"qold\")")) return rc;
   return with_dev(x, [&](auto& d) { return get_field(x, d, n, dst); });
 }
 
@@ -1084,6 +1136,7 @@ int ufx_cc_prim(ufx_cc* x) {
 
 int ufx_cc_save_old(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
+  x->qold_stale = false;
   return with_dev(x, [&](auto& d) { return run_prim(x, d, 0, 1, 0); });
 }
 
@@ -1127,17 +1180,20 @@ int ufx_cc_fvresidual(ufx_cc* x) {
 int ufx_cc_rk_update(ufx_cc* x, int stage) {
   if (int rc = check_ctx(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
+  if (int rc = check_qold(x, "ufx_cc_rk_update")) return rc;
   return with_dev(x, [&](auto& d) { return run_rk_update(x, d, RK_A[stage - 1], RK_B[stage - 1]); });
 }
 
 int ufx_cc_rk_stage(ufx_cc* x, int stage) {
   if (int rc = check_ctx(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
+  if (int rc = check_qold(x, "ufx_cc_rk_stage")) return rc;
   return with_dev(x, [&](auto& d) { return run_stage(x, d, stage, false, true, false); });
 }
 
 int ufx_cc_lusgs(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_qold(x, "ufx_cc_lusgs")) return rc;
   return with_dev(x, [&](auto& d) { return run_lusgs(x, d); });
 }
 
@@ -1177,6 +1233,9 @@ int ufx_cc_check_positivity(ufx_cc* x, int* bad) {
   HIPCHK(hipMemcpyAsync(&e, err + 2, sizeof(int), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   if (bad) *bad = e == INT_MAX ? 0 : e;
+  // a report only: the cell noted here must not become an iteration's error
+  HIPCHK(hipMemcpyAsync(err + 2, &big, sizeof(int), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
   if (e != INT_MAX) {
     set_error("Density/pressure is negative at point " + std::to_string(e));
     return UFX_ERR_STATE;
@@ -1319,12 +1378,25 @@ int ufx_cc_iterate_graph(ufx_cc* x, int iter0, int niter, double cfl, double* hi
   (void)iter0;
   x->cfl = cfl;
   HIPCHK(hipGraphLaunch(x->chunk_exec, x->stream));
+  x->qold_stale = x->p.timemode == UFX_TIME_RK3 && x->p.flow == UFX_FLOW_INVISCID && !x->p.irs;
   if (hist) {
     HIPCHK(hipMemcpyAsync(hist, x->d_hist, (size_t)niter * 8, hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     return device_error(x);
   }
   return UFX_OK;
+}
+
+int ufx_cc_history(ufx_cc* x, int n, double* out) {
+  if (int rc = check_ctx(x)) return rc;
+  if (n < 0 || n > x->hist_cap || (n > 0 && !out)) {
+    set_error("ufx_cc_history: n must be 0.." + std::to_string(x->hist_cap));
+    return UFX_ERR_ARG;
+  }
+  if (n > 0)
+    HIPCHK(hipMemcpyAsync(out, x->d_hist, (size_t)n * 8, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return device_error(x);
 }
 
 int ufx_cc_keep_residual(ufx_cc* x, int on) {
@@ -1395,6 +1467,15 @@ int ufx_cc_sync(ufx_cc* x) {
   if (int rc = check_ctx(x)) return rc;
   HIPCHK(hipStreamSynchronize(x->stream));
   return device_error(x);
+}
+
+int ufx_cc_error(ufx_cc* x, int* kind, int* face, int* cell) {
+  if (int rc = check_ctx(x)) return rc;
+  if (!kind || !face || !cell) { set_error("ufx_cc_error: null output"); return UFX_ERR_ARG; }
+  int e[4] = {0, 0, 0, 0};
+  if (int rc = read_error(x, e)) return rc;
+  decode_error(x, e, kind, face, cell);
+  return UFX_OK;
 }
 
 void* ufx_cc_stream(ufx_cc* x) { return x ? (void*)x->stream : nullptr; }
@@ -1606,6 +1687,7 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
   for (size_t i = 0; i < b->cs.size(); ++i) {
     if (!b->on[i]) continue;
     b->cs[i]->cfl = cfl;
+    b->cs[i]->qold_stale = true;  // batched cases run the fused RK3 iteration
     if (err[3 * j] != 0) {
       int rc = device_error(b->cs[i]);
       set_error("case " + std::to_string(i) + ": " + ufx_last_error());

@@ -184,6 +184,17 @@ interface
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
    end function
+   integer(c_int) function ufx_cc_history(ctx, n, hist) bind(C, name="ufx_cc_history")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      real(c_double), intent(out) :: hist(*)
+   end function
+   integer(c_int) function ufx_cc_error(ctx, kind, face, cell) bind(C, name="ufx_cc_error")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: kind, face, cell
+   end function
 
    ! ---------------------------------------------- vertex-centred (NCFVM)
    integer(c_int) function ufx_nc_geometric(nop, noc, nof, px, py, pbc, ccx, ccy, cv, &

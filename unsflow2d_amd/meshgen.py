@@ -294,6 +294,7 @@ class Deck:
     irs: int = 0
     xref: float = 0.0
     yref: float = 0.0
+    scrint: int = 100000000
 
     def text(self) -> str:
         g = "%.17g"
@@ -310,7 +311,7 @@ class Deck:
             f"MaxIter   {self.maxiter}",
             f"MinRes    {g % self.minres}",
             "SaveInt   100000000",
-            "ScrInt    100000000",
+            f"ScrInt    {self.scrint}",
             "NoIso     30",
             f"Flux      {self.flux_type}",
             f"Limiter   {self.ilimit}",
