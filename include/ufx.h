@@ -61,6 +61,17 @@ extern "C" {
 #define UFX_PREC_FP64 0
 #define UFX_PREC_MIXED 1
 
+/* Vertex (cell) order of the LU-SGS sweeps.  REFERENCE: the reference's own
+ * numbering (NCFVM/lusgs.f90:46-122, CCFVM/lusgs.F90:245-310), as a level
+ * schedule; the iterate is the reference's.  MULTICOLOUR: the vertices
+ * (cells) grouped by the colours of a greedy distance-1 colouring
+ * (ufx_mesh_get "pt_colour" / "cell_colour"), each colour one parallel step;
+ * the same LU-SGS under another ordering, so another iterate with the same
+ * converged solution, and a few launches per sweep instead of thousands of
+ * dependent levels. */
+#define UFX_LUSGS_REFERENCE 0
+#define UFX_LUSGS_MULTICOLOUR 1
+
 typedef struct ufx_mesh ufx_mesh;
 typedef struct ufx_cc ufx_cc;
 
@@ -116,7 +127,8 @@ int ufx_cc_mesh_import(int nop, int noc, int nfi, int nbf,
  * Arrays read back with ufx_mesh_get: "pt_x","pt_y","pt_cv","pt_bc",
  * "pt_nv2v","pt_v2v","pt_wx","pt_wy","pt_nv2f","pt_v2f","oldnum","fc_p1",
  * "fc_p2","fc_in","fc_out","fc_bc","fc_nx","fc_ny","fc_area","fc_cenx",
- * "fc_ceny","cell_cenx","cell_ceny","cell_cv","counts" (nop,noc,nof). */
+ * "fc_ceny","cell_cenx","cell_ceny","cell_cv","counts" (nop,noc,nof),
+ * "pt_colour". */
 int ufx_nc_geometric(int nop, int noc, int nof, const double* px, const double* py,
                      const int* pbc, const double* ccx, const double* ccy, const double* cv,
                      const int* e_p1, const int* e_p2, const int* e_in, const int* e_out,
@@ -132,7 +144,10 @@ int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count);
  * imported mesh), "oldnum","counts" (nop,noc,nof,nbf); float64 arrays: "fc_nx","fc_ny",
  * "fc_area","fc_cenx","fc_ceny","fc_cov","cell_cenx","cell_ceny","cell_cv",
  * "cell_cov","cell_r11","cell_r12","cell_r22","pt_x","pt_y","pt_wt".
- * Indices are returned 1-based. */
+ * Indices are returned 1-based.  Also (int32, colour numbers from 0) the
+ * colouring of the multicolour LU-SGS sweeps: "cell_colour" (interior cells
+ * over the interior faces) of a cell-centred mesh, "pt_colour" (vertices over
+ * the edges) of a vertex-centred one. */
 int ufx_mesh_get(const ufx_mesh* mesh, const char* name, void* dst);
 
 void ufx_mesh_destroy(ufx_mesh* mesh);
@@ -155,6 +170,7 @@ typedef struct {
   int precision;       /* UFX_PREC_* */
   double cfl_max;      /* needed by residual smoothing (CCFVM/ImpResiSmoothening.F90:17) */
   double xref, yref;   /* moment reference point (FarBC line) */
+  int lusgs_order;     /* UFX_LUSGS_REFERENCE or UFX_LUSGS_MULTICOLOUR (timemode LU-SGS) */
 } ufx_cc_params;
 
 /* Create a context on the current HIP device: uploads the mesh in SoA layout
@@ -346,6 +362,7 @@ typedef struct {
   int limiter;       /* UFX_NC_LIM_VKN (limiter_VKN, NCFVM/limiter.f90:15-98) or
                         UFX_NC_LIM_BJ (limiter_min_max, :192-260, dead code in the
                         reference, with its +1/2 at :241) */
+  int lusgs_order;   /* UFX_LUSGS_REFERENCE or UFX_LUSGS_MULTICOLOUR */
 } ufx_nc_params;
 
 #define UFX_NC_LIM_VKN 0

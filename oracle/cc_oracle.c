@@ -864,6 +864,53 @@ void orc_cc_lusgs(orc_cc* s) {
   free(upper);
 }
 
+/* lusgs2 with the cells swept in the order `order` (order[k] = the k-th
+ * cell) instead of their numbering: the forward loop visits them in that
+ * order and takes as already swept the neighbours earlier in it, the reverse
+ * loop the other way round.  The identity order gives orc_cc_lusgs; cells
+ * grouped by colour give the multicolour sweep. */
+void orc_cc_lusgs_ordered(orc_cc* s, const int* order) {
+  gas_t g = gas_of(s);
+  const double omega = 1.5;
+  int noc = s->noc;
+  orc_cc_fvresidual(s);
+  double* diag = malloc(sizeof(double) * noc);
+  double* lower = calloc((size_t)4 * noc, sizeof(double));
+  double* upper = calloc((size_t)4 * noc, sizeof(double));
+  int* rank = malloc(sizeof(int) * noc);
+  for (int k = 0; k < noc; ++k) rank[order[k]] = k;
+  for (int c = 0; c < noc; ++c) diag[c] = s->ccv[c] / s->dt[c] + 0.5 * omega * s->la[c];
+  for (int pass = 0; pass < 2; ++pass) {
+    double* w = pass == 0 ? lower : upper;
+    for (int step = 0; step < noc; ++step) {
+      int it = order[pass == 0 ? step : noc - 1 - step];
+      double cres[4] = {0, 0, 0, 0};
+      for (int k = 0; k < 4; ++k) w[it * 4 + k] = 0.0;
+      for (int j = s->c2f_ptr[it]; j < s->c2f_ptr[it + 1]; ++j) {
+        int f = s->c2f[j];
+        double nx = s->fnx[f], ny = s->fny[f];
+        int in = s->fin[f], out = s->fout[f];
+        if (out == it) { out = in; nx = -s->fnx[f]; ny = -s->fny[f]; }
+        if (out >= 0 && out < noc && (pass == 0 ? rank[out] < rank[it] : rank[out] > rank[it])) {
+          double dflux[4];
+          normal_flux(&g, &s->qold[out * 4], &w[out * 4], nx, ny, s->farea[f], dflux);
+          double lam = s->fla[f];
+          for (int k = 0; k < 4; ++k) cres[k] = cres[k] + (dflux[k] - omega * lam * w[out * 4 + k]);
+        }
+      }
+      for (int k = 0; k < 4; ++k)
+        w[it * 4 + k] = pass == 0 ? (-s->res[it * 4 + k] - 0.5 * cres[k]) / diag[it]
+                                  : lower[it * 4 + k] - 0.5 * cres[k] / diag[it];
+    }
+  }
+  for (int c = 0; c < noc; ++c)
+    for (int k = 0; k < 4; ++k) s->qc[c * 4 + k] = s->qold[c * 4 + k] + upper[c * 4 + k];
+  free(rank);
+  free(diag);
+  free(lower);
+  free(upper);
+}
+
 /* residue, CCFVM/solution.F90:136-180 */
 void orc_cc_residue(orc_cc* s, int iter) {
   double f = 0.0;

@@ -62,6 +62,9 @@ void orc_cc_limit(orc_cc* s);
 void orc_cc_fvresidual(orc_cc* s);
 void orc_cc_rk_update(orc_cc* s, int stage);
 void orc_cc_lusgs(orc_cc* s);
+/* lusgs2 with the cells swept in a given order (order[k] = k-th cell);
+ * "already swept" neighbours are those earlier in it (see cc_oracle.c). */
+void orc_cc_lusgs_ordered(orc_cc* s, const int* order);
 void orc_cc_residue(orc_cc* s, int iter);
 int orc_cc_check_positivity(const orc_cc* s);
 /* find_cl_cd, CCFVM/main.F90:263-355: out[0..6] = cl, cd (surface integral),

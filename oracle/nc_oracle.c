@@ -418,6 +418,52 @@ void orc_nc_lusgs(orc_nc* s) {
   free(dqc);
 }
 
+/* lusgs, NCFVM/lusgs.f90:4-131, with the vertices swept in the order
+ * `order` (order[k] = the k-th vertex) instead of their numbering: the
+ * forward loop visits them in that order and takes as already swept the
+ * neighbours earlier in it, the reverse loop the other way round.  With the
+ * identity order this is orc_nc_lusgs; with the vertices grouped by colour of
+ * a colouring it is the multicolour sweep, whose vertices of one colour are
+ * independent of each other. */
+void orc_nc_lusgs_ordered(orc_nc* s, const int* order) {
+  const double omega = 1.5;
+  orc_nc_fvresidual(s);
+  int n = s->nop;
+  double* dd = (double*)malloc(sizeof(double) * (size_t)n);
+  double* dqc = (double*)calloc((size_t)n * 4, sizeof(double));
+  int* rank = (int*)malloc(sizeof(int) * (size_t)n);
+  for (int k = 0; k < n; ++k) rank[order[k]] = k;
+  for (int it = 0; it < n; ++it) dd[it] = s->pcv[it] / s->dt[it] + 0.5 * omega * s->la[it];
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int step = 0; step < n; ++step) {
+      int it = order[pass == 0 ? step : n - 1 - step];
+      double cres[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int j = s->v2f_ptr[it]; j < s->v2f_ptr[it + 1]; ++j) {
+        int c = s->v2f[j];
+        double nx = s->fnx[c], ny = s->fny[c], ds = s->farea[c];
+        int p1 = s->fp1[c], p2 = s->fp2[c];
+        double sign = 1.0;
+        if (p2 == it) { p2 = p1; sign = -1.0; }
+        if (pass == 0 ? rank[p2] < rank[it] : rank[p2] > rank[it]) {
+          double f1[4], f2[4];
+          normalflux(s->qold + p2 * 4, nx, ny, ds, f1);
+          normalflux(s->qc + p2 * 4, nx, ny, ds, f2);
+          for (int iv = 0; iv < 4; ++iv)
+            cres[iv] = cres[iv] + 0.5 * (sign * (f2[iv] - f1[iv]) - omega * s->fla[c] * dqc[p2 * 4 + iv]);
+        }
+      }
+      for (int iv = 0; iv < 4; ++iv) {
+        if (pass == 0) dqc[it * 4 + iv] = (-s->res[it * 4 + iv] - cres[iv]) / dd[it];
+        else dqc[it * 4 + iv] = dqc[it * 4 + iv] - cres[iv] / dd[it];
+        s->qc[it * 4 + iv] = s->qold[it * 4 + iv] + dqc[it * 4 + iv];
+      }
+    }
+  }
+  free(rank);
+  free(dd);
+  free(dqc);
+}
+
 /* residue, NCFVM/common.f90:108-154 */
 void orc_nc_residue(orc_nc* s, int iter) {
   s->fres = 0.0;

@@ -51,6 +51,9 @@ void orc_nc_limiter(orc_nc* s);
 void orc_nc_fvresidual(orc_nc* s);
 void orc_nc_rk_update(orc_nc* s, int irk);
 void orc_nc_lusgs(orc_nc* s);
+/* lusgs with the vertices swept in a given order (order[k] = k-th vertex);
+ * "already swept" neighbours are those earlier in it (see nc_oracle.c). */
+void orc_nc_lusgs_ordered(orc_nc* s, const int* order);
 void orc_nc_residue(orc_nc* s, int iter);
 double orc_nc_cfl(double cfl_max, int iter);
 /* One iteration of NCFVM/main.f90:51-88 (timemode 1: rk3, 2: lusgs). */

@@ -58,6 +58,7 @@ for _n in ("initialize", "prim", "avg_c2v", "time_step", "save_old", "gradient",
 _lib.orc_cc_rk_update.argtypes = [C.POINTER(_State), C.c_int]
 _lib.orc_cc_residue.argtypes = [C.POINTER(_State), C.c_int]
 _lib.orc_cc_check_positivity.argtypes = [C.POINTER(_State)]
+_lib.orc_cc_lusgs_ordered.argtypes = [C.POINTER(_State), _ip]
 _lib.orc_cc_find_cl_cd.argtypes = [C.POINTER(_State), C.c_double, C.c_double, _dp, _dp, _dp, _dp,
                                   _dp]
 _lib.orc_cc_iteration.argtypes = [C.POINTER(_State), C.c_int, C.c_double, C.c_int]
@@ -148,6 +149,25 @@ class CCOracle:
     def rk_update(self, k): self._call("rk_update", k)
     def lusgs(self): self._call("lusgs")
 
+    def lusgs_ordered(self, order):
+        """lusgs2 with the cells swept in ``order`` (0-based cell of each
+        position); earlier positions count as already swept."""
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        assert o.shape == (self.noc,)
+        self._call("lusgs_ordered", o.ctypes.data_as(_ip))
+
+    def iteration_ordered(self, it, cfl, order):
+        """An LU-SGS iteration of CCFVM/main.F90:59-115 whose sweeps follow
+        ``order``."""
+        self.s.cfl = cfl
+        self.prim()
+        self.avg_c2v()
+        self.time_step(cfl)
+        self.save_old()
+        self.gradient()
+        self.lusgs_ordered(order)
+        return self.residue(it)
+
     def residue(self, it):
         _lib.orc_cc_residue(C.byref(self.s), it)
         return self.s.fres
@@ -230,6 +250,7 @@ _lib.orc_nc_iteration.argtypes = [C.POINTER(_NCState), C.c_int, C.c_int]
 _lib.orc_nc_cfl.argtypes = [C.c_double, C.c_int]
 _lib.orc_nc_cfl.restype = C.c_double
 _lib.orc_nc_forces.argtypes = [C.POINTER(_NCState), _dp, _dp]
+_lib.orc_nc_lusgs_ordered.argtypes = [C.POINTER(_NCState), _ip]
 
 
 class NCOracle:
@@ -296,6 +317,22 @@ class NCOracle:
     def iteration(self, it, timemode="lusgs"):
         _lib.orc_nc_iteration(C.byref(self.s), it, 1 if timemode == "rk3" else 2)
         return self.s.fres
+
+    def lusgs_ordered(self, cfl, order):
+        """lusgs with the vertices swept in ``order`` (0-based vertex of each
+        position); earlier positions count as already swept."""
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        assert o.shape == (self.nop,)
+        self.s.cfl = cfl
+        _lib.orc_nc_lusgs_ordered(C.byref(self.s), o.ctypes.data_as(_ip))
+
+    def iteration_ordered(self, it, order):
+        """An LU-SGS iteration of NCFVM/main.f90:51-88 whose sweeps follow
+        ``order`` (see lusgs_ordered)."""
+        cfl = self.cfl(it)
+        self.save_old()
+        self.lusgs_ordered(cfl, order)
+        return self.residue(it)
 
     def forces(self):
         """(cl, cd, fx, fy) and -Cp per wall edge (build-defined wall force)."""

@@ -41,12 +41,14 @@ def device(meta, precision="fp64"):
 def inject(s, orc, it_done, cfl):
     """Hand the oracle's state after `it_done` iterations to the device: the
     conserved state (ghosts too), the primitives (ghost ones feed the vertex
-    averages), the limiter values (ratchet) and the viscosity of the last
-    viscous pass."""
+    averages), the limiter values (ratchet) and the cell and face viscosities
+    of the last viscous pass (the time step reads the face ones,
+    CCFVM/timestep.F90:147-171)."""
     s.set("qc", orc.qc)
     s.set("qp", orc.qp)
     s.set("phi", orc.phi)
     s.set("cell_mul", orc.cmul)
+    s.set("fc_mu", orc.fmu)
     s.iter, s.cfl = it_done, cfl
 
 

@@ -60,7 +60,7 @@ class Params(C.Structure):
                 ("t_wall", C.c_double), ("flow_type", C.c_int), ("iwall", C.c_int),
                 ("flux", C.c_int), ("grad", C.c_int), ("ilimit", C.c_int), ("irs", C.c_int),
                 ("timemode", C.c_int), ("precision", C.c_int), ("cfl_max", C.c_double),
-                ("xref", C.c_double), ("yref", C.c_double)]
+                ("xref", C.c_double), ("yref", C.c_double), ("lusgs_order", C.c_int)]
 
 
 def _sig(name, args, res=C.c_int):
@@ -199,7 +199,8 @@ class Mesh:
     def get(self, name: str) -> np.ndarray:
         n = C.c_int64()
         _check(lib.ufx_mesh_size(self._h, name.encode(), C.byref(n)))
-        dt = np.int32 if name in MESH_I32 or name in MESH_I32_WALK else np.float64
+        dt = (np.int32 if name in MESH_I32 or name in MESH_I32_WALK or name == "cell_colour"
+              else np.float64)
         out = np.empty(n.value, dtype=dt)
         _check(lib.ufx_mesh_get(self._h, name.encode(), out.ctypes.data_as(_vp)))
         return out
@@ -246,11 +247,13 @@ class Config:
     cfl_ramp_steps: int = 1
     xref: float = 0.0
     yref: float = 0.0
+    lusgs_order: str = "reference"
 
     def params(self) -> Params:
         return Params(self.mach, self.aoa_deg, self.reynolds, self.t_wall, FLOW[self.flow_type],
                       self.iwall, FLUX[self.flux], GRAD[self.grad], self.ilimit, self.irs,
-                      TIME[self.timemode], PREC[self.precision], self.cfl_max, self.xref, self.yref)
+                      TIME[self.timemode], PREC[self.precision], self.cfl_max, self.xref, self.yref,
+                      {"reference": 0, "multicolour": 1}[self.lusgs_order])
 
     def cfl(self, it: int, previous: float = float("nan")) -> float:
         """CFL of iteration ``it``, CCFVM/main.F90:148-168 (the ramp holds its

@@ -87,6 +87,7 @@ struct DevT {
   double* fw;                // a1 + a2 of the diamond path per face (GG_FC)
   // LU-SGS
   const int *lu_face, *lu_cnt;   // [MAXF][noc] reference c2f order
+  const int* ccol;               // [noc] colour of each cell (multicolour LU-SGS)
   // state
   R* rec;
   R* qpc;                    // [noc][4] the interior cells' record primitives, compact

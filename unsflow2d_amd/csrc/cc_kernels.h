@@ -1203,8 +1203,11 @@ __device__ __forceinline__ void normal_flux(const Gas<double>& g, const double* 
 // One level of the forward (dir 0) or backward (dir 1) sweep of lusgs2,
 // CCFVM/lusgs.F90:244-310: every cell of a level depends only on cells of
 // earlier levels, so a level runs in parallel and the sweep reproduces the
-// reference's sequential order exactly.
-template <typename R>
+// reference's sequential order exactly.  With MC the cells are ordered by
+// colour instead of by number (multicolour form): `cells` is one colour,
+// whose cells have no face in common, and the already-swept neighbours are
+// those of lower (forward) or higher (backward) colour.
+template <typename R, bool MC>
 __global__ __launch_bounds__(BLK) void k_lusgs_level(DevT<R> d, Gas<double> g, const int* cells,
                                                      int n, int dir) {
   int t = blockIdx.x * BLK + threadIdx.x;
@@ -1222,8 +1225,12 @@ __global__ __launch_bounds__(BLK) void k_lusgs_level(DevT<R> d, Gas<double> g, c
     double nx = fr[F_NX], ny = fr[F_NY];
     int in = d.fin[f], out = d.fout[f];
     if (out == it) { out = in; nx = -(double)fr[F_NX]; ny = -(double)fr[F_NY]; }
-    // lower/upper neighbour in the reference's cell numbering
-    bool use = out < noc && (dir == 0 ? d.cref[out] < d.cref[it] : d.cref[out] > d.cref[it]);
+    // lower/upper neighbour in the reference's cell numbering (or colours)
+    bool use;
+    if constexpr (MC)
+      use = out < noc && (dir == 0 ? d.ccol[out] < d.ccol[it] : d.ccol[out] > d.ccol[it]);
+    else
+      use = out < noc && (dir == 0 ? d.cref[out] < d.cref[it] : d.cref[out] > d.cref[it]);
     if (use) {
       double qo[4], dq[4], df[4];
       ld4d(d.qold + (size_t)out * 4, qo);

@@ -14,6 +14,49 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 using namespace ufx;
 
+namespace ufx {
+
+std::vector<int> greedy_colour(int n, const std::vector<int>& ptr, const std::vector<int>& adj,
+                               int* ncolours) {
+  std::vector<int> col(n, -1);
+  std::vector<int> seen;  // seen[c] == v: colour c is taken by a neighbour of v
+  int nc = 0;
+  for (int v = 0; v < n; ++v) {
+    for (int j = ptr[v]; j < ptr[v + 1]; ++j) {
+      int o = adj[j];
+      if (o >= 0 && o < n && col[o] >= 0) seen[col[o]] = v;
+    }
+    int c = 0;
+    while (c < nc && seen[c] == v) ++c;
+    if (c == nc) {
+      ++nc;
+      seen.push_back(-1);
+    }
+    col[v] = c;
+  }
+  if (ncolours) *ncolours = nc;
+  return col;
+}
+
+std::vector<int> lusgs_colours(const Mesh& m, int* ncolours) {
+  if (m.kind == UFX_KIND_NC) return greedy_colour(m.nop, m.v2v_ptr, m.v2v, ncolours);
+  std::vector<int> ptr(m.noc + 1, 0), adj;
+  for (int f = 0; f < m.nfi; ++f) {
+    ++ptr[m.fin[f] + 1];
+    ++ptr[m.fout[f] + 1];
+  }
+  for (int c = 0; c < m.noc; ++c) ptr[c + 1] += ptr[c];
+  adj.resize(ptr[m.noc]);
+  std::vector<int> at(ptr.begin(), ptr.end() - 1);
+  for (int f = 0; f < m.nfi; ++f) {
+    adj[at[m.fin[f]]++] = m.fout[f];
+    adj[at[m.fout[f]]++] = m.fin[f];
+  }
+  return greedy_colour(m.noc, ptr, adj, ncolours);
+}
+
+}  // namespace ufx
+
 extern "C" {
 
 const char* ufx_last_error(void) { return g_err.c_str(); }
@@ -147,6 +190,7 @@ int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count) {
   if (n == "pt_x" || n == "pt_y" || n == "pt_nv2c") v = m.nop;
   if (n == "pt_v2c" || n == "pt_wt") v = (int64_t)m.v2c.size();
   if (n == "counts") v = 4;
+  if (n == "cell_colour") v = m.noc;
   if (m.kind == UFX_KIND_NC) {
     v = -1;
     if (n.rfind("fc_", 0) == 0) v = m.nof();
@@ -156,6 +200,7 @@ int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count) {
     if (n == "pt_v2v" || n == "pt_wx" || n == "pt_wy") v = (int64_t)m.v2v.size();
     if (n == "pt_v2f") v = (int64_t)m.v2f.size();
     if (n == "cell_cenx" || n == "cell_ceny" || n == "cell_cv") v = m.noc;
+    if (n == "pt_colour") v = m.nop;
     if (n == "counts") v = 3;
   }
   if (v < 0) { set_error("ufx_mesh_size: unknown array " + n); return UFX_ERR_ARG; }
@@ -198,6 +243,7 @@ int ufx_mesh_get(const ufx_mesh* mesh, const char* name, void* dst) {
     else if (n == "cell_ceny") putd(m.ccy);
     else if (n == "cell_cv") putd(m.ccv);
     else if (n == "counts") { di[0] = m.nop; di[1] = m.noc; di[2] = m.nof(); }
+    else if (n == "pt_colour") put0(lusgs_colours(m, nullptr));
     else { set_error("ufx_mesh_get: unknown array " + n); return UFX_ERR_ARG; }
     return UFX_OK;
   }
@@ -231,6 +277,7 @@ int ufx_mesh_get(const ufx_mesh* mesh, const char* name, void* dst) {
   else if (n == "pt_wt") putd(m.wt);
   else if (n == "oldnum") put1(m.oldnum);
   else if (n == "counts") { di[0] = m.nop; di[1] = m.noc; di[2] = m.nof(); di[3] = m.nbf; }
+  else if (n == "cell_colour") put0(lusgs_colours(m, nullptr));
   else { set_error("ufx_mesh_get: unknown array " + n); return UFX_ERR_ARG; }
   return UFX_OK;
 }

@@ -35,7 +35,7 @@ struct Params {
   double gamma, gamma1, znd, r_inf, u_inf, v_inf, p_inf, mref, t_wall, m_inf, rey, vll;
   double aoa, xref, yref, cfl_max;
   double qinf[4];
-  int flow, iwall, flux, grad, ilimit, irs, timemode, precision;
+  int flow, iwall, flux, grad, ilimit, irs, timemode, precision, lusgs_order;
 };
 
 // Host copies of the shared-flux residual's block face lists.
@@ -84,6 +84,9 @@ struct ufx_cc {
   // LU-SGS level schedule: cells grouped by level, forward then backward
   int* lv_cells = nullptr;
   std::vector<int> lv_fwd_off, lv_bwd_off;
+  // multicolour LU-SGS: storage cells by colour, offsets of each colour
+  int* mc_cells = nullptr;
+  std::vector<int> mc_off;
   // kernel timing
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -478,15 +481,28 @@ int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool 
 template <typename R>
 int run_lusgs(ufx_cc* x, DevT<R>& d) {
   if (int rc = run_residual_full(x, d)) return rc;
+  if (x->p.lusgs_order == UFX_LUSGS_MULTICOLOUR) {
+    // one launch per colour: forward ascending, backward descending
+    const int nc = (int)x->mc_off.size() - 1;
+    for (int dir = 0; dir < 2; ++dir)
+      for (int k = 0; k < nc; ++k) {
+        int c = dir == 0 ? k : nc - 1 - k, a = x->mc_off[c], n = x->mc_off[c + 1] - a;
+        if (n > 0)
+          LAUNCH(x, UFX_K_OTHER, (k_lusgs_level<R, true>), nblocks(n), d, x->g, x->mc_cells + a,
+                 n, dir);
+      }
+    LAUNCH(x, UFX_K_OTHER, k_lusgs_final<R>, x->nblk_cell, d);
+    return launch_check();
+  }
   int nlv = (int)x->lv_fwd_off.size() - 1;
   for (int l = 0; l < nlv; ++l) {
     int a = x->lv_fwd_off[l], n = x->lv_fwd_off[l + 1] - a;
-    LAUNCH(x, UFX_K_OTHER, k_lusgs_level<R>, nblocks(n), d, x->g, x->lv_cells + a, n, 0);
+    LAUNCH(x, UFX_K_OTHER, (k_lusgs_level<R, false>), nblocks(n), d, x->g, x->lv_cells + a, n, 0);
   }
   nlv = (int)x->lv_bwd_off.size() - 1;
   for (int l = 0; l < nlv; ++l) {
     int a = x->lv_bwd_off[l], n = x->lv_bwd_off[l + 1] - a;
-    LAUNCH(x, UFX_K_OTHER, k_lusgs_level<R>, nblocks(n), d, x->g, x->lv_cells + a, n, 1);
+    LAUNCH(x, UFX_K_OTHER, (k_lusgs_level<R, false>), nblocks(n), d, x->g, x->lv_cells + a, n, 1);
   }
   LAUNCH(x, UFX_K_OTHER, k_lusgs_final<R>, x->nblk_cell, d);
   return launch_check();
@@ -873,6 +889,10 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   if (prm->timemode != UFX_TIME_RK3 && prm->timemode != UFX_TIME_LUSGS) { set_error("Unknown time integration scheme"); return UFX_ERR_ARG; }
   if (prm->precision != UFX_PREC_FP64 && prm->precision != UFX_PREC_MIXED) { set_error("unknown precision"); return UFX_ERR_ARG; }
   if (prm->irs && !(prm->cfl_max > 0.0)) { set_error("residual smoothing needs cfl_max > 0"); return UFX_ERR_ARG; }
+  if (prm->lusgs_order != UFX_LUSGS_REFERENCE && prm->lusgs_order != UFX_LUSGS_MULTICOLOUR) {
+    set_error("ufx_cc_create: unknown LU-SGS sweep order");
+    return UFX_ERR_ARG;
+  }
   const int noc = m.noc, nof = m.nof(), nop = m.nop;
   // faces of each cell in ascending reference order (the order of every
   // per-cell sum)
@@ -1052,6 +1072,7 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
   p.flow = prm->flow_type; p.iwall = prm->iwall; p.flux = prm->flux; p.grad = prm->grad;
   p.ilimit = prm->ilimit; p.irs = prm->irs; p.timemode = prm->timemode;
   p.precision = prm->precision; p.cfl_max = prm->cfl_max;
+  p.lusgs_order = prm->lusgs_order;
   p.vll = 0.0;
   if (p.flow != UFX_FLOW_INVISCID) p.vll = 2.0 * p.gamma / (0.72 * p.rey * 0.25);
   x->g = Gas<double>{p.gamma, p.gamma1, p.znd, p.r_inf, p.u_inf, p.v_inf, p.p_inf, p.mref,
@@ -1085,6 +1106,23 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
                ? build_dev(x, x->d64, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt, sl)
                : build_dev(x, x->d32, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt, sl);
   if (!rc) rc = dupload(x, &x->lv_cells, lv_cells);
+  if (!rc && p.timemode == UFX_TIME_LUSGS && p.lusgs_order == UFX_LUSGS_MULTICOLOUR) {
+    // colours of the interior cells (reference numbering) -> storage order;
+    // each colour's cells in ascending storage order (compact blocks)
+    int ncol = 0;
+    std::vector<int> col = lusgs_colours(m, &ncol), ccol(noc);
+    for (int c = 0; c < noc; ++c) ccol[sto[c]] = col[c];
+    x->mc_off.assign(ncol + 1, 0);
+    for (int s2 = 0; s2 < noc; ++s2) x->mc_off[ccol[s2] + 1]++;
+    for (int c = 0; c < ncol; ++c) x->mc_off[c + 1] += x->mc_off[c];
+    std::vector<int> cells(std::max(1, noc)), at(x->mc_off.begin(), x->mc_off.end() - 1);
+    for (int s2 = 0; s2 < noc; ++s2) cells[at[ccol[s2]]++] = s2;
+    rc = dupload(x, &x->mc_cells, cells);
+    const int* dc = nullptr;
+    if (!rc) rc = dupload(x, &dc, ccol);
+    x->d64.ccol = dc;
+    x->d32.ccol = dc;
+  }
   if (!rc) rc = dalloc(x, &x->cirs_tmp, 8 * (size_t)noc);
   if (!rc) rc = dalloc(x, &x->bfluxv, 2 * (size_t)std::max(1, m.nbf));
   if (!rc) {

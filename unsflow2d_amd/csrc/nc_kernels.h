@@ -61,6 +61,17 @@ struct DevNC {
   double* sw_dd;        // [2 nop] diagonal
   double* sw_dq;        // [2 nop][4] dq of each position
   int sw_q;             // entry slots per position
+  // multicolour sweeps: positions 0 .. nop - 1 are the vertices grouped by
+  // colour (ascending vertex number inside a colour); per sweep direction and
+  // position, mc_q entry slots (position-major) hold the neighbours of lower
+  // (forward) or higher (backward) colour in edge order with the edge's
+  // geometry
+  const int* mc_vert;     // [nop] vertex of each position
+  const int* mc_cnt;      // [2][nop] entries of each position per direction
+  const int2* mc_eo;      // [2][mc_q][nop] (edge, bit-inverted when the vertex is p2; other end)
+  const double2* mc_nxy;  // [2][mc_q][nop] edge nx, ny
+  const double* mc_ds;    // [2][mc_q][nop] edge length
+  int mc_q;
 };
 
 __device__ __forceinline__ void ld2d(const double* p, double& a, double& b) {
@@ -571,6 +582,63 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
   int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= n) return;
   nc_lusgs_vertex(d, g, verts[t], dir);
+}
+
+// ---- multicolour sweeps
+// The sweeps of NCFVM/lusgs.f90:46-122 with the vertices ordered by colour
+// instead of by number: no edge joins two vertices of one colour, so the
+// vertices of a colour depend only on vertices of other colours and one
+// launch updates them all.  Forward (dir 0, colours ascending): the
+// already-swept neighbours are those of lower colour, whose dq this sweep
+// has written; the diagonal is formed here and kept for the reverse sweep.
+// Reverse (dir 1, colours descending): the neighbours of higher colour.  A
+// neighbour's state is rebuilt as qold + dq, the sum that defines it, so the
+// arithmetic is the reference loop's, term for term, in that vertex order.
+__global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, int n, int dir) {
+  const int t = blockIdx.x * BLK + threadIdx.x;
+  if (t >= n) return;
+  const int np = d.nop, s = s0 + t, it = d.mc_vert[s];
+  const int cnt = d.mc_cnt[(size_t)dir * np + s];
+  const size_t base = (size_t)dir * d.mc_q * np + s;
+  double cres[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < cnt; ++k) {
+    const size_t q = base + (size_t)k * np;
+    const int2 eo = d.mc_eo[q];
+    const double2 nxy = d.mc_nxy[q];
+    const double ds = d.mc_ds[q];
+    const bool first = eo.x >= 0;
+    const int e = first ? eo.x : ~eo.x, o = eo.y;
+    const double sign = first ? 1.0 : -1.0;
+    double f1[4], f2[4], qo[4], qn[4], dq[4];
+    ld4d(d.qold + (size_t)o * 4, qo);
+    ld4d(d.dqc + (size_t)o * 4, dq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) qn[i] = qo[i] + dq[i];
+    nc_normalflux(g, qo, nxy.x, nxy.y, ds, f1);
+    nc_normalflux(g, qn, nxy.x, nxy.y, ds, f2);
+    const double lam = d.fla[e];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cres[i] = cres[i] + 0.5 * (sign * (f2[i] - f1[i]) - 1.5 * lam * dq[i]);
+  }
+  double res[4], dq[4], dd;
+  if (dir == 0) {
+    dd = d.cv[it] / d.dt[it] + 0.5 * 1.5 * d.la[it];
+    d.dd[it] = dd;
+    ld4d(d.res + (size_t)it * 4, res);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dq[i] = (-res[i] - cres[i]) / dd;
+  } else {
+    dd = d.dd[it];
+    ld4d(d.dqc + (size_t)it * 4, dq);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dq[i] = dq[i] - cres[i] / dd;
+    double qo[4], q[4];
+    ld4d(d.qold + (size_t)it * 4, qo);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = qo[i] + dq[i];
+    st4d(d.qc + (size_t)it * 4, q);
+  }
+  st4d(d.dqc + (size_t)it * 4, dq);
 }
 
 // ---- level-ordered one-workgroup sweeps

@@ -45,6 +45,7 @@ struct ufx_nc {
   double *d_fe = nullptr, *d_cp = nullptr;  // [nwall][2] force terms, [nwall] -Cp
   int sweep_mode = 4;                       // 0 graph of level launches, 1 level launches, 2 one workgroup, 3 cooperative grid, 4 level-ordered one workgroup
   int grid_wgs = 8;                         // workgroups of the cooperative sweep
+  std::vector<int> mc_off;                  // multicolour sweeps: positions of each colour
   hipGraph_t sweep_graph = nullptr;         // diagonal + both sweeps, captured once
   hipGraphExec_t sweep_exec = nullptr;
   double cfl = 0.0;
@@ -175,6 +176,16 @@ int build_sweep_graph(ufx_nc* x) {
 
 int run_lusgs(ufx_nc* x, double cfl) {
   if (int rc = run_fvresidual(x, cfl, 0, 0.0, 0.0)) return rc;
+  if (x->prm.lusgs_order == UFX_LUSGS_MULTICOLOUR) {
+    // one launch per colour: forward in ascending, reverse in descending order
+    const int nc = (int)x->mc_off.size() - 1;
+    for (int dir = 0; dir < 2; ++dir)
+      for (int k = 0; k < nc; ++k) {
+        const int c = dir == 0 ? k : nc - 1 - k, s0 = x->mc_off[c], n = x->mc_off[c + 1] - s0;
+        if (n > 0) NLAUNCH(x, NK_LUSGS, k_nc_lusgs_mc, nblk_of(n), x->d, x->g, s0, n, dir);
+      }
+    return launch_ok();
+  }
   if (x->sweep_mode == 0) {
     if (!x->sweep_exec)
       if (int rc = build_sweep_graph(x)) return rc;
@@ -295,6 +306,10 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   }
   if (prm->limiter != UFX_NC_LIM_VKN && prm->limiter != UFX_NC_LIM_BJ) {
     set_error("ufx_nc_create: unknown limiter");
+    return UFX_ERR_ARG;
+  }
+  if (prm->lusgs_order != UFX_LUSGS_REFERENCE && prm->lusgs_order != UFX_LUSGS_MULTICOLOUR) {
+    set_error("ufx_nc_create: unknown LU-SGS sweep order");
     return UFX_ERR_ARG;
   }
   if (prm->timemode == UFX_TIME_RK3 && !(prm->cfl_max < 1.0)) {
@@ -486,6 +501,51 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       if (!rc) rc = nc_alloc(x, &x->d.sw_own, 4 * np);
       if (!rc) rc = nc_alloc(x, &x->d.sw_dd, np);
       if (!rc) rc = nc_alloc(x, &x->d.sw_dq, 4 * np);
+    }
+    // multicolour sweeps: vertices grouped by colour, and per direction the
+    // neighbours of lower (forward) / higher (reverse) colour in edge order
+    if (!rc && prm->timemode == UFX_TIME_LUSGS && prm->lusgs_order == UFX_LUSGS_MULTICOLOUR) {
+      int ncol = 0;
+      std::vector<int> col = lusgs_colours(m, &ncol);
+      x->mc_off.assign(ncol + 1, 0);
+      for (int p = 0; p < nop; ++p) x->mc_off[col[p] + 1]++;
+      for (int c = 0; c < ncol; ++c) x->mc_off[c + 1] += x->mc_off[c];
+      std::vector<int> mv(nop), at(x->mc_off.begin(), x->mc_off.end() - 1);
+      for (int p = 0; p < nop; ++p) mv[at[col[p]]++] = p;
+      std::vector<int> cnt(2 * (size_t)nop, 0);
+      int q = 1;
+      for (int sp = 0; sp < nop; ++sp) {
+        int p = mv[sp];
+        for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
+          int e = m.v2f[j], o = m.fp1[e] == p ? m.fp2[e] : m.fp1[e];
+          if (col[o] != col[p]) ++cnt[(col[o] < col[p] ? 0 : (size_t)nop) + sp];
+        }
+        q = std::max(q, std::max(cnt[sp], cnt[(size_t)nop + sp]));
+      }
+      const size_t nq = (size_t)q * nop;
+      std::vector<int2> eo(2 * nq, make_int2(0, 0));
+      std::vector<double2> nxy(2 * nq, make_double2(0.0, 0.0));
+      std::vector<double> dsv(2 * nq, 0.0);
+      for (int sp = 0; sp < nop; ++sp) {
+        int p = mv[sp], k[2] = {0, 0};
+        for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
+          int e = m.v2f[j];
+          bool first = m.fp1[e] == p;
+          int o = first ? m.fp2[e] : m.fp1[e];
+          if (col[o] == col[p]) continue;  // not a proper colouring: never taken
+          int dir = col[o] < col[p] ? 0 : 1;
+          size_t at2 = (size_t)dir * nq + (size_t)k[dir]++ * nop + sp;
+          eo[at2] = make_int2(first ? e : ~e, o);
+          nxy[at2] = make_double2(m.fnx[e], m.fny[e]);
+          dsv[at2] = m.farea[e];
+        }
+      }
+      x->d.mc_q = q;
+      if (!rc) rc = nc_upload(x, &x->d.mc_vert, mv);
+      if (!rc) rc = nc_upload(x, &x->d.mc_cnt, cnt);
+      if (!rc) rc = nc_upload(x, &x->d.mc_eo, eo);
+      if (!rc) rc = nc_upload(x, &x->d.mc_nxy, nxy);
+      if (!rc) rc = nc_upload(x, &x->d.mc_ds, dsv);
     }
     if (rc) { ufx_nc_destroy(x); return rc; }
     // sweep form: one workgroup over the level-ordered data (default,

@@ -66,6 +66,18 @@ struct NCRawInput {
 
 int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err);
 
+// Greedy distance-1 colouring of a graph given as adjacency lists (ptr has
+// n + 1 offsets): vertices in index order, each taking the smallest colour
+// none of its already coloured neighbours has.  No edge joins two vertices of
+// one colour, so the vertices of a colour can be updated at once in a
+// Gauss-Seidel sweep ordered by colour.  Returns colours 0..*ncolours-1.
+std::vector<int> greedy_colour(int n, const std::vector<int>& ptr, const std::vector<int>& adj,
+                               int* ncolours);
+// The colouring the LU-SGS sweeps of the multicolour form use: vertex-centred
+// meshes colour their vertices over the edges; cell-centred meshes colour the
+// interior cells over the interior faces.
+std::vector<int> lusgs_colours(const Mesh& m, int* ncolours);
+
 void set_error(const std::string& msg);
 
 }  // namespace ufx

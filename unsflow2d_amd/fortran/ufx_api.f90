@@ -19,11 +19,12 @@ integer(c_int), parameter :: UFX_TIME_RK3 = 1, UFX_TIME_LUSGS = 2
 integer(c_int), parameter :: UFX_PREC_FP64 = 0, UFX_PREC_MIXED = 1
 
 integer(c_int), parameter :: UFX_NC_LIM_VKN = 0, UFX_NC_LIM_BJ = 1
+integer(c_int), parameter :: UFX_LUSGS_REFERENCE = 0, UFX_LUSGS_MULTICOLOUR = 1
 
 ! ufx_nc_params
 type, bind(C) :: ufx_nc_params
    real(c_double) :: mach, aoa_deg, cfl_max
-   integer(c_int) :: grad, timemode, precision, flux, limiter
+   integer(c_int) :: grad, timemode, precision, flux, limiter, lusgs_order
 end type ufx_nc_params
 
 ! ufx_cc_params
@@ -31,6 +32,7 @@ type, bind(C) :: ufx_cc_params
    real(c_double) :: mach, aoa_deg, reynolds, t_wall
    integer(c_int) :: flow_type, iwall, flux, grad, ilimit, irs, timemode, precision
    real(c_double) :: cfl_max, xref, yref
+   integer(c_int) :: lusgs_order
 end type ufx_cc_params
 
 interface

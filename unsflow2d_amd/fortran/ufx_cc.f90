@@ -119,6 +119,7 @@ subroutine read_input(deck)
    prm%cfl_max = cfl_max
    prm%xref = xref
    prm%yref = yref
+   prm%lusgs_order = UFX_LUSGS_REFERENCE
 end subroutine read_input
 
 ! Read geometry.inp (CCFVM/read_input.F90:138-173), preprocess it on the host

@@ -94,6 +94,7 @@ subroutine read_input(deck, grad, limiter)
    ! the reference hard-wires van Leer; "ausm" selects the build-defined option
    prm%flux = merge(UFX_FLUX_AUSM, UFX_FLUX_VANLEER, flux_type == 'ausm')
    prm%limiter = limiter
+   prm%lusgs_order = UFX_LUSGS_REFERENCE
 end subroutine read_input
 
 ! read_grid (NCFVM/read_input.f90:89-126), then geometric

@@ -28,7 +28,7 @@ _sig("ufx_nc_geometric", [C.c_int] * 3 + [_dp, _dp, _ip, _dp, _dp, _dp] + [_ip] 
 class NCParams(C.Structure):
     _fields_ = [("mach", C.c_double), ("aoa_deg", C.c_double), ("cfl_max", C.c_double),
                 ("grad", C.c_int), ("timemode", C.c_int), ("precision", C.c_int),
-                ("flux", C.c_int), ("limiter", C.c_int)]
+                ("flux", C.c_int), ("limiter", C.c_int), ("lusgs_order", C.c_int)]
 
 
 _sig("ufx_nc_create", [_vp, C.POINTER(NCParams), C.POINTER(_vp)])
@@ -82,7 +82,7 @@ class NCMesh:
     def get(self, name: str) -> np.ndarray:
         n = C.c_int64()
         _check(lib.ufx_mesh_size(self._h, name.encode(), C.byref(n)))
-        dt = np.int32 if name in NC_MESH_I32 else np.float64
+        dt = np.int32 if name in NC_MESH_I32 or name == "pt_colour" else np.float64
         out = np.empty(n.value, dtype=dt)
         _check(lib.ufx_mesh_get(self._h, name.encode(), out.ctypes.data_as(_vp)))
         return out
@@ -114,14 +114,15 @@ class NCSolver:
     SHAPES = {"qc": 4, "qold": 4, "qp": 4, "res": 4, "phi": 4, "grad": 8, "la": 1, "dt": 1}
 
     def __init__(self, mesh: NCMesh, *, mach=0.4, aoa_deg=0.0, cfl_max=50.0, grad="gg",
-                 timemode="lusgs", flux="vanleer", limiter="vkn"):
+                 timemode="lusgs", flux="vanleer", limiter="vkn", lusgs_order="reference"):
         self.mesh = mesh
         self.counts = mesh.counts
         self.timemode = timemode
         self.cfl_max = cfl_max
         prm = NCParams(mach, aoa_deg, cfl_max, {"lsqr": 1, "gg": 2}[grad],
                        {"rk3": 1, "lusgs": 2}[timemode], 0, {"ausm": 2, "vanleer": 3}[flux],
-                       {"vkn": 0, "bj": 1}[limiter])
+                       {"vkn": 0, "bj": 1}[limiter],
+                       {"reference": 0, "multicolour": 1}[lusgs_order])
         self._h = _vp()
         _check(lib.ufx_nc_create(mesh._h, C.byref(prm), C.byref(self._h)))
         self.iter = 0
