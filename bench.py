@@ -219,13 +219,31 @@ def nc_bytes(nop: int, nof: int, w: int = 8, d: int = 8, i: int = 4) -> float:
     return nof * (2 * i + 3 * w + w) + nop * (4 * w + 8 * w + 4 * w + 2 * w + w + 4 * d + 2 * d)
 
 
+def nc_pass_bytes(nop: int, nof: int, w: int = 8, d: int = 8, i: int = 4) -> dict:
+    """Algorithmic bytes per iteration of each pass of the vertex-centred
+    LU-SGS iteration (the per-pass model: every array a pass needs read once,
+    every output written once): save_old, gradient + limiter, residual, and
+    the two sweeps (per swept edge its ends, geometry and spectral radius; per
+    vertex the residual, the three states it reads, its time-step terms and
+    the written dq and state)."""
+    return {
+        "save_old": nop * (4 * d + 4 * w + 4 * d),
+        "gradient": nof * (2 * i + 3 * w) + nop * (4 * w + w + 8 * w)
+                    + nof * 2 * i + nop * (2 * w + 4 * w + 8 * w + w + 4 * w + 4 * w),
+        "residual": nc_bytes(nop, nof, w, d, i),
+        "lusgs": 2 * (nof * (2 * i + 3 * w + w) + nop * (4 * d * 3 + 4 * d + 3 * w + 4 * d * 2)),
+    }
+
+
 def main_nc(args) -> int:
     from unsflow2d_amd import nc, nc_mesh
     t0 = time.time()
     raw = nc_mesh.config3_mesh()
     mesh = nc.NCMesh.from_raw(raw)
     cnt = mesh.counts
-    solver = nc.NCSolver(mesh, mach=0.4, cfl_max=50.0, grad="gg", timemode="lusgs")
+    solver = nc.NCSolver(mesh, mach=0.4, cfl_max=50.0, grad="gg", timemode="lusgs",
+                         lusgs_order=args.lusgs_order)
+    ncol = int(mesh.get("pt_colour").max()) + 1
     print(f"# setup {time.time() - t0:.1f}s: {cnt['nop']} vertices, {cnt['nof']} edges",
           file=sys.stderr, flush=True)
     solver.iterate(args.warmup, hist=False)
@@ -234,14 +252,15 @@ def main_nc(args) -> int:
     solver.iterate(args.steps, hist=False)
     solver.sync()
     wall = time.perf_counter() - t1
-    # per-kernel times from one more iteration with HIP events around every
-    # launch (kept out of the timed region: the LU-SGS sweeps are thousands
-    # of small launches)
+    # per-kernel times from more iterations with HIP events around every
+    # launch (kept out of the timed region; in the reference order the
+    # LU-SGS sweeps are thousands of small launches)
+    nprof = 1 if args.lusgs_order == "reference" else min(args.steps, 10)
     solver.profile(True)
-    h = solver.iterate(1)
+    h = solver.iterate(nprof)
     prof = solver.profile_read()
     solver.profile(False)
-    prof = {k: (v[0] * args.steps, v[1] * args.steps) for k, v in prof.items()}
+    prof = {k: (v[0] * args.steps / nprof, v[1] * args.steps / nprof) for k, v in prof.items()}
     if not np.all(np.isfinite(h)):
         print("# residual is not finite", file=sys.stderr)
         return 1
@@ -251,11 +270,28 @@ def main_nc(args) -> int:
     print(f"# kernel avg ms: {kern}; LU-SGS total per iteration "
           f"{prof['lusgs'][0] / args.steps:.3f} ms; iteration {ms_step:.3f} ms",
           file=sys.stderr, flush=True)
-    r_ms, r_n = prof["residual"]
-    r_avg = r_ms / max(1, r_n)
-    rb = nc_bytes(nop, nof)
-    achieved = rb / (r_avg / 1000.0) / 1e9
+    # the roofline of every pass (algorithmic bytes per iteration / its time
+    # per iteration); the line's roofline is the pass that takes longest
+    pb = nc_pass_bytes(nop, nof)
+    passes = {}
+    for k, b in pb.items():
+        ms_it = prof[k][0] / args.steps
+        if ms_it > 0:
+            passes[k] = {"ms_per_iteration": round(ms_it, 4), "bytes_per_iteration": int(b),
+                         "achieved_gbs": round(b / (ms_it / 1000.0) / 1e9, 1),
+                         "frac": round(b / (ms_it / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                         "launches_per_iteration": round(prof[k][1] / args.steps, 1)}
+    dom = max(passes, key=lambda k: passes[k]["ms_per_iteration"])
+    kname = {"lusgs": "k_nc_lusgs_mc" if args.lusgs_order == "multicolour" else "k_nc_lusgs_stream",
+             "residual": "k_nc_residual", "gradient": "k_nc_grad_lim",
+             "save_old": "k_nc_save_old"}[dom]
+    d_ms = passes[dom]["ms_per_iteration"]
+    d_n = passes[dom]["launches_per_iteration"]
+    achieved = passes[dom]["achieved_gbs"]
     cpu = None if args.no_cpu else nc_cpu_baseline(args.cpu_budget)
+    sweep = ("multicolour: one launch per colour per sweep, %d colours" % ncol
+             if args.lusgs_order == "multicolour" else
+             "level-scheduled, reference order, one workgroup per sweep")
     line = {
         "metric": "M vertex-iterations/s @2M vertices, NCFVM LU-SGS",
         "value": round(nop * args.steps / wall / 1e6, 3),
@@ -264,12 +300,16 @@ def main_nc(args) -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"NCFVM median dual, seeded O-grid 2000x1000 points around the "
                                f"unit circle ({nop} vertices, {nof} edges), M=0.4, GG + "
-                               f"Venkatakrishnan, van Leer, LU-SGS (level-scheduled, reference "
-                               f"order), tanh CFL ramp to 50", "vertices": nop, "edges": nof},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                               f"Venkatakrishnan, van Leer, LU-SGS ({sweep}), tanh CFL ramp to 50",
+                   "vertices": nop, "edges": nof, "lusgs_order": args.lusgs_order,
+                   "colours": ncol},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_nc_residual", "bytes_per_launch": int(rb),
-                     "avg_launch_ms": round(r_avg, 5)},
+                     "kernel": kname, "bytes_per_launch": int(pb[dom] / max(1.0, d_n)),
+                     "avg_launch_ms": round(d_ms / max(1.0, d_n), 5),
+                     "basis": f"{dom} pass: algorithmic bytes per iteration over its "
+                              f"{d_n:g} launches per iteration"},
+        "passes": passes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -404,6 +444,51 @@ def main_viscous(args) -> int:
     return 0
 
 
+def time_iterations(solver, args) -> tuple[float, float]:
+    """Warm-up, then K iterations timed with the host clock around them
+    (synchronised on both sides); HIP events on the library's stream bracket
+    the residual-stage launches.  Returns (wall seconds, mean stage launch
+    ms)."""
+    solver.iterate(args.warmup, hist=True)
+    solver.sync()
+    solver.profile_only("stage")
+    solver.sync()
+    t1 = time.perf_counter()
+    solver.iterate(args.steps, hist=False)
+    solver.sync()
+    wall = time.perf_counter() - t1
+    st_ms, st_n = solver.profile_read()["stage"]
+    solver.profile(False)
+    return wall, st_ms / max(1, st_n)
+
+
+def precision_errors(mesh, niter: int) -> dict:
+    """The mixed form against the FP64 form after the same `niter`
+    iterations from the free stream: force coefficients (the pressure
+    integral's and the flux-based pair the reference prints, and Cm), wall
+    -Cp, the normalised residual history and the conserved field."""
+    from unsflow2d_amd import cc
+    out, sol = {}, {}
+    for prec in ("fp64", "mixed"):
+        s = cc.Solver(mesh, cc.Config(mach=0.75, flux="roe", grad="lsqr", ilimit=1,
+                                      precision=prec))
+        h = s.iterate(niter, hist=True)
+        sol[prec] = (h, s.forces(wall=True), s.get("qc"))
+        s.close()
+    h64, f64, q64 = sol["fp64"]
+    h32, f32, q32 = sol["mixed"]
+    for k in ("cl", "cd", "cl1", "cd1", "cm"):
+        out[k] = float(abs(f32[k] - f64[k]))
+    out["cp_max"] = float(np.max(np.abs(f32["mcp"] - f64["mcp"])))
+    out["history_max_rel"] = float(np.max(np.abs(h32 - h64) / np.abs(h64)))
+    scale = np.max(np.abs(q64), axis=0)
+    out["field_linf_rel"] = float(np.max(np.max(np.abs(q32 - q64), axis=0) / scale))
+    out["residual_drop_fp64"] = float(h64[-1])
+    out["cl_fp64"], out["cd_fp64"] = float(f64["cl1"]), float(f64["cd1"])
+    out["iterations"] = niter
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,6 +512,13 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=50, help="batch: iterations per graph replay")
     ap.add_argument("--per-case", action="store_true",
                     help="batch: launch every case's kernels separately (one graph per case)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="cc: skip the mixed-precision leg and its errors against FP64")
+    ap.add_argument("--error-iters", type=int, default=2000,
+                    help="cc: iterations from the free stream before comparing mixed with FP64")
+    ap.add_argument("--lusgs-order", choices=["reference", "multicolour"], default="multicolour",
+                    help="nc: LU-SGS sweep order (reference: level schedule of the reference's "
+                         "numbering; multicolour: one launch per colour)")
     args = ap.parse_args()
     if args.precision is None:
         args.precision = "mixed" if args.workload == "batch" else "fp64"
@@ -463,18 +555,10 @@ def main() -> int:
     print(f"# setup {time.time() - t0:.1f}s: {counts['noc']} cells, {counts['nof']} faces, "
           f"{counts['nop']} vertices", file=sys.stderr, flush=True)
 
-    solver.iterate(args.warmup, hist=True)
-    solver.sync()
     # timed region: K iterations; HIP events on the library's stream bracket
     # the residual-stage launches only (an event pair around every launch
     # would itself cost about a tenth of the iteration)
-    solver.profile_only("stage")
-    solver.sync()
-    t1 = time.perf_counter()
-    solver.iterate(args.steps, hist=False)
-    solver.sync()
-    wall = time.perf_counter() - t1
-    prof = solver.profile_read()
+    wall, st_avg = time_iterations(solver, args)
     # the per-kernel breakdown, from a separate pass of K iterations with
     # every launch bracketed
     solver.profile(True)
@@ -490,8 +574,6 @@ def main() -> int:
     ms_step = wall * 1000.0 / args.steps
     value = 3.0 * noc * args.steps / wall / 1e6
     w = 8 if args.precision == "fp64" else 4
-    st_ms, st_n = prof["stage"]
-    st_avg = st_ms / max(1, st_n)
     sb = stage_bytes(noc, nof, w)
     achieved = sb / (st_avg / 1000.0) / 1e9
     it_bytes = iteration_bytes(noc, nof, nop, nv2c, w)
@@ -499,6 +581,28 @@ def main() -> int:
     print(f"# kernel avg ms: {kern}; iteration {ms_step:.3f} ms, "
           f"{it_bytes / (ms_step / 1000.0) / 1e9:.0f} GB/s algorithmic over the whole iteration",
           file=sys.stderr, flush=True)
+
+    solver.close()
+    # the mixed form (FP32 records, gradients, limiter and fluxes; FP64 state,
+    # residual sums, time steps and norms) on the same mesh, timed the same
+    # way, and its errors against FP64 after the same iterations
+    mixed = {}
+    if args.precision == "fp64" and not args.no_mixed:
+        s32 = cc.Solver(mesh, cc.Config(mach=0.75, flux="roe", grad="lsqr", ilimit=1,
+                                        precision="mixed"))
+        wall32, st32 = time_iterations(s32, args)
+        s32.close()
+        v32 = 3.0 * noc * args.steps / wall32 / 1e6
+        sb32 = stage_bytes(noc, nof, 4)
+        t0 = time.time()
+        err = precision_errors(mesh, args.error_iters)
+        print(f"# mixed: {wall32 * 1000.0 / args.steps:.3f} ms/iteration, {v32:.0f} M cell-stages/s "
+              f"({v32 / value:.3f}x); errors vs FP64 after {args.error_iters} iterations "
+              f"({time.time() - t0:.1f}s): {err}", file=sys.stderr, flush=True)
+        mixed = {"mixed_value": round(v32, 2), "mixed_speedup": round(v32 / value, 4),
+                 "mixed_ms_per_step": round(wall32 * 1000.0 / args.steps, 4),
+                 "mixed_stage_roofline_frac": round(sb32 / (st32 / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                 "mixed_errors_vs_fp64": err}
 
     cpu = None
     if not args.no_cpu:
@@ -530,6 +634,8 @@ def main() -> int:
                      "kernel": "k_stage" if args.precision == "fp64" else "k_stage_blk",
                      "bytes_per_launch": int(sb),
                      "avg_launch_ms": round(st_avg, 5)},
+        "iteration_roofline_frac": round(it_bytes / (ms_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+        **mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
