@@ -301,6 +301,21 @@ int ufx_cc_batch_iterate(ufx_cc_batch* batch, int niter, double cfl);
 int ufx_cc_batch_residual_norms(ufx_cc_batch* batch, int var, double* out);
 int ufx_cc_batch_destroy(ufx_cc_batch* batch);
 
+/* ufx_cc_residual_norm without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving or batch-driver loops, harnesses
+ * that forbid hipMalloc).  ufx_cc_residual_norm allocates its result slot on
+ * first use; this variant calls no hipMalloc, hipFree or other allocating
+ * routine: its block partials and result live in a static device array of
+ * UFX_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), which
+ * limits it to contexts of at most (UFX_NOALLOC_SCRATCH_BYTES / 8 - 32) * 256
+ * interior cells (33.5 M); larger ones are refused with UFX_ERR_ARG.  Same
+ * result as ufx_cc_residual_norm, bit for bit; synchronising.  All calls on
+ * one GPU share the array: calls that overlap on one GPU (other threads,
+ * other streams) race and must be serialised by the caller. */
+#define UFX_NOALLOC_SCRATCH_BYTES (1 << 20)
+int ufx_cc_residual_norm_noalloc(ufx_cc* ctx, int var, double* out);
+
 /* Kernel timing (no reference counterpart; instrumentation for benchmarks).
  * When enabled, a pair of HIP events is recorded on the context's stream
  * around every kernel launch; ufx_cc_profile_read (synchronising) returns,

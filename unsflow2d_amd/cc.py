@@ -94,6 +94,7 @@ _sig("ufx_cc_iterate_graph", [_vp, C.c_int, C.c_int, C.c_double, _dp])
 _sig("ufx_cc_keep_residual", [_vp, C.c_int])
 _sig("ufx_cc_history", [_vp, C.c_int, _dp])
 _sig("ufx_cc_residual_norm", [_vp, C.c_int, _dp])
+_sig("ufx_cc_residual_norm_noalloc", [_vp, C.c_int, _dp])
 _sig("ufx_cc_stream", [_vp], C.c_void_p)
 _sig("ufx_cc_profile", [_vp, C.c_int])
 _sig("ufx_cc_profile_mask", [_vp, C.c_uint])
@@ -113,6 +114,7 @@ EXPORTED = [
     "ufx_cc_limit", "ufx_cc_fvresidual", "ufx_cc_rk_update", "ufx_cc_rk_stage", "ufx_cc_lusgs",
     "ufx_cc_residue", "ufx_cc_check_positivity", "ufx_cc_forces", "ufx_cc_iterate",
     "ufx_cc_iterate_graph", "ufx_cc_history", "ufx_cc_keep_residual", "ufx_cc_residual_norm",
+    "ufx_cc_residual_norm_noalloc",
     "ufx_cc_profile_mask",
     "ufx_cc_sync", "ufx_cc_error", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
     "ufx_cc_batch_create", "ufx_cc_batch_set_active", "ufx_cc_batch_iterate",

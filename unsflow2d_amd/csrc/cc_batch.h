@@ -124,7 +124,7 @@ __global__ __launch_bounds__(BLK) void km_sumsq_var(const CaseDesc<R>* cs, const
   int c;
   const VBlock vb = case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  sumsq_var_body<R>(e.d, vb, var);
+  sumsq_var_body<R>(e.d, vb, var, e.d.part);
 }
 
 template <typename R>
@@ -132,7 +132,7 @@ __global__ void km_norm_var_final(const CaseDesc<R>* cs, const int4* map, int va
   int c;
   case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  norm_var_final_body<R>(e.d, e.nblk_cell, var, out + c);
+  norm_var_final_body<R>(e.d, e.nblk_cell, var, e.d.part, out + c);
 }
 
 // The error words of the listed cases, three ints each, into out.

@@ -1095,7 +1095,8 @@ __global__ __launch_bounds__(BLK) void k_sumsq(DevT<R> d) {
 // Block partials of the squares of one stored residual variable (var 0..3)
 // or of all four (var -1).
 template <typename R>
-__device__ __forceinline__ void sumsq_var_body(const DevT<R>& d, VBlock vb, int var) {
+__device__ __forceinline__ void sumsq_var_body(const DevT<R>& d, VBlock vb, int var,
+                                               double* part) {
   __shared__ double sh[1][BLK];
   int c = vb.cell();
   double sq = 0.0;
@@ -1108,25 +1109,28 @@ __device__ __forceinline__ void sumsq_var_body(const DevT<R>& d, VBlock vb, int 
       sq = r[var] * r[var];
   }
   block_reduce<1>(sh, &sq, OpAdd());
-  if (threadIdx.x == 0) d.part[vb.b] = sq;
+  if (threadIdx.x == 0) part[vb.b] = sq;
 }
+// block partials of the per-variable sum of squares into `part` (nblk_cell
+// doubles: the context's reduction buffer, or a caller's scratch)
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_sumsq_var(DevT<R> d, int var) {
-  sumsq_var_body<R>(d, hw_block(), var);
+__global__ __launch_bounds__(BLK) void k_sumsq_var(DevT<R> d, int var, double* part) {
+  sumsq_var_body<R>(d, hw_block(), var, part);
 }
 
 template <typename R>
 __device__ __forceinline__ void norm_var_final_body(const DevT<R>& d, int nblk, int var,
-                                                    double* out) {
+                                                    const double* part, double* out) {
   __shared__ double sh[1][BLK];
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
+  for (int b = threadIdx.x; b < nblk; b += BLK) s = s + part[b];
   block_reduce<1>(sh, &s, OpAdd());
   if (threadIdx.x == 0) out[0] = var < 0 ? sqrt(s / d.noc / 4) : sqrt(s / d.noc);
 }
 template <typename R>
-__global__ void k_norm_var_final(DevT<R> d, int nblk, int var, double* out) {
-  norm_var_final_body<R>(d, nblk, var, out);
+__global__ void k_norm_var_final(DevT<R> d, int nblk, int var, const double* part,
+                                 double* out) {
+  norm_var_final_body<R>(d, nblk, var, part, out);
 }
 
 // L2 norm (CCFVM/solution.F90:159-166): raw norm, iteration-1 normalisation,

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -53,6 +54,7 @@ struct ufx_cc {
   DevT<double> d64{};
   DevT<float> d32{};
   int nop = 0, noc = 0, nfi = 0, nbf = 0, nof = 0, ncell = 0;
+  int device = 0;                // the HIP device the context lives on
   double cfl = 0.0;              // CFL of the current iteration (param's CFL)
   bool stage_shared = true;      // residual stage with block-shared fluxes
   bool stage_local = true;       // block records staged in LDS
@@ -1036,7 +1038,8 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     std::vector<int> pb(cb.begin(), cb.end() - 1);
     for (int c = noc - 1; c >= 0; --c) lv_cells[pb[lvb[c]]++] = sto[c];
   }
-  hipError_t e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&x->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
     delete x;
@@ -1454,12 +1457,78 @@ int ufx_cc_residual_norm(ufx_cc* x, int var, double* out) {
   if (!x->d_out) HIPCHK(hipMalloc((void**)&x->d_out, 8 * sizeof(double)));
   int rc = with_dev(x, [&](auto& d) {
     using R = typename std::remove_reference_t<decltype(d)>::Real;
-    LAUNCH(x, UFX_K_NORM, k_sumsq_var<R>, x->nblk_cell, d, var);
-    LAUNCH(x, UFX_K_NORM, k_norm_var_final<R>, 1, d, x->nblk_cell, var, x->d_out);
+    LAUNCH(x, UFX_K_NORM, k_sumsq_var<R>, x->nblk_cell, d, var, d.part);
+    LAUNCH(x, UFX_K_NORM, k_norm_var_final<R>, 1, d, x->nblk_cell, var, d.part, x->d_out);
     return launch_check();
   });
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(out, x->d_out, 8, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return device_error(x);
+}
+
+// ---- residual norm without allocation
+// Scratch of ufx_cc_residual_norm_noalloc: the result slot at [0], the
+// block partials from [32] on.  Every GPU the library runs on holds its own
+// copy until the process ends; in host code the array's name is only a
+// handle, whose device address (per device) comes from hipGetSymbolAddress.
+constexpr int NOALLOC_DOUBLES = UFX_NOALLOC_SCRATCH_BYTES / 8, NOALLOC_PART = 32;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on device dev, looked up once per device
+// with that device current.
+int scratch_of(int dev, double** out) {
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    set_error("ufx_cc_residual_norm_noalloc: device number out of range");
+    return UFX_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    HIPCHK(e);
+    scratch_ptr[dev] = (double*)p;
+  }
+  *out = scratch_ptr[dev];
+  return UFX_OK;
+}
+}  // namespace
+
+int ufx_cc_residual_norm_noalloc(ufx_cc* x, int var, double* out) {
+  if (int rc = check_ctx(x)) return rc;
+  if (var < -1 || var > 3 || !out) {
+    set_error("ufx_cc_residual_norm_noalloc: var must be -1..3 and out not NULL");
+    return UFX_ERR_ARG;
+  }
+  if (x->nblk_cell > NOALLOC_DOUBLES - NOALLOC_PART) {
+    set_error("ufx_cc_residual_norm_noalloc: " + std::to_string(x->noc) + " cells need " +
+              std::to_string(x->nblk_cell) + " partial sums, more than the " +
+              std::to_string(NOALLOC_DOUBLES - NOALLOC_PART) + " the scratch holds (" +
+              std::to_string(UFX_NOALLOC_SCRATCH_BYTES) + " bytes)");
+    return UFX_ERR_ARG;
+  }
+  double* ptr = nullptr;
+  if (int rc = scratch_of(x->device, &ptr)) return rc;
+  // every partial slot and the result are written before they are read: no
+  // clearing needed
+  int rc = with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    // plain launches: the timing instrumentation would create events
+    k_sumsq_var<R><<<x->nblk_cell, BLK, 0, x->stream>>>(d, var, ptr + NOALLOC_PART);
+    k_norm_var_final<R><<<1, BLK, 0, x->stream>>>(d, x->nblk_cell, var, ptr + NOALLOC_PART, ptr);
+    return launch_check();
+  });
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(out, ptr, 8, hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   return device_error(x);
 }
