@@ -65,10 +65,12 @@ extern "C" {
  * numbering (NCFVM/lusgs.f90:46-122, CCFVM/lusgs.F90:245-310), as a level
  * schedule; the iterate is the reference's.  MULTICOLOUR: the vertices
  * (cells) grouped by the colours of a greedy distance-1 colouring
- * (ufx_mesh_get "pt_colour" / "cell_colour"), each colour one parallel step;
- * the same LU-SGS under another ordering, so another iterate with the same
- * converged solution, and a few launches per sweep instead of thousands of
- * dependent levels. */
+ * (ufx_mesh_get "pt_colour" / "cell_colour"), each colour one parallel step,
+ * a few launches per sweep instead of thousands of dependent levels.  The
+ * same LU-SGS under another ordering: another iterate with the same fixed
+ * points (a converged reference-order state stays put), but the limiters'
+ * never-reset ratchet makes the state a run converges to depend on its path,
+ * as any change of CFL schedule does. */
 #define UFX_LUSGS_REFERENCE 0
 #define UFX_LUSGS_MULTICOLOUR 1
 
@@ -255,8 +257,22 @@ int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double*
  * dominates).  Iterations iter0+1 .. iter0+niter with one constant CFL, from
  * iteration 2 on (iter0 >= 1: iteration 1 fixes the residual normalisation;
  * run it with ufx_cc_iterate).  The chunk is captured on the first call and
- * replayed while niter and cfl stay the same.  hist as ufx_cc_iterate. */
+ * replayed while niter and cfl stay the same; a call with fewer iterations
+ * than the captured chunk (the remainder of a run) launches them directly,
+ * one with more captures anew.  hist as ufx_cc_iterate. */
 int ufx_cc_iterate_graph(ufx_cc* ctx, int iter0, int niter, double cfl, double* hist);
+
+/* Form of the residual-stage kernel (no reference counterpart; a tuning
+ * knob for the autotuner, unsflow2d_amd.batch.autotune): shared = 1 evaluates
+ * every interior face flux once per block of 256 cells into LDS (an edge
+ * batch per block) and sums them per cell; shared = 0 lets every cell
+ * evaluate its own faces (each interior flux twice).  lds = 1 stages the
+ * block's cell records in LDS.  All forms give the same result to rounding.
+ * Returns UFX_ERR_UNSUPPORTED when the form's LDS exceeds 64 KB per block on
+ * this mesh.  The default at ufx_cc_create is the measured best for the
+ * precision (UFX_STAGE_KERNEL / UFX_STAGE_LDS override it). */
+int ufx_cc_set_stage_form(ufx_cc* ctx, int shared, int lds);
+int ufx_cc_get_stage_form(ufx_cc* ctx, int* shared, int* lds);
 
 /* The normalised residuals of the last whole-iteration call on the context
  * (ufx_cc_iterate, ufx_cc_iterate_graph, or the context's share of
@@ -285,7 +301,10 @@ int ufx_cc_residual_norm(ufx_cc* ctx, int var, double* out);
  * options), each already past iteration 1.  ufx_cc_batch_iterate runs niter
  * whole iterations (constant CFL) of every active case with one launch per
  * kernel for all of them, replayed from a HIP graph captured on the first
- * call; per cell it computes exactly what ufx_cc_iterate_graph computes for
+ * call (a call with fewer iterations than the captured chunk, such as the
+ * remainder of a run, launches them directly; one with more, or after a
+ * retirement or a change of stage form, captures anew); per cell it computes
+ * exactly what ufx_cc_iterate_graph computes for
  * each case alone.  It synchronises, and reports the first case whose
  * device error word is set ("case i: ...").  A case's own history buffer
  * receives its norms as with ufx_cc_iterate_graph.  Retire cases with

@@ -421,3 +421,31 @@ def test_batch_rejects_mixed_options():
     for m, s in x:
         s.close()
         m.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_follows_stage_form_changes(precision):
+    """The autotuner's knob: the residual-stage form switched between chunks
+    on every case of a batch (each switch re-captures the batch's graph; a
+    shorter chunk than the captured one runs its launches directly) keeps
+    the batch equal, bit for bit, to each case's own graph-replayed
+    iterations in the same forms."""
+    from unsflow2d_amd import cc
+    print(f"case: batch stage forms {precision}", flush=True)
+    a, b = _batch_cases(precision), _batch_cases(precision)
+    fb = cc.Batch([s for _, s in b])
+    for form, n in (("gather", 6), ("shared", 6), ("gather_lds", 6), ("gather_lds", 4),
+                    ("shared_lds", 6), ("gather", 6)):
+        for (_, sa), (_, sb) in zip(a, b):
+            sa.set_stage_form(form)
+            sb.set_stage_form(form)
+            assert sb.stage_form == form
+        for _, sa in a:
+            sa.iterate_graph(n, hist=False)
+        fb.iterate(n)
+        for i, ((_, sa), (_, sb)) in enumerate(zip(a, b)):
+            assert np.array_equal(sa.get("qc"), sb.get("qc")), f"case {i} form {form}"
+    fb.close()
+    for m, s in a + b:
+        s.close()
+        m.close()

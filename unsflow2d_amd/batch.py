@@ -63,6 +63,43 @@ def cases(n: int = 256, seed0: int = 5000, cmin: float = 5e3, cmax: float = 2e5)
     return out
 
 
+FORMS = ("gather_lds", "gather", "shared", "shared_lds")
+
+
+def autotune(fb: cc.Batch, solvers: list, iters: int = 10, reps: int = 2,
+             forms=FORMS, log=None) -> dict:
+    """Pick the residual-stage form (per-cell gather, or per-block edge
+    batches of fluxes evaluated once in LDS; each with or without the
+    block's records staged in LDS) that runs this batch fastest on this GPU.
+
+    Each candidate is set on every case, one chunk of `iters` iterations is
+    captured and replayed `reps` times, and the replays after the first are
+    timed; the fastest form is left set.  The tuning iterations are real
+    iterations of every case (they count towards MaxIter), so tuning costs
+    only the difference between the forms.  Returns the choice and the
+    measured milliseconds per iteration of every candidate (None where the
+    form does not fit in LDS on some case's mesh)."""
+    times = {}
+    for form in forms:
+        try:
+            for s in solvers:
+                s.set_stage_form(form)
+        except cc.UfxError:
+            times[form] = None
+            continue
+        fb.iterate(iters)  # capture (and run) the chunk
+        t0 = time.perf_counter()
+        for _ in range(reps - 1):
+            fb.iterate(iters)
+        times[form] = (time.perf_counter() - t0) * 1e3 / (iters * max(1, reps - 1))
+        if log:
+            log(f"autotune: {form} {times[form]:.3f} ms/iteration")
+    best = min((f for f in forms if times[f] is not None), key=lambda f: times[f])
+    for s in solvers:
+        s.set_stage_form(best)
+    return {"form": best, "ms_per_iteration": times, "iterations": iters * reps * len(forms)}
+
+
 def density_norm(solver: cc.Solver) -> float:
     """sqrt(sum res_rho^2 / noc) of the last stage's residual, reduced on
     the device."""
@@ -70,9 +107,12 @@ def density_norm(solver: cc.Solver) -> float:
 
 
 def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
-        minres: float = 1e-8, chunk: int = 50, fused: bool = True, log=None) -> dict:
+        minres: float = 1e-8, chunk: int = 50, fused: bool = True, tune: bool = False,
+        log=None) -> dict:
     """Solve every case; returns per-case iterations and final density drop
-    and the aggregate timing (setup excluded)."""
+    and the aggregate timing (setup excluded).  With ``tune`` the residual
+    stage's form is picked by :func:`autotune` on the live batch first (its
+    iterations are part of the solve and of the timing)."""
     t_setup = time.time()
     live = []
     for c in case_list:
@@ -97,6 +137,9 @@ def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
     for i, e in enumerate(live):
         e["slot"] = i
     fb = cc.Batch([e["solver"] for e in live]) if fused else None
+    tuned = None
+    if fb is not None and tune:
+        tuned = autotune(fb, [e["solver"] for e in live], log=log)
     rounds = 0
     while live:
         if fb is not None:
@@ -133,4 +176,4 @@ def run(case_list: list[Case], *, precision: str = "mixed", maxiter: int = 2000,
     iters = sum(d["iters"] * d["cells"] for d in done)
     return dict(wall=wall, setup=t_setup, solves=len(done), solves_per_s=len(done) / wall,
                 cell_iterations=iters, cell_stages_per_s=3.0 * iters / wall,
-                converged=sum(1 for d in done if d["drop"] <= minres), cases=done)
+                converged=sum(1 for d in done if d["drop"] <= minres), cases=done, tuned=tuned)

@@ -93,6 +93,8 @@ _sig("ufx_cc_iterate", [_vp, C.c_int, C.c_int, _dp, _dp])
 _sig("ufx_cc_iterate_graph", [_vp, C.c_int, C.c_int, C.c_double, _dp])
 _sig("ufx_cc_keep_residual", [_vp, C.c_int])
 _sig("ufx_cc_history", [_vp, C.c_int, _dp])
+_sig("ufx_cc_set_stage_form", [_vp, C.c_int, C.c_int])
+_sig("ufx_cc_get_stage_form", [_vp, _ip, _ip])
 _sig("ufx_cc_residual_norm", [_vp, C.c_int, _dp])
 _sig("ufx_cc_residual_norm_noalloc", [_vp, C.c_int, _dp])
 _sig("ufx_cc_stream", [_vp], C.c_void_p)
@@ -114,7 +116,7 @@ EXPORTED = [
     "ufx_cc_limit", "ufx_cc_fvresidual", "ufx_cc_rk_update", "ufx_cc_rk_stage", "ufx_cc_lusgs",
     "ufx_cc_residue", "ufx_cc_check_positivity", "ufx_cc_forces", "ufx_cc_iterate",
     "ufx_cc_iterate_graph", "ufx_cc_history", "ufx_cc_keep_residual", "ufx_cc_residual_norm",
-    "ufx_cc_residual_norm_noalloc",
+    "ufx_cc_residual_norm_noalloc", "ufx_cc_set_stage_form", "ufx_cc_get_stage_form",
     "ufx_cc_profile_mask",
     "ufx_cc_sync", "ufx_cc_error", "ufx_cc_stream", "ufx_cc_profile", "ufx_cc_profile_read",
     "ufx_cc_batch_create", "ufx_cc_batch_set_active", "ufx_cc_batch_iterate",
@@ -396,6 +398,22 @@ class Solver:
         h = np.empty(n)
         _check(lib.ufx_cc_history(self._h, n, _d(h)))
         return h
+
+    STAGE_FORMS = {"gather": (0, 0), "gather_lds": (0, 1), "shared": (1, 0),
+                   "shared_lds": (1, 1)}
+
+    def set_stage_form(self, form: str) -> None:
+        """Residual-stage kernel form: "gather" (each cell its own faces),
+        "shared" (each interior flux once per block, an edge batch in LDS),
+        with "_lds" the block's records staged in LDS."""
+        sh, lds = self.STAGE_FORMS[form]
+        _check(lib.ufx_cc_set_stage_form(self._h, sh, lds))
+
+    @property
+    def stage_form(self) -> str:
+        sh, lds = C.c_int(), C.c_int()
+        _check(lib.ufx_cc_get_stage_form(self._h, C.byref(sh), C.byref(lds)))
+        return {v: k for k, v in self.STAGE_FORMS.items()}[(sh.value, lds.value)]
 
     def keep_residual(self, on: bool = True) -> None:
         """Whole iterations also store the last stage's residual."""

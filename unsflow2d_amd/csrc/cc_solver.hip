@@ -39,6 +39,30 @@ struct Params {
   int flow, iwall, flux, grad, ilimit, irs, timemode, precision, lusgs_order;
 };
 
+// A captured chunk of whole iterations: the graph is kept with its
+// executable and both are released together (executable first), and a new
+// chunk is instantiated before the one it replaces is released.
+struct Chunk {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int n = 0;            // iterations per replay
+  double cfl = 0.0;
+  int keep = -1;        // residual kept (batched chunks)
+  bool stale = true;    // its kernel arguments no longer describe the cases
+};
+
+// Release a chunk; the first failing call's error is returned.
+hipError_t release_chunk(Chunk& c) {
+  hipError_t e = hipSuccess;
+  if (c.exec) e = hipGraphExecDestroy(c.exec);
+  if (c.graph) {
+    hipError_t e2 = hipGraphDestroy(c.graph);
+    if (e == hipSuccess) e = e2;
+  }
+  c = Chunk();
+  return e;
+}
+
 // Host copies of the shared-flux residual's block face lists.
 struct Slots {
   std::vector<int> bf_ptr, bf_face, sl_loc;
@@ -59,12 +83,11 @@ struct ufx_cc {
   bool stage_shared = true;      // residual stage with block-shared fluxes
   bool stage_local = true;       // block records staged in LDS
   size_t stage_lds = 0;          // LDS bytes per block of the stage kernel
+  size_t lds_list = 0, lds_recs = 0;  // LDS of the block flux list / records
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   // a captured chunk of whole iterations (ufx_cc_iterate_graph)
-  hipGraphExec_t chunk_exec = nullptr;
-  int chunk_n = 0;
-  double chunk_cfl = 0.0;
+  Chunk chunk;
   bool keep_res = false;           // whole iterations also store the last residual
   bool qold_stale = false;         // fused RK3 iterations ran since qold was last written
   unsigned prof_mask = ~0u;        // kernel categories timed while profiling
@@ -146,6 +169,10 @@ void prof_end(ufx_cc* x, int r) {
 namespace {
 
 int nblocks(int n) { return std::max(1, (n + BLK - 1) / BLK); }
+
+// LDS a residual-stage block may take (the kernels are launched with it as
+// dynamic shared memory)
+constexpr size_t STAGE_LDS_MAX = 64 * 1024;
 
 // Position along a Hilbert curve of order 16 of the point (x, y) of the
 // 65536 x 65536 grid.
@@ -512,10 +539,9 @@ int run_lusgs(ufx_cc* x, DevT<R>& d) {
 
 int ensure_hist(ufx_cc* x, int n) {
   if (n <= x->hist_cap) return UFX_OK;
-  // a captured chunk writes the old buffer: it goes with it
+  // a captured chunk writes the old buffer: it is never replayed again
   if (x->stream) HIPCHK(hipStreamSynchronize(x->stream));
-  if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
-  x->chunk_exec = nullptr;
+  x->chunk.stale = true;
   if (x->d_hist) HIPCHK(hipFree(x->d_hist));
   x->d_hist = nullptr;
   HIPCHK(hipMalloc((void**)&x->d_hist, (size_t)n * sizeof(double)));
@@ -869,7 +895,7 @@ int ufx_cc_destroy(ufx_cc* x) {
   if (x->stream) (void)hipStreamSynchronize(x->stream);
   for (void* p : x->allocs) (void)hipFree(p);
   if (x->d_hist) (void)hipFree(x->d_hist);
-  if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
+  (void)release_chunk(x->chunk);
   if (x->d_out) (void)hipFree(x->d_out);
   for (hipEvent_t e : x->ev_pool) (void)hipEventDestroy(e);
   if (x->stream) (void)hipStreamDestroy(x->stream);
@@ -1100,9 +1126,11 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     if (envl && std::strcmp(envl, "0") == 0) local = false;
     if (envl && std::strcmp(envl, "1") == 0) local = true;
     size_t lds_fl = (size_t)sl.bf_max * 4 * w, lds_rec = (size_t)BLK * rw * w;
-    if (shared && lds_fl + (local ? lds_rec : 0) > 64 * 1024) shared = false;
+    if (shared && lds_fl + (local ? lds_rec : 0) > STAGE_LDS_MAX) shared = false;
     x->stage_shared = shared;
     x->stage_local = local;
+    x->lds_list = lds_fl;
+    x->lds_recs = lds_rec;
     x->stage_lds = (shared ? lds_fl : 0) + (local ? lds_rec : 0);
   }
   int rc = p.precision == UFX_PREC_FP64
@@ -1393,32 +1421,39 @@ int ufx_cc_iterate_graph(ufx_cc* x, int iter0, int niter, double cfl, double* hi
   if (x->prof) { set_error("ufx_cc_iterate_graph: not while profiling"); return UFX_ERR_ARG; }
   if (int rc = ensure_hist(x, niter)) return rc;
   // the launches of a chunk depend only on the CFL and the chunk length:
-  // capture once, replay while both stay the same
-  if (!x->chunk_exec || x->chunk_n != niter || x->chunk_cfl != cfl) {
-    if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
-    x->chunk_exec = nullptr;
-    HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int k = 0; k < niter && !rc; ++k) {
-      x->cfl = cfl;
-      rc = with_dev(x, [&](auto& d) { return run_iteration(x, d, 2, k); });
+  // capture once, replay while both stay the same; a shorter chunk than the
+  // captured one (the remainder of a run) is launched directly instead of
+  // being captured anew
+  x->cfl = cfl;
+  const bool fresh = !x->chunk.exec || x->chunk.stale || x->chunk.cfl != cfl || niter > x->chunk.n;
+  if (!fresh && niter < x->chunk.n) {
+    for (int k = 0; k < niter; ++k) {
+      int rc = with_dev(x, [&](auto& d) { return run_iteration(x, d, 2, k); });
+      if (rc) return rc;
     }
-    hipGraph_t gr = nullptr;
-    hipError_t e = hipStreamEndCapture(x->stream, &gr);
-    if (rc) {
-      if (gr) (void)hipGraphDestroy(gr);
-      return rc;
+  } else {
+    if (fresh) {
+      HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int k = 0; k < niter && !rc; ++k)
+        rc = with_dev(x, [&](auto& d) { return run_iteration(x, d, 2, k); });
+      Chunk c;
+      hipError_t e = hipStreamEndCapture(x->stream, &c.graph);
+      if (!rc && e == hipSuccess) e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
+      if (rc || e != hipSuccess) {
+        (void)release_chunk(c);
+        if (rc) return rc;
+        HIPCHK(e);
+      }
+      c.n = niter;
+      c.cfl = cfl;
+      c.stale = false;
+      HIPCHK(release_chunk(x->chunk));
+      x->chunk = c;
     }
-    HIPCHK(e);
-    hipError_t ie = hipGraphInstantiate(&x->chunk_exec, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    HIPCHK(ie);
-    x->chunk_n = niter;
-    x->chunk_cfl = cfl;
+    HIPCHK(hipGraphLaunch(x->chunk.exec, x->stream));
   }
   (void)iter0;
-  x->cfl = cfl;
-  HIPCHK(hipGraphLaunch(x->chunk_exec, x->stream));
   x->qold_stale = x->p.timemode == UFX_TIME_RK3 && x->p.flow == UFX_FLOW_INVISCID && !x->p.irs;
   if (hist) {
     HIPCHK(hipMemcpyAsync(hist, x->d_hist, (size_t)niter * 8, hipMemcpyDeviceToHost, x->stream));
@@ -1440,13 +1475,33 @@ int ufx_cc_history(ufx_cc* x, int n, double* out) {
   return device_error(x);
 }
 
+int ufx_cc_set_stage_form(ufx_cc* x, int shared, int lds) {
+  if (int rc = check_ctx(x)) return rc;
+  size_t need = (shared ? x->lds_list : 0) + (lds ? x->lds_recs : 0);
+  if (need > STAGE_LDS_MAX) {
+    set_error("ufx_cc_set_stage_form: the block-shared form needs " + std::to_string(need) +
+              " bytes of LDS per block on this mesh, more than " +
+              std::to_string(STAGE_LDS_MAX));
+    return UFX_ERR_UNSUPPORTED;
+  }
+  if ((x->stage_shared != (shared != 0)) || (x->stage_local != (lds != 0))) x->chunk.stale = true;
+  x->stage_shared = shared != 0;
+  x->stage_local = lds != 0;
+  x->stage_lds = need;
+  return UFX_OK;
+}
+
+int ufx_cc_get_stage_form(ufx_cc* x, int* shared, int* lds) {
+  if (int rc = check_ctx(x)) return rc;
+  if (!shared || !lds) { set_error("ufx_cc_get_stage_form: null output"); return UFX_ERR_ARG; }
+  *shared = x->stage_shared ? 1 : 0;
+  *lds = x->stage_local ? 1 : 0;
+  return UFX_OK;
+}
+
 int ufx_cc_keep_residual(ufx_cc* x, int on) {
   if (int rc = check_ctx(x)) return rc;
-  if (x->keep_res != (on != 0)) {
-    HIPCHK(hipStreamSynchronize(x->stream));
-    if (x->chunk_exec) (void)hipGraphExecDestroy(x->chunk_exec);
-    x->chunk_exec = nullptr;
-  }
+  if (x->keep_res != (on != 0)) x->chunk.stale = true;
   x->keep_res = on != 0;
   return UFX_OK;
 }
@@ -1606,9 +1661,8 @@ struct ufx_cc_batch {
   double* d_norm = nullptr;        // per-case residual norms
   int nids = 0;
   size_t stage_lds = 0;
-  hipGraphExec_t exec = nullptr;   // a captured chunk
-  int chunk_n = 0, chunk_keep = -1;
-  double chunk_cfl = 0.0;
+  Chunk chunk;                     // a captured chunk of batched iterations
+  int chunk_form = -1;             // stage form the chunk was captured with
   bool dirty = true;               // descriptors or tables to rewrite
 };
 
@@ -1622,18 +1676,15 @@ template <typename R> DevT<R>& dev_of(ufx_cc* x);
 template <> DevT<double>& dev_of<double>(ufx_cc* x) { return x->d64; }
 template <> DevT<float>& dev_of<float>(ufx_cc* x) { return x->d32; }
 
-void batch_drop_graph(ufx_cc_batch* b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  b->exec = nullptr;
-}
-
 // Descriptors of the active cases and the block tables over them.  Inactive
 // cases are not touched (they may have been destroyed).
 template <typename R>
 int batch_write(ufx_cc_batch* b) {
   const int n = (int)b->cs.size();
   HIPCHK(hipStreamSynchronize(b->stream));
-  batch_drop_graph(b);
+  // the captured chunk's launches describe the old tables: never replayed
+  // again, released once its successor is instantiated
+  b->chunk.stale = true;
   std::vector<CaseDesc<R>> desc(n);
   std::memset((void*)desc.data(), 0, desc.size() * sizeof(CaseDesc<R>));
   std::vector<int4> map;
@@ -1760,28 +1811,54 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
     if (x->d_hist != b->hist_seen[i]) b->dirty = true;
   }
   if (!x0) return UFX_OK;
+  // the residual-stage form may have been changed (ufx_cc_set_stage_form):
+  // all cases launch as one, so they must agree
+  size_t lds = 0;
+  for (size_t i = 0; i < b->cs.size(); ++i) {
+    if (!b->on[i]) continue;
+    ufx_cc* x = b->cs[i];
+    if (x->stage_shared != x0->stage_shared || x->stage_local != x0->stage_local) {
+      set_error("ufx_cc_batch_iterate: the active cases differ in their stage form");
+      return UFX_ERR_ARG;
+    }
+    lds = std::max(lds, x->stage_lds);
+  }
+  const int form = (x0->stage_shared ? 2 : 0) | (x0->stage_local ? 1 : 0);
+  if (form != b->chunk_form || lds != b->stage_lds) b->chunk.stale = true;
+  b->stage_lds = lds;
   if (b->dirty)
     if (int rc = batch_write<R>(b)) return rc;
-  if (!b->exec || b->chunk_n != niter || b->chunk_cfl != cfl || b->chunk_keep != keep) {
-    batch_drop_graph(b);
-    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int k = 0; k < niter && !rc; ++k) rc = batch_iteration<R>(b, x0, k, cfl, keep);
-    hipGraph_t gr = nullptr;
-    hipError_t e = hipStreamEndCapture(b->stream, &gr);
-    if (rc) {
-      if (gr) (void)hipGraphDestroy(gr);
-      return rc;
+  // capture a chunk when there is none that fits (first call, new tables or
+  // stage form, another CFL or residual option, a longer chunk); a shorter
+  // chunk than the captured one (the remainder of a run) is launched directly
+  const bool fresh = !b->chunk.exec || b->chunk.stale || b->chunk.cfl != cfl ||
+                     b->chunk.keep != keep || niter > b->chunk.n;
+  if (!fresh && niter < b->chunk.n) {
+    for (int k = 0; k < niter; ++k)
+      if (int rc = batch_iteration<R>(b, x0, k, cfl, keep)) return rc;
+  } else {
+    if (fresh) {
+      HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int k = 0; k < niter && !rc; ++k) rc = batch_iteration<R>(b, x0, k, cfl, keep);
+      Chunk c;
+      hipError_t e = hipStreamEndCapture(b->stream, &c.graph);
+      if (!rc && e == hipSuccess) e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
+      if (rc || e != hipSuccess) {
+        (void)release_chunk(c);
+        if (rc) return rc;
+        HIPCHK(e);
+      }
+      c.n = niter;
+      c.cfl = cfl;
+      c.keep = keep;
+      c.stale = false;
+      HIPCHK(release_chunk(b->chunk));
+      b->chunk = c;
+      b->chunk_form = form;
     }
-    HIPCHK(e);
-    hipError_t ie = hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    HIPCHK(ie);
-    b->chunk_n = niter;
-    b->chunk_cfl = cfl;
-    b->chunk_keep = keep;
+    HIPCHK(hipGraphLaunch(b->chunk.exec, b->stream));
   }
-  HIPCHK(hipGraphLaunch(b->exec, b->stream));
   // the error words of every active case: the first failing case is reported
   const CaseDesc<R>* cs = (const CaseDesc<R>*)b->d_desc;
   km_errors<R><<<nblocks(b->nids), BLK, 0, b->stream>>>(cs, b->d_ids, b->nids, b->d_err);
@@ -1903,7 +1980,7 @@ int ufx_cc_batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
 int ufx_cc_batch_destroy(ufx_cc_batch* b) {
   if (!b) return UFX_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  batch_drop_graph(b);
+  (void)release_chunk(b->chunk);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_map) (void)hipFree(b->d_map);
   if (b->d_ids) (void)hipFree(b->d_ids);
