@@ -242,7 +242,7 @@ def main_nc(args) -> int:
     mesh = nc.NCMesh.from_raw(raw)
     cnt = mesh.counts
     solver = nc.NCSolver(mesh, mach=0.4, cfl_max=50.0, grad="gg", timemode="lusgs",
-                         lusgs_order=args.lusgs_order)
+                         lusgs_order=args.lusgs_order, precision=args.precision)
     ncol = int(mesh.get("pt_colour").max()) + 1
     print(f"# setup {time.time() - t0:.1f}s: {cnt['nop']} vertices, {cnt['nof']} edges",
           file=sys.stderr, flush=True)
@@ -272,7 +272,7 @@ def main_nc(args) -> int:
           file=sys.stderr, flush=True)
     # the roofline of every pass (algorithmic bytes per iteration / its time
     # per iteration); the line's roofline is the pass that takes longest
-    pb = nc_pass_bytes(nop, nof)
+    pb = nc_pass_bytes(nop, nof, w=8 if args.precision == "fp64" else 4)
     passes = {}
     for k, b in pb.items():
         ms_it = prof[k][0] / args.steps
@@ -297,11 +297,13 @@ def main_nc(args) -> int:
         "value": round(nop * args.steps / wall / 1e6, 3),
         "unit": "M vertex-iterations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"NCFVM median dual, seeded O-grid 2000x1000 points around the "
                                f"unit circle ({nop} vertices, {nof} edges), M=0.4, GG + "
                                f"Venkatakrishnan, van Leer, LU-SGS ({sweep}), tanh CFL ramp to 50",
                    "vertices": nop, "edges": nof, "lusgs_order": args.lusgs_order,
+                   "precision": args.precision,
                    "colours": ncol},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

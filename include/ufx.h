@@ -377,7 +377,7 @@ void* ufx_cc_stream(ufx_cc* ctx);
 /* ============================================ vertex-centred (median dual)
  * The NCFVM solver's hot path (NCFVM/main.f90:51-88).  State per vertex, in
  * the reference's breadth-first vertex order, variables of one vertex
- * contiguous. FP64. */
+ * contiguous, FP64 (the mixed form's records aside). */
 typedef struct ufx_nc ufx_nc;
 
 /* Run parameters (NCFVM/read_input.f90:23-40; the deck's Flux and Limiter
@@ -390,7 +390,12 @@ typedef struct {
   int grad;          /* UFX_GRAD_GG or UFX_GRAD_LSQR (the two sides of the
                         merge conflict at NCFVM/fvresidual.f90:11-17) */
   int timemode;      /* UFX_TIME_RK3 (needs cfl_max < 1) or UFX_TIME_LUSGS */
-  int precision;     /* UFX_PREC_FP64 */
+  int precision;     /* UFX_PREC_FP64, or UFX_PREC_MIXED: vertex records (primitives
+                        as deviations from the free stream, gradients, limiter,
+                        coordinates), edge geometry and least-squares weights in
+                        FP32, gradient and limiter in FP32; reconstruction, fluxes,
+                        residual sums, time steps, state, norms and LU-SGS in FP64.
+                        Build-defined; LU-SGS then needs UFX_LUSGS_MULTICOLOUR */
   int flux;          /* UFX_FLUX_VANLEER (the reference's) or UFX_FLUX_AUSM (build-
                         defined: the cell-centred AUSM+-up on the dual faces) */
   int limiter;       /* UFX_NC_LIM_VKN (limiter_VKN, NCFVM/limiter.f90:15-98) or

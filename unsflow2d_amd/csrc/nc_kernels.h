@@ -24,6 +24,7 @@
 namespace ufx {
 
 constexpr int NW = 18;  // vertex record width (same offsets as the cell record)
+constexpr int NW32 = 20;  // the same in FP32, padded to a multiple of 16 bytes
 constexpr double NC_EPS_SP = 1.1920928955078125e-07;  // epsilon(1.0), NCFVM/All_modules.f90:9
 
 struct NCFree {
@@ -43,6 +44,12 @@ struct DevNC {
   const int2* v2fo;          // per v2f entry: (edge, bit-inverted when the vertex is p2; other end)
   const int *v2v_ptr, *v2v;
   const double* wxy;
+  // the mixed form's records (FP32), used instead of vrec / erec / wxy
+  float* vrec32;          // [nop][NW32]
+  const float* erec32;    // [nof][4]
+  const float* wxy32;     // [..][2]
+  const float2* ggs32;    // [nop] sum over the vertex's dual faces of +-n ds (FP64 sum)
+  const double2* pxy;     // vertex coordinates in FP64 (wall forces)
   double* part;  // reduction partials [nblk][4]
   double* scal;  // [0] fres1
   int* err;
@@ -80,8 +87,42 @@ __device__ __forceinline__ void ld2d(const double* p, double& a, double& b) {
   b = v.y;
 }
 
+// Vertex records of the FP64 form (vrec, double, NW wide) and of the mixed
+// form (vrec32, float, padded to NW32 for 16-byte loads); edge records and
+// least-squares weights likewise.  The FP64 instantiation of every kernel
+// below keeps the reference's expression order (bit-exact); the mixed one
+// evaluates the same expressions in FP32 and accumulates the residual, the
+// spectral radii and the time step in FP64.
+// The mixed form stores its primitives as deviations from the free stream:
+// the limiter compares neighbour differences with single precision's epsilon
+// (NCFVM/limiter.f90:45-46, 1.19e-7), which absolute FP32 values of the
+// pressure (about 4.5 at M 0.4) cannot resolve; as deviations, differences,
+// gradients and limiter inputs keep FP32's relative precision where the flow
+// is nearly uniform.  The free stream is added back for the fluxes and the
+// boundary closures; the Green-Gauss sum, which the reference leaves without
+// a boundary closure, gets its free-stream part back from the vertex's FP64
+// sum of +-n ds (zero but for rounding inside the mesh).
+template <typename R> struct NCRec;
+template <> struct NCRec<double> {
+  static constexpr int W = NW;
+  __device__ static double* rec(const DevNC& d) { return d.vrec; }
+  __device__ static const double* edge(const DevNC& d) { return d.erec; }
+  __device__ static const double* lsq(const DevNC& d) { return d.wxy; }
+};
+template <> struct NCRec<float> {
+  static constexpr int W = NW32;
+  __device__ static float* rec(const DevNC& d) { return d.vrec32; }
+  __device__ static const float* edge(const DevNC& d) { return d.erec32; }
+  __device__ static const float* lsq(const DevNC& d) { return d.wxy32; }
+};
+
+template <typename R> __device__ __forceinline__ R nc_abs(R x);
+template <> __device__ __forceinline__ double nc_abs<double>(double x) { return fabs(x); }
+template <> __device__ __forceinline__ float nc_abs<float>(float x) { return fabsf(x); }
+
 // save_old, NCFVM/common.f90:88-104: primitives of every vertex (the only
-// primitive refresh of an iteration) and qold = qc.
+// primitive refresh of an iteration, from the FP64 state) and qold = qc.
+template <typename R>
 __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
   int p = cell_index();
   if (p >= d.nop) return;
@@ -91,8 +132,14 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
   double rho = q[1], u = q[2] / q[1], v = q[3] / q[1], et = q[0];
   double qq = u * u + v * v;
   double pr = (g.gamma - 1.0) * (et - 0.5 * rho * qq);
-  double o[4] = {rho, u, v, pr};
-  st4d(d.vrec + (size_t)p * NW + R_QP, o);
+  R o[4];
+  if constexpr (sizeof(R) == 8) {
+    o[0] = rho; o[1] = u; o[2] = v; o[3] = pr;
+  } else {
+    o[0] = (R)(rho - g.r_inf); o[1] = (R)(u - g.u_inf); o[2] = (R)(v - g.v_inf);
+    o[3] = (R)(pr - g.p_inf);
+  }
+  stv<R, 4>(NCRec<R>::rec(d) + (size_t)p * NCRec<R>::W + R_QP, o);
 }
 
 // Gradient and limiter of one vertex:
@@ -114,15 +161,19 @@ __device__ __forceinline__ void for_edges(const DevNC& d, int j0, int n, const i
   for (int k = NCU; k < n; ++k) f(d.v2fo[j0 + k]);
 }
 
-template <int GRAD, int LIM>
-__global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
+template <typename R, int GRAD, int LIM>
+__global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
+  constexpr int W = NCRec<R>::W;
   int p = cell_index();
   if (p >= d.nop) return;
-  const double* own = d.vrec + (size_t)p * NW;
-  double q[4], x, y;
-  ld4d(own + R_QP, q);
-  ld2d(own + R_CEN, x, y);
-  double gr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const R* recs = NCRec<R>::rec(d);
+  const R* erec = NCRec<R>::edge(d);
+  const R* own = recs + (size_t)p * W;
+  R q[4], xy[2];
+  ldv<R, 4>(own + R_QP, q);
+  ldv<R, 2>(own + R_CEN, xy);
+  const R x = xy[0], y = xy[1];
+  R gr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
   int2 nb[NCU];
 #pragma unroll
@@ -132,15 +183,13 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
     for_edges(d, j0, n, nb, [&](int2 eo) {
       bool first = eo.x >= 0;
       int e = first ? eo.x : ~eo.x, o = eo.y;
-      double qo[4];
-      ld4d(d.vrec + (size_t)o * NW + R_QP, qo);
-      double nx, ny, ds, pad;
-      ld2d(d.erec + (size_t)e * 4, nx, ny);
-      ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
-      double dx = nx * ds, dy = ny * ds;
+      R qo[4], er[4];
+      ldv<R, 4>(recs + (size_t)o * W + R_QP, qo);
+      ldv<R, 4>(erec + (size_t)e * 4, er);
+      R dx = er[0] * er[2], dy = er[1] * er[2];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        double var = first ? 0.5 * (q[k] + qo[k]) : 0.5 * (qo[k] + q[k]);
+        R var = first ? R(0.5) * (q[k] + qo[k]) : R(0.5) * (qo[k] + q[k]);
         if (first) {
           gr[2 * k] = gr[2 * k] + var * dx;
           gr[2 * k + 1] = gr[2 * k + 1] + var * dy;
@@ -150,134 +199,146 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d) {
         }
       }
     });
-    double v = d.cv[p];
+    if constexpr (sizeof(R) == 4) {
+      // the free-stream part of the open Green-Gauss sum
+      const float2 sv = d.ggs32[p];
+      const R qi[4] = {(R)g.r_inf, (R)g.u_inf, (R)g.v_inf, (R)g.p_inf};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        gr[2 * k] = gr[2 * k] + qi[k] * sv.x;
+        gr[2 * k + 1] = gr[2 * k + 1] + qi[k] * sv.y;
+      }
+    }
+    R v = (R)d.cv[p];
 #pragma unroll
     for (int k = 0; k < 8; ++k) gr[k] = gr[k] / v;
   } else {
+    const R* wxy = NCRec<R>::lsq(d);
     for (int j = d.v2v_ptr[p]; j < d.v2v_ptr[p + 1]; ++j) {
       int c = d.v2v[j];
-      double wx, wy, qo[4];
-      ld2d(d.wxy + (size_t)j * 2, wx, wy);
-      ld4d(d.vrec + (size_t)c * NW + R_QP, qo);
+      R w[2], qo[4];
+      ldv<R, 2>(wxy + (size_t)j * 2, w);
+      ldv<R, 4>(recs + (size_t)c * W + R_QP, qo);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        gr[2 * k] = gr[2 * k] + (qo[k] - q[k]) * wx;
-        gr[2 * k + 1] = gr[2 * k + 1] + (qo[k] - q[k]) * wy;
+        gr[2 * k] = gr[2 * k] + (qo[k] - q[k]) * w[0];
+        gr[2 * k + 1] = gr[2 * k + 1] + (qo[k] - q[k]) * w[1];
       }
     }
   }
   // difference range over the edges
-  double dumax[4], dumin[4];
+  const R eps = (R)NC_EPS_SP;
+  R dumax[4], dumin[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) { dumin[k] = NC_EPS_SP; dumax[k] = -NC_EPS_SP; }
+  for (int k = 0; k < 4; ++k) { dumin[k] = eps; dumax[k] = -eps; }
   for_edges(d, j0, n, nb, [&](int2 eo) {
     bool first = eo.x >= 0;
-    double qo[4];
-    ld4d(d.vrec + (size_t)eo.y * NW + R_QP, qo);
+    R qo[4];
+    ldv<R, 4>(recs + (size_t)eo.y * W + R_QP, qo);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      double du = first ? qo[k] - q[k] : -(q[k] - qo[k]);
+      R du = first ? qo[k] - q[k] : -(q[k] - qo[k]);
       dumax[k] = dumax[k] > du ? dumax[k] : du;
       dumin[k] = dumin[k] < du ? dumin[k] : du;
     }
   });
-  double phi[4];
-  ld4d(own + R_PHI, phi);
-  const double kappa = (double)0.01f;
-  double t = kappa * sqrt(d.cv[p]);
-  const double tol = t * t * t;
+  R phi[4];
+  ldv<R, 4>(own + R_PHI, phi);
+  const R kappa = (R)0.01f;
+  R t = kappa * fsqrt<R>((R)d.cv[p]);
+  const R tol = t * t * t;
   for_edges(d, j0, n, nb, [&](int2 eo) {
     bool first = eo.x >= 0;
-    double xo, yo;
-    ld2d(d.vrec + (size_t)eo.y * NW + R_CEN, xo, yo);
-    double d0 = first ? xo - x : x - xo, d1 = first ? yo - y : y - yo;
+    R xo[2];
+    ldv<R, 2>(recs + (size_t)eo.y * W + R_CEN, xo);
+    R d0 = first ? xo[0] - x : x - xo[0], d1 = first ? xo[1] - y : y - xo[1];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      double sum = (0.0 + gr[2 * k] * d0) + gr[2 * k + 1] * d1;
-      double ph;
+      R sum = (R(0) + gr[2 * k] * d0) + gr[2 * k + 1] * d1;
+      R ph;
       if (LIM == 0) {
-        double dl = first ? 0.5 * sum : -0.5 * sum;
-        double alfa = dl > 0.0 ? dumax[k] : dumin[k];
-        if (fabs(alfa) < NC_EPS_SP) alfa = 0.0;
-        double nr = alfa * alfa + 2.0 * dl * alfa + tol;
-        double dr = alfa * alfa + 2.0 * dl * dl + dl * alfa + tol;
+        R dl = first ? R(0.5) * sum : R(-0.5) * sum;
+        R alfa = dl > R(0) ? dumax[k] : dumin[k];
+        if (nc_abs<R>(alfa) < eps) alfa = R(0);
+        R nr = alfa * alfa + R(2) * dl * alfa + tol;
+        R dr = alfa * alfa + R(2) * dl * dl + dl * alfa + tol;
         ph = nr / dr;
-        ph = 1.0 < ph ? 1.0 : ph;
+        ph = R(1) < ph ? R(1) : ph;
       } else {
         // limiter_min_max (Barth-Jespersen), NCFVM/limiter.f90:192-260:
         // both edge ends take +1/2 of the projected gradient (:227, :241)
-        double dl = 0.5 * sum;
-        if (dl > 0.0) {
+        R dl = R(0.5) * sum;
+        if (dl > R(0)) {
           ph = dumax[k] / dl;
-          ph = 1.0 < ph ? 1.0 : ph;
-        } else if (dl < 0.0) {
+          ph = R(1) < ph ? R(1) : ph;
+        } else if (dl < R(0)) {
           ph = dumin[k] / dl;
-          ph = 1.0 < ph ? 1.0 : ph;
+          ph = R(1) < ph ? R(1) : ph;
         } else {
-          ph = 1.0;
+          ph = R(1);
         }
       }
       phi[k] = phi[k] < ph ? phi[k] : ph;
     }
   });
-  double* w = d.vrec + (size_t)p * NW;
-  st4d(w + R_G, gr);
-  st4d(w + R_G + 4, gr + 4);
-  st4d(w + R_PHI, phi);
+  R* w = NCRec<R>::rec(d) + (size_t)p * W;
+  stv<R, 4>(w + R_G, gr);
+  stv<R, 4>(w + R_G + 4, gr + 4);
+  stv<R, 4>(w + R_PHI, phi);
 }
 
 // Spectral radius of a dual face from the Roe average of its two states,
 // NCFVM/vanleer_flux.f90:103-115, times the face length.
-__device__ __forceinline__ double nc_roe_lambda(const NCFree& g, double rl, double ul, double vl,
-                                                double hl, double rr, double ur, double vr,
-                                                double hr, double nx, double ny, double area) {
-  double rt = sqrt(rr / rl);
-  double u = (ul + rt * ur) / (1.0 + rt), v = (vl + rt * vr) / (1.0 + rt);
-  double h = (hl + rt * hr) / (1.0 + rt);
-  double a = sqrt(g.gamma1 * (h - 0.5 * (u * u + v * v)));
-  return (fabs(u * nx + v * ny) + a) * area;
+template <typename R>
+__device__ __forceinline__ R nc_roe_lambda(R gm1, R rl, R ul, R vl, R hl, R rr, R ur, R vr, R hr,
+                                           R nx, R ny, R area) {
+  R rt = fsqrt<R>(rr / rl);
+  R u = (ul + rt * ur) / (R(1) + rt), v = (vl + rt * vr) / (R(1) + rt);
+  R h = (hl + rt * hr) / (R(1) + rt);
+  R a = fsqrt<R>(gm1 * (h - R(0.5) * (u * u + v * v)));
+  return (nc_abs<R>(u * nx + v * ny) + a) * area;
 }
 
 // AUSM+-up on a dual face (build-defined: the vertex-centred reference has
 // no compilable AUSM): the cell-centred routine's flux per unit length
 // (CCFVM/ausm_up_flux.F90:2-162, unit normal), with the van Leer routine's
 // Roe-averaged spectral radius.
-__device__ __forceinline__ void nc_ausm(const NCFree& g, const Gas<double>& ga, const double* ql,
-                                        const double* qr, double nx, double ny, double area,
-                                        double* F, double& ll) {
-  double ql_[4] = {ql[0], ql[1], ql[2], ql[3]}, qr_[4] = {qr[0], qr[1], qr[2], qr[3]};
-  ausm_flux<double>(ga, ql_, qr_, nx, ny, F);
-  double hl = (g.gamma * ql[3] / ql[0]) / g.gamma1 + 0.5 * (ql[1] * ql[1] + ql[2] * ql[2]);
-  double hr = (g.gamma * qr[3] / qr[0]) / g.gamma1 + 0.5 * (qr[1] * qr[1] + qr[2] * qr[2]);
-  ll = nc_roe_lambda(g, ql[0], ql[1], ql[2], hl, qr[0], qr[1], qr[2], hr, nx, ny, area);
+template <typename R>
+__device__ __forceinline__ void nc_ausm(const Gas<R>& ga, const R* ql, const R* qr, R nx, R ny,
+                                        R area, R* F, R& ll) {
+  R ql_[4] = {ql[0], ql[1], ql[2], ql[3]}, qr_[4] = {qr[0], qr[1], qr[2], qr[3]};
+  ausm_flux<R>(ga, ql_, qr_, nx, ny, F);
+  R hl = (ga.gamma * ql[3] / ql[0]) / ga.gamma1 + R(0.5) * (ql[1] * ql[1] + ql[2] * ql[2]);
+  R hr = (ga.gamma * qr[3] / qr[0]) / ga.gamma1 + R(0.5) * (qr[1] * qr[1] + qr[2] * qr[2]);
+  ll = nc_roe_lambda<R>(ga.gamma1, ql[0], ql[1], ql[2], hl, qr[0], qr[1], qr[2], hr, nx, ny, area);
 }
 
 // van Leer flux of NCFVM/vanleer_flux.f90:2-118 on one dual face between
 // the reconstructed states ql, qr (primitives): F (per unit length) and the
 // Roe-averaged spectral radius times the face length.
-__device__ __forceinline__ void nc_vanleer(const NCFree& g, const double* ql, const double* qr,
-                                           double nx, double ny, double area, double* F,
-                                           double& ll) {
-  const double gm = g.gamma, gm1 = g.gamma1;
-  double rl = ql[0], ul = ql[1], vl = ql[2], pl = ql[3];
-  double unl = ul * nx + vl * ny, ql2 = ul * ul + vl * vl;
-  double al2 = gm * pl / rl, hl = al2 / gm1 + 0.5 * ql2, cl = sqrt(al2), ml = unl / cl;
-  double rr = qr[0], ur = qr[1], vr = qr[2], pr = qr[3];
-  double unr = ur * nx + vr * ny, qr2 = ur * ur + vr * vr;
-  double ar2 = gm * pr / rr, hr = ar2 / gm1 + 0.5 * qr2, cr = sqrt(ar2), mr = unr / cr;
-  double plp = p5p<double>(ml, 0.0), prm = p5m<double>(mr, 0.0);
-  double mlp = m4p<double>(ml, 0.0), mlm = m4m<double>(ml, 0.0);
-  double mrp = m4p<double>(mr, 0.0), mrm = m4m<double>(mr, 0.0);
-  double p12 = plp * pl + prm * pr;
-  double fmp = rl * cl * mlp, fmm = rr * cr * mrm;
-  double fm12 = fmp + fmm;
-  double delta = -(gm * (cr * mrp * mrm * pr - cl * mlp * mlm * pl) / (gm - 1.0));
-  double dissi = fmp - fmm;
-  double fl[4] = {hl, 1.0, ul, vl}, fr[4] = {hr, 1.0, ur, vr};
-  double fp[4] = {delta, 0.0, nx * p12, ny * p12};
+template <typename R>
+__device__ __forceinline__ void nc_vanleer(R gm, R gm1, const R* ql, const R* qr, R nx, R ny,
+                                           R area, R* F, R& ll) {
+  R rl = ql[0], ul = ql[1], vl = ql[2], pl = ql[3];
+  R unl = ul * nx + vl * ny, ql2 = ul * ul + vl * vl;
+  R al2 = gm * pl / rl, hl = al2 / gm1 + R(0.5) * ql2, cl = fsqrt<R>(al2), ml = unl / cl;
+  R rr = qr[0], ur = qr[1], vr = qr[2], pr = qr[3];
+  R unr = ur * nx + vr * ny, qr2 = ur * ur + vr * vr;
+  R ar2 = gm * pr / rr, hr = ar2 / gm1 + R(0.5) * qr2, cr = fsqrt<R>(ar2), mr = unr / cr;
+  R plp = p5p<R>(ml, R(0)), prm = p5m<R>(mr, R(0));
+  R mlp = m4p<R>(ml, R(0)), mlm = m4m<R>(ml, R(0));
+  R mrp = m4p<R>(mr, R(0)), mrm = m4m<R>(mr, R(0));
+  R p12 = plp * pl + prm * pr;
+  R fmp = rl * cl * mlp, fmm = rr * cr * mrm;
+  R fm12 = fmp + fmm;
+  R delta = -(gm * (cr * mrp * mrm * pr - cl * mlp * mlm * pl) / (gm - R(1)));
+  R dissi = fmp - fmm;
+  R fl[4] = {hl, R(1), ul, vl}, fr[4] = {hr, R(1), ur, vr};
+  R fp[4] = {delta, R(0), nx * p12, ny * p12};
 #pragma unroll
-  for (int i = 0; i < 4; ++i) F[i] = 0.5 * fm12 * (fl[i] + fr[i]) - 0.5 * dissi * (fr[i] - fl[i]) + fp[i];
-  ll = nc_roe_lambda(g, rl, ul, vl, hl, rr, ur, vr, hr, nx, ny, area);
+  for (int i = 0; i < 4; ++i)
+    F[i] = R(0.5) * fm12 * (fl[i] + fr[i]) - R(0.5) * dissi * (fr[i] - fl[i]) + fp[i];
+  ll = nc_roe_lambda<R>(gm1, rl, ul, vl, hl, rr, ur, vr, hr, nx, ny, area);
 }
 
 // Residual of one vertex, NCFVM/fvresidual.f90:34-64 as a gather over its
@@ -285,47 +346,67 @@ __device__ __forceinline__ void nc_vanleer(const NCFree& g, const double* ql, co
 // dual face (+ into p1, - into p2, spectral radius into both), then for a
 // wall edge solid_flux (NCFVM/solid_flux.f90:1-43) and for a farfield edge
 // farfield_flux1 (NCFVM/farfield_flux.f90:1-95); then time_step2
-// (NCFVM/timestep.f90:5-37).
+// (NCFVM/timestep.f90:5-37).  The two boundary closures are evaluated in
+// FP64 in both forms (a few thousand edges).
 //   mode 0: store res, la, dt;  mode 1: also the RK stage update of
 //   NCFVM/main.f90:59-70 in place (no vertex reads another's qc in a stage).
-template <int FLUX>
-__global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, Gas<double> ga, double cfl,
-                                                     int mode, double ak, double bk) {
+template <typename R, int FLUX>
+__global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64,
+                                                     double cfl, int mode, double ak, double bk) {
+  constexpr int W = NCRec<R>::W;
   int p = cell_index();
   if (p >= d.nop) return;
+  const R* recs = NCRec<R>::rec(d);
+  const R* erec = NCRec<R>::edge(d);
   double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0;
   const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
   int2 nb[NCU];
 #pragma unroll
   for (int k = 0; k < NCU; ++k)
     if (k < n) nb[k] = d.v2fo[j0 + k];
-  double me[NW];
-  ldv<double, NW>(d.vrec + (size_t)p * NW, me);
+  R me[W];
+  ldv<R, W>(recs + (size_t)p * W, me);
   for_edges(d, j0, n, nb, [&](int2 eo) {
     bool first = eo.x >= 0;
     int e = first ? eo.x : ~eo.x;
-    double ot[NW], a[NW], b[NW];
-    ldv<double, NW>(d.vrec + (size_t)eo.y * NW, ot);
+    R ot[W], a[W], b[W];
+    ldv<R, W>(recs + (size_t)eo.y * W, ot);
 #pragma unroll
-    for (int i = 0; i < NW; ++i) {
+    for (int i = 0; i < W; ++i) {
       a[i] = first ? me[i] : ot[i];
       b[i] = first ? ot[i] : me[i];
     }
-    double nx, ny, ds, pad;
-    ld2d(d.erec + (size_t)e * 4, nx, ny);
-    ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
-    double d0 = b[R_CEN] - a[R_CEN], d1 = b[R_CEN + 1] - a[R_CEN + 1];
+    R er[4];
+    ldv<R, 4>(erec + (size_t)e * 4, er);
+    // reconstruction and flux in FP64 from the records (FP32 in the mixed
+    // form: rounding absolute states to FP32 would put a noise floor of a
+    // few 1e-7 under the residual)
+    const double nx = er[0], ny = er[1], ds = er[2];
+    double d0 = (double)b[R_CEN] - (double)a[R_CEN], d1 = (double)b[R_CEN + 1] - (double)a[R_CEN + 1];
+    if constexpr (sizeof(R) == 8) {
+      d0 = b[R_CEN] - a[R_CEN];
+      d1 = b[R_CEN + 1] - a[R_CEN + 1];
+    }
     double ql[4], qr[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      double s = (0.0 + a[R_G + 2 * k] * d0) + a[R_G + 2 * k + 1] * d1;
-      ql[k] = a[R_QP + k] + 0.5 * s * a[R_PHI + k];
-      s = (0.0 + b[R_G + 2 * k] * d0) + b[R_G + 2 * k + 1] * d1;
-      qr[k] = b[R_QP + k] - 0.5 * s * b[R_PHI + k];
+      double s = (0.0 + (double)a[R_G + 2 * k] * d0) + (double)a[R_G + 2 * k + 1] * d1;
+      ql[k] = (double)a[R_QP + k] + 0.5 * s * (double)a[R_PHI + k];
+      s = (0.0 + (double)b[R_G + 2 * k] * d0) + (double)b[R_G + 2 * k + 1] * d1;
+      qr[k] = (double)b[R_QP + k] - 0.5 * s * (double)b[R_PHI + k];
+    }
+    if constexpr (sizeof(R) == 4) {
+      // deviations -> states
+      const double qi[4] = {g.r_inf, g.u_inf, g.v_inf, g.p_inf};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        ql[k] = ql[k] + qi[k];
+        qr[k] = qr[k] + qi[k];
+      }
     }
     double F[4], ll;
-    if (FLUX == 2) nc_ausm(g, ga, ql, qr, nx, ny, ds, F, ll);
-    else nc_vanleer(g, ql, qr, nx, ny, ds, F, ll);
+    if (FLUX == 2) nc_ausm<double>(ga64, ql, qr, nx, ny, ds, F, ll);
+    else nc_vanleer<double>(g.gamma, g.gamma1, ql, qr, nx, ny, ds, F, ll);
     if (first) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i] * ds;
@@ -336,58 +417,71 @@ __global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, Gas<doub
     }
     la = la + ll;
     int bc = d.ebc[e];
-    if (bc == WALL) {
-      // solid_flux: 3/4-1/4 end-point pressures on the half-edge right normals
-      double p1w = 0.75 * a[R_QP + 3] + 0.25 * b[R_QP + 3];
-      double p2w = 0.25 * a[R_QP + 3] + 0.75 * b[R_QP + 3];
-      double wnx = 0.5 * (b[R_CEN + 1] - a[R_CEN + 1]);
-      double wny = -0.5 * (b[R_CEN] - a[R_CEN]);
-      double area = sqrt(wnx * wnx + wny * wny);
-      const double* me = first ? a : b;
-      la = la + sqrt(g.gamma * me[R_QP + 3] / me[R_QP]) * area;
-      double pw = first ? p1w : p2w;
-      r[2] = r[2] + pw * wnx;
-      r[3] = r[3] + pw * wny;
-    } else if (bc == FARF) {
-      // farfield_flux1: Riemann invariants on the half-edge left normal from
-      // the edge-averaged primitives, the same flux into both end points
-      double fnx = -0.5 * (b[R_CEN + 1] - a[R_CEN + 1]);
-      double fny = 0.5 * (b[R_CEN] - a[R_CEN]);
-      double dr = sqrt(fnx * fnx + fny * fny);
-      fnx = fnx / dr;
-      fny = fny / dr;
-      double un_inf = g.u_inf * fnx + g.v_inf * fny;
-      double rho = 0.5 * (a[R_QP] + b[R_QP]), u = 0.5 * (a[R_QP + 1] + b[R_QP + 1]);
-      double v = 0.5 * (a[R_QP + 2] + b[R_QP + 2]), pr = 0.5 * (a[R_QP + 3] + b[R_QP + 3]);
-      double as = sqrt(g.gamma * pr / rho);
-      double un = u * fnx + v * fny;
-      double rp = un + 2.0 * as / g.gamma1;
-      double rm = un_inf - 2.0 * g.a_inf / g.gamma1;
-      double unf = 0.5 * (rp + rm);
-      double af = 0.25 * (rp - rm) * g.gamma1;
-      double uf, vf, sf;
-      if (unf > 0.0) {
-        uf = u + fnx * (unf - un);
-        vf = v + fny * (unf - un);
-        sf = pr / pow(rho, g.gamma);
-      } else {
-        uf = g.u_inf + fnx * (unf - un_inf);
-        vf = g.v_inf + fny * (unf - un_inf);
-        sf = g.p_inf / pow(g.r_inf, g.gamma);
-      }
-      double rhof = pow(af * af / g.gamma / sf, 1.0 / g.gamma1);
-      double pf = rhof * af * af / g.gamma;
-      double ef = pf / g.gamma1 + 0.5 * rhof * (uf * uf + vf * vf);
-      double fl[4];
-      fl[0] = (ef + pf) * unf;
-      fl[1] = rhof * unf;
-      fl[2] = fl[1] * uf + pf * fnx;
-      fl[3] = fl[1] * vf + pf * fny;
-      double area = sqrt(fnx * fnx + fny * fny);
-      const double* me = first ? a : b;
-      la = la + (fabs(unf) + sqrt(g.gamma * me[R_QP + 3] / me[R_QP])) * area;
+    if (bc == WALL || bc == FARF) {
+      double ax = (double)a[R_CEN], ay = (double)a[R_CEN + 1];
+      double bx = (double)b[R_CEN], by = (double)b[R_CEN + 1];
+      double ap[4] = {(double)a[R_QP], (double)a[R_QP + 1], (double)a[R_QP + 2], (double)a[R_QP + 3]};
+      double bp[4] = {(double)b[R_QP], (double)b[R_QP + 1], (double)b[R_QP + 2], (double)b[R_QP + 3]};
+      if constexpr (sizeof(R) == 4) {
+        const double qi[4] = {g.r_inf, g.u_inf, g.v_inf, g.p_inf};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) r[i] = r[i] + dr * fl[i];
+        for (int k = 0; k < 4; ++k) {
+          ap[k] = ap[k] + qi[k];
+          bp[k] = bp[k] + qi[k];
+        }
+      }
+      const double* mp = first ? ap : bp;
+      if (bc == WALL) {
+        // solid_flux: 3/4-1/4 end-point pressures on the half-edge right normals
+        double p1w = 0.75 * ap[3] + 0.25 * bp[3];
+        double p2w = 0.25 * ap[3] + 0.75 * bp[3];
+        double wnx = 0.5 * (by - ay);
+        double wny = -0.5 * (bx - ax);
+        double area = sqrt(wnx * wnx + wny * wny);
+        la = la + sqrt(g.gamma * mp[3] / mp[0]) * area;
+        double pw = first ? p1w : p2w;
+        r[2] = r[2] + pw * wnx;
+        r[3] = r[3] + pw * wny;
+      } else {
+        // farfield_flux1: Riemann invariants on the half-edge left normal
+        // from the edge-averaged primitives, the same flux into both ends
+        double fnx = -0.5 * (by - ay);
+        double fny = 0.5 * (bx - ax);
+        double dr = sqrt(fnx * fnx + fny * fny);
+        fnx = fnx / dr;
+        fny = fny / dr;
+        double un_inf = g.u_inf * fnx + g.v_inf * fny;
+        double rho = 0.5 * (ap[0] + bp[0]), u = 0.5 * (ap[1] + bp[1]);
+        double v = 0.5 * (ap[2] + bp[2]), pr = 0.5 * (ap[3] + bp[3]);
+        double as = sqrt(g.gamma * pr / rho);
+        double un = u * fnx + v * fny;
+        double rp = un + 2.0 * as / g.gamma1;
+        double rm = un_inf - 2.0 * g.a_inf / g.gamma1;
+        double unf = 0.5 * (rp + rm);
+        double af = 0.25 * (rp - rm) * g.gamma1;
+        double uf, vf, sf;
+        if (unf > 0.0) {
+          uf = u + fnx * (unf - un);
+          vf = v + fny * (unf - un);
+          sf = pr / pow(rho, g.gamma);
+        } else {
+          uf = g.u_inf + fnx * (unf - un_inf);
+          vf = g.v_inf + fny * (unf - un_inf);
+          sf = g.p_inf / pow(g.r_inf, g.gamma);
+        }
+        double rhof = pow(af * af / g.gamma / sf, 1.0 / g.gamma1);
+        double pf = rhof * af * af / g.gamma;
+        double ef = pf / g.gamma1 + 0.5 * rhof * (uf * uf + vf * vf);
+        double fl[4];
+        fl[0] = (ef + pf) * unf;
+        fl[1] = rhof * unf;
+        fl[2] = fl[1] * uf + pf * fnx;
+        fl[3] = fl[1] * vf + pf * fny;
+        double area = sqrt(fnx * fnx + fny * fny);
+        la = la + (fabs(unf) + sqrt(g.gamma * mp[3] / mp[0])) * area;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = r[i] + dr * fl[i];
+      }
     }
   });
   double cvp = d.cv[p];
@@ -928,8 +1022,9 @@ __global__ __launch_bounds__(BLK) void k_nc_wall_terms(DevNC d, NCFree g, const 
   double q1[4], q2[4], x1, y1, x2, y2;
   ld4d(d.qc + (size_t)pp.x * 4, q1);
   ld4d(d.qc + (size_t)pp.y * 4, q2);
-  ld2d(d.vrec + (size_t)pp.x * NW + R_CEN, x1, y1);
-  ld2d(d.vrec + (size_t)pp.y * NW + R_CEN, x2, y2);
+  const double2 c1 = d.pxy[pp.x], c2 = d.pxy[pp.y];
+  x1 = c1.x; y1 = c1.y;
+  x2 = c2.x; y2 = c2.y;
   double pr1 = nc_pressure(g, q1), pr2 = nc_pressure(g, q2);
   double p1w = 0.75 * pr1 + 0.25 * pr2, p2w = 0.25 * pr1 + 0.75 * pr2;
   double nx = 0.5 * (y2 - y1), ny = -0.5 * (x2 - x1);

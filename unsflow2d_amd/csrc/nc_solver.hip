@@ -30,6 +30,8 @@ struct ufx_nc {
   ufx_nc_params prm{};
   NCFree g{};
   Gas<double> ga{};  // for the AUSM+-up option (mref as CCFVM/ausm_up_flux.F90:79)
+  Gas<float> gaf{};  // the same for the mixed form
+  bool mixed = false;  // FP32 records, gradients, limiter and fluxes
   DevNC d{};
   int nop = 0, nof = 0, nblk = 0;
   hipStream_t stream = nullptr;
@@ -117,29 +119,46 @@ void prof_mark(ufx_nc* x, int cat) {
     prof_mark((x), -1);                                     \
   } while (0)
 
-int run_save_old(ufx_nc* x) {
-  NLAUNCH(x, NK_SAVE, k_nc_save_old, x->nblk, x->d, x->g);
+template <typename R> const Gas<R>& nc_gas(const ufx_nc* x);
+template <> const Gas<double>& nc_gas<double>(const ufx_nc* x) { return x->ga; }
+template <> const Gas<float>& nc_gas<float>(const ufx_nc* x) { return x->gaf; }
+
+template <typename R>
+int run_save_old_r(ufx_nc* x) {
+  NLAUNCH(x, NK_SAVE, k_nc_save_old<R>, x->nblk, x->d, x->g);
   return launch_ok();
 }
 
-int run_grad_lim(ufx_nc* x) {
+template <typename R>
+int run_grad_lim_r(ufx_nc* x) {
   bool lsq = x->prm.grad == UFX_GRAD_LSQR, bj = x->prm.limiter == UFX_NC_LIM_BJ;
-  if (lsq && bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<2, 1>), x->nblk, x->d);
-  else if (lsq) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<2, 0>), x->nblk, x->d);
-  else if (bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<1, 1>), x->nblk, x->d);
-  else NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<1, 0>), x->nblk, x->d);
+  if (lsq && bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 1>), x->nblk, x->d, x->g);
+  else if (lsq) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 0>), x->nblk, x->d, x->g);
+  else if (bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 1>), x->nblk, x->d, x->g);
+  else NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 0>), x->nblk, x->d, x->g);
   return launch_ok();
 }
 
 // fvresidual (gradient, limiter, edge fluxes, time step); mode 1 fuses the
 // RK stage update with coefficients (ak, bk)
-int run_fvresidual(ufx_nc* x, double cfl, int mode, double ak, double bk) {
-  if (int rc = run_grad_lim(x)) return rc;
+template <typename R>
+int run_fvresidual_r(ufx_nc* x, double cfl, int mode, double ak, double bk) {
+  if (int rc = run_grad_lim_r<R>(x)) return rc;
+  const Gas<double>& ga = x->ga;
   if (x->prm.flux == UFX_FLUX_AUSM)
-    NLAUNCH(x, NK_RES, k_nc_residual<2>, x->nblk, x->d, x->g, x->ga, cfl, mode, ak, bk);
+    NLAUNCH(x, NK_RES, (k_nc_residual<R, 2>), x->nblk, x->d, x->g, ga, cfl, mode, ak, bk);
   else
-    NLAUNCH(x, NK_RES, k_nc_residual<3>, x->nblk, x->d, x->g, x->ga, cfl, mode, ak, bk);
+    NLAUNCH(x, NK_RES, (k_nc_residual<R, 3>), x->nblk, x->d, x->g, ga, cfl, mode, ak, bk);
   return launch_ok();
+}
+
+int run_save_old(ufx_nc* x) {
+  return x->mixed ? run_save_old_r<float>(x) : run_save_old_r<double>(x);
+}
+
+int run_fvresidual(ufx_nc* x, double cfl, int mode, double ak, double bk) {
+  return x->mixed ? run_fvresidual_r<float>(x, cfl, mode, ak, bk)
+                  : run_fvresidual_r<double>(x, cfl, mode, ak, bk);
 }
 
 // The diagonal and both sweeps as level launches (each level one kernel).
@@ -267,6 +286,43 @@ int check(ufx_nc* x) {
   return UFX_OK;
 }
 
+// The vertex records as FP64, NW per vertex, whichever form holds them.
+int read_records(ufx_nc* x, std::vector<double>& v) {
+  const size_t nop = (size_t)x->nop;
+  v.assign(nop * NW, 0.0);
+  if (!x->mixed) {
+    NCCHK(hipMemcpyAsync(v.data(), x->d.vrec, v.size() * 8, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  std::vector<float> f(nop * NW32);
+  NCCHK(hipMemcpyAsync(f.data(), x->d.vrec32, f.size() * 4, hipMemcpyDeviceToHost, x->stream));
+  NCCHK(hipStreamSynchronize(x->stream));
+  const double qi[4] = {x->g.r_inf, x->g.u_inf, x->g.v_inf, x->g.p_inf};
+  for (size_t p = 0; p < nop; ++p)
+    for (int k = 0; k < NW; ++k)
+      v[p * NW + k] = (double)f[p * NW32 + k] + (k >= R_QP && k < R_QP + 4 ? qi[k - R_QP] : 0.0);
+  return UFX_OK;
+}
+
+int write_records(ufx_nc* x, const std::vector<double>& v) {
+  const size_t nop = (size_t)x->nop;
+  if (!x->mixed) {
+    NCCHK(hipMemcpyAsync(x->d.vrec, v.data(), v.size() * 8, hipMemcpyHostToDevice, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+    return UFX_OK;
+  }
+  std::vector<float> f(nop * NW32, 0.0f);
+  const double qi[4] = {x->g.r_inf, x->g.u_inf, x->g.v_inf, x->g.p_inf};
+  for (size_t p = 0; p < nop; ++p)
+    for (int k = 0; k < NW; ++k)
+      f[p * NW32 + k] =
+          (float)(v[p * NW + k] - (k >= R_QP && k < R_QP + 4 ? qi[k - R_QP] : 0.0));
+  NCCHK(hipMemcpyAsync(x->d.vrec32, f.data(), f.size() * 4, hipMemcpyHostToDevice, x->stream));
+  NCCHK(hipStreamSynchronize(x->stream));
+  return UFX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -296,8 +352,13 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     set_error("Unknown time integration scheme");
     return UFX_ERR_ARG;
   }
-  if (prm->precision != UFX_PREC_FP64) {
-    set_error("ufx_nc_create: the vertex-centred path is FP64 only");
+  if (prm->precision != UFX_PREC_FP64 && prm->precision != UFX_PREC_MIXED) {
+    set_error("ufx_nc_create: unknown precision");
+    return UFX_ERR_ARG;
+  }
+  if (prm->precision == UFX_PREC_MIXED && prm->timemode == UFX_TIME_LUSGS &&
+      prm->lusgs_order != UFX_LUSGS_MULTICOLOUR) {
+    set_error("ufx_nc_create: the mixed form's LU-SGS runs the multicolour sweeps");
     return UFX_ERR_UNSUPPORTED;
   }
   if (prm->flux != UFX_FLUX_VANLEER && prm->flux != UFX_FLUX_AUSM) {
@@ -347,6 +408,11 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   x->ga = Gas<double>{gamma, gamma - 1.0, 1.0 / (gamma * prm->mach * prm->mach), qinf[1],
                       x->g.u_inf, x->g.v_inf, p_inf, std::max((double)0.32f, 0.5 * prm->mach),
                       1.0 / (gamma - 1.0)};
+  const Gas<double>& G = x->ga;
+  x->gaf = Gas<float>{(float)G.gamma, (float)G.gamma1, (float)G.znd, (float)G.r_inf,
+                      (float)G.u_inf, (float)G.v_inf, (float)G.p_inf, (float)G.mref,
+                      (float)G.igamma1};
+  x->mixed = prm->precision == UFX_PREC_MIXED;
   // device arrays
   std::vector<double> vrec((size_t)nop * NW, 0.0);
   const double phi0 = (double)1e20f;  // default limiter value, NCFVM/All_modules.f90:200
@@ -379,8 +445,9 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     wxy[2 * j] = m.wx[j];
     wxy[2 * j + 1] = m.wy[j];
   }
-  // LU-SGS level schedule of the sequential sweeps in vertex order
-  {
+  // LU-SGS level schedule of the sequential sweeps in vertex order (only
+  // the reference-order sweeps use it)
+  if (prm->timemode == UFX_TIME_LUSGS && prm->lusgs_order == UFX_LUSGS_REFERENCE) {
     std::vector<int> lf(nop, 0), lb(nop, 0);
     int nlf = 0, nlb = 0;
     for (int p = 0; p < nop; ++p) {
@@ -502,9 +569,26 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       if (!rc) rc = nc_alloc(x, &x->d.sw_dd, np);
       if (!rc) rc = nc_alloc(x, &x->d.sw_dq, 4 * np);
     }
+    if (rc) { ufx_nc_destroy(x); return rc; }
+    // sweep form: one workgroup over the level-ordered data (default,
+    // UFX_NC_SWEEP=stream), a replayed graph of the level launches (graph),
+    // the level launches themselves (levels), one workgroup walking the
+    // packed slots (single; needs at most SWQ swept neighbours per vertex),
+    // or one cooperative launch of several workgroups (grid)
+    const char* env = std::getenv("UFX_NC_SWEEP");
+    x->sweep_mode = 4;
+    if (env && std::strcmp(env, "graph") == 0) x->sweep_mode = 0;
+    if (env && std::strcmp(env, "levels") == 0) x->sweep_mode = 1;
+    if (env && std::strcmp(env, "single") == 0 && fits) x->sweep_mode = 2;
+    if (env && std::strcmp(env, "grid") == 0) x->sweep_mode = 3;
+    if (env && std::strcmp(env, "stream") == 0) x->sweep_mode = 4;
+    if (const char* w = std::getenv("UFX_NC_SWEEP_WGS")) x->grid_wgs = std::max(1, std::min(64, std::atoi(w)));
+  }
+  {
+    int rc = 0;
     // multicolour sweeps: vertices grouped by colour, and per direction the
     // neighbours of lower (forward) / higher (reverse) colour in edge order
-    if (!rc && prm->timemode == UFX_TIME_LUSGS && prm->lusgs_order == UFX_LUSGS_MULTICOLOUR) {
+    if (prm->timemode == UFX_TIME_LUSGS && prm->lusgs_order == UFX_LUSGS_MULTICOLOUR) {
       int ncol = 0;
       std::vector<int> col = lusgs_colours(m, &ncol);
       x->mc_off.assign(ncol + 1, 0);
@@ -548,19 +632,6 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       if (!rc) rc = nc_upload(x, &x->d.mc_ds, dsv);
     }
     if (rc) { ufx_nc_destroy(x); return rc; }
-    // sweep form: one workgroup over the level-ordered data (default,
-    // UFX_NC_SWEEP=stream), a replayed graph of the level launches (graph),
-    // the level launches themselves (levels), one workgroup walking the
-    // packed slots (single; needs at most SWQ swept neighbours per vertex),
-    // or one cooperative launch of several workgroups (grid)
-    const char* env = std::getenv("UFX_NC_SWEEP");
-    x->sweep_mode = 4;
-    if (env && std::strcmp(env, "graph") == 0) x->sweep_mode = 0;
-    if (env && std::strcmp(env, "levels") == 0) x->sweep_mode = 1;
-    if (env && std::strcmp(env, "single") == 0 && fits) x->sweep_mode = 2;
-    if (env && std::strcmp(env, "grid") == 0) x->sweep_mode = 3;
-    if (env && std::strcmp(env, "stream") == 0) x->sweep_mode = 4;
-    if (const char* w = std::getenv("UFX_NC_SWEEP_WGS")) x->grid_wgs = std::max(1, std::min(64, std::atoi(w)));
   }
   DevNC& d = x->d;
   d.nop = nop;
@@ -568,8 +639,41 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   int rc = 0;
 #define NUP(dst, src) if (!rc) rc = nc_upload(x, &d.dst, src)
 #define NAL(dst, n) if (!rc) rc = nc_alloc(x, &d.dst, (size_t)(n))
-  NUP(vrec, vrec); NUP(qc, qc); NUP(cv, m.pcv); NUP(erec, erec); NUP(ebc, m.fbc);
-  NUP(v2f_ptr, m.v2f_ptr); NUP(v2f, v2f); NUP(v2v_ptr, m.v2v_ptr); NUP(v2v, m.v2v); NUP(wxy, wxy);
+  if (x->mixed) {
+    // the records in FP32 (the FP64 ones are not allocated)
+    std::vector<float> vr32((size_t)nop * NW32, 0.0f), er32(erec.size()), w32(wxy.size());
+    for (int p = 0; p < nop; ++p)
+      for (int k = 0; k < NW; ++k) vr32[(size_t)p * NW32 + k] = (float)vrec[(size_t)p * NW + k];
+    for (size_t i = 0; i < erec.size(); ++i) er32[i] = (float)erec[i];
+    for (size_t i = 0; i < wxy.size(); ++i) w32[i] = (float)wxy[i];
+    // primitives as deviations from the free stream (the state starts
+    // there: zero), and the open Green-Gauss sum's +-n ds per vertex
+    for (int p = 0; p < nop; ++p)
+      for (int k = 0; k < 4; ++k) vr32[(size_t)p * NW32 + R_QP + k] = 0.0f;
+    std::vector<float2> ggs(nop);
+    for (int p = 0; p < nop; ++p) {
+      double sx = 0.0, sy = 0.0;
+      for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
+        int e = m.v2f[j];
+        double dx = m.fnx[e] * m.farea[e], dy = m.fny[e] * m.farea[e];
+        if (m.fp1[e] == p) { sx = sx + dx; sy = sy + dy; }
+        else { sx = sx - dx; sy = sy - dy; }
+      }
+      ggs[p] = make_float2((float)sx, (float)sy);
+    }
+    NUP(vrec32, vr32); NUP(erec32, er32); NUP(wxy32, w32); NUP(ggs32, ggs);
+  } else {
+    NUP(vrec, vrec); NUP(erec, erec); NUP(wxy, wxy);
+  }
+  NUP(qc, qc); NUP(cv, m.pcv); NUP(ebc, m.fbc);
+  NUP(v2f_ptr, m.v2f_ptr); NUP(v2f, v2f); NUP(v2v_ptr, m.v2v_ptr); NUP(v2v, m.v2v);
+  if (!rc) {
+    std::vector<double2> pxy(nop);
+    for (int p = 0; p < nop; ++p) pxy[p] = make_double2(m.px[p], m.py[p]);
+    double2* q = nullptr;
+    rc = nc_upload(x, &q, pxy);
+    d.pxy = q;
+  }
   if (!rc) {
     int2* p = nullptr;
     rc = nc_upload(x, &p, ep);
@@ -701,12 +805,11 @@ int ufx_nc_set(ufx_nc* x, const char* name, const double* src) {
     NCCHK(hipMemcpyAsync(n == "qc" ? x->d.qc : x->d.qold, src, sizeof(double) * 4 * nop,
                          hipMemcpyHostToDevice, x->stream));
   } else if (n == "phi") {
-    std::vector<double> v((size_t)nop * NW);
-    NCCHK(hipMemcpyAsync(v.data(), x->d.vrec, v.size() * 8, hipMemcpyDeviceToHost, x->stream));
-    NCCHK(hipStreamSynchronize(x->stream));
+    std::vector<double> v;
+    if (int rc = read_records(x, v)) return rc;
     for (int p = 0; p < nop; ++p)
       for (int k = 0; k < 4; ++k) v[(size_t)p * NW + R_PHI + k] = src[p * 4 + k];
-    NCCHK(hipMemcpyAsync(x->d.vrec, v.data(), v.size() * 8, hipMemcpyHostToDevice, x->stream));
+    if (int rc = write_records(x, v)) return rc;
   } else {
     set_error("ufx_nc_set: unknown or read-only array " + n);
     return UFX_ERR_ARG;
@@ -732,9 +835,8 @@ int ufx_nc_get(ufx_nc* x, const char* name, double* dst) {
   if (n == "dt") return copy(x->d.dt, nop);
   if (n == "fc_la") return copy(x->d.fla, x->nof);
   if (n == "qp" || n == "grad" || n == "phi") {
-    std::vector<double> v((size_t)nop * NW);
-    NCCHK(hipMemcpyAsync(v.data(), x->d.vrec, v.size() * 8, hipMemcpyDeviceToHost, x->stream));
-    NCCHK(hipStreamSynchronize(x->stream));
+    std::vector<double> v;
+    if (int rc = read_records(x, v)) return rc;
     int off = n == "qp" ? R_QP : n == "grad" ? R_G : R_PHI, w = n == "grad" ? 8 : 4;
     for (int p = 0; p < nop; ++p)
       for (int k = 0; k < w; ++k) dst[(size_t)p * w + k] = v[(size_t)p * NW + off + k];

@@ -114,13 +114,15 @@ class NCSolver:
     SHAPES = {"qc": 4, "qold": 4, "qp": 4, "res": 4, "phi": 4, "grad": 8, "la": 1, "dt": 1}
 
     def __init__(self, mesh: NCMesh, *, mach=0.4, aoa_deg=0.0, cfl_max=50.0, grad="gg",
-                 timemode="lusgs", flux="vanleer", limiter="vkn", lusgs_order="reference"):
+                 timemode="lusgs", flux="vanleer", limiter="vkn", lusgs_order="reference",
+                 precision="fp64"):
         self.mesh = mesh
         self.counts = mesh.counts
         self.timemode = timemode
         self.cfl_max = cfl_max
         prm = NCParams(mach, aoa_deg, cfl_max, {"lsqr": 1, "gg": 2}[grad],
-                       {"rk3": 1, "lusgs": 2}[timemode], 0, {"ausm": 2, "vanleer": 3}[flux],
+                       {"rk3": 1, "lusgs": 2}[timemode], {"fp64": 0, "mixed": 1}[precision],
+                       {"ausm": 2, "vanleer": 3}[flux],
                        {"vkn": 0, "bj": 1}[limiter],
                        {"reference": 0, "multicolour": 1}[lusgs_order])
         self._h = _vp()
