@@ -226,7 +226,8 @@ def test_nc_wall_forces_match_oracle(flux):
 
 def test_stream_sweep_beyond_the_lds_ring(monkeypatch):
     """On a 180 000-vertex O-grid some swept neighbours lie more than the
-    LDS ring's 4 096 positions back (about 0.35 % of the forward reads), so
+    LDS ring's 5 120 positions back (0.21 % of the forward and 0.26 % of the
+    backward reads; test_nc_sweep_schedule.py counts them on the host), so
     the streamed sweep reads those from HBM and the rest from the ring; its
     iterates equal the level-launch form's (per-vertex records, no ring) bit
     for bit."""
