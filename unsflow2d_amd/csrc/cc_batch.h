@@ -93,7 +93,7 @@ __global__ __launch_bounds__(BLK) void km_stage(const CaseDesc<R>* cs, const int
   int c;
   const VBlock vb = case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  stage_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res);
+  stage_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, 0);
 }
 
 template <typename R, int FLUX, bool SL>
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(BLK) void km_stage_blk(const CaseDesc<R>* cs, const
   int c;
   const VBlock vb = case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  stage_blk_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res);
+  stage_blk_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, 0);
 }
 
 // one block per case; history slot k of every case

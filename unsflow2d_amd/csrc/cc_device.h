@@ -7,6 +7,10 @@
 //                        gradient/limiter pass reads the first 8 values and
 //                        the centroid and writes values 4..15 in one piece
 //   qT   [ncell] R       temperature (fifth primitive)
+//   qpc, qTc [noc] R     the interior primitives an iteration works from:
+//                        written by the primitive refresh or by the last
+//                        stage of the previous whole iteration, copied into
+//                        rec and qT by the gradient pass
 //   gT   [noc][2] R      its gradient (only when a consumer needs it)
 //   qc   [ncell][4] f64  conserved state [E, rho, rho u, rho v] (ghosts last)
 //   qold [noc][4] f64, res [noc][4] f64, dt/dta/la [noc] f64
@@ -92,6 +96,7 @@ struct DevT {
   R* rec;
   R* qpc;                    // [noc][4] the interior cells' record primitives, compact
   R* qT;
+  R* qTc;                    // [noc] the interior cells' temperatures beside qpc
   R* gT;
   double* qc;
   double* qold;
@@ -107,10 +112,13 @@ struct DevT {
   R* fgrad;                  // nof * 10 (GG_FC)
   double* bflux;             // [2][nbf]
   double* part;              // reduction partials
+  double* rpart;             // [blocks][8] block partials of the limiter's range (max q0..3, min q0..3)
   double* scal;              // SC_* slots
   unsigned long long* range; // [8]: max of q0..3 (ordered), min of q0..3
-  int* err;                  // [0] code, [1] unused, [2] lowest 1-based reference cell with
-                             // rho <= 0 or p <= 0, [3] lowest failing face key (set_err)
+  int* err;                  // [0] code, [1] and [2] lowest 1-based reference cell with
+                             // rho <= 0 or p <= 0 among the primitives the next iteration
+                             // starts from ([1], noted by the last stage) and the current
+                             // one started from ([2]), [3] lowest failing face key (set_err)
   double* lower;             // LU-SGS work [noc][4]
   double* upper;
 };

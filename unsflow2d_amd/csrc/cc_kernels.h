@@ -88,6 +88,7 @@ __device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<doubl
         stv<R, 4>(d.rec + (size_t)c * RW + R_QP, q4);
         stv<R, 4>(d.qpc + (size_t)c * 4, q4);
         d.qT[c] = (R)p[4];
+        d.qTc[c] = (R)p[4];
         // check_positivity (CCFVM/main.F90:384-411) runs after the
         // iteration's update on these start-of-iteration values; the
         // failing cell is noted here and becomes the error at the
@@ -126,8 +127,8 @@ __device__ __forceinline__ void prim_body(const DevT<R>& d, VBlock vb, Gas<doubl
         a = sh[k][w] > a ? sh[k][w] : a;
         b = sh[k][BLK / 64 + w] < b ? sh[k][BLK / 64 + w] : b;
       }
-      d.part[(size_t)vb.b * 8 + k] = a;
-      d.part[(size_t)vb.b * 8 + 4 + k] = b;
+      d.rpart[(size_t)vb.b * 8 + k] = a;
+      d.rpart[(size_t)vb.b * 8 + 4 + k] = b;
     }
 }
 template <typename R>
@@ -144,7 +145,7 @@ __device__ __forceinline__ void range_final_body(const DevT<R>& d, VBlock vb, in
   for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
   // 64 bytes per block partial, four of them in flight per thread (max and
   // min do not depend on the order)
-  const double2* p2 = reinterpret_cast<const double2*>(d.part);
+  const double2* p2 = reinterpret_cast<const double2*>(d.rpart);
 #pragma unroll 4
   for (int b = threadIdx.x; b < nblk; b += BLK) {
     const double2 v0 = p2[(size_t)b * 4], v1 = p2[(size_t)b * 4 + 1];
@@ -191,7 +192,7 @@ __device__ __forceinline__ void avg_c2v_body(const DevT<R>& d, VBlock vb) {
     R q[4];
     ldv<R, 4>(d.qpc + (size_t)(w0 + i) * 4, q);
     stv<R, 4>(s_q + i * 4, q);
-    s_t[i] = d.qT[w0 + i];
+    s_t[i] = d.qTc[w0 + i];
     if (VISC) s_mu[i] = d.cmul[w0 + i];
   }
   __syncthreads();
@@ -221,7 +222,7 @@ __device__ __forceinline__ void avg_c2v_body(const DevT<R>& d, VBlock vb) {
     } else {
       if (c < d.noc) ldv<R, 4>(d.qpc + (size_t)c * 4, q);
       else ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q);
-      t = d.qT[c];
+      t = c < d.noc ? d.qTc[c] : d.qT[c];
       if (VISC) mu = d.cmul[c];
     }
     w = w + cw;
@@ -255,7 +256,7 @@ __device__ __forceinline__ void range_final_small(const DevT<R>& d, int nblk) {
   __shared__ double shw[8][BLK / 64];
   double mx[4], mn[4];
   for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
-  const double2* p2 = reinterpret_cast<const double2*>(d.part);
+  const double2* p2 = reinterpret_cast<const double2*>(d.rpart);
 #pragma unroll 1
   for (int b = threadIdx.x; b < nblk; b += BLK) {
     const double2 v0 = p2[(size_t)b * 4], v1 = p2[(size_t)b * 4 + 1];
@@ -340,14 +341,15 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
   R own[RW];
   R tc = R(0);
   if (c < d.noc) {
-    // the fields this kernel reads: primitives, limiter values, centroid,
+    // the fields this kernel reads: primitives (the compact copy, which the
+    // last stage of a whole iteration refreshes), limiter values, centroid,
     // and the old gradient only for a limiter-only call
     const R* rc = d.rec + (size_t)c * RW;
-    static_assert(R_PHI == R_QP + 4, "primitives and limiter values are adjacent");
-    ldv<R, 8>(rc + R_QP, own + R_QP);
+    ldv<R, 4>(d.qpc + (size_t)c * 4, own + R_QP);
+    ldv<R, 4>(rc + R_PHI, own + R_PHI);
     ldv<R, 2>(rc + R_CEN, own + R_CEN);
     if (GRAD == G_NONE) ldv<R, 8>(rc + R_G, own + R_G);
-    if (WITH_T) tc = d.qT[c];
+    tc = d.qTc[c];
   }
   if (SL) {
     if (c < d.noc) {
@@ -411,6 +413,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
     const bool loc = SL && l < (unsigned)cnt;
     if (GRAD == G_LSQR || LIMIT) {
       if (loc) ldv<R, 4>(s_qp + l * 4, nq);
+      else if (!bnd) ldv<R, 4>(d.qpc + (size_t)n * 4, nq);
       else ldv<R, 4>(d.rec + (size_t)n * RW + R_QP, nq);
     }
     if (GRAD == G_LSQR) {
@@ -421,7 +424,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
       R alfa1 = dx / r11 / r11;
       R alfa2 = bnd ? (dy - dx * r12 / r11) / r22 / r22 : (dy - dx * r12 / r11) / (r22 * r22);
       R wx = alfa1 - alfa2 * r12 / r11, wy = alfa2;
-      R nt = WITH_T ? (loc ? s_t[l] : d.qT[n]) : R(0);
+      R nt = WITH_T ? (loc ? s_t[l] : (bnd ? d.qT[n] : d.qTc[n])) : R(0);
       if (in_side) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -514,8 +517,13 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
       }
     }
   }
-  static_assert(R_G == R_PHI + 4, "limiter values and gradient are adjacent");
-  if (GRAD != G_NONE || LIMIT) stv<R, 12>(d.rec + (size_t)c * RW + R_PHI, own + R_PHI);
+  // the record's primitives are refreshed here, with the limiter values
+  // and the gradient, for the stages' neighbour gathers
+  static_assert(R_PHI == R_QP + 4 && R_G == R_PHI + 4, "record fields are adjacent");
+  if (GRAD != G_NONE || LIMIT) {
+    stv<R, 16>(d.rec + (size_t)c * RW + R_QP, own + R_QP);
+    d.qT[c] = tc;
+  }
   if (TSTEP) {
     d.la[c] = la;
     double dt = cfl * d.cv[c] / la;
@@ -610,89 +618,6 @@ __device__ __forceinline__ bool face_flux(const Gas<R>& g, R* ql, R* qr, R nx, R
   return true;
 }
 
-// Boundary faces of a residual evaluation, CCFVM/fvresidual.F90:27-130, one
-// thread per boundary face (in double): left state by the full-step
-// reconstruction of the inner cell, farfield state or wall ghost (built
-// from the inner cell's current conserved state), flux, ghost update and,
-// for walls, the momentum flux of the force sums.  The flux is stored for
-// k_stage, which adds it first, as the reference does.
-template <typename R, int FLUX>
-__device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<double> g,
-                                           int viscous_wall, int iwall, double t_wall, int src) {
-  constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
-  int k = vb.b * BLK + threadIdx.x;
-  if (k >= d.nbf || d.err[0]) return;
-  int bc = d.fbc[k];
-  double F[4] = {0.0, 0.0, 0.0, 0.0};
-  if (bc == FARF || bc == WALL) {
-    int f = d.bface[k], c = d.bcell[k], gh = d.noc + k;
-    R own[RW], fr[FW];
-    ldv<R, RW>(d.rec + (size_t)c * RW, own);
-    face_rec(d, f, fr);
-    double nx = fr[F_NX], ny = fr[F_NY], area = fr[F_AREA];
-    double dx = (double)fr[F_CX] - (double)own[R_CEN], dy = (double)fr[F_CY] - (double)own[R_CEN + 1];
-    double ql[4], qr[4], gq[4], gp[5];
-    for (int i = 0; i < 4; ++i)
-      ql[i] = (double)own[R_QP + i] +
-              (double)own[R_PHI + i] * ((double)own[R_G + 2 * i] * dx + (double)own[R_G + 2 * i + 1] * dy);
-    bool ok;
-    if (bc == FARF) {
-      farfield_state(g, ql, nx, ny, qr);
-      ok = face_flux<double, FLUX>(g, ql, qr, nx, ny, area, F);
-      prim2con(g, qr, gq);
-      con2prim(g, gq, gp);
-    } else {
-      double q[4];
-      ld4d(stage_in(d, src) + (size_t)c * 4, q);
-      if (!viscous_wall) {
-        // slip_wall, CCFVM/boundCond.F90:46-81
-        double rho = q[1], u = q[2] / q[1], v = q[3] / q[1];
-        double un = u * nx + v * ny;
-        gq[0] = q[0];
-        gq[1] = q[1];
-        gq[2] = rho * u - 2.0 * un * nx;
-        gq[3] = rho * v - 2.0 * un * ny;
-        con2prim(g, gq, gp);
-      } else if (iwall == 1) {
-        // adiabatic no-slip, CCFVM/boundCond.F90:31-37
-        gq[0] = q[0];
-        gq[1] = q[1];
-        gq[2] = -q[2];
-        gq[3] = -q[3];
-        con2prim(g, gq, gp);
-      } else {
-        // isothermal no-slip, CCFVM/boundCond.F90:19-30 (the ghost density
-        // slot receives p/(rho znd), as in the reference)
-        double pin[5];
-        con2prim(g, q, pin);
-        double tg = 2.0 * t_wall - pin[4];
-        if (tg <= 0.0) tg = 0.01 * t_wall;
-        gp[0] = pin[3] / (pin[0] * g.znd);
-        gp[1] = -pin[1];
-        gp[2] = -pin[2];
-        gp[3] = pin[3];
-        gp[4] = tg;
-        prim2con(g, gp, gq);
-      }
-      for (int i = 0; i < 4; ++i) qr[i] = gp[i];
-      ok = face_flux<double, FLUX>(g, ql, qr, nx, ny, area, F);
-      d.bflux[k] = F[2];
-      d.bflux[d.nbf + k] = F[3];
-    }
-    if (!ok) set_err(d, DERR_ROE, k);
-    st4d(d.qc + (size_t)gh * 4, gq);
-    R gp4[4] = {(R)gp[0], (R)gp[1], (R)gp[2], (R)gp[3]};
-    stv<R, 4>(d.rec + (size_t)gh * RW + R_QP, gp4);
-    d.qT[gh] = (R)gp[4];
-  }
-  st4d(d.bres + (size_t)k * 4, F);
-}
-template <typename R, int FLUX>
-__global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int viscous_wall,
-                                               int iwall, double t_wall, int src) {
-  bface_body<R, FLUX>(d, hw_block(), g, viscous_wall, iwall, t_wall, src);
-}
-
 // Where a stage update reads and writes its states.  src 0 (routine by
 // routine): q = a qold + b (qc - dt/A R) in qc.  Whole RK3 iterations keep
 // the iteration's start state in qc and the stage states in qs, so no copy
@@ -710,6 +635,108 @@ __device__ __forceinline__ const double* stage_in(const DevT<R>& d, int src) {
 template <typename R>
 __device__ __forceinline__ double* stage_out(const DevT<R>& d, int src) {
   return src == 1 || src == 2 ? d.qs : d.qc;
+}
+
+// The boundary conditions of a residual evaluation.
+struct BndOpts {
+  Gas<double> g;
+  int viscous_wall, iwall;
+  double t_wall;
+};
+
+// Boundary face k of a residual evaluation, CCFVM/fvresidual.F90:27-130, in
+// double: left state by the full-step reconstruction of the inner cell
+// (record `own`), farfield state or wall ghost (built from the inner cell's
+// current conserved state), flux F, ghost update and, for walls, the
+// momentum flux of the force sums.  F = 0 for other boundary kinds.
+template <typename R, int FLUX>
+__device__ __forceinline__ void bface_one(const DevT<R>& d, const BndOpts& bo, int k,
+                                          const R* own, int src, double* F) {
+  constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
+  const Gas<double>& g = bo.g;
+  int bc = d.fbc[k];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) F[i] = 0.0;
+  if (bc != FARF && bc != WALL) return;
+  int f = d.bface[k], c = d.bcell[k], gh = d.noc + k;
+  R fr[FW];
+  face_rec(d, f, fr);
+  double nx = fr[F_NX], ny = fr[F_NY], area = fr[F_AREA];
+  double dx = (double)fr[F_CX] - (double)own[R_CEN], dy = (double)fr[F_CY] - (double)own[R_CEN + 1];
+  double ql[4], qr[4], gq[4], gp[5];
+  for (int i = 0; i < 4; ++i)
+    ql[i] = (double)own[R_QP + i] +
+            (double)own[R_PHI + i] * ((double)own[R_G + 2 * i] * dx + (double)own[R_G + 2 * i + 1] * dy);
+  bool ok;
+  if (bc == FARF) {
+    farfield_state(g, ql, nx, ny, qr);
+    ok = face_flux<double, FLUX>(g, ql, qr, nx, ny, area, F);
+    prim2con(g, qr, gq);
+    con2prim(g, gq, gp);
+  } else {
+    double q[4];
+    ld4d(stage_in(d, src) + (size_t)c * 4, q);
+    if (!bo.viscous_wall) {
+      // slip_wall, CCFVM/boundCond.F90:46-81
+      double rho = q[1], u = q[2] / q[1], v = q[3] / q[1];
+      double un = u * nx + v * ny;
+      gq[0] = q[0];
+      gq[1] = q[1];
+      gq[2] = rho * u - 2.0 * un * nx;
+      gq[3] = rho * v - 2.0 * un * ny;
+      con2prim(g, gq, gp);
+    } else if (bo.iwall == 1) {
+      // adiabatic no-slip, CCFVM/boundCond.F90:31-37
+      gq[0] = q[0];
+      gq[1] = q[1];
+      gq[2] = -q[2];
+      gq[3] = -q[3];
+      con2prim(g, gq, gp);
+    } else {
+      // isothermal no-slip, CCFVM/boundCond.F90:19-30 (the ghost density
+      // slot receives p/(rho znd), as in the reference)
+      double pin[5];
+      con2prim(g, q, pin);
+      double tg = 2.0 * bo.t_wall - pin[4];
+      if (tg <= 0.0) tg = 0.01 * bo.t_wall;
+      gp[0] = pin[3] / (pin[0] * g.znd);
+      gp[1] = -pin[1];
+      gp[2] = -pin[2];
+      gp[3] = pin[3];
+      gp[4] = tg;
+      prim2con(g, gp, gq);
+    }
+    for (int i = 0; i < 4; ++i) qr[i] = gp[i];
+    ok = face_flux<double, FLUX>(g, ql, qr, nx, ny, area, F);
+    d.bflux[k] = F[2];
+    d.bflux[d.nbf + k] = F[3];
+  }
+  if (!ok) set_err(d, DERR_ROE, k);
+  st4d(d.qc + (size_t)gh * 4, gq);
+  R gp4[4] = {(R)gp[0], (R)gp[1], (R)gp[2], (R)gp[3]};
+  stv<R, 4>(d.rec + (size_t)gh * RW + R_QP, gp4);
+  d.qT[gh] = (R)gp[4];
+}
+
+// The boundary faces alone, one thread per face; the flux is stored for the
+// stage kernel, which adds it first, as the reference does.
+template <typename R, int FLUX>
+__device__ __forceinline__ void bface_body(const DevT<R>& d, VBlock vb, Gas<double> g,
+                                           int viscous_wall, int iwall, double t_wall, int src) {
+  constexpr int RW = Layout<R>::RW;
+  int k = vb.b * BLK + threadIdx.x;
+  if (k >= d.nbf || d.err[0]) return;
+  const BndOpts bo{g, viscous_wall, iwall, t_wall};
+  R own[RW];
+  ldv<R, RW>(d.rec + (size_t)d.bcell[k] * RW, own);
+  double F[4];
+  bface_one<R, FLUX>(d, bo, k, own, src, F);
+  st4d(d.bres + (size_t)k * 4, F);
+}
+template <typename R, int FLUX>
+__global__ __launch_bounds__(BLK) void k_bface(DevT<R> d, Gas<double> g, int viscous_wall,
+                                               int iwall, double t_wall, int src) {
+  bface_body<R, FLUX>(d, hw_block(), g, viscous_wall, iwall, t_wall, src);
 }
 
 // Copy the records of the block's cells [base, base + cnt) into LDS with
@@ -734,10 +761,152 @@ __device__ __forceinline__ void load_rec(const DevT<R>& d, const R* srec, int ba
   else ldv<R, RW>(d.rec + (size_t)n * RW, out);
 }
 
-// Residual of one cell, CCFVM/fvresidual.F90:16-171, as a gather over its
+// Residual of cell c, CCFVM/fvresidual.F90:16-171, as a gather over its
 // faces: the boundary-face fluxes from k_bface first, then the interior
 // faces with recon (:190-252) on both sides, each flux evaluated from the
-// same inputs by both adjacent cells.
+// same inputs by both adjacent cells.  Returns the lowest failing face key
+// (set_err), INT_MAX when every flux succeeded.  SL: records of the block's
+// cells [base, base + BLK) are read from srec.
+template <typename R, int FLUX, bool SL>
+__device__ __forceinline__ int gather_residual(const DevT<R>& d, const R* srec, int base, int c,
+                                               const Gas<R>& gr, double* r) {
+  constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
+  const int noc = d.noc;
+  int info = ld_nt(d.cnf + c), nf = info & 0xff, nb = info >> 8;
+  R own[RW];
+  load_rec<R, SL>(d, srec, base, c, own);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) r[i] = 0.0;
+  int ekey = INT_MAX;
+  for (int s = nf - nb; s < nf; ++s) {
+    int k = ld_nt(d.sl_nbr + s * noc + c) - noc;
+    double F[4];
+    ld4d(d.bres + (size_t)k * 4, F);
+    for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i];
+  }
+  // interior faces, CCFVM/fvresidual.F90:133-171
+  const R zhi = R(1) / R(3);
+  int sf[MAXF], sn[MAXF];
+#pragma unroll
+  for (int s = 0; s < MAXF; ++s)
+    if (s < nf - nb) {
+      sf[s] = ld_nt(d.sl_face + s * noc + c);
+      sn[s] = ld_nt(d.sl_nbr + s * noc + c);
+    }
+#pragma unroll
+  for (int s = 0; s < MAXF; ++s) {
+    if (s >= nf - nb) break;
+    int f = sf[s];
+    int n = sn[s];
+    bool in_side = n >= 0;
+    n = in_side ? n : ~n;
+    R nr[RW], fr[FW];
+    load_rec<R, SL>(d, srec, base, n, nr);
+    face_rec(d, f, fr);
+    R ql[4], qr[4], F[4];
+    // side a = the face's `in` cell, side b = its `out` cell
+    R ax = in_side ? own[R_CEN] : nr[R_CEN], ay = in_side ? own[R_CEN + 1] : nr[R_CEN + 1];
+    R bx = in_side ? nr[R_CEN] : own[R_CEN], by = in_side ? nr[R_CEN + 1] : own[R_CEN + 1];
+    R dlx = fr[F_CX] - ax, dly = fr[F_CY] - ay, drx = fr[F_CX] - bx, dry = fr[F_CY] - by;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      R qa = in_side ? own[R_QP + i] : nr[R_QP + i];
+      R qb = in_side ? nr[R_QP + i] : own[R_QP + i];
+      R gax = in_side ? own[R_G + 2 * i] : nr[R_G + 2 * i];
+      R gay = in_side ? own[R_G + 2 * i + 1] : nr[R_G + 2 * i + 1];
+      R gbx = in_side ? nr[R_G + 2 * i] : own[R_G + 2 * i];
+      R gby = in_side ? nr[R_G + 2 * i + 1] : own[R_G + 2 * i + 1];
+      R pa = in_side ? own[R_PHI + i] : nr[R_PHI + i];
+      R pb = in_side ? nr[R_PHI + i] : own[R_PHI + i];
+      R alfa = gax * dlx + gay * dly;
+      R beta = qb - qa;
+      ql[i] = qa + R(0.5) * pa * (zhi * beta + (R(1) - zhi) * alfa);
+      alfa = gbx * drx + gby * dry;
+      beta = qa - qb;
+      qr[i] = qb + R(0.5) * pb * (zhi * beta + (R(1) - zhi) * alfa);
+    }
+    if (!face_flux<R, FLUX>(gr, ql, qr, fr[F_NX], fr[F_NY], fr[F_AREA], F))
+      ekey = min(ekey, d.nbf + d.fref[f]);
+    if (in_side) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = r[i] + (double)F[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = r[i] - (double)F[i];
+    }
+  }
+  return ekey;
+}
+
+// The primitive refresh of the next iteration (CCFVM/main.F90:62-67), done
+// by the last stage of a whole RK3 iteration from the state it has just
+// written, so that the next iteration needs no pass of its own for it: the
+// compact primitives and temperature qpc, qTc (the gradient pass copies them
+// into the records and qT, which this stage's other cells are still
+// gathering, and which stay the reference's qp until then), the
+// positivity note for the next iteration (err[1], moved up by the norm) and
+// the block's partial of the limiter's global range over its cells and the
+// ghosts of their boundary faces (written by this stage's k_bface).  Called
+// by every thread of the block; `have`: the thread's cell is interior and
+// its new state is q.
+template <typename R>
+__device__ __forceinline__ void prim_epilogue(const DevT<R>& d, VBlock vb, const Gas<double>& g,
+                                              int c, bool have, const double* q) {
+  constexpr int RW = Layout<R>::RW;
+  __shared__ double shr[8][BLK / 64];
+  double mx[4], mn[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { mx[k] = -INFINITY; mn[k] = INFINITY; }
+  if (have) {
+    double p[5];
+    con2prim(g, q, p);
+    R q4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q4[k] = (R)p[k];
+    stv<R, 4>(d.qpc + (size_t)c * 4, q4);
+    d.qTc[c] = (R)p[4];
+    if (p[0] <= 0.0 || p[3] <= 0.0) atomicMin(&d.err[1], d.cref[c] + 1);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mx[k] = mn[k] = (double)q4[k];
+    const int info = ld_nt(d.cnf + c), nf = info & 0xff, nb = info >> 8;
+    for (int s = nf - nb; s < nf; ++s) {
+      const int gh = ld_nt(d.sl_nbr + s * d.noc + c);
+      R gq[4];
+      ldv<R, 4>(d.rec + (size_t)gh * RW + R_QP, gq);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double v = (double)gq[k];
+        mx[k] = v > mx[k] ? v : mx[k];
+        mn[k] = v < mn[k] ? v : mn[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double a = __shfl_xor(mx[k], off), b = __shfl_xor(mn[k], off);
+      mx[k] = a > mx[k] ? a : mx[k];
+      mn[k] = b < mn[k] ? b : mn[k];
+    }
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      shr[k][wave] = mx[k];
+      shr[4 + k][wave] = mn[k];
+    }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    const int k = threadIdx.x;
+    double a = shr[k][0];
+    for (int w = 1; w < BLK / 64; ++w)
+      a = k < 4 ? (shr[k][w] > a ? shr[k][w] : a) : (shr[k][w] < a ? shr[k][w] : a);
+    d.rpart[(size_t)vb.b * 8 + k] = a;
+  }
+}
+
+// Residual stage over the block's cells (gather_residual), then
 //   mode & 3 = 0: store the residual only;
 //              1: apply the stage update of CCFVM/main.F90:92-101 (no other
 //                 cell reads this cell's state during a stage);
@@ -745,12 +914,13 @@ __device__ __forceinline__ void load_rec(const DevT<R>& d, const R* srec, int ba
 //   mode >> 2 (src): where the update's states live, stage_old/in/out.
 // SL: the block's records are first copied to LDS, so the neighbours inside
 // the block (most of them, the cells being Hilbert-ordered) are read from
-// there instead of through L2.
+// there instead of through L2.  epi: the last stage of a whole iteration
+// also refreshes the primitives (prim_epilogue).
 template <typename R, int FLUX, bool SL>
 __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
-                                           double ak, double bk, int write_res) {
+                                           double ak, double bk, int write_res,
+                                           const Gas<double>& g, int epi) {
   const int op = mode & 3, src = mode >> 2;
-  constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);  // [BLK][RW] when SL
   __shared__ double sh[1][BLK];
@@ -760,75 +930,15 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
     stage_block_recs(d, srec, base, min(BLK, d.noc - base));
     __syncthreads();
   }
-  double sq = 0.0;
+  double sq = 0.0, q[4] = {0.0, 0.0, 0.0, 0.0};
+  bool have = false;
   if (c < d.noc && !d.err[0]) {
-    const int noc = d.noc;
-    int info = ld_nt(d.cnf + c), nf = info & 0xff, nb = info >> 8;
-    R own[RW];
-    load_rec<R, SL>(d, srec, base, c, own);
-    double r[4] = {0.0, 0.0, 0.0, 0.0};
-    int ekey = INT_MAX;  // lowest failing face key (set_err)
-    for (int s = nf - nb; s < nf; ++s) {
-      int k = ld_nt(d.sl_nbr + s * noc + c) - noc;
-      double F[4];
-      ld4d(d.bres + (size_t)k * 4, F);
-      for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i];
-    }
-    // interior faces, CCFVM/fvresidual.F90:133-171
-    const R zhi = R(1) / R(3);
-    int sf[MAXF], sn[MAXF];
-#pragma unroll
-    for (int s = 0; s < MAXF; ++s)
-      if (s < nf - nb) {
-        sf[s] = ld_nt(d.sl_face + s * noc + c);
-        sn[s] = ld_nt(d.sl_nbr + s * noc + c);
-      }
-#pragma unroll
-    for (int s = 0; s < MAXF; ++s) {
-      if (s >= nf - nb) break;
-      int f = sf[s];
-      int n = sn[s];
-      bool in_side = n >= 0;
-      n = in_side ? n : ~n;
-      R nr[RW], fr[FW];
-      load_rec<R, SL>(d, srec, base, n, nr);
-      face_rec(d, f, fr);
-      R ql[4], qr[4], F[4];
-      // side a = the face's `in` cell, side b = its `out` cell
-      R ax = in_side ? own[R_CEN] : nr[R_CEN], ay = in_side ? own[R_CEN + 1] : nr[R_CEN + 1];
-      R bx = in_side ? nr[R_CEN] : own[R_CEN], by = in_side ? nr[R_CEN + 1] : own[R_CEN + 1];
-      R dlx = fr[F_CX] - ax, dly = fr[F_CY] - ay, drx = fr[F_CX] - bx, dry = fr[F_CY] - by;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        R qa = in_side ? own[R_QP + i] : nr[R_QP + i];
-        R qb = in_side ? nr[R_QP + i] : own[R_QP + i];
-        R gax = in_side ? own[R_G + 2 * i] : nr[R_G + 2 * i];
-        R gay = in_side ? own[R_G + 2 * i + 1] : nr[R_G + 2 * i + 1];
-        R gbx = in_side ? nr[R_G + 2 * i] : own[R_G + 2 * i];
-        R gby = in_side ? nr[R_G + 2 * i + 1] : own[R_G + 2 * i + 1];
-        R pa = in_side ? own[R_PHI + i] : nr[R_PHI + i];
-        R pb = in_side ? nr[R_PHI + i] : own[R_PHI + i];
-        R alfa = gax * dlx + gay * dly;
-        R beta = qb - qa;
-        ql[i] = qa + R(0.5) * pa * (zhi * beta + (R(1) - zhi) * alfa);
-        alfa = gbx * drx + gby * dry;
-        beta = qa - qb;
-        qr[i] = qb + R(0.5) * pb * (zhi * beta + (R(1) - zhi) * alfa);
-      }
-      if (!face_flux<R, FLUX>(gr, ql, qr, fr[F_NX], fr[F_NY], fr[F_AREA], F))
-        ekey = min(ekey, d.nbf + d.fref[f]);
-      if (in_side) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) r[i] = r[i] + (double)F[i];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) r[i] = r[i] - (double)F[i];
-      }
-    }
+    double r[4];
+    const int ekey = gather_residual<R, FLUX, SL>(d, srec, base, c, gr, r);
     if (ekey != INT_MAX) set_err(d, DERR_ROE, ekey);
     if (op == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
     if (op >= 1) {
-      double dta = ld_nt(d.dta + c), q[4], o[4];
+      double dta = ld_nt(d.dta + c), o[4];
       ld4d_nt(stage_old(d, src) + (size_t)c * 4, o);
       if (src == 1) {
 #pragma unroll
@@ -839,6 +949,7 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
       st4d_nt(stage_out(d, src) + (size_t)c * 4, q);
+      have = true;
     }
     if (op == 2)
       for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];
@@ -847,11 +958,12 @@ __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> g
     block_reduce<1>(sh, &sq, OpAdd());
     if (threadIdx.x == 0) d.part[vb.b] = sq;
   }
+  if (epi) prim_epilogue<R>(d, vb, g, c, have, q);
 }
 template <typename R, int FLUX, bool SL>
 __global__ __launch_bounds__(BLK) void k_stage(DevT<R> d, Gas<R> gr, int mode, double ak, double bk,
-                                               int write_res) {
-  stage_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res);
+                                               int write_res, Gas<double> g, int epi) {
+  stage_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res, g, epi);
 }
 
 // Flux through interior face `f` from the records of its `in` cell (a) and
@@ -887,7 +999,8 @@ __device__ __forceinline__ bool face_flux_ab(const Gas<R>& gr, const R* ra, cons
 // LDS: [BLK][RW] records when SL, then [bf_max][4] fluxes.
 template <typename R, int FLUX, bool SL>
 __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
-                                               double ak, double bk, int write_res) {
+                                               double ak, double bk, int write_res,
+                                               const Gas<double>& g, int epi) {
   const int op = mode & 3, src = mode >> 2;
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
@@ -914,7 +1027,8 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
   }
   __syncthreads();
   const int c = base + threadIdx.x;
-  double sq = 0.0;
+  double sq = 0.0, q[4] = {0.0, 0.0, 0.0, 0.0};
+  bool have = false;
   if (run && c < d.noc) {
     const int noc = d.noc;
     int info = ld_nt(d.cnf + c), nf = info & 0xff, nb = info >> 8;
@@ -945,7 +1059,7 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
     }
     if (op == 0 || write_res) st4d_nt(d.res + (size_t)c * 4, r);
     if (op >= 1) {
-      double dta = ld_nt(d.dta + c), q[4], o[4];
+      double dta = ld_nt(d.dta + c), o[4];
       ld4d_nt(stage_old(d, src) + (size_t)c * 4, o);
       if (src == 1) {
 #pragma unroll
@@ -956,6 +1070,7 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
 #pragma unroll
       for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, o[i], bk, q[i], dta, r[i]);
       st4d_nt(stage_out(d, src) + (size_t)c * 4, q);
+      have = true;
     }
     if (op == 2)
       for (int i = 0; i < 4; ++i) sq = sq + r[i] * r[i];
@@ -964,11 +1079,13 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
     block_reduce<1>(sh, &sq, OpAdd());
     if (threadIdx.x == 0) d.part[vb.b] = sq;
   }
+  if (epi) prim_epilogue<R>(d, vb, g, c, have, q);
 }
 template <typename R, int FLUX, bool SL>
 __global__ __launch_bounds__(BLK) void k_stage_blk(DevT<R> d, Gas<R> gr, int mode, double ak,
-                                                   double bk, int write_res) {
-  stage_blk_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res);
+                                                   double bk, int write_res, Gas<double> g,
+                                                   int epi) {
+  stage_blk_body<R, FLUX, SL>(d, hw_block(), gr, mode, ak, bk, write_res, g, epi);
 }
 
 // Sutherland viscosity of the interior cells from the current state,
@@ -1146,8 +1263,15 @@ __device__ __forceinline__ void norm_final_body(const DevT<R>& d, VBlock vb, int
   for (int b = threadIdx.x; b < nblk; b += BLK) s = s + d.part[b];
   block_reduce<1>(sh, &s, OpAdd());
   if (threadIdx.x == 0) {
-    // the positivity check of the iteration, CCFVM/main.F90:113
+    // the positivity check of the iteration, CCFVM/main.F90:113, on the
+    // primitives it started from (err[2], kept for the report); the note
+    // taken on the primitives the next iteration starts from (err[1], when
+    // the last stage refreshed them) moves up
     if (d.err[0] == 0 && d.err[2] != INT_MAX) d.err[0] = DERR_POS;
+    if (d.err[0] == 0) {
+      d.err[2] = d.err[1];
+      d.err[1] = INT_MAX;
+    }
     double fres = sqrt(s / d.noc / 4);
     d.scal[SC_NORM] = fres;
     if (first) d.scal[SC_FRES1] = fres;
@@ -1158,6 +1282,23 @@ __device__ __forceinline__ void norm_final_body(const DevT<R>& d, VBlock vb, int
 template <typename R>
 __global__ void k_norm_final(DevT<R> d, int nblk, int first, double* hist, int k) {
   norm_final_body<R>(d, hw_block(), nblk, first, hist, k);
+}
+
+// The working primitives set back to the records' (the reference's current
+// qp and temperature), and the positivity note taken ahead on the ones the
+// next iteration would have started from dropped (a primitive refresh notes
+// it again), before a routine call after a whole iteration that refreshed
+// them ahead.
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_prims_current(DevT<R> d) {
+  constexpr int RW = Layout<R>::RW;
+  int c = blockIdx.x * BLK + threadIdx.x;
+  if (c == 0) d.err[2] = INT_MAX;
+  if (c >= d.noc) return;
+  R q[4];
+  ldv<R, 4>(d.rec + (size_t)c * RW + R_QP, q);
+  stv<R, 4>(d.qpc + (size_t)c * 4, q);
+  d.qTc[c] = d.qT[c];
 }
 
 template <typename R>

@@ -49,6 +49,7 @@ struct Chunk {
   double cfl = 0.0;
   int keep = -1;        // residual kept (batched chunks)
   bool stale = true;    // its kernel arguments no longer describe the cases
+  bool prims_next = false;  // a replay leaves the next iteration's primitives in qpc
 };
 
 // Release a chunk; the first failing call's error is returned.
@@ -90,6 +91,9 @@ struct ufx_cc {
   Chunk chunk;
   bool keep_res = false;           // whole iterations also store the last residual
   bool qold_stale = false;         // fused RK3 iterations ran since qold was last written
+  bool prims_next = false;         // qpc/qTc hold the next iteration's primitives (written by
+                                   // the last stage of a whole iteration; see prim_epilogue)
+  bool prim_epi = true;            // whole RK3 iterations refresh them in their last stage
   unsigned prof_mask = ~0u;        // kernel categories timed while profiling
   double* d_out = nullptr;         // small device results (residual norms)
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
@@ -333,6 +337,23 @@ int run_prim(ufx_cc* x, DevT<R>& d, int do_prim, int do_old, int do_range) {
   return launch_check();
 }
 
+// Before a routine call: the working primitives back to the current ones
+// when a whole iteration refreshed them ahead (k_prims_current).
+int prims_current(ufx_cc* x) {
+  if (!x->prims_next) return UFX_OK;
+  x->prims_next = false;
+  return with_dev(x, [&](auto& d) {
+    using R = typename std::remove_reference_t<decltype(d)>::Real;
+    LAUNCH(x, UFX_K_OTHER, k_prims_current<R>, x->nblk_cell, d);
+    return launch_check();
+  });
+}
+
+int check_routine(ufx_cc* x) {
+  if (int rc = check_ctx(x)) return rc;
+  return prims_current(x);
+}
+
 template <typename R>
 int run_range_final(ufx_cc* x, DevT<R>& d) {
   LAUNCH(x, UFX_K_RANGE, k_range_final<R>, 1, d, x->nblk_all);
@@ -355,11 +376,11 @@ int run_avg(ufx_cc* x, DevT<R>& d) {
 // avg_c2v with the limiter's range reduction riding along (k_prim's block
 // partials must be complete, as for run_range_final).
 template <typename R>
-int run_avg_range(ufx_cc* x, DevT<R>& d) {
+int run_avg_range(ufx_cc* x, DevT<R>& d, int nparts) {
   if (x->p.flow != UFX_FLOW_INVISCID)
-    LAUNCH(x, UFX_K_AVG, (k_avg_c2v_range<R, true>), x->nblk_pt + 1, d, x->nblk_all);
+    LAUNCH(x, UFX_K_AVG, (k_avg_c2v_range<R, true>), x->nblk_pt + 1, d, nparts);
   else
-    LAUNCH(x, UFX_K_AVG, (k_avg_c2v_range<R, false>), x->nblk_pt + 1, d, x->nblk_all);
+    LAUNCH(x, UFX_K_AVG, (k_avg_c2v_range<R, false>), x->nblk_pt + 1, d, nparts);
   return launch_check();
 }
 
@@ -407,7 +428,8 @@ int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, dou
 }
 
 template <typename R>
-int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int write_res) {
+int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int write_res,
+                     int epi = 0) {
   const int src = mode >> 2;
   int visc = x->p.flow != UFX_FLOW_INVISCID;
   const Gas<R>& gr = gas_of<R>(x);
@@ -418,16 +440,16 @@ int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int 
       LAUNCH(x, UFX_K_BFACE, (k_bface<R, F>), nbb, d, x->g, visc, x->p.iwall, x->p.t_wall, src); \
     if (x->stage_shared && x->stage_local)                                                      \
       LAUNCH_LDS(x, UFX_K_STAGE, (k_stage_blk<R, F, true>), x->nblk_cell, x->stage_lds, d, gr,  \
-                 mode, ak, bk, write_res);                                                      \
+                 mode, ak, bk, write_res, x->g, epi);                                           \
     else if (x->stage_shared)                                                                   \
       LAUNCH_LDS(x, UFX_K_STAGE, (k_stage_blk<R, F, false>), x->nblk_cell, x->stage_lds, d, gr, \
-                 mode, ak, bk, write_res);                                                      \
+                 mode, ak, bk, write_res, x->g, epi);                                           \
     else if (x->stage_local)                                                                    \
       LAUNCH_LDS(x, UFX_K_STAGE, (k_stage<R, F, true>), x->nblk_cell, x->stage_lds, d, gr,      \
-                 mode, ak, bk, write_res);                                                      \
+                 mode, ak, bk, write_res, x->g, epi);                                           \
     else                                                                                        \
       LAUNCH(x, UFX_K_STAGE, (k_stage<R, F, false>), x->nblk_cell, d, gr, mode, ak, bk,         \
-             write_res);                                                                        \
+             write_res, x->g, epi);                                                             \
   } while (0)
   switch (x->p.flux) {
     case UFX_FLUX_ROE: KS(1); break;
@@ -489,13 +511,15 @@ const double RK_B[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
 // One Runge-Kutta stage: residual and update fused in one pass when nothing
 // else contributes to the residual, else the full residual then the update.
 // whole: a stage of a whole iteration (its states in qc and qs, see
-// stage_old/in/out), else routine by routine (qold and qc).
+// stage_old/in/out), else routine by routine (qold and qc); epi: the stage
+// also refreshes the next iteration's primitives (prim_epilogue).
 template <typename R>
-int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool whole) {
+int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool whole,
+              bool epi = false) {
   double ak = RK_A[stage - 1], bk = RK_B[stage - 1];
   if (x->p.flow == UFX_FLOW_INVISCID && !x->p.irs)
     return run_stage_kernel(x, d, (norm ? 2 : 1) | ((whole ? stage : 0) << 2), ak, bk,
-                            write_res ? 1 : 0);
+                            write_res ? 1 : 0, epi ? 1 : 0);
   if (int rc = run_residual_full(x, d)) return rc;
   if (int rc = run_rk_update(x, d, ak, bk)) return rc;
   if (norm) {
@@ -558,21 +582,31 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
   // positivity and the global range (re-armed by the previous norm)
   // the fused RK3 stages keep the start state in qc (no qold copy)
   const bool fused = x->p.timemode == UFX_TIME_RK3 && !visc && !x->p.irs;
+  // the last stage of a fused iteration refreshes the next iteration's
+  // primitives itself (not with the diamond-path gradient, whose face pass
+  // reads the records' primitives before the gradient pass copies them in)
+  const bool epi = fused && x->p.grad != UFX_GRAD_GGFC && x->prim_epi;
   x->qold_stale = fused;
-  if (int rc = run_prim(x, d, 1, fused ? 2 : 1, lim ? 1 : 0)) return rc;
+  const bool refresh = !(epi && x->prims_next);
+  x->prims_next = false;
+  if (refresh)
+    if (int rc = run_prim(x, d, 1, fused ? 2 : 1, lim ? 1 : 0)) return rc;
   // vertex averages, with the range reduction as one more block
-  if (int rc = lim ? run_avg_range(x, d) : run_avg(x, d)) return rc;
+  if (int rc = lim ? run_avg_range(x, d, refresh ? x->nblk_all : x->nblk_cell) : run_avg(x, d))
+    return rc;
   // gradient, limiter and time step in one pass over the cells; the
   // temperature gradient only when the viscous flux needs it
   if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
   if (x->p.timemode == UFX_TIME_RK3) {
     for (int s = 1; s <= 3; ++s)
-      if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res, fused)) return rc;
+      if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res, fused, s == 3 && epi))
+        return rc;
   } else {
     if (int rc = run_lusgs(x, d)) return rc;
     LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
   }
   LAUNCH(x, UFX_K_NORM, k_norm_final<R>, 1, d, x->nblk_cell, iter == 1, x->d_hist, k);
+  x->prims_next = epi;
   return launch_check();
 }
 
@@ -731,6 +765,7 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
     for (int s = 0; s < noc; ++s)
       for (int k = 0; k < 4; ++k) qpc[(size_t)s * 4 + k] = rec[(size_t)s * RW + R_QP + k];
     HIPCHK(up(d.qpc, qpc.data(), qpc.size() * sizeof(R)));
+    HIPCHK(up(d.qTc, t.data(), (size_t)noc * sizeof(R)));
     HIPCHK(hipStreamSynchronize(x->stream));
     return UFX_OK;
   }
@@ -870,15 +905,15 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     std::vector<R> qpc((size_t)noc * 4, R(0));
     UP(qpc, qpc);
   }
-  AL(fw, nof); AL(qT, ncell); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc); AL(qs, 4 * (size_t)noc);
+  AL(fw, nof); AL(qT, ncell); AL(qTc, noc); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc); AL(qs, 4 * (size_t)noc);
   AL(res, 4 * (size_t)noc); AL(dt, noc); AL(dta, noc); AL(la, noc); AL(fla, nof); AL(fmu, nof);
   AL(cmul, ncell); AL(prec, (size_t)nop * PW); AL(fgrad, (size_t)nof * 10);
   AL(bflux, 2 * (size_t)std::max(1, nbf)); AL(bres, 4 * (size_t)std::max(1, nbf));
-  AL(part, (size_t)std::max(x->nblk_all, 1) * 8);
+  AL(part, (size_t)std::max(x->nblk_all, 1) * 8); AL(rpart, (size_t)std::max(x->nblk_all, 1) * 8);
   AL(scal, SC_NSCAL); AL(range, 8); AL(err, 4); AL(lower, 4 * (size_t)noc); AL(upper, 4 * (size_t)noc);
 #undef UP
 #undef AL
-  int init[4] = {0, 0, INT_MAX, INT_MAX};
+  int init[4] = {0, INT_MAX, INT_MAX, INT_MAX};
   HIPCHK(hipMemcpyAsync(d.err, init, sizeof(init), hipMemcpyHostToDevice, x->stream));
   unsigned long long rg[8] = {0, 0, 0, 0, ~0ULL, ~0ULL, ~0ULL, ~0ULL};
   HIPCHK(hipMemcpyAsync(d.range, rg, sizeof(rg), hipMemcpyHostToDevice, x->stream));
@@ -1121,6 +1156,10 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     bool shared = p.precision == UFX_PREC_MIXED;
     if (env && std::strcmp(env, "gather") == 0) shared = false;
     if (env && std::strcmp(env, "shared") == 0) shared = true;
+    // UFX_PRIM_EPILOGUE=0: whole iterations refresh the primitives in a pass
+    // of their own (same results; for comparison)
+    const char* enve = std::getenv("UFX_PRIM_EPILOGUE");
+    x->prim_epi = !(enve && std::strcmp(enve, "0") == 0);
     const char* envl = std::getenv("UFX_STAGE_LDS");
     bool local = p.precision == UFX_PREC_FP64;
     if (envl && std::strcmp(envl, "0") == 0) local = false;
@@ -1181,7 +1220,7 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
 }
 
 int ufx_cc_set(ufx_cc* x, const char* name, const double* src) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   if (!name || !src) { set_error("ufx_cc_set: null argument"); return UFX_ERR_ARG; }
   std::string n(name);
   int rc = with_dev(x, [&](auto& d) { return set_field(x, d, n, src); });
@@ -1199,23 +1238,23 @@ int ufx_cc_get(ufx_cc* x, const char* name, double* dst) {
 }
 
 int ufx_cc_prim(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   return with_dev(x, [&](auto& d) { return run_prim(x, d, 1, 0, 0); });
 }
 
 int ufx_cc_save_old(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   x->qold_stale = false;
   return with_dev(x, [&](auto& d) { return run_prim(x, d, 0, 1, 0); });
 }
 
 int ufx_cc_avg_c2v(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   return with_dev(x, [&](auto& d) { return run_avg(x, d); });
 }
 
 int ufx_cc_time_step(ufx_cc* x, double cfl) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   x->cfl = cfl;
   return with_dev(x, [&](auto& d) {
     using R = typename std::remove_reference_t<decltype(d)>::Real;
@@ -1224,7 +1263,7 @@ int ufx_cc_time_step(ufx_cc* x, double cfl) {
 }
 
 int ufx_cc_gradient(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   return with_dev(x, [&](auto& d) {
     if (x->p.ilimit)
       if (int rc = run_range(x, d)) return rc;
@@ -1233,7 +1272,7 @@ int ufx_cc_gradient(ufx_cc* x) {
 }
 
 int ufx_cc_limit(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   return with_dev(x, [&](auto& d) {
     using R = typename std::remove_reference_t<decltype(d)>::Real;
     if (int rc = run_range(x, d)) return rc;
@@ -1242,32 +1281,32 @@ int ufx_cc_limit(ufx_cc* x) {
 }
 
 int ufx_cc_fvresidual(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   return with_dev(x, [&](auto& d) { return run_residual_full(x, d); });
 }
 
 int ufx_cc_rk_update(ufx_cc* x, int stage) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
   if (int rc = check_qold(x, "ufx_cc_rk_update")) return rc;
   return with_dev(x, [&](auto& d) { return run_rk_update(x, d, RK_A[stage - 1], RK_B[stage - 1]); });
 }
 
 int ufx_cc_rk_stage(ufx_cc* x, int stage) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
   if (int rc = check_qold(x, "ufx_cc_rk_stage")) return rc;
   return with_dev(x, [&](auto& d) { return run_stage(x, d, stage, false, true, false); });
 }
 
 int ufx_cc_lusgs(ufx_cc* x) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   if (int rc = check_qold(x, "ufx_cc_lusgs")) return rc;
   return with_dev(x, [&](auto& d) { return run_lusgs(x, d); });
 }
 
 int ufx_cc_residue(ufx_cc* x, int iter, double* fres, double* fres_raw) {
-  if (int rc = check_ctx(x)) return rc;
+  if (int rc = check_routine(x)) return rc;
   if (int rc = ensure_hist(x, 1)) return rc;
   int rc = with_dev(x, [&](auto& d) {
     using R = typename std::remove_reference_t<decltype(d)>::Real;
@@ -1289,7 +1328,9 @@ int ufx_cc_residue(ufx_cc* x, int iter, double* fres, double* fres_raw) {
 int ufx_cc_check_positivity(ufx_cc* x, int* bad) {
   if (int rc = check_ctx(x)) return rc;
   int* err = err_word(x);
-  int big = INT_MAX;
+  int big = INT_MAX, held = INT_MAX;
+  HIPCHK(hipMemcpyAsync(&held, err + 2, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
   HIPCHK(hipMemcpyAsync(err + 2, &big, sizeof(int), hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   int rc = with_dev(x, [&](auto& d) {
@@ -1303,7 +1344,8 @@ int ufx_cc_check_positivity(ufx_cc* x, int* bad) {
   HIPCHK(hipStreamSynchronize(x->stream));
   if (bad) *bad = e == INT_MAX ? 0 : e;
   // a report only: the cell noted here must not become an iteration's error
-  HIPCHK(hipMemcpyAsync(err + 2, &big, sizeof(int), hipMemcpyHostToDevice, x->stream));
+  // (the note an iteration may have taken ahead is put back)
+  HIPCHK(hipMemcpyAsync(err + 2, &held, sizeof(int), hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
   if (e != INT_MAX) {
     set_error("Density/pressure is negative at point " + std::to_string(e));
@@ -1433,6 +1475,9 @@ int ufx_cc_iterate_graph(ufx_cc* x, int iter0, int niter, double cfl, double* hi
     }
   } else {
     if (fresh) {
+      // the chunk's first iteration refreshes the primitives itself, so a
+      // replay does not depend on what ran before it
+      x->prims_next = false;
       HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
       int rc = 0;
       for (int k = 0; k < niter && !rc; ++k)
@@ -1448,10 +1493,12 @@ int ufx_cc_iterate_graph(ufx_cc* x, int iter0, int niter, double cfl, double* hi
       c.n = niter;
       c.cfl = cfl;
       c.stale = false;
+      c.prims_next = x->prims_next;
       HIPCHK(release_chunk(x->chunk));
       x->chunk = c;
     }
     HIPCHK(hipGraphLaunch(x->chunk.exec, x->stream));
+    x->prims_next = x->chunk.prims_next;
   }
   (void)iter0;
   x->qold_stale = x->p.timemode == UFX_TIME_RK3 && x->p.flow == UFX_FLOW_INVISCID && !x->p.irs;
@@ -1805,6 +1852,8 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
       return UFX_ERR_ARG;
     }
     if (x->prof) { set_error("ufx_cc_batch_iterate: a case is profiling"); return UFX_ERR_ARG; }
+    // batched iterations refresh the primitives at their top
+    if (int rc = prims_current(x)) return rc;
     // the case's own queued work first
     HIPCHK(hipStreamSynchronize(x->stream));
     if (int rc = ensure_hist(x, niter)) return rc;
