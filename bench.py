@@ -20,6 +20,7 @@ sources, timed on this host on a bounded sample).
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import subprocess
@@ -39,40 +40,133 @@ def stage_bytes(noc: int, nof: int, w: int, d: int = 8, i: int = 4) -> float:
     """Algorithmic bytes of one residual-stage launch, averaged over the three
     stages: face connectivity and geometry once per face, cell state once per
     cell (primitives, gradients, limiter values, centroid), the old and the
-    current conserved state, dt/area, and the written state."""
+    current conserved state, dt/area, and the written state; the last stage
+    also writes the next iteration's primitives and temperature (5w per
+    cell, the primitive refresh it does for the next iteration)."""
     faces = nof * (2 * i + 5 * w)
     cells = noc * (2 * w + 4 * w + 8 * w + 4 * w) + noc * 4 * d + noc * w + noc * 4 * d
     prev_q = noc * 4 * d * 2.0 / 3.0
-    return faces + cells + prev_q
+    epilogue = noc * 5 * w / 3.0
+    return faces + cells + prev_q + epilogue
+
+
+def pass_bytes(noc: int, nof: int, nop: int, nv2c: int, w: int, d: int = 8, i: int = 4) -> dict:
+    """Algorithmic bytes per launch of the passes of a whole RK3 iteration, by
+    profiling category: the vertex averages (P2), the gradient pass with the
+    limiter and the time step whose face spectral radii it evaluates itself
+    (P3 + P4) and a residual stage (P5, averaged over the three)."""
+    p2 = nop * i + nv2c * (i + w) + noc * 4 * w + nop * 4 * w
+    p3 = nof * (4 * i + 3 * w) + nop * 4 * w + noc * 2 * w
+    p4 = nof * 2 * i + noc * (2 * w + 3 * w + 4 * w + 4 * w) + nof * 2 * w + noc * (8 * w + 4 * w)
+    return {"avg_c2v": p2, "gradient": p3 + p4, "stage": stage_bytes(noc, nof, w, d, i)}
 
 
 def iteration_bytes(noc: int, nof: int, nop: int, nv2c: int, w: int, d: int = 8, i: int = 4) -> float:
-    """Algorithmic bytes of one RK3 iteration (all passes)."""
+    """Algorithmic bytes of one RK3 iteration (all passes of the reference's
+    call structure: primitive refresh P1, vertex averages P2, spectral radii
+    P3, gradient and limiter P4, three residual stages P5)."""
     p1 = noc * (4 * d + 4 * w)
     p2 = nop * i + nv2c * (i + w) + noc * 4 * w + nop * 4 * w
     p3 = nof * (4 * i + 3 * w) + nop * 4 * w + noc * 2 * w
     p4 = nof * 2 * i + noc * (2 * w + 3 * w + 4 * w + 4 * w) + nof * 2 * w + noc * (8 * w + 4 * w)
-    return p1 + p2 + p3 + p4 + 3 * stage_bytes(noc, nof, w, d, i)
+    faces = nof * (2 * i + 5 * w)
+    cells = noc * (2 * w + 4 * w + 8 * w + 4 * w) + noc * 4 * d + noc * w + noc * 4 * d
+    return p1 + p2 + p3 + p4 + 3 * (faces + cells) + 2 * noc * 4 * d
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_bump2M_hbm_traffic.md")
 
 
-def measured_traffic(precision: str) -> int | None:
-    """HBM bytes per launch of the stage kernel, from the committed rocprofv3
-    --pmc summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
-    same counter reads for a plain copy of known size): lines
+STAGE_KERNELS = {"gather": ("k_stage", "false"), "gather_lds": ("k_stage", "true"),
+                 "shared": ("k_stage_blk", "false"), "shared_lds": ("k_stage_blk", "true")}
+
+
+def kernel_name(category: str, precision: str, form: str) -> str:
+    """The kernel that runs a profiling category of the Roe/LSQ/limited
+    whole iteration, as rocprofv3 names it."""
+    r = "double" if precision == "fp64" else "float"
+    if category == "stage":
+        k, lds = STAGE_KERNELS[form]
+        return f"ufx::{k}<{r},1,{lds}>"
+    if category == "gradient":
+        return f"ufx::k_cell<{r},1,false,true,2>"
+    return f"ufx::k_avg_c2v_range<{r},false>"
+
+
+def measured_traffic(precision: str, kernel: str) -> int | None:
+    """HBM bytes per launch of `kernel`, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the same
+    counter reads for a plain copy of known size): lines
     ``traffic <precision> <kernel> <bytes>``.  None when not measured."""
     import re
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 m = re.match(r"^traffic (\S+) (\S+) (\d+)\s*$", line.strip())
-                if m and m.group(1) == precision:
+                if m and m.group(1) == precision and m.group(2) == kernel:
                     return int(m.group(3))
     except OSError:
         pass
     return None
+
+
+class ClockSampler:
+    """The GPU's graphics and memory clocks (amdsmi), sampled every `period`
+    seconds in a thread while a timed region runs.  Reported beside the
+    figures: the same kernels have run at very different speeds on boxes
+    whose graphics clock under load differed.  Best effort: without amdsmi
+    the summary is None."""
+
+    def __init__(self, period: float = 0.01):
+        import threading
+        self.period = period
+        self.gfx, self.mem = [], []
+        self._stop = threading.Event()
+        self._thread = None
+        self._h = None
+        self.max_gfx = None
+        try:
+            import amdsmi
+            amdsmi.amdsmi_init()
+            self._smi = amdsmi
+            self._h = amdsmi.amdsmi_get_processor_handles()[0]
+            self.max_gfx = amdsmi.amdsmi_get_clock_info(self._h, amdsmi.AmdSmiClkType.GFX)["max_clk"]
+        except Exception:
+            self._h = None
+
+    def _run(self):
+        smi = self._smi
+        while not self._stop.is_set():
+            try:
+                self.gfx.append(smi.amdsmi_get_clock_info(self._h, smi.AmdSmiClkType.GFX)["clk"])
+                self.mem.append(smi.amdsmi_get_clock_info(self._h, smi.AmdSmiClkType.MEM)["clk"])
+            except Exception:
+                return
+            self._stop.wait(self.period)
+
+    def __enter__(self):
+        import threading
+        if self._h is not None:
+            self._stop.clear()
+            self._thread = threading.Thread(target=self._run, daemon=True)
+            self._thread.start()
+        return self
+
+    def __exit__(self, *exc):
+        if self._thread is not None:
+            self._stop.set()
+            self._thread.join()
+            self._thread = None
+        return False
+
+    def summary(self) -> dict | None:
+        if not self.gfx:
+            return None
+        g = np.array(self.gfx, dtype=float)
+        m = np.array(self.mem, dtype=float)
+        return {"gfx_mhz_median": float(np.median(g)), "gfx_mhz_min": float(g.min()),
+                "gfx_mhz_max": float(g.max()), "gfx_mhz_peak": self.max_gfx,
+                "mem_mhz_median": float(np.median(m)) if len(m) else None, "samples": len(g)}
 
 
 def cpu_baseline(budget_s: float = 20.0) -> dict | None:
@@ -488,20 +582,22 @@ def main_viscous(args) -> int:
     return 0
 
 
-def time_iterations(solver, args) -> tuple[float, float]:
+def time_iterations(solver, args, category: str = "stage", clocks=None) -> tuple[float, float]:
     """Warm-up, then K iterations timed with the host clock around them
     (synchronised on both sides); HIP events on the library's stream bracket
-    the residual-stage launches.  Returns (wall seconds, mean stage launch
-    ms)."""
+    the launches of one kernel category (an event pair around every launch
+    would itself cost about a tenth of the iteration).  Returns (wall
+    seconds, mean launch ms of that category)."""
     solver.iterate(args.warmup, hist=True)
     solver.sync()
-    solver.profile_only("stage")
+    solver.profile_only(category)
     solver.sync()
-    t1 = time.perf_counter()
-    solver.iterate(args.steps, hist=False)
-    solver.sync()
-    wall = time.perf_counter() - t1
-    st_ms, st_n = solver.profile_read()["stage"]
+    with clocks if clocks is not None else contextlib.nullcontext():
+        t1 = time.perf_counter()
+        solver.iterate(args.steps, hist=False)
+        solver.sync()
+        wall = time.perf_counter() - t1
+    st_ms, st_n = solver.profile_read()[category]
     solver.profile(False)
     return wall, st_ms / max(1, st_n)
 
@@ -557,7 +653,7 @@ def main() -> int:
     ap.add_argument("--per-case", action="store_true",
                     help="batch: launch every case's kernels separately (one graph per case)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="batch: keep the default stage form instead of autotuning it")
+                    help="keep the default residual-stage form instead of tuning it on this GPU")
     ap.add_argument("--seq-window", type=int, default=100, metavar="N",
                     help="batch: also time every case alone, one after another, eager, over N "
                          "iterations scaled to MaxIter (0: skip)")
@@ -604,32 +700,45 @@ def main() -> int:
     print(f"# setup {time.time() - t0:.1f}s: {counts['noc']} cells, {counts['nof']} faces, "
           f"{counts['nop']} vertices", file=sys.stderr, flush=True)
 
-    # timed region: K iterations; HIP events on the library's stream bracket
-    # the residual-stage launches only (an event pair around every launch
-    # would itself cost about a tenth of the iteration)
-    wall, st_avg = time_iterations(solver, args)
-    # the per-kernel breakdown, from a separate pass of K iterations with
-    # every launch bracketed
+    noc, nof, nop = counts["noc"], counts["nof"], counts["nop"]
+    w = 8 if args.precision == "fp64" else 4
+    pbytes = pass_bytes(noc, nof, nop, nv2c, w)
+    say = lambda m: print(f"# {m}", file=sys.stderr, flush=True)
+    # the residual-stage form that runs fastest on this GPU (the tuning
+    # iterations are real iterations; they precede the warm-up)
+    tune = None if args.no_tune else solver.tune_stage_form(log=say)
+    form = solver.stage_form
+    # the per-kernel breakdown, from a pass of K iterations with every launch
+    # bracketed; the pass with the largest share is the roofline's kernel
+    solver.iterate(args.warmup, hist=False)
     solver.profile(True)
     solver.iterate(args.steps, hist=False)
     prof_all = solver.profile_read()
     solver.profile(False)
+    dominant = max(pbytes, key=lambda k: prof_all[k][0])
+    # timed region: K iterations; HIP events on the library's stream bracket
+    # the dominant pass's launches only
+    clocks = ClockSampler()
+    wall, k_avg = time_iterations(solver, args, dominant, clocks)
     hist = solver.iterate(1, hist=True)
     if not np.all(np.isfinite(hist)):
         print("# residual is not finite", file=sys.stderr)
         return 1
 
-    noc, nof, nop = counts["noc"], counts["nof"], counts["nop"]
     ms_step = wall * 1000.0 / args.steps
     value = 3.0 * noc * args.steps / wall / 1e6
-    w = 8 if args.precision == "fp64" else 4
-    sb = stage_bytes(noc, nof, w)
-    achieved = sb / (st_avg / 1000.0) / 1e9
+    kb = pbytes[dominant]
+    achieved = kb / (k_avg / 1000.0) / 1e9
     it_bytes = iteration_bytes(noc, nof, nop, nv2c, w)
     kern = {k: round(v[0] / max(1, v[1]), 4) for k, v in prof_all.items() if v[1]}
-    print(f"# kernel avg ms: {kern}; iteration {ms_step:.3f} ms, "
-          f"{it_bytes / (ms_step / 1000.0) / 1e9:.0f} GB/s algorithmic over the whole iteration",
-          file=sys.stderr, flush=True)
+    per_it = {k: round(v[0] / args.steps, 4) for k, v in prof_all.items() if v[1]}
+    say(f"stage form {form}; kernel avg ms: {kern}; per iteration: {per_it}; iteration "
+        f"{ms_step:.3f} ms, {it_bytes / (ms_step / 1000.0) / 1e9:.0f} GB/s algorithmic over the "
+        f"whole iteration; clocks {clocks.summary()}")
+    passes = {k: {"ms_per_iteration": per_it[k], "avg_launch_ms": kern[k],
+                  "bytes_per_launch": int(pbytes[k]),
+                  "frac": round(pbytes[k] / (kern[k] / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}
+              for k in pbytes if k in kern}
 
     solver.close()
     # the mixed form (FP32 records, gradients, limiter and fluxes; FP64 state,
@@ -639,17 +748,20 @@ def main() -> int:
     if args.precision == "fp64" and not args.no_mixed:
         s32 = cc.Solver(mesh, cc.Config(mach=0.75, flux="roe", grad="lsqr", ilimit=1,
                                         precision="mixed"))
-        wall32, st32 = time_iterations(s32, args)
+        tune32 = None if args.no_tune else s32.tune_stage_form(log=say)
+        wall32, st32 = time_iterations(s32, args, "stage")
+        form32 = s32.stage_form
         s32.close()
         v32 = 3.0 * noc * args.steps / wall32 / 1e6
         sb32 = stage_bytes(noc, nof, 4)
         t0 = time.time()
         err = precision_errors(mesh, args.error_iters)
-        print(f"# mixed: {wall32 * 1000.0 / args.steps:.3f} ms/iteration, {v32:.0f} M cell-stages/s "
-              f"({v32 / value:.3f}x); errors vs FP64 after {args.error_iters} iterations "
-              f"({time.time() - t0:.1f}s): {err}", file=sys.stderr, flush=True)
+        say(f"mixed ({form32}): {wall32 * 1000.0 / args.steps:.3f} ms/iteration, {v32:.0f} M "
+            f"cell-stages/s ({v32 / value:.3f}x); errors vs FP64 after {args.error_iters} "
+            f"iterations ({time.time() - t0:.1f}s): {err}")
         mixed = {"mixed_value": round(v32, 2), "mixed_speedup": round(v32 / value, 4),
                  "mixed_ms_per_step": round(wall32 * 1000.0 / args.steps, 4),
+                 "mixed_stage_form": form32,
                  "mixed_stage_roofline_frac": round(sb32 / (st32 / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
                  "mixed_errors_vs_fp64": err}
 
@@ -679,11 +791,16 @@ def main() -> int:
                    "cells": noc, "faces": nof, "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.precision),
-                     "kernel": "k_stage" if args.precision == "fp64" else "k_stage_blk",
-                     "bytes_per_launch": int(sb),
-                     "avg_launch_ms": round(st_avg, 5)},
+                     "traffic": measured_traffic(args.precision,
+                                                 kernel_name(dominant, args.precision, form)),
+                     "kernel": kernel_name(dominant, args.precision, form),
+                     "bytes_per_launch": int(kb),
+                     "avg_launch_ms": round(k_avg, 5)},
         "iteration_roofline_frac": round(it_bytes / (ms_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+        "passes": passes,
+        "stage_form": form,
+        "stage_form_tuning": tune,
+        "gpu_clocks": clocks.summary(),
         **mixed,
         "cpu_baseline": cpu,
     }

@@ -1,15 +1,19 @@
-"""Whole RK3 iterations whose last stage refreshes the next iteration's
-primitives.
+"""Whole RK3 iterations fused across the reference's passes: the last stage
+refreshes the next iteration's primitives, and the gradient pass sums the
+face spectral radii it evaluates itself.
 
 The reference refreshes the primitives at the top of every iteration
 (con2prim, CCFVM/main.F90:62-67) from the state the previous iteration's
 last stage wrote.  The device's last stage of a whole inviscid RK3 iteration
 does that refresh itself from the state it has just computed (the compact
 primitives, the temperature, the limiter's range partials and the positivity
-note), so the next iteration starts without a pass of its own for it;
-UFX_PRIM_EPILOGUE=0 selects the separate pass.  Checked here:
+note), so the next iteration starts without a pass of its own for it.  The time
+step's face spectral radii (CCFVM/timestep.F90:147-171) are evaluated by
+the gradient pass, which already visits every face of its cells, instead of
+a face pass that stores them.  UFX_ITER_FUSION=0 selects the separate
+passes.  Checked here:
 * both schedules give the same state, limiter values, gradients,
-  primitives, residual, history and forces bit for bit, eagerly and from
+  primitives, residual, time steps, history and forces bit for bit, eagerly and from
   replayed graphs, over the fluxes, both precisions, both stage forms and
   the gradient methods that use it;
 * a routine call between whole iterations sees the reference's current
@@ -27,9 +31,9 @@ pytestmark = pytest.mark.gpu
 
 def _solver(mesh, monkeypatch, epi, form=None, **cfg):
     from unsflow2d_amd import cc
-    monkeypatch.setenv("UFX_PRIM_EPILOGUE", "1" if epi else "0")
+    monkeypatch.setenv("UFX_ITER_FUSION", "1" if epi else "0")
     s = cc.Solver(mesh, cc.Config(**cfg))
-    monkeypatch.delenv("UFX_PRIM_EPILOGUE")
+    monkeypatch.delenv("UFX_ITER_FUSION")
     if form:
         s.set_stage_form(form)
     s.keep_residual(True)
@@ -39,7 +43,7 @@ def _solver(mesh, monkeypatch, epi, form=None, **cfg):
 def _state(s):
     f = s.forces()
     return {"qc": s.get("qc"), "phi": s.get("phi"), "grad": s.get("grad"), "qp": s.get("qp"),
-            "res": s.get("res"), "forces": np.array([f[k] for k in ("cl", "cd", "cl1", "cd1")])}
+            "res": s.get("res"), "dt": s.get("dt"), "la": s.get("la"), "forces": np.array([f[k] for k in ("cl", "cd", "cl1", "cd1")])}
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])

@@ -17,6 +17,7 @@ import ctypes as C
 import dataclasses
 import math
 import os
+import time
 
 import numpy as np
 
@@ -414,6 +415,34 @@ class Solver:
         sh, lds = C.c_int(), C.c_int()
         _check(lib.ufx_cc_get_stage_form(self._h, C.byref(sh), C.byref(lds)))
         return {v: k for k, v in self.STAGE_FORMS.items()}[(sh.value, lds.value)]
+
+    def tune_stage_form(self, iters: int = 8, reps: int = 3, log=None) -> dict:
+        """Pick the residual-stage form that runs whole iterations of this
+        context fastest on this GPU: each form runs one untimed and
+        ``reps - 1`` timed batches of ``iters`` iterations (host clock,
+        synchronised).  The tuning iterations are real iterations: they
+        advance the state and the iteration count.  The fastest form is left
+        set; returns it with every form's milliseconds per iteration (None
+        where the form does not fit in LDS on this mesh)."""
+        times = {}
+        for form in self.STAGE_FORMS:
+            try:
+                self.set_stage_form(form)
+            except UfxError:
+                times[form] = None
+                continue
+            self.iterate(iters, hist=False)
+            self.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps - 1):
+                self.iterate(iters, hist=False)
+            self.sync()
+            times[form] = (time.perf_counter() - t0) * 1e3 / (iters * max(1, reps - 1))
+            if log:
+                log(f"stage form {form}: {times[form]:.4f} ms/iteration")
+        best = min((f for f in times if times[f] is not None), key=lambda f: times[f])
+        self.set_stage_form(best)
+        return {"form": best, "ms_per_iteration": times}
 
     def keep_residual(self, on: bool = True) -> None:
         """Whole iterations also store the last stage's residual."""

@@ -37,9 +37,25 @@ __device__ __forceinline__ void face_prims(const DevT<R>& d, int f, R* fq) {
   for (int k = 0; k < NV; ++k) fq[k] = R(0.5) * (a[k] + b[k]);
 }
 
-// Spectral radius of every face, CCFVM/timestep.F90:147-171 (spectral02):
-// inviscid |u.n| + a from the face-averaged primitives, times the face
-// length, plus the viscous term.
+// Spectral radius of face f, CCFVM/timestep.F90:147-171 (spectral02):
+// inviscid |u.n| + a from the face-averaged primitives fq, times the face
+// length, plus the viscous term (mu: the face viscosity when vll != 0).
+// Nothing is contracted, so the face pass and the gradient pass that sums
+// the radii itself give the same bits.
+template <typename R>
+__device__ __forceinline__ double face_lambda_val(const Gas<double>& g, const R* fq, const R* fr,
+                                                  double vll, double mu) {
+#pragma clang fp contract(off)
+  double r0 = fq[0], u0 = fq[1], v0 = fq[2], p0 = fq[3];
+  double a0 = sqrt(g.gamma * p0 / r0);
+  double un = u0 * (double)fr[F_NX] + v0 * (double)fr[F_NY];
+  double ll = (double)fr[F_AREA] * (fabs(un) + a0);
+  if (vll != 0.0) ll = ll + vll * mu / r0;
+  return ll;
+}
+
+// The spectral radius of every face, stored (for the time step of a routine
+// call, LU-SGS and the residual smoothing).
 template <typename R>
 __device__ __forceinline__ void face_lambda_body(const DevT<R>& d, VBlock vb, Gas<double> g,
                                                  double vll) {
@@ -48,12 +64,7 @@ __device__ __forceinline__ void face_lambda_body(const DevT<R>& d, VBlock vb, Ga
   R fq[4], fr[Layout<R>::FW];
   face_prims<R, 4>(d, f, fq);
   face_rec(d, f, fr);
-  double r0 = fq[0], u0 = fq[1], v0 = fq[2], p0 = fq[3];
-  double a0 = sqrt(g.gamma * p0 / r0);
-  double un = u0 * (double)fr[F_NX] + v0 * (double)fr[F_NY];
-  double ll = (double)fr[F_AREA] * (fabs(un) + a0);
-  if (vll != 0.0) ll = ll + vll * d.fmu[f] / r0;
-  d.fla[f] = ll;
+  d.fla[f] = face_lambda_val<R>(g, fq, fr, vll, vll != 0.0 ? d.fmu[f] : 0.0);
 }
 template <typename R>
 __global__ __launch_bounds__(BLK) void k_face_lambda(DevT<R> d, Gas<double> g, double vll) {
@@ -324,8 +335,9 @@ enum { G_NONE = 0, G_LSQR = 1, G_GG = 2, G_GGFC = 3 };
 //             GG:   :95-156 (boundary faces with -n, /area);
 //             GGFC: :159-323 cell part, sum of face gradients x co-area / cov;
 //   limiter   Venkatakrishnan with the ratchet, CCFVM/limiter.F90:123-296;
-//   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177.
-template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
+//   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177, with the
+//             face radii of k_face_lambda (TSTEP 1) or evaluated here (2).
+template <typename R, int GRAD, bool WITH_T, bool LIMIT, int TSTEP>
 __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll,
                                           double cfl) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
@@ -390,7 +402,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
     n = in_side ? n : ~n;
     bool bnd = n >= noc;
     R fr[FW];
-    if (GRAD == G_GG) {
+    if (GRAD == G_GG || TSTEP == 2) {
       face_rec(d, f, fr);
       fcx[s] = fr[F_CX];
       fcy[s] = fr[F_CY];
@@ -401,13 +413,14 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
       fcy[s] = fc[1];
     }
     R fq[5];
-    if (GRAD == G_GG || (LIMIT && bnd)) {
+    if (GRAD == G_GG || (LIMIT && bnd) || TSTEP == 2) {
       if (GRAD == G_GG) face_prims<R, 5>(d, f, fq);
       else face_prims<R, 4>(d, f, fq);
     }
-    // cell spectral radius: the face radii of k_face_lambda summed in
-    // ascending face order, CCFVM/timestep.F90:171-173
-    if (TSTEP) la = la + d.fla[f];
+    // cell spectral radius: the face radii summed in ascending face order,
+    // CCFVM/timestep.F90:171-173
+    if (TSTEP == 1) la = la + d.fla[f];
+    if (TSTEP == 2) la = la + face_lambda_val<R>(g, fq, fr, vll, vll != 0.0 ? d.fmu[f] : 0.0);
     R nq[4];
     const unsigned l = (unsigned)(n - base);
     const bool loc = SL && l < (unsigned)cnt;
@@ -531,7 +544,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
     d.dta[c] = dt / d.cv[c];
   }
 }
-template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
+template <typename R, int GRAD, bool WITH_T, bool LIMIT, int TSTEP>
 __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl) {
   cell_body<R, GRAD, WITH_T, LIMIT, TSTEP>(d, hw_block(), g, vll, cfl);
 }
@@ -804,26 +817,23 @@ __device__ __forceinline__ int gather_residual(const DevT<R>& d, const R* srec, 
     load_rec<R, SL>(d, srec, base, n, nr);
     face_rec(d, f, fr);
     R ql[4], qr[4], F[4];
-    // side a = the face's `in` cell, side b = its `out` cell
-    R ax = in_side ? own[R_CEN] : nr[R_CEN], ay = in_side ? own[R_CEN + 1] : nr[R_CEN + 1];
-    R bx = in_side ? nr[R_CEN] : own[R_CEN], by = in_side ? nr[R_CEN + 1] : own[R_CEN + 1];
-    R dlx = fr[F_CX] - ax, dly = fr[F_CY] - ay, drx = fr[F_CX] - bx, dry = fr[F_CY] - by;
+    // recon of this cell's side and of the neighbour's side, then the
+    // face's `in` side as ql and its `out` side as qr: the same arithmetic
+    // as reconstructing the in and out sides directly, with only the
+    // results selected
+    R dox = fr[F_CX] - own[R_CEN], doy = fr[F_CY] - own[R_CEN + 1];
+    R dnx = fr[F_CX] - nr[R_CEN], dny = fr[F_CY] - nr[R_CEN + 1];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      R qa = in_side ? own[R_QP + i] : nr[R_QP + i];
-      R qb = in_side ? nr[R_QP + i] : own[R_QP + i];
-      R gax = in_side ? own[R_G + 2 * i] : nr[R_G + 2 * i];
-      R gay = in_side ? own[R_G + 2 * i + 1] : nr[R_G + 2 * i + 1];
-      R gbx = in_side ? nr[R_G + 2 * i] : own[R_G + 2 * i];
-      R gby = in_side ? nr[R_G + 2 * i + 1] : own[R_G + 2 * i + 1];
-      R pa = in_side ? own[R_PHI + i] : nr[R_PHI + i];
-      R pb = in_side ? nr[R_PHI + i] : own[R_PHI + i];
-      R alfa = gax * dlx + gay * dly;
-      R beta = qb - qa;
-      ql[i] = qa + R(0.5) * pa * (zhi * beta + (R(1) - zhi) * alfa);
-      alfa = gbx * drx + gby * dry;
-      beta = qa - qb;
-      qr[i] = qb + R(0.5) * pb * (zhi * beta + (R(1) - zhi) * alfa);
+      R qo = own[R_QP + i], qn = nr[R_QP + i];
+      R alfa = own[R_G + 2 * i] * dox + own[R_G + 2 * i + 1] * doy;
+      R beta = qn - qo;
+      R so = qo + R(0.5) * own[R_PHI + i] * (zhi * beta + (R(1) - zhi) * alfa);
+      alfa = nr[R_G + 2 * i] * dnx + nr[R_G + 2 * i + 1] * dny;
+      beta = qo - qn;
+      R sn = qn + R(0.5) * nr[R_PHI + i] * (zhi * beta + (R(1) - zhi) * alfa);
+      ql[i] = in_side ? so : sn;
+      qr[i] = in_side ? sn : so;
     }
     if (!face_flux<R, FLUX>(gr, ql, qr, fr[F_NX], fr[F_NY], fr[F_AREA], F))
       ekey = min(ekey, d.nbf + d.fref[f]);
