@@ -52,13 +52,13 @@ def stage_bytes(noc: int, nof: int, w: int, d: int = 8, i: int = 4) -> float:
 
 def pass_bytes(noc: int, nof: int, nop: int, nv2c: int, w: int, d: int = 8, i: int = 4) -> dict:
     """Algorithmic bytes per launch of the passes of a whole RK3 iteration, by
-    profiling category: the vertex averages (P2), the gradient pass with the
-    limiter and the time step whose face spectral radii it evaluates itself
-    (P3 + P4) and a residual stage (P5, averaged over the three)."""
+    profiling category: the vertex averages (P2), the face spectral radii
+    (P3), the gradient pass with the limiter and the time step (P4) and a
+    residual stage (P5, averaged over the three)."""
     p2 = nop * i + nv2c * (i + w) + noc * 4 * w + nop * 4 * w
     p3 = nof * (4 * i + 3 * w) + nop * 4 * w + noc * 2 * w
     p4 = nof * 2 * i + noc * (2 * w + 3 * w + 4 * w + 4 * w) + nof * 2 * w + noc * (8 * w + 4 * w)
-    return {"avg_c2v": p2, "gradient": p3 + p4, "stage": stage_bytes(noc, nof, w, d, i)}
+    return {"avg_c2v": p2, "time_step": p3, "gradient": p4, "stage": stage_bytes(noc, nof, w, d, i)}
 
 
 def iteration_bytes(noc: int, nof: int, nop: int, nv2c: int, w: int, d: int = 8, i: int = 4) -> float:
@@ -89,7 +89,9 @@ def kernel_name(category: str, precision: str, form: str) -> str:
         k, lds = STAGE_KERNELS[form]
         return f"ufx::{k}<{r},1,{lds}>"
     if category == "gradient":
-        return f"ufx::k_cell<{r},1,false,true,2>"
+        return f"ufx::k_cell<{r},1,false,true,true>"
+    if category == "time_step":
+        return f"ufx::k_face_lambda<{r}>"
     return f"ufx::k_avg_c2v_range<{r},false>"
 
 

@@ -1,17 +1,13 @@
-"""Whole RK3 iterations fused across the reference's passes: the last stage
-refreshes the next iteration's primitives, and the gradient pass sums the
-face spectral radii it evaluates itself.
+"""Whole RK3 iterations whose last stage refreshes the next iteration's
+primitives.
 
 The reference refreshes the primitives at the top of every iteration
 (con2prim, CCFVM/main.F90:62-67) from the state the previous iteration's
 last stage wrote.  The device's last stage of a whole inviscid RK3 iteration
 does that refresh itself from the state it has just computed (the compact
 primitives, the temperature, the limiter's range partials and the positivity
-note), so the next iteration starts without a pass of its own for it.  The time
-step's face spectral radii (CCFVM/timestep.F90:147-171) are evaluated by
-the gradient pass, which already visits every face of its cells, instead of
-a face pass that stores them.  UFX_ITER_FUSION=0 selects the separate
-passes.  Checked here:
+note), so the next iteration starts without a pass of its own for it;
+UFX_ITER_FUSION=0 selects the separate pass.  Checked here:
 * both schedules give the same state, limiter values, gradients,
   primitives, residual, time steps, history and forces bit for bit, eagerly and from
   replayed graphs, over the fluxes, both precisions, both stage forms and

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(BLK) void km_face_lambda(const CaseDesc<R>* cs, con
   face_lambda_body<R>(e.d, vb, e.g, e.vll);
 }
 
-template <typename R, int GRAD, bool WITH_T, bool LIMIT, int TSTEP>
+template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
 __global__ __launch_bounds__(BLK) void km_cell(const CaseDesc<R>* cs, const int4* map,
                                                double cfl) {
   int c;

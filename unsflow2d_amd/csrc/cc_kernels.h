@@ -40,8 +40,6 @@ __device__ __forceinline__ void face_prims(const DevT<R>& d, int f, R* fq) {
 // Spectral radius of face f, CCFVM/timestep.F90:147-171 (spectral02):
 // inviscid |u.n| + a from the face-averaged primitives fq, times the face
 // length, plus the viscous term (mu: the face viscosity when vll != 0).
-// Nothing is contracted, so the face pass and the gradient pass that sums
-// the radii itself give the same bits.
 template <typename R>
 __device__ __forceinline__ double face_lambda_val(const Gas<double>& g, const R* fq, const R* fr,
                                                   double vll, double mu) {
@@ -335,9 +333,8 @@ enum { G_NONE = 0, G_LSQR = 1, G_GG = 2, G_GGFC = 3 };
 //             GG:   :95-156 (boundary faces with -n, /area);
 //             GGFC: :159-323 cell part, sum of face gradients x co-area / cov;
 //   limiter   Venkatakrishnan with the ratchet, CCFVM/limiter.F90:123-296;
-//   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177, with the
-//             face radii of k_face_lambda (TSTEP 1) or evaluated here (2).
-template <typename R, int GRAD, bool WITH_T, bool LIMIT, int TSTEP>
+//   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177.
+template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
 __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll,
                                           double cfl) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
@@ -402,7 +399,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
     n = in_side ? n : ~n;
     bool bnd = n >= noc;
     R fr[FW];
-    if (GRAD == G_GG || TSTEP == 2) {
+    if (GRAD == G_GG) {
       face_rec(d, f, fr);
       fcx[s] = fr[F_CX];
       fcy[s] = fr[F_CY];
@@ -413,14 +410,13 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
       fcy[s] = fc[1];
     }
     R fq[5];
-    if (GRAD == G_GG || (LIMIT && bnd) || TSTEP == 2) {
+    if (GRAD == G_GG || (LIMIT && bnd)) {
       if (GRAD == G_GG) face_prims<R, 5>(d, f, fq);
       else face_prims<R, 4>(d, f, fq);
     }
-    // cell spectral radius: the face radii summed in ascending face order,
-    // CCFVM/timestep.F90:171-173
-    if (TSTEP == 1) la = la + d.fla[f];
-    if (TSTEP == 2) la = la + face_lambda_val<R>(g, fq, fr, vll, vll != 0.0 ? d.fmu[f] : 0.0);
+    // cell spectral radius: the face radii of k_face_lambda summed in
+    // ascending face order, CCFVM/timestep.F90:171-173
+    if (TSTEP) la = la + d.fla[f];
     R nq[4];
     const unsigned l = (unsigned)(n - base);
     const bool loc = SL && l < (unsigned)cnt;
@@ -544,7 +540,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
     d.dta[c] = dt / d.cv[c];
   }
 }
-template <typename R, int GRAD, bool WITH_T, bool LIMIT, int TSTEP>
+template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
 __global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl) {
   cell_body<R, GRAD, WITH_T, LIMIT, TSTEP>(d, hw_block(), g, vll, cfl);
 }

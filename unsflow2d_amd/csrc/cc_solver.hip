@@ -93,8 +93,7 @@ struct ufx_cc {
   bool qold_stale = false;         // fused RK3 iterations ran since qold was last written
   bool prims_next = false;         // qpc/qTc hold the next iteration's primitives (written by
                                    // the last stage of a whole iteration; see prim_epilogue)
-  bool iter_fusion = true;         // whole RK3 iterations: primitive refresh in the last stage,
-                                   // face spectral radii inside the gradient pass
+  bool iter_fusion = true;         // whole RK3 iterations refresh the primitives in their last stage
   unsigned prof_mask = ~0u;        // kernel categories timed while profiling
   double* d_out = nullptr;         // small device results (residual norms)
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
@@ -386,32 +385,27 @@ int run_avg_range(ufx_cc* x, DevT<R>& d, int nparts) {
 }
 
 // k_cell with the options of one call; with_t when the temperature gradient
-// has a consumer (the drop-in gradient routines always produce it).  tstep:
-// 0 no time step, 1 with the face radii of a k_face_lambda pass (stored for
-// routine calls, LU-SGS and the residual smoothing), 2 with the radii summed
-// by k_cell itself (whole RK3 iterations, which need them nowhere else).
+// has a consumer (the drop-in gradient routines always produce it); tstep:
+// with the time step from the face radii of a k_face_lambda pass.  (Summing
+// radii evaluated inside k_cell instead saves that pass but doubled the
+// prec gathers' HBM traffic, the vertex averages no longer staying in L2
+// beside k_cell's other streams: same time, 1.36 instead of 1.09 GB.)
 template <typename R, int GR>
-int run_cell_g(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, int tstep, double cfl) {
+int run_cell_g(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl) {
   double vll = x->p.vll;
   int cat = GR != G_NONE ? UFX_K_GRAD : (tstep ? UFX_K_TSTEP : UFX_K_GRAD);
-  if (with_t && tstep == 2) tstep = 1;
-  if (tstep == 1) {
+  if (tstep) {
     LAUNCH(x, UFX_K_TSTEP, k_face_lambda<R>, x->nblk_face, d, x->g, vll);
     if (int rc = launch_check()) return rc;
   }
 #define KC(WT, LM, TS) \
   LAUNCH(x, cat, (k_cell<R, GR, WT, LM, TS>), x->nblk_cell, d, x->g, vll, cfl)
   if (with_t) {
-    if (limit) { if (tstep) KC(true, true, 1); else KC(true, true, 0); }
-    else { if (tstep) KC(true, false, 1); else KC(true, false, 0); }
-  } else if (limit) {
-    if (tstep == 2) KC(false, true, 2);
-    else if (tstep) KC(false, true, 1);
-    else KC(false, true, 0);
+    if (limit) { if (tstep) KC(true, true, true); else KC(true, true, false); }
+    else { if (tstep) KC(true, false, true); else KC(true, false, false); }
   } else {
-    if (tstep == 2) KC(false, false, 2);
-    else if (tstep) KC(false, false, 1);
-    else KC(false, false, 0);
+    if (limit) { if (tstep) KC(false, true, true); else KC(false, true, false); }
+    else { if (tstep) KC(false, false, true); else KC(false, false, false); }
   }
 #undef KC
   return launch_check();
@@ -427,7 +421,7 @@ int run_ggfc_face(ufx_cc* x, DevT<R>& d) {
 // and the time step (the reference calls limit from inside each gradient
 // routine when ILIMIT is set).
 template <typename R>
-int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, int tstep, double cfl) {
+int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl) {
   switch (x->p.grad) {
     case UFX_GRAD_GG: return run_cell_g<R, G_GG>(x, d, with_t, limit, tstep, cfl);
     case UFX_GRAD_GGFC:
@@ -606,7 +600,7 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
     return rc;
   // gradient, limiter and time step in one pass over the cells; the
   // temperature gradient only when the viscous flux needs it
-  if (int rc = run_gradient(x, d, visc, lim, fused && x->iter_fusion ? 2 : 1, x->cfl)) return rc;
+  if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
   if (x->p.timemode == UFX_TIME_RK3) {
     for (int s = 1; s <= 3; ++s)
       if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res, fused, s == 3 && epi))
@@ -1166,9 +1160,8 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     bool shared = p.precision == UFX_PREC_MIXED;
     if (env && std::strcmp(env, "gather") == 0) shared = false;
     if (env && std::strcmp(env, "shared") == 0) shared = true;
-    // UFX_ITER_FUSION=0: whole iterations refresh the primitives and
-    // evaluate the face spectral radii in passes of their own (same results;
-    // for comparison)
+    // UFX_ITER_FUSION=0: whole iterations refresh the primitives in a pass
+    // of their own (same results; for comparison)
     const char* enve = std::getenv("UFX_ITER_FUSION");
     x->iter_fusion = !(enve && std::strcmp(enve, "0") == 0);
     const char* envl = std::getenv("UFX_STAGE_LDS");

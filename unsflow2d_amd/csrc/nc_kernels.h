@@ -71,13 +71,18 @@ struct DevNC {
   // multicolour sweeps: positions 0 .. nop - 1 are the vertices grouped by
   // colour (ascending vertex number inside a colour); per sweep direction and
   // position, mc_q entry slots (position-major) hold the neighbours of lower
-  // (forward) or higher (backward) colour in edge order with the edge's
-  // geometry
+  // (forward) or higher (backward) colour in edge order, as positions, with
+  // the edge's geometry.  The sweeps keep their state by position: a block's
+  // neighbours in another colour then lie in a narrow window of that
+  // colour's positions instead of all over the vertex numbering
   const int* mc_vert;     // [nop] vertex of each position
   const int* mc_cnt;      // [2][nop] entries of each position per direction
   const int2* mc_eo;      // [2][mc_q][nop] (edge, bit-inverted when the vertex is p2; other end)
   const double2* mc_nxy;  // [2][mc_q][nop] edge nx, ny
   const double* mc_ds;    // [2][mc_q][nop] edge length
+  double* mc_qold;        // [nop][4] old state by position (the forward sweep writes it)
+  double* mc_dq;          // [nop][4] dq by position
+  double* mc_dd;          // [nop] diagonal by position
   int mc_q;
 };
 
@@ -488,6 +493,10 @@ __global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, Gas<doub
   double dt = cfl * cvp / la;
   d.la[p] = la;
   d.dt[p] = dt;
+  // the LU-SGS diagonal D = V/dt + omega/2 lambda (omega = 1.5,
+  // NCFVM/lusgs.f90:4-131; the multicolour sweeps read
+  // it once per vertex instead of gathering its three inputs)
+  if (mode == 0) d.dd[p] = cvp / dt + 0.5 * 1.5 * la;
   st4d(d.res + (size_t)p * 4, r);
   if (mode == 1) {
     double dtbyarea = dt / cvp, q[4], o[4];
@@ -704,8 +713,8 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, 
     const int e = first ? eo.x : ~eo.x, o = eo.y;
     const double sign = first ? 1.0 : -1.0;
     double f1[4], f2[4], qo[4], qn[4], dq[4];
-    ld4d(d.qold + (size_t)o * 4, qo);
-    ld4d(d.dqc + (size_t)o * 4, dq);
+    ld4d(d.mc_qold + (size_t)o * 4, qo);
+    ld4d(d.mc_dq + (size_t)o * 4, dq);
 #pragma unroll
     for (int i = 0; i < 4; ++i) qn[i] = qo[i] + dq[i];
     nc_normalflux(g, qo, nxy.x, nxy.y, ds, f1);
@@ -716,23 +725,28 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, 
   }
   double res[4], dq[4], dd;
   if (dir == 0) {
-    dd = d.cv[it] / d.dt[it] + 0.5 * 1.5 * d.la[it];
-    d.dd[it] = dd;
+    // the vertex's own data, once per sweep pair: its old state and
+    // diagonal go to position order for its neighbours and the backward sweep
+    double qo[4];
+    ld4d(d.qold + (size_t)it * 4, qo);
+    st4d(d.mc_qold + (size_t)s * 4, qo);
+    dd = d.dd[it];
+    d.mc_dd[s] = dd;
     ld4d(d.res + (size_t)it * 4, res);
 #pragma unroll
     for (int i = 0; i < 4; ++i) dq[i] = (-res[i] - cres[i]) / dd;
   } else {
-    dd = d.dd[it];
-    ld4d(d.dqc + (size_t)it * 4, dq);
+    dd = d.mc_dd[s];
+    ld4d(d.mc_dq + (size_t)s * 4, dq);
 #pragma unroll
     for (int i = 0; i < 4; ++i) dq[i] = dq[i] - cres[i] / dd;
     double qo[4], q[4];
-    ld4d(d.qold + (size_t)it * 4, qo);
+    ld4d(d.mc_qold + (size_t)s * 4, qo);
 #pragma unroll
     for (int i = 0; i < 4; ++i) q[i] = qo[i] + dq[i];
     st4d(d.qc + (size_t)it * 4, q);
   }
-  st4d(d.dqc + (size_t)it * 4, dq);
+  st4d(d.mc_dq + (size_t)s * 4, dq);
 }
 
 // ---- level-ordered one-workgroup sweeps

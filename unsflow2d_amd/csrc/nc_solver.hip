@@ -43,7 +43,8 @@ struct ufx_nc {
   int *d_fwd_off = nullptr, *d_bwd_off = nullptr;  // the same on the device
   SweepSlot* d_slots = nullptr;             // packed sweep slots, forward then backward
   int nwall = 0;                            // wall edges (bc 1001)
-  int* d_wedge = nullptr;                   // their edge numbers, ascending
+  int* d_wedge = nullptr;                   // their stored edges, in ascending reference order
+  std::vector<int> es;                      // storage slot of each reference edge
   double *d_fe = nullptr, *d_cp = nullptr;  // [nwall][2] force terms, [nwall] -Cp
   int sweep_mode = 4;                       // 0 graph of level launches, 1 level launches, 2 one workgroup, 3 cooperative grid, 4 level-ordered one workgroup
   int grid_wgs = 8;                         // workgroups of the cooperative sweep
@@ -413,6 +414,21 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
                       (float)G.u_inf, (float)G.v_inf, (float)G.p_inf, (float)G.mref,
                       (float)G.igamma1};
   x->mixed = prm->precision == UFX_PREC_MIXED;
+  // edges are stored in first-touch order over the vertices (each vertex's
+  // edges in its list order), so the edges a block of vertices gathers lie
+  // in a few contiguous runs instead of all over the reference numbering;
+  // every per-vertex sum still runs over the vertex's list, in ascending
+  // reference edge order
+  std::vector<int>& es = x->es;
+  es.assign(nof, -1);
+  {
+    int ne = 0;
+    for (int p = 0; p < nop; ++p)
+      for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j)
+        if (es[m.v2f[j]] < 0) es[m.v2f[j]] = ne++;
+    for (int e = 0; e < nof; ++e)
+      if (es[e] < 0) es[e] = ne++;
+  }
   // device arrays
   std::vector<double> vrec((size_t)nop * NW, 0.0);
   const double phi0 = (double)1e20f;  // default limiter value, NCFVM/All_modules.f90:200
@@ -425,11 +441,14 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   for (int p = 0; p < nop; ++p)
     for (int k = 0; k < 4; ++k) qc[(size_t)p * 4 + k] = qinf[k];
   std::vector<int2> ep(nof);
+  std::vector<int> ebc(std::max(1, nof), 0);
   for (int e = 0; e < nof; ++e) {
-    erec[(size_t)e * 4] = m.fnx[e];
-    erec[(size_t)e * 4 + 1] = m.fny[e];
-    erec[(size_t)e * 4 + 2] = m.farea[e];
-    ep[e] = make_int2(m.fp1[e], m.fp2[e]);
+    const size_t s = es[e];
+    erec[s * 4] = m.fnx[e];
+    erec[s * 4 + 1] = m.fny[e];
+    erec[s * 4 + 2] = m.farea[e];
+    ep[s] = make_int2(m.fp1[e], m.fp2[e]);
+    ebc[s] = m.fbc[e];
   }
   std::vector<int> v2f(m.v2f.size());
   std::vector<int2> v2fo(m.v2f.size());
@@ -437,7 +456,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
       int e = m.v2f[j];
       bool first = m.fp1[e] == p;
-      v2f[j] = first ? e : ~e;
+      v2f[j] = first ? es[e] : ~es[e];
       v2fo[j] = make_int2(v2f[j], first ? m.fp2[e] : m.fp1[e]);
     }
   std::vector<double> wxy(2 * m.v2v.size());
@@ -509,7 +528,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
         int o = first ? m.fp2[e] : m.fp1[e];
         if (dir == 0 ? o < p : o > p) {
           if (cnt == SWQ) { fits = false; break; }
-          sl.eo[cnt++] = make_int2(first ? e : ~e, o);
+          sl.eo[cnt++] = make_int2(first ? es[e] : ~es[e], o);
         }
       }
       sl.cnt = cnt;
@@ -554,7 +573,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
           const int e = eo.x >= 0 ? eo.x : ~eo.x;
           const int on = pos[(s < (size_t)nop ? 0 : nop) + eo.y];
           sw_nb[k * np + s] = eo.x >= 0 ? on : ~on;
-          sw_pe[k * np + s] = make_int2(e, eo.y);
+          sw_pe[k * np + s] = make_int2(es[e], eo.y);
           sw_ent[k * SWE * np + s] = make_double2(m.fnx[e], m.fny[e]);
           sw_ent[(k * SWE + 1) * np + s] = make_double2(m.farea[e], 0.0);
         }
@@ -594,8 +613,9 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       x->mc_off.assign(ncol + 1, 0);
       for (int p = 0; p < nop; ++p) x->mc_off[col[p] + 1]++;
       for (int c = 0; c < ncol; ++c) x->mc_off[c + 1] += x->mc_off[c];
-      std::vector<int> mv(nop), at(x->mc_off.begin(), x->mc_off.end() - 1);
+      std::vector<int> mv(nop), at(x->mc_off.begin(), x->mc_off.end() - 1), mpos(nop);
       for (int p = 0; p < nop; ++p) mv[at[col[p]]++] = p;
+      for (int sp = 0; sp < nop; ++sp) mpos[mv[sp]] = sp;
       std::vector<int> cnt(2 * (size_t)nop, 0);
       int q = 1;
       for (int sp = 0; sp < nop; ++sp) {
@@ -619,7 +639,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
           if (col[o] == col[p]) continue;  // not a proper colouring: never taken
           int dir = col[o] < col[p] ? 0 : 1;
           size_t at2 = (size_t)dir * nq + (size_t)k[dir]++ * nop + sp;
-          eo[at2] = make_int2(first ? e : ~e, o);
+          eo[at2] = make_int2(first ? es[e] : ~es[e], mpos[o]);
           nxy[at2] = make_double2(m.fnx[e], m.fny[e]);
           dsv[at2] = m.farea[e];
         }
@@ -630,6 +650,9 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       if (!rc) rc = nc_upload(x, &x->d.mc_eo, eo);
       if (!rc) rc = nc_upload(x, &x->d.mc_nxy, nxy);
       if (!rc) rc = nc_upload(x, &x->d.mc_ds, dsv);
+      if (!rc) rc = nc_alloc(x, &x->d.mc_qold, 4 * (size_t)nop);
+      if (!rc) rc = nc_alloc(x, &x->d.mc_dq, 4 * (size_t)nop);
+      if (!rc) rc = nc_alloc(x, &x->d.mc_dd, (size_t)nop);
     }
     if (rc) { ufx_nc_destroy(x); return rc; }
   }
@@ -665,7 +688,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   } else {
     NUP(vrec, vrec); NUP(erec, erec); NUP(wxy, wxy);
   }
-  NUP(qc, qc); NUP(cv, m.pcv); NUP(ebc, m.fbc);
+  NUP(qc, qc); NUP(cv, m.pcv); NUP(ebc, ebc);
   NUP(v2f_ptr, m.v2f_ptr); NUP(v2f, v2f); NUP(v2v_ptr, m.v2v_ptr); NUP(v2v, m.v2v);
   if (!rc) {
     std::vector<double2> pxy(nop);
@@ -690,7 +713,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   {
     std::vector<int> wedge;
     for (int e = 0; e < nof; ++e)
-      if (m.fbc[e] == 1001) wedge.push_back(e);
+      if (m.fbc[e] == 1001) wedge.push_back(es[e]);
     x->nwall = (int)wedge.size();
     if (!rc) rc = nc_upload(x, &x->d_wedge, wedge);
     if (!rc) rc = nc_alloc(x, &x->d_fe, 2 * (size_t)x->nwall + 2);
@@ -833,7 +856,13 @@ int ufx_nc_get(ufx_nc* x, const char* name, double* dst) {
   if (n == "res") return copy(x->d.res, 4 * (size_t)nop);
   if (n == "la") return copy(x->d.la, nop);
   if (n == "dt") return copy(x->d.dt, nop);
-  if (n == "fc_la") return copy(x->d.fla, x->nof);
+  if (n == "fc_la") {
+    std::vector<double> t(x->nof);
+    if (int rc = copy(x->d.fla, x->nof)) return rc;
+    std::copy(dst, dst + x->nof, t.begin());
+    for (int e = 0; e < x->nof; ++e) dst[e] = t[x->es[e]];
+    return UFX_OK;
+  }
   if (n == "qp" || n == "grad" || n == "phi") {
     std::vector<double> v;
     if (int rc = read_records(x, v)) return rc;
