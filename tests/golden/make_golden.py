@@ -49,6 +49,10 @@ CASES = {
     "cc_tiny_ausm": (TINY, dict(mach=0.5, flux_type="ausm", cfl_min=0.05, cfl_max=0.05),
                      10, 2, -1.0, False, True),
     "cc_tiny_gg": (TINY, dict(mach=0.5, grad_type="gg"), 10, 2, -1.0, False, True),
+    # quadrilateral cells, and quadrilaterals mixed with triangles
+    "cc_tiny_quad": (dict(TINY, cells="quad"), dict(mach=0.5), 10, 2, -1.0, True, True),
+    "cc_tiny_mixed_cells": (dict(TINY, cells="mixed"), dict(mach=0.5, grad_type="gg"), 10, 2, -1.0,
+                            True, True),
     "cc_tiny_ggfc": (TINY, dict(mach=0.5, grad_type="ggfc"), 10, 2, -1.0, False, True),
     "cc_visc_ggfc_irs": (VISC, dict(mach=0.3, flow_type="laminar", reynolds=1000.0, iwall=1,
                                     grad_type="ggfc", irs=1, cfl_min=0.5, cfl_max=0.8,

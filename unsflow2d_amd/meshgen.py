@@ -48,7 +48,8 @@ class RawMesh:
     fbnd: np.ndarray         # (nbf, 3) int32: p1 p2 in, 1-based
     fbn: np.ndarray          # (nbf, 2) float64: nx ny (unnormalised)
     fbc: np.ndarray          # (nbf,) int32: 1001 wall / 2001 farfield
-    tri: np.ndarray          # (noc, 3) int32, counter-clockwise vertex ids, 1-based
+    tri: np.ndarray          # (noc, k) int32, counter-clockwise vertex ids, 1-based (k = 3,
+                             # or 4 with -1 padding for triangles among quadrilaterals)
 
     @property
     def nop(self) -> int:
@@ -100,11 +101,15 @@ def _graded_eta(ny: int, first: float | None, growth: float | None) -> np.ndarra
 def bump_channel(nx: int, ny: int, seed: int, *, h: float = 0.05,
                  length: float = 3.0, xb: float = 1.5, width: float = 0.2,
                  jitter: float = 0.2, jitter_y: bool = True,
-                 first: float | None = None, growth: float | None = None) -> RawMesh:
+                 first: float | None = None, growth: float | None = None,
+                 cells: str = "tri") -> RawMesh:
     """Seeded jittered-triangle bump channel (see module docstring).
 
     ``first``/``growth`` grade the wall-normal spacing (anisotropic boundary
     layer); ``jitter_y=False`` jitters only in the streamwise direction.
+    ``cells="quad"`` keeps the quadrilaterals whole, ``cells="mixed"`` splits
+    a seeded half of them (the reference's solver takes any polygon; the
+    triangulated channel is the default and the workloads').
     """
     rng = np.random.default_rng(seed)
     nxp, nyp = nx + 1, ny + 1
@@ -150,10 +155,24 @@ def bump_channel(nx: int, ny: int, seed: int, *, h: float = 0.05,
     diag = rng.integers(0, 2, size=a.shape)  # 0: a-c diagonal, 1: b-d diagonal
     t0 = np.where(diag[:, None] == 0, np.stack([a, b, c], 1), np.stack([a, b, d], 1))
     t1 = np.where(diag[:, None] == 0, np.stack([a, c, d], 1), np.stack([b, c, d], 1))
-    tri = np.empty((2 * a.size, 3), dtype=np.int64)
-    tri[0::2] = t0
-    tri[1::2] = t1
-    return _from_triangles(xs, ys, pbc, tri, wall_of=lambda p, q: (jj[p] == 0) & (jj[q] == 0))
+    wall_of = lambda p, q: (jj[p] == 0) & (jj[q] == 0)
+    if cells == "tri":
+        tri = np.empty((2 * a.size, 3), dtype=np.int64)
+        tri[0::2] = t0
+        tri[1::2] = t1
+        return _from_triangles(xs, ys, pbc, tri, wall_of=wall_of)
+    if cells not in ("quad", "mixed"):
+        raise ValueError(f"cells must be tri, quad or mixed, not {cells!r}")
+    quad = np.stack([a, b, c, d], 1)
+    split = (rng.integers(0, 2, size=a.shape) == 1) if cells == "mixed" else np.zeros(a.shape, bool)
+    polys = []
+    for k in range(a.size):
+        if split[k]:
+            polys.append(list(t0[k]))
+            polys.append(list(t1[k]))
+        else:
+            polys.append(list(quad[k]))
+    return _from_polygons(xs, ys, pbc, polys, wall_of=wall_of)
 
 
 def _from_triangles(xs, ys, pbc, tri, wall_of) -> RawMesh:
@@ -203,6 +222,66 @@ def _from_triangles(xs, ys, pbc, tri, wall_of) -> RawMesh:
                    cen=np.ascontiguousarray(cen), area=np.ascontiguousarray(area),
                    fint=(fint + 1).astype(np.int32), fbnd=(fbnd + 1).astype(np.int32),
                    fbn=np.ascontiguousarray(fbn), fbc=fbc, tri=(tri + 1).astype(np.int32))
+
+
+def _from_polygons(xs, ys, pbc, polys, wall_of) -> RawMesh:
+    """Build faces from counter-clockwise polygons (lists of 0-based vertex
+    ids), numbered as :func:`_from_triangles` numbers them: in order of
+    first appearance walking the cells in order and their edges (v0v1, v1v2,
+    ..., v(k-1)v0); the first cell to meet an edge is its ``in`` cell.  Area
+    and centroid by the shoelace formula."""
+    nc = len(polys)
+    ha, hb, hc = [], [], []
+    cen = np.empty((nc, 2))
+    area = np.empty(nc)
+    for c, pv in enumerate(polys):
+        k = len(pv)
+        px, py = xs[pv], ys[pv]
+        qx, qy = np.roll(px, -1), np.roll(py, -1)
+        cr = px * qy - qx * py
+        a2 = cr.sum()
+        area[c] = 0.5 * a2
+        cen[c, 0] = ((px + qx) * cr).sum() / (3.0 * a2)
+        cen[c, 1] = ((py + qy) * cr).sum() / (3.0 * a2)
+        for i in range(k):
+            ha.append(pv[i])
+            hb.append(pv[(i + 1) % k])
+            hc.append(c)
+    if np.any(area <= 0):
+        raise ValueError("generator produced a non-positive cell area")
+    ha, hb, hc = np.array(ha), np.array(hb), np.array(hc)
+    nop = xs.shape[0]
+    key = np.minimum(ha, hb) * nop + np.maximum(ha, hb)
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    start = np.flatnonzero(np.r_[True, ks[1:] != ks[:-1]])
+    cnt = np.diff(np.r_[start, ks.size])
+    if np.any(cnt > 2):
+        raise ValueError("non-manifold edge")
+    first = order[start]
+    second = np.where(cnt == 2, order[np.minimum(start + 1, ks.size - 1)], -1)
+    is_int = cnt == 2
+    fi_first = first[is_int]
+    fi_second = second[is_int]
+    o = np.argsort(fi_first, kind="stable")
+    fi_first, fi_second = fi_first[o], fi_second[o]
+    fint = np.stack([ha[fi_first], hb[fi_first], hc[fi_first], hc[fi_second]], 1)
+    fb_first = np.sort(first[~is_int])
+    p1, p2 = ha[fb_first], hb[fb_first]
+    fbnd = np.stack([p1, p2, hc[fb_first]], 1)
+    fbn = np.stack([ys[p2] - ys[p1], -(xs[p2] - xs[p1])], 1)
+    fbc = np.where(wall_of(p1, p2), WALL, FARFIELD).astype(np.int32)
+    kmax = max(len(pv) for pv in polys)
+    tri = np.full((nc, kmax), -1, dtype=np.int64)
+    for c, pv in enumerate(polys):
+        tri[c, :len(pv)] = pv
+    return RawMesh(x=np.ascontiguousarray(xs, dtype=np.float64),
+                   y=np.ascontiguousarray(ys, dtype=np.float64),
+                   pbc=pbc.astype(np.int32),
+                   cen=np.ascontiguousarray(cen), area=np.ascontiguousarray(area),
+                   fint=(fint + 1).astype(np.int32), fbnd=(fbnd + 1).astype(np.int32),
+                   fbn=np.ascontiguousarray(fbn), fbc=fbc,
+                   tri=np.where(tri >= 0, tri + 1, -1).astype(np.int32))
 
 
 def config_mesh(config: int, scale: float = 1.0, seed: int | None = None) -> RawMesh:
