@@ -100,6 +100,22 @@ int ufx_cc_geometric(int nop, int noc, int nfi, int nbf,
                      const double* fb_nx, const double* fb_ny, const int* fb_bc,
                      ufx_mesh** mesh);
 
+/* The same preprocessing with its per-face, per-cell and per-vertex passes
+ * on the current GPU (ghost centres and cell_face_norm, face co-volumes and
+ * their cell sums, LSQR_Coeff, pseudo_Laplacian; CCFVM/geometric.F90:62-198,
+ * 619-674, 898-1013); the connectivity and the breadth-first numbering
+ * (:200-616) run on the host between them.  Every array equals
+ * ufx_cc_geometric's bit for bit.  Allocates and frees its own device
+ * memory; UFX_ERR_HIP when a HIP call fails. */
+int ufx_cc_geometric_device(int nop, int noc, int nfi, int nbf,
+                            const double* px, const double* py,
+                            const double* ccx, const double* ccy, const double* cv,
+                            const int* fi_p1, const int* fi_p2, const int* fi_in,
+                            const int* fi_out, const int* fi_bc,
+                            const int* fb_p1, const int* fb_p2, const int* fb_in,
+                            const double* fb_nx, const double* fb_ny, const int* fb_bc,
+                            ufx_mesh** mesh);
+
 /* Build a mesh directly from arrays the reference holds after its own
  * `geometric` (CCFVM/types/grid.F90 fields fc%pt,in,out,bc,nx,ny,area,cen,cov;
  * cell%cen,cv,cov,r11,r12,r22,nc2c,c2f; pt%x,y,v2c,wt).  nof = nfi + nbf

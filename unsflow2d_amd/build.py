@@ -53,10 +53,11 @@ def build_lib() -> str:
     os.makedirs(bdir, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include")]
     objs = []
-    # the vertex-centred kernels evaluate every expression as the reference
-    # does (no a*b+c contraction), so their FP64 results follow it bit for bit
-    extra = {"nc_solver.hip": ["-ffp-contract=off"]}
-    for src in ("cc_solver.hip", "nc_solver.hip"):
+    # the vertex-centred kernels and the device preprocessing evaluate every
+    # expression as the reference does (no a*b+c contraction), so their FP64
+    # results follow it bit for bit
+    extra = {"nc_solver.hip": ["-ffp-contract=off"], "cc_mesh_dev.hip": ["-ffp-contract=off"]}
+    for src in ("cc_solver.hip", "nc_solver.hip", "cc_mesh_dev.hip"):
         obj = os.path.join(bdir, src + ".o")
         _run([HIPCC, *HIP_FLAGS, *extra.get(src, []), *inc, "-c", os.path.join(CSRC, src), "-o",
               obj])

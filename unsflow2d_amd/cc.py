@@ -73,6 +73,7 @@ def _sig(name, args, res=C.c_int):
 
 _sig("ufx_last_error", [], C.c_char_p)
 _sig("ufx_cc_geometric", [C.c_int] * 4 + [_dp] * 5 + [_ip] * 5 + [_ip] * 3 + [_dp] * 2 + [_ip, C.POINTER(_vp)])
+_sig("ufx_cc_geometric_device", [C.c_int] * 4 + [_dp] * 5 + [_ip] * 5 + [_ip] * 3 + [_dp] * 2 + [_ip, C.POINTER(_vp)])
 _sig("ufx_cc_mesh_import", [C.c_int] * 4 + [_dp] * 2 + [_ip] * 5 + [_dp] * 6 + [_dp] * 7 + [_ip] * 3 + [_ip] * 2 + [_dp, C.POINTER(_vp)])
 _sig("ufx_mesh_size", [_vp, C.c_char_p, C.POINTER(C.c_int64)])
 _sig("ufx_mesh_get", [_vp, C.c_char_p, _vp])
@@ -111,7 +112,7 @@ _sig("ufx_cc_batch_destroy", [_vp])
 KCAT = ["prim", "range", "avg_c2v", "time_step", "gradient", "stage", "norm", "other", "bface"]
 
 EXPORTED = [
-    "ufx_last_error", "ufx_cc_geometric", "ufx_cc_mesh_import", "ufx_mesh_size", "ufx_mesh_get",
+    "ufx_last_error", "ufx_cc_geometric", "ufx_cc_geometric_device", "ufx_cc_mesh_import", "ufx_mesh_size", "ufx_mesh_get",
     "ufx_mesh_destroy", "ufx_cc_create", "ufx_cc_destroy", "ufx_cc_set", "ufx_cc_get",
     "ufx_cc_prim", "ufx_cc_avg_c2v", "ufx_cc_time_step", "ufx_cc_save_old", "ufx_cc_gradient",
     "ufx_cc_limit", "ufx_cc_fvresidual", "ufx_cc_rk_update", "ufx_cc_rk_stage", "ufx_cc_lusgs",
@@ -154,8 +155,10 @@ class Mesh:
         self._h = handle
 
     @classmethod
-    def from_raw(cls, raw: meshgen.RawMesh) -> "Mesh":
-        """Preprocess a geometry.inp-form mesh (CCFVM/geometric.F90:1-57)."""
+    def from_raw(cls, raw: meshgen.RawMesh, device: bool = False) -> "Mesh":
+        """Preprocess a geometry.inp-form mesh (CCFVM/geometric.F90:1-57);
+        with ``device`` its per-face, per-cell and per-vertex passes run on
+        the GPU (the same arrays bit for bit)."""
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
         keep = dict(px=f64(raw.x), py=f64(raw.y), cx=f64(raw.cen[:, 0]), cy=f64(raw.cen[:, 1]),
@@ -165,11 +168,11 @@ class Mesh:
                     bnx=f64(raw.fbn[:, 0]), bny=f64(raw.fbn[:, 1]), bc=i32(raw.fbc))
         h = _vp()
         k = keep
-        _check(lib.ufx_cc_geometric(raw.nop, raw.noc, raw.nfi, raw.nbf, _d(k["px"]), _d(k["py"]),
-                                    _d(k["cx"]), _d(k["cy"]), _d(k["cv"]), _i(k["a"]), _i(k["b"]),
-                                    _i(k["i"]), _i(k["o"]), None, _i(k["ba"]), _i(k["bb"]),
-                                    _i(k["bi"]), _d(k["bnx"]), _d(k["bny"]), _i(k["bc"]),
-                                    C.byref(h)))
+        fn = lib.ufx_cc_geometric_device if device else lib.ufx_cc_geometric
+        _check(fn(raw.nop, raw.noc, raw.nfi, raw.nbf, _d(k["px"]), _d(k["py"]), _d(k["cx"]),
+                  _d(k["cy"]), _d(k["cv"]), _i(k["a"]), _i(k["b"]), _i(k["i"]), _i(k["o"]), None,
+                  _i(k["ba"]), _i(k["bb"]), _i(k["bi"]), _d(k["bnx"]), _d(k["bny"]), _i(k["bc"]),
+                  C.byref(h)))
         return cls(h)
 
     @classmethod

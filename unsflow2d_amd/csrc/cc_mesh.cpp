@@ -56,7 +56,9 @@ void push_unique(std::vector<int>& v, int val) {
 
 }  // namespace
 
-int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
+// The input arrays, checked, 0-based, ghosts attached (read_grid,
+// CCFVM/read_input.F90:179-203).
+int cc_geom_read(const RawInput& in, Mesh& m, std::string& err) {
   const int nop = in.nop, noc = in.noc, nfi = in.nfi, nbf = in.nbf;
   if (nop <= 0 || noc <= 0 || nfi < 0 || nbf < 0) {
     err = "invalid mesh counts";
@@ -110,10 +112,19 @@ int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
     m.fnx[i] = in.fb_nx[k];
     m.fny[i] = in.fb_ny[k];
     m.fbc[i] = in.fb_bc[k];
+  }
+  return UFX_OK;
+}
+
+// Ghost centres (read_grid's mirror) and cell_face_norm, per face.
+void cc_geom_faces(Mesh& m) {
+  const int noc = m.noc, nfi = m.nfi, nbf = m.nbf, nof = m.nof();
+  for (int k = 0; k < nbf; ++k) {
     // ghost centre = mirror of the inner centre about the face line
     // (pt2linemirror, CCFVM/types/tools.F90:30-46), file endpoint order
+    const int i = nfi + k, p1 = m.fp1[i], p2 = m.fp2[i], ci = m.fin[i];
     double sx = m.px[p1], sy = m.py[p1], ex = m.px[p2], ey = m.py[p2];
-    double px = m.ccx[ci - 1], py = m.ccy[ci - 1];
+    double px = m.ccx[ci], py = m.ccy[ci];
     double dx = ex - sx, dy = ey - sy;
     double x = sx - px, y = sy - py;
     double r = 1.0 / (dx * dx + dy * dy);
@@ -162,6 +173,12 @@ int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
     m.fnx[i] = m.fnx[i] / m.farea[i];
     m.fny[i] = m.fny[i] / m.farea[i];
   }
+}
+
+// Connectivity and the breadth-first numbering; the vertex -> cell lists go
+// to m.v2c_ptr / m.v2c (the pseudo-Laplacian weights follow their order).
+int cc_geom_connect(Mesh& m, std::string& err) {
+  const int nop = m.nop, noc = m.noc, nfi = m.nfi, nof = m.nof();
 
   // ---- cell_2_cell (CCFVM/geometric.F90:200-245), interior faces only
   std::vector<std::vector<int>> c2c(noc);
@@ -256,6 +273,18 @@ int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
   m.c2f.resize(m.c2f_ptr[noc]);
   for (int i = 0; i < noc; ++i)
     std::memcpy(&m.c2f[m.c2f_ptr[i]], c2f[i].data(), c2f[i].size() * sizeof(int));
+  m.v2c_ptr.assign(nop + 1, 0);
+  for (int i = 0; i < nop; ++i) m.v2c_ptr[i + 1] = m.v2c_ptr[i] + (int)v2c[i].size();
+  m.v2c.resize(m.v2c_ptr[nop]);
+  for (int i = 0; i < nop; ++i)
+    if (!v2c[i].empty())
+      std::memcpy(&m.v2c[m.v2c_ptr[i]], v2c[i].data(), v2c[i].size() * sizeof(int));
+  return UFX_OK;
+}
+
+// Face co-volumes, least-squares factors and pseudo-Laplacian weights.
+int cc_geom_cells(Mesh& m, std::string& err) {
+  const int nop = m.nop, noc = m.noc, nfi = m.nfi, nof = m.nof(), ncell = m.ncell();
 
   // ---- face_co_volume (CCFVM/geometric.F90:619-674)
   m.ccov.assign(ncell, 0.0);
@@ -302,13 +331,11 @@ int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
   }
 
   // ---- pseudo_Laplacian (CCFVM/geometric.F90:962-1013)
-  m.v2c_ptr.assign(nop + 1, 0);
-  for (int i = 0; i < nop; ++i) m.v2c_ptr[i + 1] = m.v2c_ptr[i] + (int)v2c[i].size();
-  m.v2c.resize(m.v2c_ptr[nop]);
   m.wt.resize(m.v2c_ptr[nop]);
   for (int i = 0; i < nop; ++i) {
     double ixx = 0.0, iyy = 0.0, ixy = 0.0, rx = 0.0, ry = 0.0;
-    for (int cc : v2c[i]) {
+    for (int j = m.v2c_ptr[i]; j < m.v2c_ptr[i + 1]; ++j) {
+      const int cc = m.v2c[j];
       double dx = m.ccx[cc] - m.px[i], dy = m.ccy[cc] - m.py[i];
       rx = rx + dx; ry = ry + dy;
       ixx = ixx + dx * dx;
@@ -319,15 +346,20 @@ int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
     if (det < 0.0) { err = "Determinant -ve @ pseudo_Laplacian"; return UFX_ERR_MESH; }
     double gx = (ixy * ry - iyy * rx) / det;
     double gy = (ixy * rx - ixx * ry) / det;
-    int o = m.v2c_ptr[i];
-    for (size_t j = 0; j < v2c[i].size(); ++j) {
-      int cc = v2c[i][j];
+    for (int j = m.v2c_ptr[i]; j < m.v2c_ptr[i + 1]; ++j) {
+      const int cc = m.v2c[j];
       double dx = m.ccx[cc] - m.px[i], dy = m.ccy[cc] - m.py[i];
-      m.v2c[o + j] = cc;
-      m.wt[o + j] = 1.0 + gx * dx + gy * dy;
+      m.wt[j] = 1.0 + gx * dx + gy * dy;
     }
   }
   return UFX_OK;
+}
+
+int cc_geometric(const RawInput& in, Mesh& m, std::string& err) {
+  if (int rc = cc_geom_read(in, m, err)) return rc;
+  cc_geom_faces(m);
+  if (int rc = cc_geom_connect(m, err)) return rc;
+  return cc_geom_cells(m, err);
 }
 
 }  // namespace ufx

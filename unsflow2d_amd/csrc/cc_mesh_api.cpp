@@ -57,18 +57,18 @@ std::vector<int> lusgs_colours(const Mesh& m, int* ncolours) {
 
 }  // namespace ufx
 
-extern "C" {
+namespace ufx {
+namespace {
 
-const char* ufx_last_error(void) { return g_err.c_str(); }
-
-int ufx_cc_geometric(int nop, int noc, int nfi, int nbf, const double* px, const double* py,
-                     const double* ccx, const double* ccy, const double* cv, const int* fi_p1,
-                     const int* fi_p2, const int* fi_in, const int* fi_out, const int* fi_bc,
-                     const int* fb_p1, const int* fb_p2, const int* fb_in, const double* fb_nx,
-                     const double* fb_ny, const int* fb_bc, ufx_mesh** mesh) {
+int cc_geometric_entry(const char* who, bool device, int nop, int noc, int nfi, int nbf,
+                       const double* px, const double* py, const double* ccx, const double* ccy,
+                       const double* cv, const int* fi_p1, const int* fi_p2, const int* fi_in,
+                       const int* fi_out, const int* fi_bc, const int* fb_p1, const int* fb_p2,
+                       const int* fb_in, const double* fb_nx, const double* fb_ny,
+                       const int* fb_bc, ufx_mesh** mesh) {
   if (!mesh || !px || !py || !ccx || !ccy || !cv || (nfi > 0 && (!fi_p1 || !fi_p2 || !fi_in || !fi_out)) ||
       (nbf > 0 && (!fb_p1 || !fb_p2 || !fb_in || !fb_nx || !fb_ny || !fb_bc))) {
-    set_error("ufx_cc_geometric: null argument");
+    set_error(std::string(who) + ": null argument");
     return UFX_ERR_ARG;
   }
   RawInput in;
@@ -79,7 +79,7 @@ int ufx_cc_geometric(int nop, int noc, int nfi, int nbf, const double* px, const
   in.fb_bc = fb_bc;
   ufx_mesh* m = new ufx_mesh();
   std::string err;
-  int rc = cc_geometric(in, m->m, err);
+  int rc = device ? cc_geometric_device(in, m->m, err) : cc_geometric(in, m->m, err);
   if (rc) {
     delete m;
     set_error(err);
@@ -87,6 +87,35 @@ int ufx_cc_geometric(int nop, int noc, int nfi, int nbf, const double* px, const
   }
   *mesh = m;
   return UFX_OK;
+}
+
+}  // namespace
+}  // namespace ufx
+
+extern "C" {
+
+const char* ufx_last_error(void) { return g_err.c_str(); }
+
+int ufx_cc_geometric(int nop, int noc, int nfi, int nbf, const double* px, const double* py,
+                     const double* ccx, const double* ccy, const double* cv, const int* fi_p1,
+                     const int* fi_p2, const int* fi_in, const int* fi_out, const int* fi_bc,
+                     const int* fb_p1, const int* fb_p2, const int* fb_in, const double* fb_nx,
+                     const double* fb_ny, const int* fb_bc, ufx_mesh** mesh) {
+  return ufx::cc_geometric_entry("ufx_cc_geometric", false, nop, noc, nfi, nbf, px, py, ccx, ccy, cv,
+                            fi_p1, fi_p2, fi_in, fi_out, fi_bc, fb_p1, fb_p2, fb_in, fb_nx, fb_ny,
+                            fb_bc, mesh);
+}
+
+int ufx_cc_geometric_device(int nop, int noc, int nfi, int nbf, const double* px,
+                            const double* py, const double* ccx, const double* ccy,
+                            const double* cv, const int* fi_p1, const int* fi_p2,
+                            const int* fi_in, const int* fi_out, const int* fi_bc,
+                            const int* fb_p1, const int* fb_p2, const int* fb_in,
+                            const double* fb_nx, const double* fb_ny, const int* fb_bc,
+                            ufx_mesh** mesh) {
+  return ufx::cc_geometric_entry("ufx_cc_geometric_device", true, nop, noc, nfi, nbf, px, py, ccx,
+                            ccy, cv, fi_p1, fi_p2, fi_in, fi_out, fi_bc, fb_p1, fb_p2, fb_in,
+                            fb_nx, fb_ny, fb_bc, mesh);
 }
 
 int ufx_nc_geometric(int nop, int noc, int nof, const double* px, const double* py,

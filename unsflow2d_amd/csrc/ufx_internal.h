@@ -52,6 +52,17 @@ struct Mesh {
 };
 
 int cc_geometric(const RawInput& in, Mesh& m, std::string& err);
+// Its phases (cc_geometric = read, faces, connect, cells): the checked input
+// with ghosts attached; per face the ghost centres and cell_face_norm; the
+// connectivity and breadth-first numbering (vertex -> cell lists into
+// m.v2c_ptr / m.v2c); per cell and vertex the co-volumes, least-squares
+// factors and pseudo-Laplacian weights.  The per-face and per-cell phases
+// also run on the GPU (cc_geometric_device, cc_mesh_dev.hip).
+int cc_geom_read(const RawInput& in, Mesh& m, std::string& err);
+void cc_geom_faces(Mesh& m);
+int cc_geom_connect(Mesh& m, std::string& err);
+int cc_geom_cells(Mesh& m, std::string& err);
+int cc_geometric_device(const RawInput& in, Mesh& m, std::string& err);
 
 // Raw vertex-centred mesh as NCFVM/read_input.f90:101-112 reads it (1-based,
 // 0 = no cell on that side of an edge).
