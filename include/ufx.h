@@ -152,6 +152,19 @@ int ufx_nc_geometric(int nop, int noc, int nof, const double* px, const double* 
                      const int* e_p1, const int* e_p2, const int* e_in, const int* e_out,
                      const int* e_bc, ufx_mesh** mesh);
 
+/* The same preprocessing with its per-edge and per-vertex passes on the
+ * current GPU (dual-face normals and lengths, the co-volume check, median-
+ * dual volumes, least-squares weights; NCFVM/geometric.f90:198-313,
+ * 668-804); the neighbour lists and the breadth-first numbering (:52-180,
+ * 347-379) run on the host first.  Every array equals ufx_nc_geometric's bit
+ * for bit.  Allocates and frees its own device memory; UFX_ERR_HIP when a
+ * HIP call fails. */
+int ufx_nc_geometric_device(int nop, int noc, int nof, const double* px, const double* py,
+                            const int* pbc, const double* ccx, const double* ccy,
+                            const double* cv, const int* e_p1, const int* e_p2,
+                            const int* e_in, const int* e_out, const int* e_bc,
+                            ufx_mesh** mesh);
+
 /* Element count of a named mesh array (see ufx_mesh_get). */
 int ufx_mesh_size(const ufx_mesh* mesh, const char* name, int64_t* count);
 

@@ -83,3 +83,44 @@ def test_device_preprocessing_reports_reference_errors():
             msgs.append(str(ei.value))
         print(f"  {msgs}", flush=True)
         assert msgs[0] == msgs[1] == "[mesh] geom,triarea,  -ve area"
+
+
+def _nc_same(raw, label):
+    from unsflow2d_amd import nc
+    t0 = time.perf_counter()
+    host = nc.NCMesh.from_raw(raw)
+    t1 = time.perf_counter()
+    dev = nc.NCMesh.from_raw(raw, device=True)
+    t2 = time.perf_counter()
+    print(f"  {label}: host {t1 - t0:.3f}s, device passes {t2 - t1:.3f}s", flush=True)
+    for name in nc.NC_MESH_I32 + nc.NC_MESH_F64:
+        a, b = host.get(name), dev.get(name)
+        assert a.dtype == b.dtype, name
+        assert np.array_equal(a, b), f"{label}: {name} differs"
+
+
+@pytest.mark.parametrize("shape", [(24, 10, 3, 1.2), (200, 100, 5, 1.02), (2000, 1000, 3, 1.005)])
+def test_nc_device_preprocessing_equals_host(shape):
+    """The vertex-centred preprocessing (ufx_nc_geometric_device: dual-face
+    normals, median-dual volumes, co-volume check, least-squares weights on
+    the GPU) against the host form, up to the 2 000 000-vertex O-grid."""
+    from unsflow2d_amd import nc_mesh
+    ni, nj, seed, stretch = shape
+    print(f"case: nc device preprocessing, o-grid {ni}x{nj} seed {seed}", flush=True)
+    _nc_same(nc_mesh.o_grid(ni, nj, seed, stretch=stretch), f"o-grid {ni}x{nj}")
+
+
+def test_nc_device_preprocessing_reports_reference_errors():
+    """A cell centroid moved far out of place makes an edge's co-volume
+    non-positive: the reference's STOP message names the same edge on the
+    device as on the host."""
+    from unsflow2d_amd import nc, nc_mesh
+    raw = nc_mesh.o_grid(24, 10, 3, stretch=1.2)
+    raw.cen[7] = raw.cen[7] + 50.0
+    msgs = []
+    for device in (False, True):
+        with pytest.raises(nc.UfxError) as ei:
+            nc.NCMesh.from_raw(raw, device=device)
+        msgs.append(str(ei.value))
+    print(f"  {msgs}", flush=True)
+    assert msgs[0] == msgs[1] and "-ve face co-volume" in msgs[0]

@@ -56,8 +56,9 @@ def build_lib() -> str:
     # the vertex-centred kernels and the device preprocessing evaluate every
     # expression as the reference does (no a*b+c contraction), so their FP64
     # results follow it bit for bit
-    extra = {"nc_solver.hip": ["-ffp-contract=off"], "cc_mesh_dev.hip": ["-ffp-contract=off"]}
-    for src in ("cc_solver.hip", "nc_solver.hip", "cc_mesh_dev.hip"):
+    nofma = ["-ffp-contract=off"]
+    extra = {"nc_solver.hip": nofma, "cc_mesh_dev.hip": nofma, "nc_mesh_dev.hip": nofma}
+    for src in ("cc_solver.hip", "nc_solver.hip", "cc_mesh_dev.hip", "nc_mesh_dev.hip"):
         obj = os.path.join(bdir, src + ".o")
         _run([HIPCC, *HIP_FLAGS, *extra.get(src, []), *inc, "-c", os.path.join(CSRC, src), "-o",
               obj])

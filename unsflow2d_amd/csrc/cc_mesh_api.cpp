@@ -89,6 +89,31 @@ int cc_geometric_entry(const char* who, bool device, int nop, int noc, int nfi, 
   return UFX_OK;
 }
 
+
+int nc_geometric_entry(const char* who, bool device, int nop, int noc, int nof, const double* px,
+                       const double* py, const int* pbc, const double* ccx, const double* ccy,
+                       const double* cv, const int* e_p1, const int* e_p2, const int* e_in,
+                       const int* e_out, const int* e_bc, ufx_mesh** mesh) {
+  if (!mesh || !px || !py || !ccx || !ccy || !cv || !e_p1 || !e_p2 || !e_in || !e_out || !e_bc) {
+    set_error(std::string(who) + ": null argument");
+    return UFX_ERR_ARG;
+  }
+  NCRawInput in;
+  in.nop = nop; in.noc = noc; in.nof = nof;
+  in.px = px; in.py = py; in.pbc = pbc; in.ccx = ccx; in.ccy = ccy; in.cv = cv;
+  in.e_p1 = e_p1; in.e_p2 = e_p2; in.e_in = e_in; in.e_out = e_out; in.e_bc = e_bc;
+  ufx_mesh* m = new ufx_mesh();
+  std::string err;
+  int rc = device ? nc_geometric_device(in, m->m, err) : nc_geometric(in, m->m, err);
+  if (rc) {
+    delete m;
+    set_error(err);
+    return rc;
+  }
+  *mesh = m;
+  return UFX_OK;
+}
+
 }  // namespace
 }  // namespace ufx
 
@@ -122,24 +147,17 @@ int ufx_nc_geometric(int nop, int noc, int nof, const double* px, const double* 
                      const int* pbc, const double* ccx, const double* ccy, const double* cv,
                      const int* e_p1, const int* e_p2, const int* e_in, const int* e_out,
                      const int* e_bc, ufx_mesh** mesh) {
-  if (!mesh || !px || !py || !ccx || !ccy || !cv || !e_p1 || !e_p2 || !e_in || !e_out || !e_bc) {
-    set_error("ufx_nc_geometric: null argument");
-    return UFX_ERR_ARG;
-  }
-  NCRawInput in;
-  in.nop = nop; in.noc = noc; in.nof = nof;
-  in.px = px; in.py = py; in.pbc = pbc; in.ccx = ccx; in.ccy = ccy; in.cv = cv;
-  in.e_p1 = e_p1; in.e_p2 = e_p2; in.e_in = e_in; in.e_out = e_out; in.e_bc = e_bc;
-  ufx_mesh* m = new ufx_mesh();
-  std::string err;
-  int rc = nc_geometric(in, m->m, err);
-  if (rc) {
-    delete m;
-    set_error(err);
-    return rc;
-  }
-  *mesh = m;
-  return UFX_OK;
+  return ufx::nc_geometric_entry("ufx_nc_geometric", false, nop, noc, nof, px, py, pbc, ccx, ccy,
+                                 cv, e_p1, e_p2, e_in, e_out, e_bc, mesh);
+}
+
+int ufx_nc_geometric_device(int nop, int noc, int nof, const double* px, const double* py,
+                            const int* pbc, const double* ccx, const double* ccy,
+                            const double* cv, const int* e_p1, const int* e_p2,
+                            const int* e_in, const int* e_out, const int* e_bc,
+                            ufx_mesh** mesh) {
+  return ufx::nc_geometric_entry("ufx_nc_geometric_device", true, nop, noc, nof, px, py, pbc,
+                                 ccx, ccy, cv, e_p1, e_p2, e_in, e_out, e_bc, mesh);
 }
 
 int ufx_cc_mesh_import(int nop, int noc, int nfi, int nbf, const double* px, const double* py,

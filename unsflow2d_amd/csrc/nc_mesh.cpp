@@ -23,7 +23,9 @@
 
 namespace ufx {
 
-int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err) {
+// The checked input, the neighbour lists, the breadth-first numbering and
+// the edges of each vertex (connectivity).
+int nc_geom_connect(const NCRawInput& in, Mesh& m, std::string& err) {
   const int nop = in.nop, noc = in.noc, nof = in.nof;
   if (nop <= 0 || noc <= 0 || nof <= 0) {
     err = "nc_geometric: empty mesh";
@@ -105,6 +107,28 @@ int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err) {
     m.fp2[i] = newnum[in.e_p2[i] - 1];
   }
 
+  // vertex_2_face, :355-374
+  {
+    std::vector<std::vector<int>> vf(nop);
+    for (int i = 0; i < nof; ++i) {
+      vf[m.fp1[i]].push_back(i);
+      vf[m.fp2[i]].push_back(i);
+    }
+    m.v2f_ptr.assign(nop + 1, 0);
+    for (int i = 0; i < nop; ++i) {
+      for (int f : vf[i]) m.v2f.push_back(f);
+      m.v2f_ptr[i + 1] = (int)m.v2f.size();
+    }
+  }
+
+  return UFX_OK;
+}
+
+// Dual-face normals, median-dual volumes, the co-volume check and the
+// least-squares weights (per edge and per vertex).
+int nc_geom_numeric(Mesh& m, std::string& err) {
+  const int nop = m.nop, nof = m.nfi;
+
   // cell_face_norm_c2f_dist, :198-255: dual-face normal of each edge, the
   // sum of the half normals centroid(in)->midpoint and midpoint->centroid(out),
   // each turned to point along p1->p2
@@ -183,20 +207,6 @@ int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err) {
     }
   }
 
-  // vertex_2_face, :355-374
-  {
-    std::vector<std::vector<int>> vf(nop);
-    for (int i = 0; i < nof; ++i) {
-      vf[m.fp1[i]].push_back(i);
-      vf[m.fp2[i]].push_back(i);
-    }
-    m.v2f_ptr.assign(nop + 1, 0);
-    for (int i = 0; i < nop; ++i) {
-      for (int f : vf[i]) m.v2f.push_back(f);
-      m.v2f_ptr[i + 1] = (int)m.v2f.size();
-    }
-  }
-
   // face_co_volume, :668-738: the co-volume itself feeds nothing; its sign
   // check stops the reference
   for (int i = 0; i < nof; ++i) {
@@ -255,6 +265,11 @@ int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err) {
     }
   }
   return UFX_OK;
+}
+
+int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err) {
+  if (int rc = nc_geom_connect(in, m, err)) return rc;
+  return nc_geom_numeric(m, err);
 }
 
 }  // namespace ufx

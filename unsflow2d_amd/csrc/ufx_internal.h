@@ -76,6 +76,11 @@ struct NCRawInput {
 };
 
 int nc_geometric(const NCRawInput& in, Mesh& m, std::string& err);
+// Its phases: connectivity and numbering, then the per-edge and per-vertex
+// geometry (also on the GPU, nc_geometric_device, nc_mesh_dev.hip).
+int nc_geom_connect(const NCRawInput& in, Mesh& m, std::string& err);
+int nc_geom_numeric(Mesh& m, std::string& err);
+int nc_geometric_device(const NCRawInput& in, Mesh& m, std::string& err);
 
 // Greedy distance-1 colouring of a graph given as adjacency lists (ptr has
 // n + 1 offsets): vertices in index order, each taking the smallest colour

@@ -23,6 +23,8 @@ _dp = C.POINTER(C.c_double)
 
 _sig("ufx_nc_geometric", [C.c_int] * 3 + [_dp, _dp, _ip, _dp, _dp, _dp] + [_ip] * 5
      + [C.POINTER(_vp)])
+_sig("ufx_nc_geometric_device", [C.c_int] * 3 + [_dp, _dp, _ip, _dp, _dp, _dp] + [_ip] * 5
+     + [C.POINTER(_vp)])
 
 
 class NCParams(C.Structure):
@@ -46,7 +48,7 @@ _sig("ufx_nc_profile", [_vp, C.c_int])
 _sig("ufx_nc_profile_read", [_vp, _dp, _ip])
 _sig("ufx_nc_forces", [_vp, _dp, _dp])
 
-NC_EXPORTED = ["ufx_nc_geometric", "ufx_nc_create", "ufx_nc_destroy", "ufx_nc_save_old",
+NC_EXPORTED = ["ufx_nc_geometric", "ufx_nc_geometric_device", "ufx_nc_create", "ufx_nc_destroy", "ufx_nc_save_old",
                "ufx_nc_fvresidual", "ufx_nc_rk_stage", "ufx_nc_lusgs", "ufx_nc_residue",
                "ufx_nc_iterate", "ufx_nc_set", "ufx_nc_get", "ufx_nc_sync", "ufx_nc_profile",
                "ufx_nc_profile_read", "ufx_nc_forces"]
@@ -65,7 +67,9 @@ class NCMesh:
         self._h = handle
 
     @classmethod
-    def from_raw(cls, raw: nc_mesh.NCRawMesh) -> "NCMesh":
+    def from_raw(cls, raw: nc_mesh.NCRawMesh, device: bool = False) -> "NCMesh":
+        """Preprocess (NCFVM/geometric.f90:1-46); with ``device`` the per-edge
+        and per-vertex passes run on the GPU (the same arrays bit for bit)."""
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
         k = dict(px=f64(raw.x), py=f64(raw.y), pbc=i32(raw.pbc), cx=f64(raw.cen[:, 0]),
@@ -73,10 +77,10 @@ class NCMesh:
                  b=i32(raw.edge[:, 1]), i=i32(raw.edge[:, 2]), o=i32(raw.edge[:, 3]),
                  bc=i32(raw.ebc))
         h = _vp()
-        _check(lib.ufx_nc_geometric(raw.nop, raw.noc, raw.nof, _d(k["px"]), _d(k["py"]),
-                                    _i(k["pbc"]), _d(k["cx"]), _d(k["cy"]), _d(k["cv"]),
-                                    _i(k["a"]), _i(k["b"]), _i(k["i"]), _i(k["o"]), _i(k["bc"]),
-                                    C.byref(h)))
+        fn = lib.ufx_nc_geometric_device if device else lib.ufx_nc_geometric
+        _check(fn(raw.nop, raw.noc, raw.nof, _d(k["px"]), _d(k["py"]), _i(k["pbc"]), _d(k["cx"]),
+                  _d(k["cy"]), _d(k["cv"]), _i(k["a"]), _i(k["b"]), _i(k["i"]), _i(k["o"]),
+                  _i(k["bc"]), C.byref(h)))
         return cls(h)
 
     def get(self, name: str) -> np.ndarray:
