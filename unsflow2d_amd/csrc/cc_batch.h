@@ -44,13 +44,16 @@ __global__ __launch_bounds__(BLK) void km_prim(const CaseDesc<R>* cs, const int4
   prim_body<R>(e.d, vb, e.g, do_prim, do_old, do_range);
 }
 
-// one block per case
+// one block per case; the partials come from the primitive refresh (one
+// per block of all cells) or from the previous iteration's last stage (one
+// per block of interior cells)
 template <typename R>
-__global__ __launch_bounds__(BLK) void km_range_final(const CaseDesc<R>* cs, const int4* map) {
+__global__ __launch_bounds__(BLK) void km_range_final(const CaseDesc<R>* cs, const int4* map,
+                                                      int from_stage) {
   int c;
   const VBlock vb = case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  range_final_body<R>(e.d, vb, e.nblk_all);
+  range_final_body<R>(e.d, vb, from_stage ? e.nblk_cell : e.nblk_all);
 }
 
 template <typename R, bool VISC>
@@ -89,21 +92,21 @@ __global__ __launch_bounds__(BLK) void km_bface(const CaseDesc<R>* cs, const int
 
 template <typename R, int FLUX, bool SL>
 __global__ __launch_bounds__(BLK) void km_stage(const CaseDesc<R>* cs, const int4* map, int mode,
-                                                double ak, double bk, int write_res) {
+                                                double ak, double bk, int write_res, int epi) {
   int c;
   const VBlock vb = case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  stage_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, 0);
+  stage_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, epi);
 }
 
 template <typename R, int FLUX, bool SL>
 __global__ __launch_bounds__(BLK) void km_stage_blk(const CaseDesc<R>* cs, const int4* map,
                                                     int mode, double ak, double bk,
-                                                    int write_res) {
+                                                    int write_res, int epi) {
   int c;
   const VBlock vb = case_block(map, c);
   const CaseDesc<R> e = cs[c];
-  stage_blk_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, 0);
+  stage_blk_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, epi);
 }
 
 // one block per case; history slot k of every case

@@ -1794,16 +1794,18 @@ int batch_write(ufx_cc_batch* b) {
 }
 
 // One inviscid RK3 iteration of every active case, as run_iteration does it
-// for one context (the first active case's options hold for all).
+// for one context (the first active case's options hold for all): the
+// primitive refresh only when `refresh` (the first iteration of a call), the
+// last stage refreshing them for the next iteration.
 template <typename R>
-int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep) {
+int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep, bool refresh) {
   const CaseDesc<R>* cs = (const CaseDesc<R>*)b->d_desc;
   const hipStream_t s = b->stream;
   auto M = [&](int t) { return (const int4*)(b->d_map + b->off[t]); };
   auto G = [&](int t) { return dim3((unsigned)b->cnt[t]); };
   const bool lim = x0->p.ilimit != 0;
-  km_prim<R><<<G(T_ALL), BLK, 0, s>>>(cs, M(T_ALL), 1, 2, lim ? 1 : 0);
-  if (lim) km_range_final<R><<<G(T_CASE), BLK, 0, s>>>(cs, M(T_CASE));
+  if (refresh) km_prim<R><<<G(T_ALL), BLK, 0, s>>>(cs, M(T_ALL), 1, 2, lim ? 1 : 0);
+  if (lim) km_range_final<R><<<G(T_CASE), BLK, 0, s>>>(cs, M(T_CASE), refresh ? 0 : 1);
   km_avg_c2v<R, false><<<G(T_PT), BLK, 0, s>>>(cs, M(T_PT));
   km_face_lambda<R><<<G(T_FACE), BLK, 0, s>>>(cs, M(T_FACE));
 #define KMC(GR)                                                                            \
@@ -1817,18 +1819,21 @@ int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep) {
   for (int st = 1; st <= 3; ++st) {
     const double ak = RK_A[st - 1], bk = RK_B[st - 1];
     const int mode = (st == 3 ? 2 : 1) | (st << 2), wr = st == 3 && keep ? 1 : 0;
+    const int epi = st == 3 ? 1 : 0;
     const size_t lds = b->stage_lds;
 #define KMS(F)                                                                                  \
   do {                                                                                          \
     if (b->cnt[T_BF]) km_bface<R, F><<<G(T_BF), BLK, 0, s>>>(cs, M(T_BF), 0, st);               \
     if (x0->stage_shared && x0->stage_local)                                                    \
-      km_stage_blk<R, F, true><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr);    \
+      km_stage_blk<R, F, true><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr,     \
+                                                           epi);                                \
     else if (x0->stage_shared)                                                                  \
-      km_stage_blk<R, F, false><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr);   \
+      km_stage_blk<R, F, false><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr,    \
+                                                            epi);                               \
     else if (x0->stage_local)                                                                   \
-      km_stage<R, F, true><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr);        \
+      km_stage<R, F, true><<<G(T_CELL), BLK, lds, s>>>(cs, M(T_CELL), mode, ak, bk, wr, epi);   \
     else                                                                                        \
-      km_stage<R, F, false><<<G(T_CELL), BLK, 0, s>>>(cs, M(T_CELL), mode, ak, bk, wr);         \
+      km_stage<R, F, false><<<G(T_CELL), BLK, 0, s>>>(cs, M(T_CELL), mode, ak, bk, wr, epi);    \
   } while (0)
     switch (x0->p.flux) {
       case UFX_FLUX_ROE: KMS(1); break;
@@ -1888,12 +1893,13 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
                      b->chunk.keep != keep || niter > b->chunk.n;
   if (!fresh && niter < b->chunk.n) {
     for (int k = 0; k < niter; ++k)
-      if (int rc = batch_iteration<R>(b, x0, k, cfl, keep)) return rc;
+      if (int rc = batch_iteration<R>(b, x0, k, cfl, keep, k == 0)) return rc;
   } else {
     if (fresh) {
       HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
       int rc = 0;
-      for (int k = 0; k < niter && !rc; ++k) rc = batch_iteration<R>(b, x0, k, cfl, keep);
+      for (int k = 0; k < niter && !rc; ++k)
+        rc = batch_iteration<R>(b, x0, k, cfl, keep, k == 0);
       Chunk c;
       hipError_t e = hipStreamEndCapture(b->stream, &c.graph);
       if (!rc && e == hipSuccess) e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
@@ -1925,6 +1931,7 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
     if (!b->on[i]) continue;
     b->cs[i]->cfl = cfl;
     b->cs[i]->qold_stale = true;  // batched cases run the fused RK3 iteration
+    b->cs[i]->prims_next = true;  // and its last stage refreshes the primitives
     if (err[3 * j] != 0) {
       int rc = device_error(b->cs[i]);
       set_error("case " + std::to_string(i) + ": " + ufx_last_error());
