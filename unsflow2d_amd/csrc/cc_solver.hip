@@ -1861,8 +1861,9 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
       return UFX_ERR_ARG;
     }
     if (x->prof) { set_error("ufx_cc_batch_iterate: a case is profiling"); return UFX_ERR_ARG; }
-    // batched iterations refresh the primitives at their top
-    if (int rc = prims_current(x)) return rc;
+    // the batch's first iteration refreshes the primitives from the state
+    // (and notes their positivity again), whatever the context held
+    x->prims_next = false;
     // the case's own queued work first
     HIPCHK(hipStreamSynchronize(x->stream));
     if (int rc = ensure_hist(x, niter)) return rc;
