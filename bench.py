@@ -279,13 +279,14 @@ def copy_probe(mib: int) -> int:
 
 def nc_cpu_baseline(budget_s: float = 20.0) -> dict | None:
     """The reference vertex-centred solver (oracle/_ref/nc_fast) on a bounded
-    sample of the vertex-centred workload: the same generator at 200 x 100
-    points (20 000 vertices), LU-SGS, GG, single thread."""
+    sample of the vertex-centred workload: the same generator at 500 x 250
+    points (125 000 vertices, a working set well beyond one core's 32 MiB of
+    L3), LU-SGS, GG, single thread."""
     exe = os.path.join(ROOT, "oracle", "_ref", "nc_fast", "ref_nc_driver")
     if not os.path.exists(exe):
         return None
     from unsflow2d_amd import nc_mesh
-    m = nc_mesh.o_grid(200, 100, 3, stretch=1.005 ** (999 / 99))
+    m = nc_mesh.o_grid(500, 250, 3, stretch=1.005 ** (999 / 249))
     env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
     with tempfile.TemporaryDirectory() as tmp:
         nc_mesh.write_geometry_inp(m, os.path.join(tmp, "geometry.inp"))
@@ -302,7 +303,7 @@ def nc_cpu_baseline(budget_s: float = 20.0) -> dict | None:
     value = m.nop / (ms / 1000.0) / 1e6
     return {"value": round(value, 4), "unit": "M vertex-iterations/s", "cores": 1,
             "kind": "reference",
-            "sample": f"reference NCFVM (amdflang -O3 -march=x86-64-v3), O-grid 200x100 points "
+            "sample": f"reference NCFVM (amdflang -O3 -march=x86-64-v3), O-grid 500x250 points "
                       f"({m.nop} vertices), M=0.4, GG + limiter, van Leer, LU-SGS, {nit} timed "
                       f"iterations after {warm} warm-up, {ms:.2f} ms/iteration, 1 thread"}
 
@@ -402,7 +403,8 @@ def main_nc(args) -> int:
                    "precision": args.precision,
                    "colours": ncol},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("nc-" + args.precision, "ufx::" + kname),
                      "kernel": kname, "bytes_per_launch": int(pb[dom] / max(1.0, d_n)),
                      "avg_launch_ms": round(d_ms / max(1.0, d_n), 5),
                      "basis": f"{dom} pass: algorithmic bytes per iteration over its "
