@@ -20,7 +20,6 @@ sources, timed on this host on a bounded sample).
 from __future__ import annotations
 
 import argparse
-import contextlib
 import json
 import os
 import subprocess
@@ -110,65 +109,6 @@ def measured_traffic(precision: str, kernel: str) -> int | None:
     except OSError:
         pass
     return None
-
-
-class ClockSampler:
-    """The GPU's graphics and memory clocks (amdsmi), sampled every `period`
-    seconds in a thread while a timed region runs.  Reported beside the
-    figures: the same kernels have run at very different speeds on boxes
-    whose graphics clock under load differed.  Best effort: without amdsmi
-    the summary is None."""
-
-    def __init__(self, period: float = 0.01):
-        import threading
-        self.period = period
-        self.gfx, self.mem = [], []
-        self._stop = threading.Event()
-        self._thread = None
-        self._h = None
-        self.max_gfx = None
-        try:
-            import amdsmi
-            amdsmi.amdsmi_init()
-            self._smi = amdsmi
-            self._h = amdsmi.amdsmi_get_processor_handles()[0]
-            self.max_gfx = amdsmi.amdsmi_get_clock_info(self._h, amdsmi.AmdSmiClkType.GFX)["max_clk"]
-        except Exception:
-            self._h = None
-
-    def _run(self):
-        smi = self._smi
-        while not self._stop.is_set():
-            try:
-                self.gfx.append(smi.amdsmi_get_clock_info(self._h, smi.AmdSmiClkType.GFX)["clk"])
-                self.mem.append(smi.amdsmi_get_clock_info(self._h, smi.AmdSmiClkType.MEM)["clk"])
-            except Exception:
-                return
-            self._stop.wait(self.period)
-
-    def __enter__(self):
-        import threading
-        if self._h is not None:
-            self._stop.clear()
-            self._thread = threading.Thread(target=self._run, daemon=True)
-            self._thread.start()
-        return self
-
-    def __exit__(self, *exc):
-        if self._thread is not None:
-            self._stop.set()
-            self._thread.join()
-            self._thread = None
-        return False
-
-    def summary(self) -> dict | None:
-        if not self.gfx:
-            return None
-        g = np.array(self.gfx, dtype=float)
-        m = np.array(self.mem, dtype=float)
-        return {"gfx_mhz_median": float(np.median(g)), "gfx_mhz_min": float(g.min()),
-                "gfx_mhz_max": float(g.max()), "gfx_mhz_peak": self.max_gfx,
-                "mem_mhz_median": float(np.median(m)) if len(m) else None, "samples": len(g)}
 
 
 def cpu_baseline(budget_s: float = 20.0) -> dict | None:
@@ -586,7 +526,7 @@ def main_viscous(args) -> int:
     return 0
 
 
-def time_iterations(solver, args, category: str = "stage", clocks=None) -> tuple[float, float]:
+def time_iterations(solver, args, category: str = "stage") -> tuple[float, float]:
     """Warm-up, then K iterations timed with the host clock around them
     (synchronised on both sides); HIP events on the library's stream bracket
     the launches of one kernel category (an event pair around every launch
@@ -596,11 +536,10 @@ def time_iterations(solver, args, category: str = "stage", clocks=None) -> tuple
     solver.sync()
     solver.profile_only(category)
     solver.sync()
-    with clocks if clocks is not None else contextlib.nullcontext():
-        t1 = time.perf_counter()
-        solver.iterate(args.steps, hist=False)
-        solver.sync()
-        wall = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    solver.iterate(args.steps, hist=False)
+    solver.sync()
+    wall = time.perf_counter() - t1
     st_ms, st_n = solver.profile_read()[category]
     solver.profile(False)
     return wall, st_ms / max(1, st_n)
@@ -722,8 +661,7 @@ def main() -> int:
     dominant = max(pbytes, key=lambda k: prof_all[k][0])
     # timed region: K iterations; HIP events on the library's stream bracket
     # the dominant pass's launches only
-    clocks = ClockSampler()
-    wall, k_avg = time_iterations(solver, args, dominant, clocks)
+    wall, k_avg = time_iterations(solver, args, dominant)
     hist = solver.iterate(1, hist=True)
     if not np.all(np.isfinite(hist)):
         print("# residual is not finite", file=sys.stderr)
@@ -738,7 +676,7 @@ def main() -> int:
     per_it = {k: round(v[0] / args.steps, 4) for k, v in prof_all.items() if v[1]}
     say(f"stage form {form}; kernel avg ms: {kern}; per iteration: {per_it}; iteration "
         f"{ms_step:.3f} ms, {it_bytes / (ms_step / 1000.0) / 1e9:.0f} GB/s algorithmic over the "
-        f"whole iteration; clocks {clocks.summary()}")
+        f"whole iteration")
     passes = {k: {"ms_per_iteration": per_it[k], "avg_launch_ms": kern[k],
                   "bytes_per_launch": int(pbytes[k]),
                   "frac": round(pbytes[k] / (kern[k] / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -804,7 +742,6 @@ def main() -> int:
         "passes": passes,
         "stage_form": form,
         "stage_form_tuning": tune,
-        "gpu_clocks": clocks.summary(),
         **mixed,
         "cpu_baseline": cpu,
     }
