@@ -6,11 +6,12 @@
 // sum keeps the reference's order.
 //
 // HBM layout (vertices in the reference's breadth-first order):
-//   vrec [nop][18] f64  primitives rho,u,v,p | gradient (d/dx, d/dy per
-//                       variable) | limiter values | x, y
+//   vrec [nop][18] f64  primitives rho,u,v,p | limiter values | gradient
+//                       (d/dx, d/dy per variable) | x, y
 //   qc, qold, res [nop][4] f64 (conserved order E, rho, rho u, rho v)
 //   la, dt, cv [nop] f64;  fla [nof] f64
-//   erec [nof][4] f64   dual-face nx, ny, length, pad;  ep [nof] int2 (p1, p2)
+//   erec [nof][4] f64   dual-face nx, ny, length, pad;  ep [nof] int2 (p1, p2);
+//                       edges stored in first-touch vertex order
 //   v2f [..] int        edges of a vertex, bit-inverted when it is p2
 //   v2fo [..] int2      the same with the vertex at the other end
 //   v2v [..] int, wxy [..][2] f64   least-squares neighbours and weights
