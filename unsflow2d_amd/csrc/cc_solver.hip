@@ -178,45 +178,6 @@ int nblocks(int n) { return std::max(1, (n + BLK - 1) / BLK); }
 // dynamic shared memory)
 constexpr size_t STAGE_LDS_MAX = 64 * 1024;
 
-// Position along a Hilbert curve of order 16 of the point (x, y) of the
-// 65536 x 65536 grid.
-uint32_t hilbert_d(uint32_t x, uint32_t y) {
-  uint32_t d = 0;
-  for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
-    uint32_t rx = (x & s) ? 1 : 0, ry = (y & s) ? 1 : 0;
-    d += s * s * ((3 * rx) ^ ry);
-    if (ry == 0) {
-      if (rx == 1) {
-        x = 65535u - x;
-        y = 65535u - y;
-      }
-      uint32_t t = x; x = y; y = t;
-    }
-  }
-  return d;
-}
-
-// Cells ordered along the Hilbert curve through their centroids (ties keep
-// the reference numbering).
-std::vector<int> hilbert_order(const double* cx, const double* cy, int n) {
-  double x0 = cx[0], x1 = cx[0], y0 = cy[0], y1 = cy[0];
-  for (int i = 0; i < n; ++i) {
-    x0 = std::min(x0, cx[i]); x1 = std::max(x1, cx[i]);
-    y0 = std::min(y0, cy[i]); y1 = std::max(y1, cy[i]);
-  }
-  double sc = 65535.0 / std::max(std::max(x1 - x0, y1 - y0), 1e-300);
-  std::vector<std::pair<uint32_t, int>> key(n);
-  for (int i = 0; i < n; ++i) {
-    uint32_t ix = (uint32_t)std::min(65535.0, std::max(0.0, (cx[i] - x0) * sc));
-    uint32_t iy = (uint32_t)std::min(65535.0, std::max(0.0, (cy[i] - y0) * sc));
-    key[i] = {hilbert_d(ix, iy), i};
-  }
-  std::sort(key.begin(), key.end());
-  std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = key[i].second;
-  return order;
-}
-
 int launch_check() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

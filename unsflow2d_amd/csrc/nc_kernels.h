@@ -5,7 +5,8 @@
 // same order (its v2f list is in ascending edge order), so each per-vertex
 // sum keeps the reference's order.
 //
-// HBM layout (vertices in the reference's breadth-first order):
+// HBM layout (vertices in storage order, a Hilbert curve through them; vref
+// holds their reference numbers):
 //   vrec [nop][18] f64  primitives rho,u,v,p | limiter values | gradient
 //                       (d/dx, d/dy per variable) | x, y
 //   qc, qold, res [nop][4] f64 (conserved order E, rho, rho u, rho v)
@@ -39,12 +40,17 @@ struct DevNC {
   const double* cv;
   const double* erec;
   const int2* ep;
-  const int* ebc;
+  const int* ebc;            // per edge: its index in the boundary-edge list, or -1
+  const int* bedge;          // wall and farfield edges (stored edge numbers)
+  const int* bkind;          // their kind (WALL or FARF)
+  double* bclos;             // [nbnd][2 ends][5] closure terms (momentum/flux 4, radius 1)
+  int nbnd;
   double* fla;
   const int *v2f_ptr, *v2f;
   const int2* v2fo;          // per v2f entry: (edge, bit-inverted when the vertex is p2; other end)
   const int *v2v_ptr, *v2v;
   const double* wxy;
+  const int* vref;           // reference number of each stored vertex
   // the mixed form's records (FP32), used instead of vrec / erec / wxy
   float* vrec32;          // [nop][NW32]
   const float* erec32;    // [nof][4]
@@ -91,6 +97,23 @@ __device__ __forceinline__ void ld2d(const double* p, double& a, double& b) {
   double2 v = *reinterpret_cast<const double2*>(p);
   a = v.x;
   b = v.y;
+}
+
+// Copy n values (a multiple of 16 bytes, 16-byte aligned on both sides)
+// from global memory into LDS, the block's threads striding over them.
+template <typename R>
+__device__ __forceinline__ void stage_block(R* dst, const R* src, int n) {
+  constexpr int V = 16 / (int)sizeof(R);
+  const int n4 = n / V;
+  if constexpr (sizeof(R) == 8) {
+    const double2* s2 = reinterpret_cast<const double2*>(src);
+    double2* d2 = reinterpret_cast<double2*>(dst);
+    for (int i = threadIdx.x; i < n4; i += BLK) d2[i] = s2[i];
+  } else {
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    for (int i = threadIdx.x; i < n4; i += BLK) d4[i] = s4[i];
+  }
 }
 
 // Vertex records of the FP64 form (vrec, double, NW wide) and of the mixed
@@ -148,6 +171,90 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
   stv<R, 4>(NCRec<R>::rec(d) + (size_t)p * NCRec<R>::W + R_QP, o);
 }
 
+// The boundary closures of one wall or farfield edge, for both of its ends
+// (side 0: p1, side 1: p2): the terms the residual adds right after the
+// edge's van Leer flux, each already rounded as the reference's expression
+// rounds it: for the wall (solid_flux, NCFVM/solid_flux.f90:1-43) the
+// momentum terms pw n of its end and the spectral radius term; for the
+// farfield (farfield_flux1, NCFVM/farfield_flux.f90:1-95) the flux times
+// the half-edge length, the same into both ends, and the spectral radius
+// term of each end.  Evaluated in FP64 in both forms.
+template <typename R>
+__device__ __forceinline__ void nc_closure(const DevNC& d, const NCFree& g, int k) {
+  constexpr int W = NCRec<R>::W;
+  const R* recs = NCRec<R>::rec(d);
+  const int e = d.bedge[k], bc = d.bkind[k];
+  const int2 pp = d.ep[e];
+  const R* a = recs + (size_t)pp.x * W;
+  const R* b = recs + (size_t)pp.y * W;
+  double ax = (double)a[R_CEN], ay = (double)a[R_CEN + 1];
+  double bx = (double)b[R_CEN], by = (double)b[R_CEN + 1];
+  double ap[4] = {(double)a[R_QP], (double)a[R_QP + 1], (double)a[R_QP + 2], (double)a[R_QP + 3]};
+  double bp[4] = {(double)b[R_QP], (double)b[R_QP + 1], (double)b[R_QP + 2], (double)b[R_QP + 3]};
+  if constexpr (sizeof(R) == 4) {
+    const double qi[4] = {g.r_inf, g.u_inf, g.v_inf, g.p_inf};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ap[i] = ap[i] + qi[i];
+      bp[i] = bp[i] + qi[i];
+    }
+  }
+  double* c0 = d.bclos + (size_t)k * 10;
+  double* c1 = c0 + 5;
+  if (bc == WALL) {
+    // 3/4-1/4 end-point pressures on the half-edge right normals
+    double p1w = 0.75 * ap[3] + 0.25 * bp[3];
+    double p2w = 0.25 * ap[3] + 0.75 * bp[3];
+    double wnx = 0.5 * (by - ay);
+    double wny = -0.5 * (bx - ax);
+    double area = sqrt(wnx * wnx + wny * wny);
+    c0[0] = 0.0; c0[1] = 0.0; c0[2] = p1w * wnx; c0[3] = p1w * wny;
+    c1[0] = 0.0; c1[1] = 0.0; c1[2] = p2w * wnx; c1[3] = p2w * wny;
+    c0[4] = sqrt(g.gamma * ap[3] / ap[0]) * area;
+    c1[4] = sqrt(g.gamma * bp[3] / bp[0]) * area;
+  } else {
+    // Riemann invariants on the half-edge left normal from the
+    // edge-averaged primitives
+    double fnx = -0.5 * (by - ay);
+    double fny = 0.5 * (bx - ax);
+    double dr = sqrt(fnx * fnx + fny * fny);
+    fnx = fnx / dr;
+    fny = fny / dr;
+    double un_inf = g.u_inf * fnx + g.v_inf * fny;
+    double rho = 0.5 * (ap[0] + bp[0]), u = 0.5 * (ap[1] + bp[1]);
+    double v = 0.5 * (ap[2] + bp[2]), pr = 0.5 * (ap[3] + bp[3]);
+    double as = sqrt(g.gamma * pr / rho);
+    double un = u * fnx + v * fny;
+    double rp = un + 2.0 * as / g.gamma1;
+    double rm = un_inf - 2.0 * g.a_inf / g.gamma1;
+    double unf = 0.5 * (rp + rm);
+    double af = 0.25 * (rp - rm) * g.gamma1;
+    double uf, vf, sf;
+    if (unf > 0.0) {
+      uf = u + fnx * (unf - un);
+      vf = v + fny * (unf - un);
+      sf = pr / pow(rho, g.gamma);
+    } else {
+      uf = g.u_inf + fnx * (unf - un_inf);
+      vf = g.v_inf + fny * (unf - un_inf);
+      sf = g.p_inf / pow(g.r_inf, g.gamma);
+    }
+    double rhof = pow(af * af / g.gamma / sf, 1.0 / g.gamma1);
+    double pf = rhof * af * af / g.gamma;
+    double ef = pf / g.gamma1 + 0.5 * rhof * (uf * uf + vf * vf);
+    double fl[4];
+    fl[0] = (ef + pf) * unf;
+    fl[1] = rhof * unf;
+    fl[2] = fl[1] * uf + pf * fnx;
+    fl[3] = fl[1] * vf + pf * fny;
+    double area = sqrt(fnx * fnx + fny * fny);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c0[i] = c1[i] = dr * fl[i];
+    c0[4] = (fabs(unf) + sqrt(g.gamma * ap[3] / ap[0])) * area;
+    c1[4] = (fabs(unf) + sqrt(g.gamma * bp[3] / bp[0])) * area;
+  }
+}
+
 // Gradient and limiter of one vertex:
 //   GRAD 1  Gradient_GG, NCFVM/gradient.f90:35-75 (edge midpoint values on
 //           the dual faces, no boundary closure), / dual volume;
@@ -170,14 +277,40 @@ __device__ __forceinline__ void for_edges(const DevNC& d, int j0, int n, const i
 template <typename R, int GRAD, int LIM>
 __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
   constexpr int W = NCRec<R>::W;
-  int p = cell_index();
-  if (p >= d.nop) return;
+  // the primitives and coordinates of the block's vertices, staged in LDS:
+  // the vertices are stored along a Hilbert curve, so most neighbours lie in
+  // the same block
+  __shared__ R s_qp[BLK * 4];
+  __shared__ R s_xy[BLK * 2];
+  // blocks past the vertices' form the boundary closures of this residual
+  // evaluation (they read only primitives and coordinates, which this
+  // kernel does not write)
+  const int nvb = (d.nop + BLK - 1) / BLK;
+  if ((int)blockIdx.x >= nvb) {
+    const int k = ((int)blockIdx.x - nvb) * BLK + (int)threadIdx.x;
+    if (k < d.nbnd) nc_closure<R>(d, g, k);
+    return;
+  }
+  const int base = xcd_block(blockIdx.x, nvb) * BLK;
+  const int p = base + (int)threadIdx.x;
+  const int cnt = min(BLK, d.nop - base);
   const R* recs = NCRec<R>::rec(d);
   const R* erec = NCRec<R>::edge(d);
   const R* own = recs + (size_t)p * W;
   R q[4], xy[2];
-  ldv<R, 4>(own + R_QP, q);
-  ldv<R, 2>(own + R_CEN, xy);
+  if (p < d.nop) {
+    ldv<R, 4>(own + R_QP, q);
+    ldv<R, 2>(own + R_CEN, xy);
+    stv<R, 4>(s_qp + threadIdx.x * 4, q);
+    stv<R, 2>(s_xy + threadIdx.x * 2, xy);
+  }
+  __syncthreads();
+  if (p >= d.nop) return;
+  auto nbr_qp = [&](int o, R* qo) {
+    const unsigned l = (unsigned)(o - base);
+    if (l < (unsigned)cnt) ldv<R, 4>(s_qp + l * 4, qo);
+    else ldv<R, 4>(recs + (size_t)o * W + R_QP, qo);
+  };
   const R x = xy[0], y = xy[1];
   R gr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
@@ -190,7 +323,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
       bool first = eo.x >= 0;
       int e = first ? eo.x : ~eo.x, o = eo.y;
       R qo[4], er[4];
-      ldv<R, 4>(recs + (size_t)o * W + R_QP, qo);
+      nbr_qp(o, qo);
       ldv<R, 4>(erec + (size_t)e * 4, er);
       R dx = er[0] * er[2], dy = er[1] * er[2];
 #pragma unroll
@@ -224,7 +357,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
       int c = d.v2v[j];
       R w[2], qo[4];
       ldv<R, 2>(wxy + (size_t)j * 2, w);
-      ldv<R, 4>(recs + (size_t)c * W + R_QP, qo);
+      nbr_qp(c, qo);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         gr[2 * k] = gr[2 * k] + (qo[k] - q[k]) * w[0];
@@ -240,7 +373,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
   for_edges(d, j0, n, nb, [&](int2 eo) {
     bool first = eo.x >= 0;
     R qo[4];
-    ldv<R, 4>(recs + (size_t)eo.y * W + R_QP, qo);
+    nbr_qp(eo.y, qo);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       R du = first ? qo[k] - q[k] : -(q[k] - qo[k]);
@@ -256,7 +389,9 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
   for_edges(d, j0, n, nb, [&](int2 eo) {
     bool first = eo.x >= 0;
     R xo[2];
-    ldv<R, 2>(recs + (size_t)eo.y * W + R_CEN, xo);
+    const unsigned l = (unsigned)(eo.y - base);
+    if (l < (unsigned)cnt) ldv<R, 2>(s_xy + l * 2, xo);
+    else ldv<R, 2>(recs + (size_t)eo.y * W + R_CEN, xo);
     R d0 = first ? xo[0] - x : x - xo[0], d1 = first ? xo[1] - y : y - xo[1];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -356,50 +491,58 @@ __device__ __forceinline__ void nc_vanleer(R gm, R gm1, const R* ql, const R* qr
 // FP64 in both forms (a few thousand edges).
 //   mode 0: store res, la, dt;  mode 1: also the RK stage update of
 //   NCFVM/main.f90:59-70 in place (no vertex reads another's qc in a stage).
+// Built for 4 waves per SIMD (128 registers, a few values spilled): the
+// third wave's latency hiding pays more than the spills cost.
 template <typename R, int FLUX>
-__global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64,
-                                                     double cfl, int mode, double ak, double bk) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4)))
+void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode, double ak,
+                   double bk) {
   constexpr int W = NCRec<R>::W;
-  int p = cell_index();
-  if (p >= d.nop) return;
+  // the block's vertex records, staged in LDS with one coalesced copy: the
+  // vertices are stored along a Hilbert curve, so most neighbours a vertex
+  // gathers lie in its own block, and its own record is read from here on
+  // every edge instead of being held in registers
+  __shared__ R s_rec[BLK * W];
+  const int base = xcd_block(blockIdx.x, gridDim.x) * BLK;
+  const int p = base + (int)threadIdx.x;
+  const int cnt = min(BLK, d.nop - base);
   const R* recs = NCRec<R>::rec(d);
+  stage_block<R>(s_rec, recs + (size_t)base * W, cnt * W);
+  __syncthreads();
+  if (p >= d.nop) return;
   const R* erec = NCRec<R>::edge(d);
+  const R* me = s_rec + threadIdx.x * W;
   double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0;
   const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
-  int2 nb[NCU];
-#pragma unroll
-  for (int k = 0; k < NCU; ++k)
-    if (k < n) nb[k] = d.v2fo[j0 + k];
-  R me[W];
-  ldv<R, W>(recs + (size_t)p * W, me);
-  for_edges(d, j0, n, nb, [&](int2 eo) {
-    bool first = eo.x >= 0;
-    int e = first ? eo.x : ~eo.x;
-    R ot[W], a[W], b[W];
-    ldv<R, W>(recs + (size_t)eo.y * W, ot);
-#pragma unroll
-    for (int i = 0; i < W; ++i) {
-      a[i] = first ? me[i] : ot[i];
-      b[i] = first ? ot[i] : me[i];
-    }
+  auto edge = [&](int2 eo) {
+    const bool first = eo.x >= 0;
+    const int e = first ? eo.x : ~eo.x;
+    const unsigned l = (unsigned)(eo.y - base);
+    const R* op = l < (unsigned)cnt ? s_rec + l * W : recs + (size_t)eo.y * W;
+    R ot[W];
+    ldv<R, W>(op, ot);
     R er[4];
     ldv<R, 4>(erec + (size_t)e * 4, er);
     // reconstruction and flux in FP64 from the records (FP32 in the mixed
     // form: rounding absolute states to FP32 would put a noise floor of a
-    // few 1e-7 under the residual)
+    // few 1e-7 under the residual).  a = p1's record, b = p2's: the edge
+    // vector b - a, then each end's extrapolation to the dual face, p1 with
+    // +, p2 with -, exactly as the reference writes them
     const double nx = er[0], ny = er[1], ds = er[2];
-    double d0 = (double)b[R_CEN] - (double)a[R_CEN], d1 = (double)b[R_CEN + 1] - (double)a[R_CEN + 1];
-    if constexpr (sizeof(R) == 8) {
-      d0 = b[R_CEN] - a[R_CEN];
-      d1 = b[R_CEN + 1] - a[R_CEN + 1];
-    }
+    const double mx = (double)me[R_CEN], my = (double)me[R_CEN + 1];
+    const double ox = (double)ot[R_CEN], oy = (double)ot[R_CEN + 1];
+    const double d0 = first ? ox - mx : mx - ox, d1 = first ? oy - my : my - oy;
     double ql[4], qr[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      double s = (0.0 + (double)a[R_G + 2 * k] * d0) + (double)a[R_G + 2 * k + 1] * d1;
-      ql[k] = (double)a[R_QP + k] + 0.5 * s * (double)a[R_PHI + k];
-      s = (0.0 + (double)b[R_G + 2 * k] * d0) + (double)b[R_G + 2 * k + 1] * d1;
-      qr[k] = (double)b[R_QP + k] - 0.5 * s * (double)b[R_PHI + k];
+      double s = (0.0 + (double)me[R_G + 2 * k] * d0) + (double)me[R_G + 2 * k + 1] * d1;
+      double t = 0.5 * s * (double)me[R_PHI + k];
+      const double so = first ? (double)me[R_QP + k] + t : (double)me[R_QP + k] - t;
+      s = (0.0 + (double)ot[R_G + 2 * k] * d0) + (double)ot[R_G + 2 * k + 1] * d1;
+      t = 0.5 * s * (double)ot[R_PHI + k];
+      const double sn = first ? (double)ot[R_QP + k] - t : (double)ot[R_QP + k] + t;
+      ql[k] = first ? so : sn;
+      qr[k] = first ? sn : so;
     }
     if constexpr (sizeof(R) == 4) {
       // deviations -> states
@@ -422,74 +565,27 @@ __global__ __launch_bounds__(BLK) void k_nc_residual(DevNC d, NCFree g, Gas<doub
       for (int i = 0; i < 4; ++i) r[i] = r[i] - F[i] * ds;
     }
     la = la + ll;
-    int bc = d.ebc[e];
-    if (bc == WALL || bc == FARF) {
-      double ax = (double)a[R_CEN], ay = (double)a[R_CEN + 1];
-      double bx = (double)b[R_CEN], by = (double)b[R_CEN + 1];
-      double ap[4] = {(double)a[R_QP], (double)a[R_QP + 1], (double)a[R_QP + 2], (double)a[R_QP + 3]};
-      double bp[4] = {(double)b[R_QP], (double)b[R_QP + 1], (double)b[R_QP + 2], (double)b[R_QP + 3]};
-      if constexpr (sizeof(R) == 4) {
-        const double qi[4] = {g.r_inf, g.u_inf, g.v_inf, g.p_inf};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          ap[k] = ap[k] + qi[k];
-          bp[k] = bp[k] + qi[k];
-        }
-      }
-      const double* mp = first ? ap : bp;
-      if (bc == WALL) {
-        // solid_flux: 3/4-1/4 end-point pressures on the half-edge right normals
-        double p1w = 0.75 * ap[3] + 0.25 * bp[3];
-        double p2w = 0.25 * ap[3] + 0.75 * bp[3];
-        double wnx = 0.5 * (by - ay);
-        double wny = -0.5 * (bx - ax);
-        double area = sqrt(wnx * wnx + wny * wny);
-        la = la + sqrt(g.gamma * mp[3] / mp[0]) * area;
-        double pw = first ? p1w : p2w;
-        r[2] = r[2] + pw * wnx;
-        r[3] = r[3] + pw * wny;
+    // a wall or farfield edge's closure (solid_flux, farfield_flux1), added
+    // right after the edge's flux as the reference does; its terms were
+    // formed per boundary edge and end by the gradient launch
+    const int bk = d.ebc[e];
+    if (bk >= 0) {
+      const double* c = d.bclos + ((size_t)bk * 2 + (first ? 0 : 1)) * 5;
+      if (d.bkind[bk] == WALL) {
+        r[2] = r[2] + c[2];
+        r[3] = r[3] + c[3];
       } else {
-        // farfield_flux1: Riemann invariants on the half-edge left normal
-        // from the edge-averaged primitives, the same flux into both ends
-        double fnx = -0.5 * (by - ay);
-        double fny = 0.5 * (bx - ax);
-        double dr = sqrt(fnx * fnx + fny * fny);
-        fnx = fnx / dr;
-        fny = fny / dr;
-        double un_inf = g.u_inf * fnx + g.v_inf * fny;
-        double rho = 0.5 * (ap[0] + bp[0]), u = 0.5 * (ap[1] + bp[1]);
-        double v = 0.5 * (ap[2] + bp[2]), pr = 0.5 * (ap[3] + bp[3]);
-        double as = sqrt(g.gamma * pr / rho);
-        double un = u * fnx + v * fny;
-        double rp = un + 2.0 * as / g.gamma1;
-        double rm = un_inf - 2.0 * g.a_inf / g.gamma1;
-        double unf = 0.5 * (rp + rm);
-        double af = 0.25 * (rp - rm) * g.gamma1;
-        double uf, vf, sf;
-        if (unf > 0.0) {
-          uf = u + fnx * (unf - un);
-          vf = v + fny * (unf - un);
-          sf = pr / pow(rho, g.gamma);
-        } else {
-          uf = g.u_inf + fnx * (unf - un_inf);
-          vf = g.v_inf + fny * (unf - un_inf);
-          sf = g.p_inf / pow(g.r_inf, g.gamma);
-        }
-        double rhof = pow(af * af / g.gamma / sf, 1.0 / g.gamma1);
-        double pf = rhof * af * af / g.gamma;
-        double ef = pf / g.gamma1 + 0.5 * rhof * (uf * uf + vf * vf);
-        double fl[4];
-        fl[0] = (ef + pf) * unf;
-        fl[1] = rhof * unf;
-        fl[2] = fl[1] * uf + pf * fnx;
-        fl[3] = fl[1] * vf + pf * fny;
-        double area = sqrt(fnx * fnx + fny * fny);
-        la = la + (fabs(unf) + sqrt(g.gamma * mp[3] / mp[0])) * area;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) r[i] = r[i] + dr * fl[i];
+        for (int i = 0; i < 4; ++i) r[i] = r[i] + c[i];
       }
+      la = la + c[4];
     }
-  });
+  };
+  int2 nb[NCU];
+#pragma unroll
+  for (int k = 0; k < NCU; ++k)
+    if (k < n) nb[k] = d.v2fo[j0 + k];
+  for_edges(d, j0, n, nb, edge);
   double cvp = d.cv[p];
   double dt = cfl * cvp / la;
   d.la[p] = la;
@@ -524,7 +620,7 @@ __global__ __launch_bounds__(BLK) void k_nc_norm(DevNC d) {
     for (int j = 0; j < 4; ++j) fr = fr + r[j] * r[j];
     s = fr;
     mx = sqrt(fr);
-    imx = p;
+    imx = d.vref[p];
     dmn = d.dt[p];
   }
   double v[1] = {s};
@@ -634,7 +730,7 @@ __device__ __forceinline__ void nc_normalflux(const NCFree& g, const double* q, 
 
 // The update of one vertex in the forward (dir 0) or backward (dir 1)
 // sweep, NCFVM/lusgs.f90:46-122: the off-diagonal sum over the neighbours
-// already swept (lower-numbered forward, higher-numbered backward), then
+// already swept (lower reference number forward, higher backward), then
 // dq and qc = qold + dq.
 __device__ __forceinline__ void nc_lusgs_vertex(const DevNC& d, const NCFree& g, int it, int dir) {
   double cres[4] = {0.0, 0.0, 0.0, 0.0};
@@ -647,7 +743,7 @@ __device__ __forceinline__ void nc_lusgs_vertex(const DevNC& d, const NCFree& g,
     bool first = eo.x >= 0;
     int e = first ? eo.x : ~eo.x, o = eo.y;
     double sign = first ? 1.0 : -1.0;
-    if (dir == 0 ? o < it : o > it) {
+    if (dir == 0 ? d.vref[o] < d.vref[it] : d.vref[o] > d.vref[it]) {
       double nx, ny, ds, pad, f1[4], f2[4], qo[4], qn[4], dq[4];
       ld2d(d.erec + (size_t)e * 4, nx, ny);
       ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
@@ -698,6 +794,7 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
 // Reverse (dir 1, colours descending): the neighbours of higher colour.  A
 // neighbour's state is rebuilt as qold + dq, the sum that defines it, so the
 // arithmetic is the reference loop's, term for term, in that vertex order.
+constexpr int MCU = 8;
 __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, int n, int dir) {
   const int t = blockIdx.x * BLK + threadIdx.x;
   if (t >= n) return;
@@ -705,11 +802,7 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, 
   const int cnt = d.mc_cnt[(size_t)dir * np + s];
   const size_t base = (size_t)dir * d.mc_q * np + s;
   double cres[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k = 0; k < cnt; ++k) {
-    const size_t q = base + (size_t)k * np;
-    const int2 eo = d.mc_eo[q];
-    const double2 nxy = d.mc_nxy[q];
-    const double ds = d.mc_ds[q];
+  auto term = [&](int2 eo, double2 nxy, double ds) {
     const bool first = eo.x >= 0;
     const int e = first ? eo.x : ~eo.x, o = eo.y;
     const double sign = first ? 1.0 : -1.0;
@@ -723,6 +816,28 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, 
     const double lam = d.fla[e];
 #pragma unroll
     for (int i = 0; i < 4; ++i) cres[i] = cres[i] + 0.5 * (sign * (f2[i] - f1[i]) - 1.5 * lam * dq[i]);
+  };
+  // the first MCU entries' slot data are loaded together, ahead of the
+  // gathers they select; the sum runs over the entries in order
+  {
+    int2 eo[MCU];
+    double2 nxy[MCU];
+    double dsv[MCU];
+#pragma unroll
+    for (int k = 0; k < MCU; ++k)
+      if (k < cnt) {
+        const size_t q = base + (size_t)k * np;
+        eo[k] = d.mc_eo[q];
+        nxy[k] = d.mc_nxy[q];
+        dsv[k] = d.mc_ds[q];
+      }
+#pragma unroll
+    for (int k = 0; k < MCU; ++k)
+      if (k < cnt) term(eo[k], nxy[k], dsv[k]);
+  }
+  for (int k = MCU; k < cnt; ++k) {
+    const size_t q = base + (size_t)k * np;
+    term(d.mc_eo[q], d.mc_nxy[q], d.mc_ds[q]);
   }
   double res[4], dq[4], dd;
   if (dir == 0) {

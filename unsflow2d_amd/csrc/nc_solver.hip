@@ -44,7 +44,8 @@ struct ufx_nc {
   SweepSlot* d_slots = nullptr;             // packed sweep slots, forward then backward
   int nwall = 0;                            // wall edges (bc 1001)
   int* d_wedge = nullptr;                   // their stored edges, in ascending reference order
-  std::vector<int> es;                      // storage slot of each reference edge
+  std::vector<int> es, es_ref;              // storage slot of each reference edge, and back
+  std::vector<int> rv, sv;                  // reference vertex of each storage slot, and back
   double *d_fe = nullptr, *d_cp = nullptr;  // [nwall][2] force terms, [nwall] -Cp
   int sweep_mode = 4;                       // 0 graph of level launches, 1 level launches, 2 one workgroup, 3 cooperative grid, 4 level-ordered one workgroup
   int grid_wgs = 8;                         // workgroups of the cooperative sweep
@@ -130,13 +131,15 @@ int run_save_old_r(ufx_nc* x) {
   return launch_ok();
 }
 
+// gradient + limiter, with extra blocks for the boundary closures
 template <typename R>
 int run_grad_lim_r(ufx_nc* x) {
   bool lsq = x->prm.grad == UFX_GRAD_LSQR, bj = x->prm.limiter == UFX_NC_LIM_BJ;
-  if (lsq && bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 1>), x->nblk, x->d, x->g);
-  else if (lsq) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 0>), x->nblk, x->d, x->g);
-  else if (bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 1>), x->nblk, x->d, x->g);
-  else NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 0>), x->nblk, x->d, x->g);
+  const int nb = x->nblk + (x->d.nbnd + BLK - 1) / BLK;
+  if (lsq && bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 1>), nb, x->d, x->g);
+  else if (lsq) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 0>), nb, x->d, x->g);
+  else if (bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 1>), nb, x->d, x->g);
+  else NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 0>), nb, x->d, x->g);
   return launch_ok();
 }
 
@@ -324,6 +327,54 @@ int write_records(ufx_nc* x, const std::vector<double>& v) {
   return UFX_OK;
 }
 
+// The mesh with its vertices renumbered to storage order (rv: reference
+// vertex of each storage slot, sv: the inverse): per-vertex arrays and
+// lists moved, vertex numbers in the lists and edge end points mapped.
+// Edges keep their reference numbers, and every list keeps its order.
+Mesh nc_storage_mesh(const Mesh& m, const std::vector<int>& rv, const std::vector<int>& sv) {
+  const int nop = m.nop, nof = m.nof();
+  Mesh s;
+  s.kind = m.kind;
+  s.nop = nop;
+  s.noc = m.noc;
+  s.nfi = m.nfi;
+  s.nbf = m.nbf;
+  s.fnx = m.fnx;
+  s.fny = m.fny;
+  s.farea = m.farea;
+  s.fbc = m.fbc;
+  s.fp1.resize(nof);
+  s.fp2.resize(nof);
+  for (int e = 0; e < nof; ++e) {
+    s.fp1[e] = sv[m.fp1[e]];
+    s.fp2[e] = sv[m.fp2[e]];
+  }
+  s.px.resize(nop);
+  s.py.resize(nop);
+  s.pcv.resize(nop);
+  s.v2f_ptr.assign(nop + 1, 0);
+  s.v2v_ptr.assign(nop + 1, 0);
+  s.v2f.reserve(m.v2f.size());
+  s.v2v.reserve(m.v2v.size());
+  s.wx.reserve(m.wx.size());
+  s.wy.reserve(m.wy.size());
+  for (int i = 0; i < nop; ++i) {
+    const int r = rv[i];
+    s.px[i] = m.px[r];
+    s.py[i] = m.py[r];
+    s.pcv[i] = m.pcv[r];
+    for (int j = m.v2f_ptr[r]; j < m.v2f_ptr[r + 1]; ++j) s.v2f.push_back(m.v2f[j]);
+    for (int j = m.v2v_ptr[r]; j < m.v2v_ptr[r + 1]; ++j) {
+      s.v2v.push_back(sv[m.v2v[j]]);
+      s.wx.push_back(m.wx[j]);
+      s.wy.push_back(m.wy[j]);
+    }
+    s.v2f_ptr[i + 1] = (int)s.v2f.size();
+    s.v2v_ptr[i + 1] = (int)s.v2v.size();
+  }
+  return s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -343,8 +394,8 @@ int ufx_nc_destroy(ufx_nc* x) {
 
 int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) {
   if (!mesh || !prm || !out) { set_error("ufx_nc_create: null argument"); return UFX_ERR_ARG; }
-  const Mesh& m = mesh->m;
-  if (m.kind != UFX_KIND_NC) { set_error("ufx_nc_create: not a vertex-centred mesh"); return UFX_ERR_ARG; }
+  const Mesh& m0 = mesh->m;
+  if (m0.kind != UFX_KIND_NC) { set_error("ufx_nc_create: not a vertex-centred mesh"); return UFX_ERR_ARG; }
   if (prm->grad != UFX_GRAD_GG && prm->grad != UFX_GRAD_LSQR) {
     set_error("ufx_nc_create: the vertex-centred path has the GG and LSQR gradients");
     return UFX_ERR_ARG;
@@ -381,7 +432,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   }
   ufx_nc* x = new ufx_nc();
   x->prm = *prm;
-  const int nop = m.nop, nof = m.nof();
+  const int nop = m0.nop, nof = m0.nof();
   x->nop = nop;
   x->nof = nof;
   x->nblk = nblk_of(nop);
@@ -414,6 +465,28 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
                       (float)G.u_inf, (float)G.v_inf, (float)G.p_inf, (float)G.mref,
                       (float)G.igamma1};
   x->mixed = prm->precision == UFX_PREC_MIXED;
+  // vertices are stored in Hilbert-curve order of their coordinates, so a
+  // block of consecutive vertices is a compact patch of the mesh and most of
+  // the neighbours it gathers lie in the same block (in the breadth-first
+  // reference numbering a vertex's neighbours lie a whole front away);
+  // UFX_NC_ORDER=reference keeps the reference numbering.  Every sum runs
+  // in the reference's order whatever the storage order, and the sweeps of
+  // the reference-order LU-SGS compare reference numbers.
+  std::vector<int>& rv = x->rv;
+  std::vector<int>& sv = x->sv;
+  {
+    const char* ord = std::getenv("UFX_NC_ORDER");
+    if (ord && std::strcmp(ord, "reference") == 0) {
+      rv.resize(nop);
+      for (int i = 0; i < nop; ++i) rv[i] = i;
+    } else {
+      rv = hilbert_order(m0.px.data(), m0.py.data(), nop);
+    }
+    sv.assign(nop, 0);
+    for (int i = 0; i < nop; ++i) sv[rv[i]] = i;
+  }
+  const Mesh ms = nc_storage_mesh(m0, rv, sv);
+  const Mesh& m = ms;
   // edges are stored in first-touch order over the vertices (each vertex's
   // edges in its list order), so the edges a block of vertices gathers lie
   // in a few contiguous runs instead of all over the reference numbering;
@@ -428,6 +501,8 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
         if (es[m.v2f[j]] < 0) es[m.v2f[j]] = ne++;
     for (int e = 0; e < nof; ++e)
       if (es[e] < 0) es[e] = ne++;
+    x->es_ref.assign(nof, 0);
+    for (int e = 0; e < nof; ++e) x->es_ref[es[e]] = e;
   }
   // device arrays
   std::vector<double> vrec((size_t)nop * NW, 0.0);
@@ -441,14 +516,22 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   for (int p = 0; p < nop; ++p)
     for (int k = 0; k < 4; ++k) qc[(size_t)p * 4 + k] = qinf[k];
   std::vector<int2> ep(nof);
-  std::vector<int> ebc(std::max(1, nof), 0);
+  std::vector<int> ebc(std::max(1, nof), -1), bedge, bkind;
   for (int e = 0; e < nof; ++e) {
     const size_t s = es[e];
     erec[s * 4] = m.fnx[e];
     erec[s * 4 + 1] = m.fny[e];
     erec[s * 4 + 2] = m.farea[e];
     ep[s] = make_int2(m.fp1[e], m.fp2[e]);
-    ebc[s] = m.fbc[e];
+  }
+  // the wall and farfield edges, whose closures the residual adds
+  for (int s = 0; s < nof; ++s) {
+    const int e = x->es_ref[s];
+    if (m.fbc[e] == WALL || m.fbc[e] == FARF) {
+      ebc[s] = (int)bedge.size();
+      bedge.push_back(s);
+      bkind.push_back(m.fbc[e]);
+    }
   }
   std::vector<int> v2f(m.v2f.size());
   std::vector<int2> v2fo(m.v2f.size());
@@ -469,17 +552,21 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   if (prm->timemode == UFX_TIME_LUSGS && prm->lusgs_order == UFX_LUSGS_REFERENCE) {
     std::vector<int> lf(nop, 0), lb(nop, 0);
     int nlf = 0, nlb = 0;
-    for (int p = 0; p < nop; ++p) {
+    // (vertices visited in reference order; p is a storage slot, r its
+    // reference number)
+    for (int r = 0; r < nop; ++r) {
+      const int p = sv[r];
       int l = 0;
       for (int j = m.v2v_ptr[p]; j < m.v2v_ptr[p + 1]; ++j)
-        if (m.v2v[j] < p) l = std::max(l, lf[m.v2v[j]] + 1);
+        if (rv[m.v2v[j]] < r) l = std::max(l, lf[m.v2v[j]] + 1);
       lf[p] = l;
       nlf = std::max(nlf, l + 1);
     }
-    for (int p = nop - 1; p >= 0; --p) {
+    for (int r = nop - 1; r >= 0; --r) {
+      const int p = sv[r];
       int l = 0;
       for (int j = m.v2v_ptr[p]; j < m.v2v_ptr[p + 1]; ++j)
-        if (m.v2v[j] > p) l = std::max(l, lb[m.v2v[j]] + 1);
+        if (rv[m.v2v[j]] > r) l = std::max(l, lb[m.v2v[j]] + 1);
       lb[p] = l;
       nlb = std::max(nlb, l + 1);
     }
@@ -492,8 +579,8 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     x->lv_fwd_off = cf;
     x->lv_bwd_off = cb;
     std::vector<int> pf(cf.begin(), cf.end() - 1), pb(cb.begin(), cb.end() - 1);
-    for (int p = 0; p < nop; ++p) verts[pf[lf[p]]++] = p;
-    for (int p = nop - 1; p >= 0; --p) verts[pb[lb[p]]++] = p;
+    for (int r = 0; r < nop; ++r) verts[pf[lf[sv[r]]]++] = sv[r];
+    for (int r = nop - 1; r >= 0; --r) verts[pb[lb[sv[r]]]++] = sv[r];
     // inside a level the vertices are independent, so any order gives the
     // same bits; ordering them by swept-neighbour count (most first) lets a
     // wave's lanes share their count and skip the unused entry slots
@@ -501,7 +588,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       std::vector<int> nlo(nop, 0), nhi(nop, 0);
       for (int p = 0; p < nop; ++p)
         for (int j = m.v2v_ptr[p]; j < m.v2v_ptr[p + 1]; ++j)
-          ++(m.v2v[j] < p ? nlo[p] : nhi[p]);
+          ++(rv[m.v2v[j]] < rv[p] ? nlo[p] : nhi[p]);
       auto sort_levels = [&](const std::vector<int>& off, const std::vector<int>& cnt) {
         for (size_t l = 0; l + 1 < off.size(); ++l)
           std::stable_sort(verts.begin() + off[l], verts.begin() + off[l + 1],
@@ -526,7 +613,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
         int e = m.v2f[j];
         bool first = m.fp1[e] == p;
         int o = first ? m.fp2[e] : m.fp1[e];
-        if (dir == 0 ? o < p : o > p) {
+        if (dir == 0 ? rv[o] < rv[p] : rv[o] > rv[p]) {
           if (cnt == SWQ) { fits = false; break; }
           sl.eo[cnt++] = make_int2(first ? es[e] : ~es[e], o);
         }
@@ -552,7 +639,7 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
           int e = m.v2f[j];
           bool first = m.fp1[e] == p;
           int o = first ? m.fp2[e] : m.fp1[e];
-          if (dir == 0 ? o < p : o > p) {
+          if (dir == 0 ? rv[o] < rv[p] : rv[o] > rv[p]) {
             eo_all.push_back(make_int2(first ? e : ~e, o));
             ++cnt;
           }
@@ -608,8 +695,13 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     // multicolour sweeps: vertices grouped by colour, and per direction the
     // neighbours of lower (forward) / higher (reverse) colour in edge order
     if (prm->timemode == UFX_TIME_LUSGS && prm->lusgs_order == UFX_LUSGS_MULTICOLOUR) {
+      // the colouring of the reference numbering (ufx_mesh_get "pt_colour"),
+      // carried to storage slots; inside a colour the vertices are
+      // independent and taken in storage order
       int ncol = 0;
-      std::vector<int> col = lusgs_colours(m, &ncol);
+      const std::vector<int> col0 = lusgs_colours(m0, &ncol);
+      std::vector<int> col(nop);
+      for (int p = 0; p < nop; ++p) col[p] = col0[rv[p]];
       x->mc_off.assign(ncol + 1, 0);
       for (int p = 0; p < nop; ++p) x->mc_off[col[p] + 1]++;
       for (int c = 0; c < ncol; ++c) x->mc_off[c + 1] += x->mc_off[c];
@@ -688,7 +780,9 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   } else {
     NUP(vrec, vrec); NUP(erec, erec); NUP(wxy, wxy);
   }
-  NUP(qc, qc); NUP(cv, m.pcv); NUP(ebc, ebc);
+  NUP(qc, qc); NUP(cv, m.pcv); NUP(ebc, ebc); NUP(vref, rv);
+  NUP(bedge, bedge); NUP(bkind, bkind); NAL(bclos, 10 * bedge.size());
+  d.nbnd = (int)bedge.size();
   NUP(v2f_ptr, m.v2f_ptr); NUP(v2f, v2f); NUP(v2v_ptr, m.v2v_ptr); NUP(v2v, m.v2v);
   if (!rc) {
     std::vector<double2> pxy(nop);
@@ -824,14 +918,20 @@ int ufx_nc_set(ufx_nc* x, const char* name, const double* src) {
   if (!name || !src) { set_error("ufx_nc_set: null argument"); return UFX_ERR_ARG; }
   std::string n(name);
   const int nop = x->nop;
+  // arrays come in the reference's vertex numbering; the device holds them
+  // by storage slot
   if (n == "qc" || n == "qold") {
-    NCCHK(hipMemcpyAsync(n == "qc" ? x->d.qc : x->d.qold, src, sizeof(double) * 4 * nop,
+    std::vector<double> t(4 * (size_t)nop);
+    for (int p = 0; p < nop; ++p)
+      for (int k = 0; k < 4; ++k) t[(size_t)p * 4 + k] = src[(size_t)x->rv[p] * 4 + k];
+    NCCHK(hipMemcpyAsync(n == "qc" ? x->d.qc : x->d.qold, t.data(), sizeof(double) * 4 * nop,
                          hipMemcpyHostToDevice, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
   } else if (n == "phi") {
     std::vector<double> v;
     if (int rc = read_records(x, v)) return rc;
     for (int p = 0; p < nop; ++p)
-      for (int k = 0; k < 4; ++k) v[(size_t)p * NW + R_PHI + k] = src[p * 4 + k];
+      for (int k = 0; k < 4; ++k) v[(size_t)p * NW + R_PHI + k] = src[(size_t)x->rv[p] * 4 + k];
     if (int rc = write_records(x, v)) return rc;
   } else {
     set_error("ufx_nc_set: unknown or read-only array " + n);
@@ -851,11 +951,21 @@ int ufx_nc_get(ufx_nc* x, const char* name, double* dst) {
     NCCHK(hipStreamSynchronize(x->stream));
     return UFX_OK;
   };
-  if (n == "qc") return copy(x->d.qc, 4 * (size_t)nop);
-  if (n == "qold") return copy(x->d.qold, 4 * (size_t)nop);
-  if (n == "res") return copy(x->d.res, 4 * (size_t)nop);
-  if (n == "la") return copy(x->d.la, nop);
-  if (n == "dt") return copy(x->d.dt, nop);
+  // per-vertex arrays (w values each) from storage slots to the reference's
+  // vertex numbering
+  auto vertex_array = [&](const double* src, int w) -> int {
+    std::vector<double> t((size_t)w * nop);
+    NCCHK(hipMemcpyAsync(t.data(), src, t.size() * 8, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+    for (int p = 0; p < nop; ++p)
+      for (int k = 0; k < w; ++k) dst[(size_t)x->rv[p] * w + k] = t[(size_t)p * w + k];
+    return UFX_OK;
+  };
+  if (n == "qc") return vertex_array(x->d.qc, 4);
+  if (n == "qold") return vertex_array(x->d.qold, 4);
+  if (n == "res") return vertex_array(x->d.res, 4);
+  if (n == "la") return vertex_array(x->d.la, 1);
+  if (n == "dt") return vertex_array(x->d.dt, 1);
   if (n == "fc_la") {
     std::vector<double> t(x->nof);
     if (int rc = copy(x->d.fla, x->nof)) return rc;
@@ -868,7 +978,7 @@ int ufx_nc_get(ufx_nc* x, const char* name, double* dst) {
     if (int rc = read_records(x, v)) return rc;
     int off = n == "qp" ? R_QP : n == "grad" ? R_G : R_PHI, w = n == "grad" ? 8 : 4;
     for (int p = 0; p < nop; ++p)
-      for (int k = 0; k < w; ++k) dst[(size_t)p * w + k] = v[(size_t)p * NW + off + k];
+      for (int k = 0; k < w; ++k) dst[(size_t)x->rv[p] * w + k] = v[(size_t)p * NW + off + k];
     return UFX_OK;
   }
   set_error("ufx_nc_get: unknown array " + n);

@@ -94,6 +94,11 @@ std::vector<int> greedy_colour(int n, const std::vector<int>& ptr, const std::ve
 // interior cells over the interior faces.
 std::vector<int> lusgs_colours(const Mesh& m, int* ncolours);
 
+// Points (x[i], y[i]) ordered along a Hilbert curve of order 16 over their
+// bounding square (ties keep the given order): storage orders whose blocks of
+// consecutive points are compact patches of the mesh.
+std::vector<int> hilbert_order(const double* x, const double* y, int n);
+
 void set_error(const std::string& msg);
 
 }  // namespace ufx
