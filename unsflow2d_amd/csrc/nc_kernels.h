@@ -482,6 +482,53 @@ __device__ __forceinline__ void nc_vanleer(R gm, R gm1, const R* ql, const R* qr
   ll = nc_roe_lambda<R>(gm1, rl, ul, vl, hl, rr, ur, vr, hr, nx, ny, area);
 }
 
+// residue, NCFVM/common.f90:108-154: the partials of one block of vertices
+// (`blk`: its index in vertex order) -- sum |res|^2, the largest per-vertex
+// |res| with its (first, by reference number) vertex, and min dt.  Every
+// thread of the block calls it; `have`: the thread has a vertex, whose
+// residual is r and time step dt.  sh and si: LDS of 2 BLK doubles and BLK
+// ints.
+__device__ __forceinline__ void nc_norm_partial(const DevNC& d, double (*sh)[BLK], int* si, int blk,
+                                                bool have, const double* r, double dt, int vref) {
+  double s = 0.0, mx = 0.0, dmn = 1.0e20;
+  int imx = 0x7fffffff;
+  if (have) {
+    double fr = 0.0;
+    for (int j = 0; j < 4; ++j) fr = fr + r[j] * r[j];
+    s = fr;
+    mx = sqrt(fr);
+    imx = vref;
+    dmn = dt;
+  }
+  double v[1] = {s};
+  block_reduce<1>(sh, v, OpAdd());
+  // argmax, first index on ties (the reference's strict '>' scan)
+  __syncthreads();
+  sh[0][threadIdx.x] = mx;
+  si[threadIdx.x] = imx;
+  sh[1][threadIdx.x] = dmn;
+  __syncthreads();
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double m2 = sh[0][threadIdx.x + w];
+      int i2 = si[threadIdx.x + w];
+      if (m2 > sh[0][threadIdx.x] || (m2 == sh[0][threadIdx.x] && i2 < si[threadIdx.x])) {
+        sh[0][threadIdx.x] = m2;
+        si[threadIdx.x] = i2;
+      }
+      double d2 = sh[1][threadIdx.x + w];
+      sh[1][threadIdx.x] = d2 < sh[1][threadIdx.x] ? d2 : sh[1][threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.part[blk * 4 + 0] = v[0];
+    d.part[blk * 4 + 1] = sh[0][0];
+    d.part[blk * 4 + 2] = (double)si[0];
+    d.part[blk * 4 + 3] = sh[1][0];
+  }
+}
+
 // Residual of one vertex, NCFVM/fvresidual.f90:34-64 as a gather over its
 // edges in edge order: reconstruction of both ends (:40-54), van Leer on the
 // dual face (+ into p1, - into p2, spectral radius into both), then for a
@@ -502,17 +549,20 @@ void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode, do
   // vertices are stored along a Hilbert curve, so most neighbours a vertex
   // gathers lie in its own block, and its own record is read from here on
   // every edge instead of being held in registers
-  __shared__ R s_rec[BLK * W];
+  __shared__ __align__(16) R s_rec[BLK * W];
+  static_assert(BLK * W * sizeof(R) >= BLK * (2 * sizeof(double) + sizeof(int)),
+                "the norm's reduction reuses the staged records' LDS");
   const int base = xcd_block(blockIdx.x, gridDim.x) * BLK;
   const int p = base + (int)threadIdx.x;
   const int cnt = min(BLK, d.nop - base);
   const R* recs = NCRec<R>::rec(d);
   stage_block<R>(s_rec, recs + (size_t)base * W, cnt * W);
   __syncthreads();
-  if (p >= d.nop) return;
+  const bool have = p < d.nop;
   const R* erec = NCRec<R>::edge(d);
   const R* me = s_rec + threadIdx.x * W;
-  double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0;
+  double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0, dt = 0.0;
+  if (have) {
   const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
   auto edge = [&](int2 eo) {
     const bool first = eo.x >= 0;
@@ -587,7 +637,7 @@ void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode, do
     if (k < n) nb[k] = d.v2fo[j0 + k];
   for_edges(d, j0, n, nb, edge);
   double cvp = d.cv[p];
-  double dt = cfl * cvp / la;
+  dt = cfl * cvp / la;
   d.la[p] = la;
   d.dt[p] = dt;
   // the LU-SGS diagonal D = V/dt + omega/2 lambda (omega = 1.5,
@@ -603,53 +653,28 @@ void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode, do
     for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dtbyarea * r[i]);
     st4d(d.qc + (size_t)p * 4, q);
   }
+  }
+  // the residual norm's partials of this block from the registers (the
+  // staged records' LDS reused), as k_nc_norm forms them from memory
+  __syncthreads();
+  double(*sh)[BLK] = reinterpret_cast<double(*)[BLK]>(s_rec);
+  int* si = reinterpret_cast<int*>(s_rec) + 4 * BLK;
+  nc_norm_partial(d, sh, si, base / BLK, have, r, dt, have ? d.vref[p] : 0);
 }
 
-// residue, NCFVM/common.f90:108-154: block partials of sum |res|^2, the
-// largest per-vertex |res| with its (first) vertex, and min dt.
+// The norm partials from the stored residual and time steps (the residual
+// pass forms the same partials itself; this is for ufx_nc_residue).
 __global__ __launch_bounds__(BLK) void k_nc_norm(DevNC d) {
   __shared__ double sh[2][BLK];
   __shared__ int si[BLK];
-  int p = blockIdx.x * BLK + threadIdx.x;
-  double s = 0.0, mx = 0.0, dmn = 1.0e20;
-  int imx = 0x7fffffff;
-  if (p < d.nop) {
-    double r[4];
+  const int p = blockIdx.x * BLK + threadIdx.x;
+  const bool have = p < d.nop;
+  double r[4] = {0.0, 0.0, 0.0, 0.0}, dt = 0.0;
+  if (have) {
     ld4d(d.res + (size_t)p * 4, r);
-    double fr = 0.0;
-    for (int j = 0; j < 4; ++j) fr = fr + r[j] * r[j];
-    s = fr;
-    mx = sqrt(fr);
-    imx = d.vref[p];
-    dmn = d.dt[p];
+    dt = d.dt[p];
   }
-  double v[1] = {s};
-  block_reduce<1>(sh, v, OpAdd());
-  // argmax, first index on ties (the reference's strict '>' scan)
-  __syncthreads();
-  sh[0][threadIdx.x] = mx;
-  si[threadIdx.x] = imx;
-  sh[1][threadIdx.x] = dmn;
-  __syncthreads();
-  for (int w = BLK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      double m2 = sh[0][threadIdx.x + w];
-      int i2 = si[threadIdx.x + w];
-      if (m2 > sh[0][threadIdx.x] || (m2 == sh[0][threadIdx.x] && i2 < si[threadIdx.x])) {
-        sh[0][threadIdx.x] = m2;
-        si[threadIdx.x] = i2;
-      }
-      double d2 = sh[1][threadIdx.x + w];
-      sh[1][threadIdx.x] = d2 < sh[1][threadIdx.x] ? d2 : sh[1][threadIdx.x];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    d.part[blockIdx.x * 4 + 0] = v[0];
-    d.part[blockIdx.x * 4 + 1] = sh[0][0];
-    d.part[blockIdx.x * 4 + 2] = (double)si[0];
-    d.part[blockIdx.x * 4 + 3] = sh[1][0];
-  }
+  nc_norm_partial(d, sh, si, blockIdx.x, have, r, dt, have ? d.vref[p] : 0);
 }
 
 // Final reduction: hist[k] = {fres (normalised by the iteration-1 value),
@@ -660,13 +685,28 @@ __global__ __launch_bounds__(BLK) void k_nc_norm_final(DevNC d, int nblk, int fi
   __shared__ int si[BLK];
   double s = 0.0, mx = 0.0, dmn = 1.0e20;
   int imx = 0x7fffffff;
-  for (int b = threadIdx.x; b < nblk; b += BLK) {
-    s = s + d.part[b * 4];
-    double m2 = d.part[b * 4 + 1];
-    int i2 = (int)d.part[b * 4 + 2];
-    if (m2 > mx || (m2 == mx && i2 < imx)) { mx = m2; imx = i2; }
-    double d2 = d.part[b * 4 + 3];
-    dmn = d2 < dmn ? d2 : dmn;
+  // a thread's partials in block order, NU of them loaded together
+  constexpr int NU = 8;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += NU * BLK) {
+    double2 pa[NU], pb[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int b = b0 + u * BLK;
+      if (b < nblk) {
+        pa[u] = reinterpret_cast<const double2*>(d.part)[2 * b];
+        pb[u] = reinterpret_cast<const double2*>(d.part)[2 * b + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      if (b0 + u * BLK >= nblk) break;
+      s = s + pa[u].x;
+      double m2 = pa[u].y;
+      int i2 = (int)pb[u].x;
+      if (m2 > mx || (m2 == mx && i2 < imx)) { mx = m2; imx = i2; }
+      double d2 = pb[u].y;
+      dmn = d2 < dmn ? d2 : dmn;
+    }
   }
   double v[1] = {s};
   block_reduce<1>(sh, v, OpAdd());

@@ -274,8 +274,11 @@ int ensure_hist(ufx_nc* x, int n) {
   return UFX_OK;
 }
 
-int run_norm(ufx_nc* x, int iter, int k) {
-  NLAUNCH(x, NK_NORM, k_nc_norm, x->nblk, x->d);
+// residue: the block partials from the stored residual (`partials`), or
+// those the last residual pass formed itself (whole iterations), then the
+// final reduction into hist[k]
+int run_norm(ufx_nc* x, int iter, int k, bool partials) {
+  if (partials) NLAUNCH(x, NK_NORM, k_nc_norm, x->nblk, x->d);
   NLAUNCH(x, NK_NORM, k_nc_norm_final, 1, x->d, x->nblk, iter == 1 ? 1 : 0, x->d_hist, k);
   return launch_ok();
 }
@@ -853,7 +856,7 @@ int ufx_nc_lusgs(ufx_nc* x, double cfl) {
 int ufx_nc_residue(ufx_nc* x, int iter, double* out) {
   if (int rc = check(x)) return rc;
   if (int rc = ensure_hist(x, 1)) return rc;
-  if (int rc = run_norm(x, iter, 0)) return rc;
+  if (int rc = run_norm(x, iter, 0, true)) return rc;
   double h[5];
   NCCHK(hipMemcpyAsync(h, x->d_hist, sizeof(h), hipMemcpyDeviceToHost, x->stream));
   NCCHK(hipStreamSynchronize(x->stream));
@@ -903,7 +906,7 @@ int ufx_nc_iterate(ufx_nc* x, int iter0, int niter, double* hist) {
     } else {
       if (int rc = run_lusgs(x, cfl)) return rc;
     }
-    if (int rc = run_norm(x, iter, k)) return rc;
+    if (int rc = run_norm(x, iter, k, false)) return rc;
   }
   if (hist) {
     NCCHK(hipMemcpyAsync(hist, x->d_hist, sizeof(double) * 5 * (size_t)niter,
