@@ -97,6 +97,9 @@ struct DevT {
   R* qpc;                    // [noc][4] the interior cells' record primitives, compact
   R* qT;
   R* qTc;                    // [noc] the interior cells' temperatures beside qpc
+  R* phc;                    // [noc][4] the interior cells' limiter values, compact (the
+                             // ratchet's previous values; the records hold a copy)
+  const R* cenc;             // [ncell][2] centroids, compact
   R* gT;
   double* qc;
   double* qold;

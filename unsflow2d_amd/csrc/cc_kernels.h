@@ -350,14 +350,15 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
   R own[RW];
   R tc = R(0);
   if (c < d.noc) {
-    // the fields this kernel reads: primitives (the compact copy, which the
-    // last stage of a whole iteration refreshes), limiter values, centroid,
-    // and the old gradient only for a limiter-only call
-    const R* rc = d.rec + (size_t)c * RW;
+    // the fields this kernel reads, from compact arrays (whole lines of
+    // them, where the records would give a few values per line): the
+    // primitives (which the last stage of a whole iteration refreshes), the
+    // limiter values, the centroid; the old gradient only for a
+    // limiter-only call
     ldv<R, 4>(d.qpc + (size_t)c * 4, own + R_QP);
-    ldv<R, 4>(rc + R_PHI, own + R_PHI);
-    ldv<R, 2>(rc + R_CEN, own + R_CEN);
-    if (GRAD == G_NONE) ldv<R, 8>(rc + R_G, own + R_G);
+    ldv<R, 4>(d.phc + (size_t)c * 4, own + R_PHI);
+    ldv<R, 2>(d.cenc + (size_t)c * 2, own + R_CEN);
+    if (GRAD == G_NONE) ldv<R, 8>(d.rec + (size_t)c * RW + R_G, own + R_G);
     tc = d.qTc[c];
   }
   if (SL) {
@@ -428,7 +429,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
     if (GRAD == G_LSQR) {
       R ncen[2];
       if (loc) ldv<R, 2>(s_cen + l * 2, ncen);
-      else ldv<R, 2>(d.rec + (size_t)n * RW + R_CEN, ncen);
+      else ldv<R, 2>(d.cenc + (size_t)n * 2, ncen);
       R dx = ncen[0] - cx, dy = ncen[1] - cy;
       R alfa1 = dx / r11 / r11;
       R alfa2 = bnd ? (dy - dx * r12 / r11) / r22 / r22 : (dy - dx * r12 / r11) / (r22 * r22);
@@ -533,6 +534,7 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
     stv<R, 16>(d.rec + (size_t)c * RW + R_QP, own + R_QP);
     d.qT[c] = tc;
   }
+  if (LIMIT) stv<R, 4>(d.phc + (size_t)c * 4, own + R_PHI);
   if (TSTEP) {
     d.la[c] = la;
     double dt = cfl * d.cv[c] / la;

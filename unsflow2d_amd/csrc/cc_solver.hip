@@ -731,6 +731,10 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
       for (int k = 0; k < 4; ++k) qpc[(size_t)s * 4 + k] = rec[(size_t)s * RW + R_QP + k];
     HIPCHK(up(d.qpc, qpc.data(), qpc.size() * sizeof(R)));
     HIPCHK(up(d.qTc, t.data(), (size_t)noc * sizeof(R)));
+    std::vector<R> phc((size_t)noc * 4);
+    for (int s = 0; s < noc; ++s)
+      for (int k = 0; k < 4; ++k) phc[(size_t)s * 4 + k] = rec[(size_t)s * RW + R_PHI + k];
+    HIPCHK(up(d.phc, phc.data(), phc.size() * sizeof(R)));
     HIPCHK(hipStreamSynchronize(x->stream));
     return UFX_OK;
   }
@@ -826,13 +830,19 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     r[R_CEN] = (R)m.ccx[x->ref_of_sto[c]];
     r[R_CEN + 1] = (R)m.ccy[x->ref_of_sto[c]];
   }
+  // compact copies the gradient pass reads instead of the records
+  std::vector<R> phc((size_t)noc * 4, R(1)), cenc((size_t)ncell * 2);
+  for (int c = 0; c < ncell; ++c) {
+    cenc[(size_t)c * 2] = rec[(size_t)c * RW + R_CEN];
+    cenc[(size_t)c * 2 + 1] = rec[(size_t)c * RW + R_CEN + 1];
+  }
   std::vector<double> qc((size_t)ncell * 4);
   for (int c = 0; c < ncell; ++c)
     for (int k = 0; k < 4; ++k) qc[(size_t)c * 4 + k] = x->p.qinf[k];
   int rc;
 #define UP(dst, src) if ((rc = dupload(x, &d.dst, src))) return rc
 #define AL(dst, n) if ((rc = dalloc(x, &d.dst, (size_t)(n)))) return rc
-  UP(frec, frec); UP(fcen, fcen);
+  UP(frec, frec); UP(fcen, fcen); UP(phc, phc); UP(cenc, cenc);
   {
     int2* p = nullptr;
     if ((rc = dupload(x, &p, fpt))) return rc;
