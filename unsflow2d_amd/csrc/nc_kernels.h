@@ -41,6 +41,11 @@ struct DevNC {
   const double* erec;
   const int2* ep;
   const int* ebc;            // per edge: its index in the boundary-edge list, or -1
+  // block-shared residual: per block of BLK vertices the edges they touch
+  // (ascending stored edge number), and per v2f entry its edge's position
+  // in its vertex's block list (bit-inverted when the vertex is p2)
+  const int *be_ptr, *be_edge, *sl_loc;
+  int be_max;
   const int* bedge;          // wall and farfield edges (stored edge numbers)
   const int* bkind;          // their kind (WALL or FARF)
   double* bclos;             // [nbnd][2 ends][5] closure terms (momentum/flux 4, radius 1)
@@ -529,6 +534,49 @@ __device__ __forceinline__ void nc_norm_partial(const DevNC& d, double (*sh)[BLK
   }
 }
 
+// A wall or farfield edge's closure (solid_flux, farfield_flux1), added
+// right after the edge's flux as the reference does (bi: the edge's index
+// in the boundary-edge list, -1 for other edges); its terms were formed per
+// boundary edge by the gradient launch.
+__device__ __forceinline__ void nc_add_closure(const DevNC& d, int bi, bool first, double* r,
+                                               double& la) {
+  if (bi < 0) return;
+  const double* c = d.bclos + ((size_t)bi * 2 + (first ? 0 : 1)) * 5;
+  if (d.bkind[bi] == WALL) {
+    r[2] = r[2] + c[2];
+    r[3] = r[3] + c[3];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = r[i] + c[i];
+  }
+  la = la + c[4];
+}
+
+// The end of a vertex's residual: time_step2 (NCFVM/timestep.f90:5-37), the
+// LU-SGS diagonal D = V/dt + omega/2 lambda (omega = 1.5,
+// NCFVM/lusgs.f90:4-131; the multicolour sweeps read it once per vertex
+// instead of gathering its three inputs), the residual, and for mode 1 the
+// RK stage update of NCFVM/main.f90:59-70 in place.  Returns dt.
+__device__ __forceinline__ double nc_vertex_finish(const DevNC& d, int p, const double* r,
+                                                   double la, double cfl, int mode, double ak,
+                                                   double bk) {
+  const double cvp = d.cv[p];
+  const double dt = cfl * cvp / la;
+  d.la[p] = la;
+  d.dt[p] = dt;
+  if (mode == 0) d.dd[p] = cvp / dt + 0.5 * 1.5 * la;
+  st4d(d.res + (size_t)p * 4, r);
+  if (mode == 1) {
+    double dtbyarea = dt / cvp, q[4], o[4];
+    ld4d(d.qc + (size_t)p * 4, q);
+    ld4d(d.qold + (size_t)p * 4, o);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dtbyarea * r[i]);
+    st4d(d.qc + (size_t)p * 4, q);
+  }
+  return dt;
+}
+
 // Residual of one vertex, NCFVM/fvresidual.f90:34-64 as a gather over its
 // edges in edge order: reconstruction of both ends (:40-54), van Leer on the
 // dual face (+ into p1, - into p2, spectral radius into both), then for a
@@ -615,44 +663,14 @@ void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode, do
       for (int i = 0; i < 4; ++i) r[i] = r[i] - F[i] * ds;
     }
     la = la + ll;
-    // a wall or farfield edge's closure (solid_flux, farfield_flux1), added
-    // right after the edge's flux as the reference does; its terms were
-    // formed per boundary edge and end by the gradient launch
-    const int bk = d.ebc[e];
-    if (bk >= 0) {
-      const double* c = d.bclos + ((size_t)bk * 2 + (first ? 0 : 1)) * 5;
-      if (d.bkind[bk] == WALL) {
-        r[2] = r[2] + c[2];
-        r[3] = r[3] + c[3];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) r[i] = r[i] + c[i];
-      }
-      la = la + c[4];
-    }
+    nc_add_closure(d, d.ebc[e], first, r, la);
   };
   int2 nb[NCU];
 #pragma unroll
   for (int k = 0; k < NCU; ++k)
     if (k < n) nb[k] = d.v2fo[j0 + k];
   for_edges(d, j0, n, nb, edge);
-  double cvp = d.cv[p];
-  dt = cfl * cvp / la;
-  d.la[p] = la;
-  d.dt[p] = dt;
-  // the LU-SGS diagonal D = V/dt + omega/2 lambda (omega = 1.5,
-  // NCFVM/lusgs.f90:4-131; the multicolour sweeps read
-  // it once per vertex instead of gathering its three inputs)
-  if (mode == 0) d.dd[p] = cvp / dt + 0.5 * 1.5 * la;
-  st4d(d.res + (size_t)p * 4, r);
-  if (mode == 1) {
-    double dtbyarea = dt / cvp, q[4], o[4];
-    ld4d(d.qc + (size_t)p * 4, q);
-    ld4d(d.qold + (size_t)p * 4, o);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) q[i] = ak * o[i] + bk * (q[i] - dtbyarea * r[i]);
-    st4d(d.qc + (size_t)p * 4, q);
-  }
+  dt = nc_vertex_finish(d, p, r, la, cfl, mode, ak, bk);
   }
   // the residual norm's partials of this block from the registers (the
   // staged records' LDS reused), as k_nc_norm forms them from memory
@@ -739,6 +757,93 @@ __global__ __launch_bounds__(BLK) void k_nc_norm_final(DevNC d, int nblk, int fi
     h[3] = sh[0][0] > 0.0 ? (double)(si[0] + 1) : 0.0;
     h[4] = sh[1][0];
   }
+}
+
+// The same residual with each edge's flux evaluated once per block instead
+// of once per end: the edges the block's vertices touch (a list per block,
+// `be_ptr`/`be_edge`; vertices are stored along a Hilbert curve, so most
+// edges have both ends in the block) are evaluated by the block's threads
+// into LDS -- F ds and the spectral radius, from p1's and p2's records
+// exactly as the reference reconstructs them -- then every vertex sums its
+// edges from LDS in its own edge order (`sl_loc`: the entry's position in
+// the block's list, bit-inverted when the vertex is p2), with the
+// boundary closures after their edges.  The same sums in the same order as
+// the per-vertex gather: the results are identical.
+template <typename R, int FLUX>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4)))
+void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode, double ak,
+                       double bk) {
+  constexpr int W = NCRec<R>::W;
+  extern __shared__ __align__(16) double s_f[];  // [be_max][5]: F ds (4), radius
+  int* s_bi = reinterpret_cast<int*>(s_f + (size_t)d.be_max * 5);  // [be_max] boundary index
+  const int nvb = (d.nop + BLK - 1) / BLK;
+  const int blk = xcd_block(blockIdx.x, nvb);
+  const int base = blk * BLK;
+  const int p = base + (int)threadIdx.x;
+  const R* recs = NCRec<R>::rec(d);
+  const R* erec = NCRec<R>::edge(d);
+  const int e0 = d.be_ptr[blk], ne = d.be_ptr[blk + 1] - e0;
+  for (int k = threadIdx.x; k < ne; k += BLK) {
+    const int e = d.be_edge[e0 + k];
+    const int2 pp = d.ep[e];
+    R a[W], b[W], er[4];
+    ldv<R, W>(recs + (size_t)pp.x * W, a);
+    ldv<R, W>(recs + (size_t)pp.y * W, b);
+    ldv<R, 4>(erec + (size_t)e * 4, er);
+    const double nx = er[0], ny = er[1], ds = er[2];
+    const double d0 = (double)b[R_CEN] - (double)a[R_CEN];
+    const double d1 = (double)b[R_CEN + 1] - (double)a[R_CEN + 1];
+    double ql[4], qr[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double t = (0.0 + (double)a[R_G + 2 * i] * d0) + (double)a[R_G + 2 * i + 1] * d1;
+      ql[i] = (double)a[R_QP + i] + 0.5 * t * (double)a[R_PHI + i];
+      t = (0.0 + (double)b[R_G + 2 * i] * d0) + (double)b[R_G + 2 * i + 1] * d1;
+      qr[i] = (double)b[R_QP + i] - 0.5 * t * (double)b[R_PHI + i];
+    }
+    if constexpr (sizeof(R) == 4) {
+      const double qi[4] = {g.r_inf, g.u_inf, g.v_inf, g.p_inf};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        ql[i] = ql[i] + qi[i];
+        qr[i] = qr[i] + qi[i];
+      }
+    }
+    double F[4], ll;
+    if (FLUX == 2) nc_ausm<double>(ga64, ql, qr, nx, ny, ds, F, ll);
+    else nc_vanleer<double>(g.gamma, g.gamma1, ql, qr, nx, ny, ds, F, ll);
+    double* o = s_f + (size_t)k * 5;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[i] = F[i] * ds;
+    o[4] = ll;
+    s_bi[k] = d.ebc[e];
+    d.fla[e] = ll;
+  }
+  __syncthreads();
+  const bool have = p < d.nop;
+  double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0, dt = 0.0;
+  if (have) {
+    for (int j = d.v2f_ptr[p]; j < d.v2f_ptr[p + 1]; ++j) {
+      const int loc = d.sl_loc[j];
+      const bool first = loc >= 0;
+      const int k = first ? loc : ~loc;
+      const double* o = s_f + (size_t)k * 5;
+      if (first) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = r[i] + o[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = r[i] - o[i];
+      }
+      la = la + o[4];
+      nc_add_closure(d, s_bi[k], first, r, la);
+    }
+    dt = nc_vertex_finish(d, p, r, la, cfl, mode, ak, bk);
+  }
+  __syncthreads();
+  double(*sh)[BLK] = reinterpret_cast<double(*)[BLK]>(s_f);
+  int* si = reinterpret_cast<int*>(s_f + 2 * BLK);
+  nc_norm_partial(d, sh, si, blk, have, r, dt, have ? d.vref[p] : 0);
 }
 
 // LU-SGS of NCFVM/lusgs.f90:4-131 (omega = 1.5): diagonal D = V/dt +

@@ -49,6 +49,7 @@ struct ufx_nc {
   double *d_fe = nullptr, *d_cp = nullptr;  // [nwall][2] force terms, [nwall] -Cp
   int sweep_mode = 4;                       // 0 graph of level launches, 1 level launches, 2 one workgroup, 3 cooperative grid, 4 level-ordered one workgroup
   int grid_wgs = 8;                         // workgroups of the cooperative sweep
+  size_t res_lds = 0;                       // LDS of the block-shared residual (0: per-vertex gather form)
   std::vector<int> mc_off;                  // multicolour sweeps: positions of each colour
   hipGraph_t sweep_graph = nullptr;         // diagonal + both sweeps, captured once
   hipGraphExec_t sweep_exec = nullptr;
@@ -149,10 +150,19 @@ template <typename R>
 int run_fvresidual_r(ufx_nc* x, double cfl, int mode, double ak, double bk) {
   if (int rc = run_grad_lim_r<R>(x)) return rc;
   const Gas<double>& ga = x->ga;
-  if (x->prm.flux == UFX_FLUX_AUSM)
+  if (x->res_lds > 0) {
+    const size_t lds = x->res_lds;
+    prof_mark(x, NK_RES);
+    if (x->prm.flux == UFX_FLUX_AUSM)
+      k_nc_residual_blk<R, 2><<<x->nblk, BLK, lds, x->stream>>>(x->d, x->g, ga, cfl, mode, ak, bk);
+    else
+      k_nc_residual_blk<R, 3><<<x->nblk, BLK, lds, x->stream>>>(x->d, x->g, ga, cfl, mode, ak, bk);
+    prof_mark(x, -1);
+  } else if (x->prm.flux == UFX_FLUX_AUSM) {
     NLAUNCH(x, NK_RES, (k_nc_residual<R, 2>), x->nblk, x->d, x->g, ga, cfl, mode, ak, bk);
-  else
+  } else {
     NLAUNCH(x, NK_RES, (k_nc_residual<R, 3>), x->nblk, x->d, x->g, ga, cfl, mode, ak, bk);
+  }
   return launch_ok();
 }
 
@@ -545,6 +555,40 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       v2f[j] = first ? es[e] : ~es[e];
       v2fo[j] = make_int2(v2f[j], first ? m.fp2[e] : m.fp1[e]);
     }
+  // block-shared residual: per block of BLK vertices the stored edges its
+  // vertices touch, ascending, and per v2f entry the position of its edge
+  // in that list (bit-inverted when the vertex is p2)
+  std::vector<int> be_ptr(x->nblk + 1, 0), be_edge, sl_loc(m.v2f.size());
+  int be_max = 0;
+  {
+    std::vector<int> list;
+    for (int b = 0; b < x->nblk; ++b) {
+      const int p0 = b * BLK, p1 = std::min(nop, p0 + BLK);
+      list.clear();
+      for (int p = p0; p < p1; ++p)
+        for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) list.push_back(es[m.v2f[j]]);
+      std::sort(list.begin(), list.end());
+      list.erase(std::unique(list.begin(), list.end()), list.end());
+      for (int p = p0; p < p1; ++p)
+        for (int j = m.v2f_ptr[p]; j < m.v2f_ptr[p + 1]; ++j) {
+          const int e = m.v2f[j];
+          const int k = (int)(std::lower_bound(list.begin(), list.end(), es[e]) - list.begin());
+          sl_loc[j] = m.fp1[e] == p ? k : ~k;
+        }
+      be_edge.insert(be_edge.end(), list.begin(), list.end());
+      be_ptr[b + 1] = (int)be_edge.size();
+      be_max = std::max(be_max, (int)list.size());
+    }
+  }
+  {
+    // per edge 5 doubles and a boundary index; at least what the norm's
+    // block reduction reuses
+    const size_t lds = std::max((size_t)be_max * (5 * sizeof(double) + sizeof(int)),
+                                (size_t)BLK * (2 * sizeof(double) + sizeof(int)));
+    const char* form = std::getenv("UFX_NC_RESIDUAL");
+    const bool gather = form && std::strcmp(form, "gather") == 0;
+    if (!gather && lds <= 64 * 1024) x->res_lds = lds;
+  }
   std::vector<double> wxy(2 * m.v2v.size());
   for (size_t j = 0; j < m.v2v.size(); ++j) {
     wxy[2 * j] = m.wx[j];
@@ -785,6 +829,8 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
   }
   NUP(qc, qc); NUP(cv, m.pcv); NUP(ebc, ebc); NUP(vref, rv);
   NUP(bedge, bedge); NUP(bkind, bkind); NAL(bclos, 10 * bedge.size());
+  NUP(be_ptr, be_ptr); NUP(be_edge, be_edge); NUP(sl_loc, sl_loc);
+  d.be_max = be_max;
   d.nbnd = (int)bedge.size();
   NUP(v2f_ptr, m.v2f_ptr); NUP(v2f, v2f); NUP(v2v_ptr, m.v2v_ptr); NUP(v2v, m.v2v);
   if (!rc) {
