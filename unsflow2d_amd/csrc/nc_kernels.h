@@ -92,9 +92,11 @@ struct DevNC {
   const int2* mc_eo;      // [2][mc_q][nop] (edge, bit-inverted when the vertex is p2; other end)
   const double2* mc_nxy;  // [2][mc_q][nop] edge nx, ny
   const double* mc_ds;    // [2][mc_q][nop] edge length
-  double* mc_qold;        // [nop][4] old state by position (the forward sweep writes it)
+  const int* mc_pos;      // [nop] colour position of each vertex (null: no multicolour sweeps)
+  double* mc_qold;        // [nop][4] old state by position (save_old writes it)
+  double* mc_res;         // [nop][4] residual by position (the residual pass writes it)
   double* mc_dq;          // [nop][4] dq by position
-  double* mc_dd;          // [nop] diagonal by position
+  double* mc_dd;          // [nop] diagonal by position (the residual pass writes it)
   int mc_q;
 };
 
@@ -163,6 +165,8 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
   double q[4];
   ld4d(d.qc + (size_t)p * 4, q);
   st4d(d.qold + (size_t)p * 4, q);
+  // the multicolour sweeps read the old state by colour position
+  if (d.mc_pos) st4d(d.mc_qold + (size_t)d.mc_pos[p] * 4, q);
   double rho = q[1], u = q[2] / q[1], v = q[3] / q[1], et = q[0];
   double qq = u * u + v * v;
   double pr = (g.gamma - 1.0) * (et - 0.5 * rho * qq);
@@ -566,6 +570,12 @@ __device__ __forceinline__ double nc_vertex_finish(const DevNC& d, int p, const 
   d.dt[p] = dt;
   if (mode == 0) d.dd[p] = cvp / dt + 0.5 * 1.5 * la;
   st4d(d.res + (size_t)p * 4, r);
+  if (d.mc_pos && mode == 0) {
+    // the multicolour sweeps read residual and diagonal by colour position
+    const size_t s = (size_t)d.mc_pos[p];
+    st4d(d.mc_res + s * 4, r);
+    d.mc_dd[s] = d.dd[p];
+  }
   if (mode == 1) {
     double dtbyarea = dt / cvp, q[4], o[4];
     ld4d(d.qc + (size_t)p * 4, q);
@@ -986,14 +996,10 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, 
   }
   double res[4], dq[4], dd;
   if (dir == 0) {
-    // the vertex's own data, once per sweep pair: its old state and
-    // diagonal go to position order for its neighbours and the backward sweep
-    double qo[4];
-    ld4d(d.qold + (size_t)it * 4, qo);
-    st4d(d.mc_qold + (size_t)s * 4, qo);
-    dd = d.dd[it];
-    d.mc_dd[s] = dd;
-    ld4d(d.res + (size_t)it * 4, res);
+    // the vertex's own residual and diagonal, in position order (the
+    // residual pass wrote them there, as save_old the old state)
+    dd = d.mc_dd[s];
+    ld4d(d.mc_res + (size_t)s * 4, res);
 #pragma unroll
     for (int i = 0; i < 4; ++i) dq[i] = (-res[i] - cres[i]) / dd;
   } else {

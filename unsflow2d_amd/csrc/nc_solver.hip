@@ -51,6 +51,7 @@ struct ufx_nc {
   int grid_wgs = 8;                         // workgroups of the cooperative sweep
   size_t res_lds = 0;                       // LDS of the block-shared residual (0: per-vertex gather form)
   std::vector<int> mc_off;                  // multicolour sweeps: positions of each colour
+  std::vector<int> mc_pos;                  // and the position of each stored vertex
   hipGraph_t sweep_graph = nullptr;         // diagonal + both sweeps, captured once
   hipGraphExec_t sweep_exec = nullptr;
   double cfl = 0.0;
@@ -790,6 +791,9 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       if (!rc) rc = nc_upload(x, &x->d.mc_nxy, nxy);
       if (!rc) rc = nc_upload(x, &x->d.mc_ds, dsv);
       if (!rc) rc = nc_alloc(x, &x->d.mc_qold, 4 * (size_t)nop);
+      if (!rc) rc = nc_alloc(x, &x->d.mc_res, 4 * (size_t)nop);
+      if (!rc) rc = nc_upload(x, &x->d.mc_pos, mpos);
+      x->mc_pos = mpos;
       if (!rc) rc = nc_alloc(x, &x->d.mc_dq, 4 * (size_t)nop);
       if (!rc) rc = nc_alloc(x, &x->d.mc_dd, (size_t)nop);
     }
@@ -976,6 +980,15 @@ int ufx_nc_set(ufx_nc* x, const char* name, const double* src) {
     NCCHK(hipMemcpyAsync(n == "qc" ? x->d.qc : x->d.qold, t.data(), sizeof(double) * 4 * nop,
                          hipMemcpyHostToDevice, x->stream));
     NCCHK(hipStreamSynchronize(x->stream));
+    if (n == "qold" && !x->mc_pos.empty()) {
+      // the multicolour sweeps' copy by colour position
+      std::vector<double> u(4 * (size_t)nop);
+      for (int p = 0; p < nop; ++p)
+        for (int k = 0; k < 4; ++k) u[(size_t)x->mc_pos[p] * 4 + k] = t[(size_t)p * 4 + k];
+      NCCHK(hipMemcpyAsync(x->d.mc_qold, u.data(), sizeof(double) * 4 * nop, hipMemcpyHostToDevice,
+                           x->stream));
+      NCCHK(hipStreamSynchronize(x->stream));
+    }
   } else if (n == "phi") {
     std::vector<double> v;
     if (int rc = read_records(x, v)) return rc;
