@@ -36,6 +36,13 @@ struct NCFree {
 struct DevNC {
   int nop, nof;
   double* vrec;
+  // compact per-vertex arrays beside the records: the primitives save_old
+  // writes, the limiter values, the coordinates (the gradient pass reads
+  // these and copies primitives and limiter values into the records)
+  double *vqp, *vphc;
+  const double* vxy;
+  float *vqp32, *vphc32;
+  const float* vxy32;
   double *qc, *qold, *res, *la, *dt, *dd, *dqc;
   const double* cv;
   const double* erec;
@@ -142,12 +149,18 @@ template <typename R> struct NCRec;
 template <> struct NCRec<double> {
   static constexpr int W = NW;
   __device__ static double* rec(const DevNC& d) { return d.vrec; }
+  __device__ static double* qp(const DevNC& d) { return d.vqp; }
+  __device__ static double* phc(const DevNC& d) { return d.vphc; }
+  __device__ static const double* xy(const DevNC& d) { return d.vxy; }
   __device__ static const double* edge(const DevNC& d) { return d.erec; }
   __device__ static const double* lsq(const DevNC& d) { return d.wxy; }
 };
 template <> struct NCRec<float> {
   static constexpr int W = NW32;
   __device__ static float* rec(const DevNC& d) { return d.vrec32; }
+  __device__ static float* qp(const DevNC& d) { return d.vqp32; }
+  __device__ static float* phc(const DevNC& d) { return d.vphc32; }
+  __device__ static const float* xy(const DevNC& d) { return d.vxy32; }
   __device__ static const float* edge(const DevNC& d) { return d.erec32; }
   __device__ static const float* lsq(const DevNC& d) { return d.wxy32; }
 };
@@ -177,7 +190,7 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
     o[0] = (R)(rho - g.r_inf); o[1] = (R)(u - g.u_inf); o[2] = (R)(v - g.v_inf);
     o[3] = (R)(pr - g.p_inf);
   }
-  stv<R, 4>(NCRec<R>::rec(d) + (size_t)p * NCRec<R>::W + R_QP, o);
+  stv<R, 4>(NCRec<R>::qp(d) + (size_t)p * 4, o);
 }
 
 // The boundary closures of one wall or farfield edge, for both of its ends
@@ -190,16 +203,16 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
 // term of each end.  Evaluated in FP64 in both forms.
 template <typename R>
 __device__ __forceinline__ void nc_closure(const DevNC& d, const NCFree& g, int k) {
-  constexpr int W = NCRec<R>::W;
-  const R* recs = NCRec<R>::rec(d);
+  const R* qps = NCRec<R>::qp(d);
+  const R* xys = NCRec<R>::xy(d);
   const int e = d.bedge[k], bc = d.bkind[k];
   const int2 pp = d.ep[e];
-  const R* a = recs + (size_t)pp.x * W;
-  const R* b = recs + (size_t)pp.y * W;
-  double ax = (double)a[R_CEN], ay = (double)a[R_CEN + 1];
-  double bx = (double)b[R_CEN], by = (double)b[R_CEN + 1];
-  double ap[4] = {(double)a[R_QP], (double)a[R_QP + 1], (double)a[R_QP + 2], (double)a[R_QP + 3]};
-  double bp[4] = {(double)b[R_QP], (double)b[R_QP + 1], (double)b[R_QP + 2], (double)b[R_QP + 3]};
+  const R* a = qps + (size_t)pp.x * 4;
+  const R* b = qps + (size_t)pp.y * 4;
+  double ax = (double)xys[(size_t)pp.x * 2], ay = (double)xys[(size_t)pp.x * 2 + 1];
+  double bx = (double)xys[(size_t)pp.y * 2], by = (double)xys[(size_t)pp.y * 2 + 1];
+  double ap[4] = {(double)a[0], (double)a[1], (double)a[2], (double)a[3]};
+  double bp[4] = {(double)b[0], (double)b[1], (double)b[2], (double)b[3]};
   if constexpr (sizeof(R) == 4) {
     const double qi[4] = {g.r_inf, g.u_inf, g.v_inf, g.p_inf};
 #pragma unroll
@@ -303,13 +316,13 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
   const int base = xcd_block(blockIdx.x, nvb) * BLK;
   const int p = base + (int)threadIdx.x;
   const int cnt = min(BLK, d.nop - base);
-  const R* recs = NCRec<R>::rec(d);
+  const R* qps = NCRec<R>::qp(d);
+  const R* xys = NCRec<R>::xy(d);
   const R* erec = NCRec<R>::edge(d);
-  const R* own = recs + (size_t)p * W;
   R q[4], xy[2];
   if (p < d.nop) {
-    ldv<R, 4>(own + R_QP, q);
-    ldv<R, 2>(own + R_CEN, xy);
+    ldv<R, 4>(qps + (size_t)p * 4, q);
+    ldv<R, 2>(xys + (size_t)p * 2, xy);
     stv<R, 4>(s_qp + threadIdx.x * 4, q);
     stv<R, 2>(s_xy + threadIdx.x * 2, xy);
   }
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
   auto nbr_qp = [&](int o, R* qo) {
     const unsigned l = (unsigned)(o - base);
     if (l < (unsigned)cnt) ldv<R, 4>(s_qp + l * 4, qo);
-    else ldv<R, 4>(recs + (size_t)o * W + R_QP, qo);
+    else ldv<R, 4>(qps + (size_t)o * 4, qo);
   };
   const R x = xy[0], y = xy[1];
   R gr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -391,7 +404,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
     }
   });
   R phi[4];
-  ldv<R, 4>(own + R_PHI, phi);
+  ldv<R, 4>(NCRec<R>::phc(d) + (size_t)p * 4, phi);
   const R kappa = (R)0.01f;
   R t = kappa * fsqrt<R>((R)d.cv[p]);
   const R tol = t * t * t;
@@ -400,7 +413,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
     R xo[2];
     const unsigned l = (unsigned)(eo.y - base);
     if (l < (unsigned)cnt) ldv<R, 2>(s_xy + l * 2, xo);
-    else ldv<R, 2>(recs + (size_t)eo.y * W + R_CEN, xo);
+    else ldv<R, 2>(xys + (size_t)eo.y * 2, xo);
     R d0 = first ? xo[0] - x : x - xo[0], d1 = first ? xo[1] - y : y - xo[1];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -431,10 +444,17 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
       phi[k] = phi[k] < ph ? phi[k] : ph;
     }
   });
-  R* w = NCRec<R>::rec(d) + (size_t)p * W;
-  stv<R, 4>(w + R_G, gr);
-  stv<R, 4>(w + R_G + 4, gr + 4);
-  stv<R, 4>(w + R_PHI, phi);
+  // the record the residual gathers: primitives, limiter values and
+  // gradient in one piece; and the limiter values' compact copy
+  static_assert(R_PHI == R_QP + 4 && R_G == R_PHI + 4, "record fields are adjacent");
+  R o16[16];
+  ldv<R, 4>(s_qp + threadIdx.x * 4, o16);  // (read back: q need not stay live)
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o16[4 + k] = phi[k];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o16[8 + k] = gr[k];
+  stv<R, 16>(NCRec<R>::rec(d) + (size_t)p * W + R_QP, o16);
+  stv<R, 4>(NCRec<R>::phc(d) + (size_t)p * 4, phi);
 }
 
 // Spectral radius of a dual face from the Roe average of its two states,

@@ -304,39 +304,71 @@ int check(ufx_nc* x) {
   return UFX_OK;
 }
 
-// The vertex records as FP64, NW per vertex, whichever form holds them.
+// The vertex records as FP64, NW per vertex, whichever form holds them;
+// their primitives and limiter values from the compact copies, which are
+// current (the records' copies are refreshed by the gradient pass).
 int read_records(ufx_nc* x, std::vector<double>& v) {
   const size_t nop = (size_t)x->nop;
   v.assign(nop * NW, 0.0);
+  std::vector<double> qp(nop * 4), ph(nop * 4);
   if (!x->mixed) {
     NCCHK(hipMemcpyAsync(v.data(), x->d.vrec, v.size() * 8, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipMemcpyAsync(qp.data(), x->d.vqp, qp.size() * 8, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipMemcpyAsync(ph.data(), x->d.vphc, ph.size() * 8, hipMemcpyDeviceToHost, x->stream));
     NCCHK(hipStreamSynchronize(x->stream));
-    return UFX_OK;
+  } else {
+    std::vector<float> f(nop * NW32), fq(nop * 4), fp(nop * 4);
+    NCCHK(hipMemcpyAsync(f.data(), x->d.vrec32, f.size() * 4, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipMemcpyAsync(fq.data(), x->d.vqp32, fq.size() * 4, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipMemcpyAsync(fp.data(), x->d.vphc32, fp.size() * 4, hipMemcpyDeviceToHost, x->stream));
+    NCCHK(hipStreamSynchronize(x->stream));
+    const double qi[4] = {x->g.r_inf, x->g.u_inf, x->g.v_inf, x->g.p_inf};
+    for (size_t p = 0; p < nop; ++p) {
+      for (int k = 0; k < NW; ++k) v[p * NW + k] = (double)f[p * NW32 + k];
+      for (int k = 0; k < 4; ++k) {
+        qp[p * 4 + k] = (double)fq[p * 4 + k] + qi[k];
+        ph[p * 4 + k] = (double)fp[p * 4 + k];
+      }
+    }
   }
-  std::vector<float> f(nop * NW32);
-  NCCHK(hipMemcpyAsync(f.data(), x->d.vrec32, f.size() * 4, hipMemcpyDeviceToHost, x->stream));
-  NCCHK(hipStreamSynchronize(x->stream));
-  const double qi[4] = {x->g.r_inf, x->g.u_inf, x->g.v_inf, x->g.p_inf};
   for (size_t p = 0; p < nop; ++p)
-    for (int k = 0; k < NW; ++k)
-      v[p * NW + k] = (double)f[p * NW32 + k] + (k >= R_QP && k < R_QP + 4 ? qi[k - R_QP] : 0.0);
+    for (int k = 0; k < 4; ++k) {
+      v[p * NW + R_QP + k] = qp[p * 4 + k];
+      v[p * NW + R_PHI + k] = ph[p * 4 + k];
+    }
   return UFX_OK;
 }
 
+// The records, and the compact primitives and limiter values, from FP64
+// records.
 int write_records(ufx_nc* x, const std::vector<double>& v) {
   const size_t nop = (size_t)x->nop;
+  std::vector<double> qp(nop * 4), ph(nop * 4);
+  for (size_t p = 0; p < nop; ++p)
+    for (int k = 0; k < 4; ++k) {
+      qp[p * 4 + k] = v[p * NW + R_QP + k];
+      ph[p * 4 + k] = v[p * NW + R_PHI + k];
+    }
   if (!x->mixed) {
     NCCHK(hipMemcpyAsync(x->d.vrec, v.data(), v.size() * 8, hipMemcpyHostToDevice, x->stream));
+    NCCHK(hipMemcpyAsync(x->d.vqp, qp.data(), qp.size() * 8, hipMemcpyHostToDevice, x->stream));
+    NCCHK(hipMemcpyAsync(x->d.vphc, ph.data(), ph.size() * 8, hipMemcpyHostToDevice, x->stream));
     NCCHK(hipStreamSynchronize(x->stream));
     return UFX_OK;
   }
-  std::vector<float> f(nop * NW32, 0.0f);
+  std::vector<float> f(nop * NW32, 0.0f), fq(nop * 4), fp(nop * 4);
   const double qi[4] = {x->g.r_inf, x->g.u_inf, x->g.v_inf, x->g.p_inf};
-  for (size_t p = 0; p < nop; ++p)
+  for (size_t p = 0; p < nop; ++p) {
     for (int k = 0; k < NW; ++k)
-      f[p * NW32 + k] =
-          (float)(v[p * NW + k] - (k >= R_QP && k < R_QP + 4 ? qi[k - R_QP] : 0.0));
+      f[p * NW32 + k] = (float)(v[p * NW + k] - (k >= R_QP && k < R_QP + 4 ? qi[k - R_QP] : 0.0));
+    for (int k = 0; k < 4; ++k) {
+      fq[p * 4 + k] = (float)(qp[p * 4 + k] - qi[k]);
+      fp[p * 4 + k] = (float)ph[p * 4 + k];
+    }
+  }
   NCCHK(hipMemcpyAsync(x->d.vrec32, f.data(), f.size() * 4, hipMemcpyHostToDevice, x->stream));
+  NCCHK(hipMemcpyAsync(x->d.vqp32, fq.data(), fq.size() * 4, hipMemcpyHostToDevice, x->stream));
+  NCCHK(hipMemcpyAsync(x->d.vphc32, fp.data(), fp.size() * 4, hipMemcpyHostToDevice, x->stream));
   NCCHK(hipStreamSynchronize(x->stream));
   return UFX_OK;
 }
@@ -828,8 +860,20 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
       ggs[p] = make_float2((float)sx, (float)sy);
     }
     NUP(vrec32, vr32); NUP(erec32, er32); NUP(wxy32, w32); NUP(ggs32, ggs);
+    std::vector<float> q32((size_t)nop * 4, 0.0f), p32((size_t)nop * 4, (float)phi0), x32((size_t)nop * 2);
+    for (int p = 0; p < nop; ++p) {
+      x32[(size_t)p * 2] = (float)m.px[p];
+      x32[(size_t)p * 2 + 1] = (float)m.py[p];
+    }
+    NUP(vqp32, q32); NUP(vphc32, p32); NUP(vxy32, x32);
   } else {
     NUP(vrec, vrec); NUP(erec, erec); NUP(wxy, wxy);
+    std::vector<double> q64((size_t)nop * 4, 0.0), p64((size_t)nop * 4, phi0), x64((size_t)nop * 2);
+    for (int p = 0; p < nop; ++p) {
+      x64[(size_t)p * 2] = m.px[p];
+      x64[(size_t)p * 2 + 1] = m.py[p];
+    }
+    NUP(vqp, q64); NUP(vphc, p64); NUP(vxy, x64);
   }
   NUP(qc, qc); NUP(cv, m.pcv); NUP(ebc, ebc); NUP(vref, rv);
   NUP(bedge, bedge); NUP(bkind, bkind); NAL(bclos, 10 * bedge.size());
