@@ -64,8 +64,9 @@ extern "C" {
 /* Vertex (cell) order of the LU-SGS sweeps.  REFERENCE: the reference's own
  * numbering (NCFVM/lusgs.f90:46-122, CCFVM/lusgs.F90:245-310), as a level
  * schedule; the iterate is the reference's.  MULTICOLOUR: the vertices
- * (cells) grouped by the colours of a greedy distance-1 colouring
- * (ufx_mesh_get "pt_colour" / "cell_colour"), each colour one parallel step,
+ * (cells) grouped by the colours of a greedy distance-1 colouring in
+ * smallest-last order (ufx_mesh_get "pt_colour" / "cell_colour"), each colour
+ * one parallel step,
  * a few launches per sweep instead of thousands of dependent levels.  The
  * same LU-SGS under another ordering: another iterate with the same fixed
  * points (a converged reference-order state stays put), but the limiters'

@@ -5,8 +5,9 @@ CCFVM/lusgs.F90:207-379) with the unknowns ordered by colour instead of by
 number.  Checked here without a GPU:
 * the colourings the library hands out are proper (no edge or interior face
   joins two vertices / cells of one colour), use few colours, and follow the
-  greedy rule (each vertex takes the smallest colour its lower-numbered
-  neighbours leave free);
+  greedy rule in smallest-last order (vertices taken out one by one at least
+  remaining degree, lowest number on ties; coloured in reverse, each taking
+  the smallest colour its already coloured neighbours leave free);
 * the oracle's ordered sweep with the identity order is the reference-order
   sweep bit for bit (that one is pinned by the reference's fixtures), and with
   a colour order it is invariant to the order inside each colour -- the
@@ -31,8 +32,25 @@ def nc_case(ni=24, nj=10, seed=3, stretch=1.2):
 
 
 def greedy_reference(n, nbrs):
+    """Smallest-last order with a heap of (remaining degree, vertex), then
+    greedy colouring in the reverse of that order."""
+    import heapq
+    deg = [len(nbrs[v]) for v in range(n)]
+    heap = [(deg[v], v) for v in range(n)]
+    heapq.heapify(heap)
+    out, order = [False] * n, []
+    while heap:
+        d, v = heapq.heappop(heap)
+        if out[v] or d != deg[v]:
+            continue
+        out[v] = True
+        order.append(v)
+        for u in nbrs[v]:
+            if not out[u]:
+                deg[u] -= 1
+                heapq.heappush(heap, (deg[u], u))
     col = [-1] * n
-    for v in range(n):
+    for v in reversed(order):
         taken = {col[o] for o in nbrs[v] if col[o] >= 0}
         c = 0
         while c in taken:
@@ -47,7 +65,7 @@ def test_vertex_colouring_is_proper_and_greedy(shape):
     col = mesh.get("pt_colour")
     p1, p2 = a["fc_p1"] - 1, a["fc_p2"] - 1
     assert np.all(col[p1] != col[p2])
-    assert col.min() == 0 and col.max() + 1 <= 8
+    assert col.min() == 0 and col.max() + 1 <= 5
     ptr = np.r_[0, np.cumsum(a["pt_nv2v"])]
     nbrs = [a["pt_v2v"][ptr[v]:ptr[v + 1]] - 1 for v in range(len(col))]
     np.testing.assert_array_equal(col, greedy_reference(len(col), nbrs))

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -20,10 +21,35 @@ namespace ufx {
 
 std::vector<int> greedy_colour(int n, const std::vector<int>& ptr, const std::vector<int>& adj,
                                int* ncolours) {
+  // smallest-last order: repeatedly take out a vertex of least remaining
+  // degree (ties: the lowest number); colour in the reverse of that order
+  std::vector<int> deg(n, 0), order;
+  std::vector<char> out(n, 0);
+  order.reserve(n);
+  using Item = std::pair<int, int>;
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> pq;
+  for (int v = 0; v < n; ++v) {
+    for (int j = ptr[v]; j < ptr[v + 1]; ++j)
+      if (adj[j] >= 0 && adj[j] < n) ++deg[v];
+    pq.push({deg[v], v});
+  }
+  while (!pq.empty()) {
+    const Item it = pq.top();
+    pq.pop();
+    const int v = it.second;
+    if (out[v] || it.first != deg[v]) continue;
+    out[v] = 1;
+    order.push_back(v);
+    for (int j = ptr[v]; j < ptr[v + 1]; ++j) {
+      const int u = adj[j];
+      if (u >= 0 && u < n && !out[u]) pq.push({--deg[u], u});
+    }
+  }
+  std::reverse(order.begin(), order.end());
   std::vector<int> col(n, -1);
   std::vector<int> seen;  // seen[c] == v: colour c is taken by a neighbour of v
   int nc = 0;
-  for (int v = 0; v < n; ++v) {
+  for (int v : order) {
     for (int j = ptr[v]; j < ptr[v + 1]; ++j) {
       int o = adj[j];
       if (o >= 0 && o < n && col[o] >= 0) seen[col[o]] = v;

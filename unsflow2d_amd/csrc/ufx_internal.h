@@ -83,10 +83,14 @@ int nc_geom_numeric(Mesh& m, std::string& err);
 int nc_geometric_device(const NCRawInput& in, Mesh& m, std::string& err);
 
 // Greedy distance-1 colouring of a graph given as adjacency lists (ptr has
-// n + 1 offsets): vertices in index order, each taking the smallest colour
-// none of its already coloured neighbours has.  No edge joins two vertices of
-// one colour, so the vertices of a colour can be updated at once in a
-// Gauss-Seidel sweep ordered by colour.  Returns colours 0..*ncolours-1.
+// n + 1 offsets) in smallest-last order (the vertices taken out one by one,
+// each time one of least remaining degree, the lowest-numbered on ties; then
+// coloured in the reverse of that order, each taking the smallest colour its
+// already coloured neighbours leave free).  No edge joins two vertices of one
+// colour, so the vertices of a colour can be updated at once in a
+// Gauss-Seidel sweep ordered by colour; the order keeps the colours few and
+// even (four on triangulated O-grids, where index order leaves six or seven,
+// the last ones nearly empty).  Returns colours 0..*ncolours-1.
 std::vector<int> greedy_colour(int n, const std::vector<int>& ptr, const std::vector<int>& adj,
                                int* ncolours);
 // The colouring the LU-SGS sweeps of the multicolour form use: vertex-centred
