@@ -320,7 +320,7 @@ def main_nc(args) -> int:
                          "launches_per_iteration": round(prof[k][1] / args.steps, 1)}
     dom = max(passes, key=lambda k: passes[k]["ms_per_iteration"])
     kname = {"lusgs": "k_nc_lusgs_mc" if args.lusgs_order == "multicolour" else "k_nc_lusgs_stream",
-             "residual": "k_nc_residual", "gradient": "k_nc_grad_lim",
+             "residual": "k_nc_residual_blk", "gradient": "k_nc_grad_lim",
              "save_old": "k_nc_save_old"}[dom]
     d_ms = passes[dom]["ms_per_iteration"]
     d_n = passes[dom]["launches_per_iteration"]
