@@ -1004,7 +1004,10 @@ __device__ __forceinline__ bool face_flux_ab(const Gas<R>& gr, const R* ra, cons
 // it in LDS.  Phase 2: each cell sums the boundary fluxes and then its
 // interior fluxes from LDS in its slot order (the reference's accumulation
 // order), then applies the stage update as k_stage does.  SL as in k_stage.
-// LDS: [BLK][RW] records when SL, then [bf_max][4] fluxes.
+// LDS: [BLK][RW] records when SL, then [bf_max][4] fluxes; when SL and the
+// list has at most 2 BLK faces, the fluxes are held in registers until every
+// thread is done with the records and then written over them, so the block
+// needs no more LDS than its records.
 template <typename R, int FLUX, bool SL>
 __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
                                                double ak, double bk, int write_res,
@@ -1022,6 +1025,31 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
   __syncthreads();
   const bool run = s_err == 0;  // uniform over the block
   const int j0 = d.bf_ptr[b], nj = run ? d.bf_ptr[b + 1] - j0 : 0;
+  if (SL && d.bf_max <= 2 * BLK) {
+    R F2[2][4];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int j = threadIdx.x + k * BLK;
+      if (j < nj) {
+        const int2 ab = d.bf_ab[j0 + j];
+        R ra[RW], rb[RW], fr[FW];
+        ldv<R, FW>(d.bf_rec + (size_t)(j0 + j) * FW, fr);
+        load_rec<R, SL>(d, srec, base, ab.x, ra);
+        load_rec<R, SL>(d, srec, base, ab.y, rb);
+        if (!face_flux_ab<R, FLUX>(gr, ra, rb, fr, F2[k]))
+          set_err(d, DERR_ROE, d.nbf + d.bf_fref[j0 + j]);
+      }
+    }
+    __syncthreads();
+    fl = srec;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int j = threadIdx.x + k * BLK;
+      if (j < nj)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fl[j * 4 + i] = F2[k][i];
+    }
+  } else
   for (int j = threadIdx.x; j < nj; j += BLK) {
     const int2 ab = d.bf_ab[j0 + j];
     const int ca = ab.x, cb = ab.y;

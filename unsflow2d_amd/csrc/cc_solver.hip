@@ -85,6 +85,7 @@ struct ufx_cc {
   bool stage_local = true;       // block records staged in LDS
   size_t stage_lds = 0;          // LDS bytes per block of the stage kernel
   size_t lds_list = 0, lds_recs = 0;  // LDS of the block flux list / records
+  bool lds_overlay = false;            // the flux list fits over the records (at most 2 BLK faces)
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   // a captured chunk of whole iterations (ufx_cc_iterate_graph)
@@ -123,6 +124,16 @@ struct ufx_cc {
   struct Rec { int cat; size_t a, b; };
   std::vector<Rec> recs;
 };
+
+namespace {
+// LDS per block of a residual-stage form: the block's flux list (shared),
+// its records (local), both -- or, when the list fits over the records, the
+// records alone.
+size_t stage_lds_need(const ufx_cc* x, bool shared, bool local) {
+  if (shared && local && x->lds_overlay) return std::max(x->lds_list, x->lds_recs);
+  return (shared ? x->lds_list : 0) + (local ? x->lds_recs : 0);
+}
+}  // namespace
 
 namespace {
 
@@ -1140,12 +1151,13 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     if (envl && std::strcmp(envl, "0") == 0) local = false;
     if (envl && std::strcmp(envl, "1") == 0) local = true;
     size_t lds_fl = (size_t)sl.bf_max * 4 * w, lds_rec = (size_t)BLK * rw * w;
-    if (shared && lds_fl + (local ? lds_rec : 0) > STAGE_LDS_MAX) shared = false;
-    x->stage_shared = shared;
-    x->stage_local = local;
     x->lds_list = lds_fl;
     x->lds_recs = lds_rec;
-    x->stage_lds = (shared ? lds_fl : 0) + (local ? lds_rec : 0);
+    x->lds_overlay = sl.bf_max <= 2 * BLK;
+    if (shared && stage_lds_need(x, shared, local) > STAGE_LDS_MAX) shared = false;
+    x->stage_shared = shared;
+    x->stage_local = local;
+    x->stage_lds = stage_lds_need(x, shared, local);
   }
   int rc = p.precision == UFX_PREC_FP64
                ? build_dev(x, x->d64, m, sl_face, sl_nbr, cnf, lu_face, lu_cnt, sl)
@@ -1499,7 +1511,7 @@ int ufx_cc_history(ufx_cc* x, int n, double* out) {
 
 int ufx_cc_set_stage_form(ufx_cc* x, int shared, int lds) {
   if (int rc = check_ctx(x)) return rc;
-  size_t need = (shared ? x->lds_list : 0) + (lds ? x->lds_recs : 0);
+  size_t need = stage_lds_need(x, shared != 0, lds != 0);
   if (need > STAGE_LDS_MAX) {
     set_error("ufx_cc_set_stage_form: the block-shared form needs " + std::to_string(need) +
               " bytes of LDS per block on this mesh, more than " +
