@@ -275,6 +275,33 @@ def test_stage_kernel_forms_agree(precision, monkeypatch):
             _check(f"qc vs oracle {key}", q, orc.qc, 1e-11)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_serpentine_traversal_equals_forward(precision, monkeypatch):
+    """The face-radius pass and RK stages 1 and 3 walk their blocks from the
+    far end of each XCD's slice (so each pass starts on the data the one
+    before touched last); every per-cell result is independent of the order
+    in which blocks run, so state, primitives, limiter values, gradients,
+    residual and time steps equal those of all-forward passes
+    (UFX_SERPENTINE=0) bit for bit, eagerly and from a HIP graph; the
+    residual norm, whose block partials are summed in block order, to 1e-13."""
+    print(f"case: serpentine vs forward traversal {precision}, mesh 80x40 seed 13", flush=True)
+    from unsflow2d_amd import cc, meshgen
+    mesh = cc.Mesh.from_raw(meshgen.bump_channel(80, 40, 13, h=0.08))
+    cfg = cc.Config(mach=0.7, precision=precision)
+    runs = {}
+    for serp in ("1", "0"):
+        monkeypatch.setenv("UFX_SERPENTINE", serp)
+        s = cc.Solver(mesh, cfg)
+        h = np.concatenate([s.iterate(6), s.iterate_graph(6)])
+        runs[serp] = (h, {k: s.get(k) for k in ("qc", "qp", "phi", "grad", "res", "dt")})
+        s.close()
+    monkeypatch.delenv("UFX_SERPENTINE")
+    (h1, a1), (h0, a0) = runs["1"], runs["0"]
+    for k in a0:
+        _check(k, a1[k], a0[k], 0)
+    _check("history", h1, h0, 1e-13)
+
+
 def test_mixed_converged_forces_match_fp64():
     """The mixed form's converged force coefficients (the reference's
     pressure-integrated CL/CD and the flux-based CL1/CD1 it prints,

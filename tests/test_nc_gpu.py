@@ -244,6 +244,33 @@ def test_nc_storage_orders_and_residual_forms_agree(monkeypatch, timemode, order
         _check(f"{key} history", h, h0, 1e-13)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nc_serpentine_traversal_equals_forward(monkeypatch, precision):
+    """The gradient pass and every other colour launch of the multicolour
+    sweeps walk their blocks from the far end of each XCD's slice (so each
+    starts on the data the pass before touched last); every per-vertex
+    result is independent of the order in which blocks run, and the norm's
+    partials are combined in vertex-block order, so everything equals the
+    all-forward passes (UFX_SERPENTINE=0) bit for bit."""
+    from unsflow2d_amd import nc, nc_mesh
+    print(f"case: nc serpentine vs forward, multicolour {precision}, o-grid 40x16 seed 6",
+          flush=True)
+    m = nc.NCMesh.from_raw(nc_mesh.o_grid(40, 16, 6, stretch=1.15))
+    runs = {}
+    for serp in ("1", "0"):
+        monkeypatch.setenv("UFX_SERPENTINE", serp)
+        s = nc.NCSolver(m, mach=0.5, aoa_deg=1.5, cfl_max=50.0, grad="gg", timemode="lusgs",
+                        lusgs_order="multicolour", precision=precision)
+        h = s.iterate(10)
+        runs[serp] = (h, {k: s.get(k) for k in ("qc", "phi", "res", "dt", "fc_la")})
+        s.close()
+    monkeypatch.delenv("UFX_SERPENTINE")
+    (h1, a1), (h0, a0) = runs["1"], runs["0"]
+    for k in a0:
+        _check(k, a1[k], a0[k], 0)
+    _check("history", h1, h0, 0)
+
+
 @pytest.mark.parametrize("flux", ["vanleer", "ausm"])
 def test_nc_wall_forces_match_oracle(flux):
     """The wall force (the wall closure of solid_flux summed in edge order)

@@ -208,6 +208,16 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// The same with each XCD walking its slice from the far end when `rev`: a
+// kernel that follows one which walked forward then starts on the data the
+// other touched last, which the memory-side cache still holds.
+__device__ __forceinline__ int xcd_block_dir(int b, int nblk, bool rev) {
+  int q = nblk / NXCD, r = nblk % NXCD;
+  int x = b % NXCD, i = b / NXCD;
+  if (rev) i = (x < r ? q + 1 : q) - 1 - i;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 __device__ __forceinline__ int cell_index() {
   return xcd_block(blockIdx.x, gridDim.x) * BLK + threadIdx.x;
 }

@@ -919,7 +919,8 @@ __device__ __forceinline__ void prim_epilogue(const DevT<R>& d, VBlock vb, const
 //              1: apply the stage update of CCFVM/main.F90:92-101 (no other
 //                 cell reads this cell's state during a stage);
 //              2: 1 and a block partial of the squared residual;
-//   mode >> 2 (src): where the update's states live, stage_old/in/out.
+//   (mode >> 2) & 3 (src): where the update's states live, stage_old/in/out;
+//   mode & 16: the XCDs walk their slices of blocks from the far end.
 // SL: the block's records are first copied to LDS, so the neighbours inside
 // the block (most of them, the cells being Hilbert-ordered) are read from
 // there instead of through L2.  epi: the last stage of a whole iteration
@@ -928,11 +929,12 @@ template <typename R, int FLUX, bool SL>
 __device__ __forceinline__ void stage_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
                                            double ak, double bk, int write_res,
                                            const Gas<double>& g, int epi) {
-  const int op = mode & 3, src = mode >> 2;
+  const int op = mode & 3, src = (mode >> 2) & 3;
+  const bool rev = (mode & 16) != 0;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);  // [BLK][RW] when SL
   __shared__ double sh[1][BLK];
-  const int base = xcd_block(vb.b, vb.n) * BLK;
+  const int base = xcd_block_dir(vb.b, vb.n, rev) * BLK;
   const int c = base + threadIdx.x;
   if (SL) {
     stage_block_recs(d, srec, base, min(BLK, d.noc - base));
@@ -1012,7 +1014,8 @@ template <typename R, int FLUX, bool SL>
 __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<R> gr, int mode,
                                                double ak, double bk, int write_res,
                                                const Gas<double>& g, int epi) {
-  const int op = mode & 3, src = mode >> 2;
+  const int op = mode & 3, src = (mode >> 2) & 3;
+  const bool rev = (mode & 16) != 0;
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   extern __shared__ double lds_raw[];
   R* srec = reinterpret_cast<R*>(lds_raw);
@@ -1020,7 +1023,7 @@ __device__ __forceinline__ void stage_blk_body(const DevT<R>& d, VBlock vb, Gas<
   __shared__ double sh[1][BLK];
   __shared__ int s_err;
   if (threadIdx.x == 0) s_err = d.err[0];
-  const int b = xcd_block(vb.b, vb.n), base = b * BLK;
+  const int b = xcd_block_dir(vb.b, vb.n, rev), base = b * BLK;
   if (SL) stage_block_recs(d, srec, base, min(BLK, d.noc - base));
   __syncthreads();
   const bool run = s_err == 0;  // uniform over the block

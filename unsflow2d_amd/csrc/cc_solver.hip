@@ -86,6 +86,11 @@ struct ufx_cc {
   size_t stage_lds = 0;          // LDS bytes per block of the stage kernel
   size_t lds_list = 0, lds_recs = 0;  // LDS of the block flux list / records
   bool lds_overlay = false;            // the flux list fits over the records (at most 2 BLK faces)
+  // alternate traversal: in RK stages 1 and 3 the XCDs walk their slices of
+  // blocks from the far end, so each stage starts on the data the pass
+  // before it touched last, which the 256 MB memory-side cache still holds
+  // (UFX_SERPENTINE=0: all passes forward)
+  int stage_rev = 10;                  // bit s: stage s walks its blocks from the far end
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   // a captured chunk of whole iterations (ufx_cc_iterate_graph)
@@ -406,7 +411,7 @@ int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, dou
 template <typename R>
 int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int write_res,
                      int epi = 0) {
-  const int src = mode >> 2;
+  const int src = (mode >> 2) & 3;
   int visc = x->p.flow != UFX_FLOW_INVISCID;
   const Gas<R>& gr = gas_of<R>(x);
   int nbb = nblocks(x->nbf);
@@ -494,8 +499,9 @@ int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool 
               bool epi = false) {
   double ak = RK_A[stage - 1], bk = RK_B[stage - 1];
   if (x->p.flow == UFX_FLOW_INVISCID && !x->p.irs)
-    return run_stage_kernel(x, d, (norm ? 2 : 1) | ((whole ? stage : 0) << 2), ak, bk,
-                            write_res ? 1 : 0, epi ? 1 : 0);
+    return run_stage_kernel(x, d, (norm ? 2 : 1) | ((whole ? stage : 0) << 2) |
+                                      ((x->stage_rev >> stage) & 1 ? 16 : 0),
+                            ak, bk, write_res ? 1 : 0, epi ? 1 : 0);
   if (int rc = run_residual_full(x, d)) return rc;
   if (int rc = run_rk_update(x, d, ak, bk)) return rc;
   if (norm) {
@@ -1144,6 +1150,8 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     if (env && std::strcmp(env, "shared") == 0) shared = true;
     // UFX_ITER_FUSION=0: whole iterations refresh the primitives in a pass
     // of their own (same results; for comparison)
+    if (const char* r = std::getenv("UFX_SERPENTINE"); r && std::strcmp(r, "0") == 0)
+      x->stage_rev = 0;
     const char* enve = std::getenv("UFX_ITER_FUSION");
     x->iter_fusion = !(enve && std::strcmp(enve, "0") == 0);
     const char* envl = std::getenv("UFX_STAGE_LDS");
@@ -1801,7 +1809,8 @@ int batch_iteration(ufx_cc_batch* b, ufx_cc* x0, int k, double cfl, int keep, bo
 #undef KMC
   for (int st = 1; st <= 3; ++st) {
     const double ak = RK_A[st - 1], bk = RK_B[st - 1];
-    const int mode = (st == 3 ? 2 : 1) | (st << 2), wr = st == 3 && keep ? 1 : 0;
+    const int mode = (st == 3 ? 2 : 1) | (st << 2) | ((x0->stage_rev >> st) & 1 ? 16 : 0),
+              wr = st == 3 && keep ? 1 : 0;
     const int epi = st == 3 ? 1 : 0;
     const size_t lds = b->stage_lds;
 #define KMS(F)                                                                                  \

@@ -297,7 +297,7 @@ __device__ __forceinline__ void for_edges(const DevNC& d, int j0, int n, const i
 }
 
 template <typename R, int GRAD, int LIM>
-__global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
+__global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g, int rev) {
   constexpr int W = NCRec<R>::W;
   // the primitives and coordinates of the block's vertices, staged in LDS:
   // the vertices are stored along a Hilbert curve, so most neighbours lie in
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g) {
     if (k < d.nbnd) nc_closure<R>(d, g, k);
     return;
   }
-  const int base = xcd_block(blockIdx.x, nvb) * BLK;
+  const int base = xcd_block_dir(blockIdx.x, nvb, rev != 0) * BLK;
   const int p = base + (int)threadIdx.x;
   const int cnt = min(BLK, d.nop - base);
   const R* qps = NCRec<R>::qp(d);
@@ -970,8 +970,9 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
 // neighbour's state is rebuilt as qold + dq, the sum that defines it, so the
 // arithmetic is the reference loop's, term for term, in that vertex order.
 constexpr int MCU = 8;
-__global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, int n, int dir) {
-  const int t = blockIdx.x * BLK + threadIdx.x;
+__global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, int n, int dir,
+                                                     int rev) {
+  const int t = xcd_block_dir(blockIdx.x, gridDim.x, rev != 0) * BLK + threadIdx.x;
   if (t >= n) return;
   const int np = d.nop, s = s0 + t, it = d.mc_vert[s];
   const int cnt = d.mc_cnt[(size_t)dir * np + s];

@@ -50,6 +50,11 @@ struct ufx_nc {
   int sweep_mode = 4;                       // 0 graph of level launches, 1 level launches, 2 one workgroup, 3 cooperative grid, 4 level-ordered one workgroup
   int grid_wgs = 8;                         // workgroups of the cooperative sweep
   size_t res_lds = 0;                       // LDS of the block-shared residual (0: per-vertex gather form)
+  // alternate traversal: the gradient pass and every other colour launch of
+  // the sweeps walk their blocks from the far end of each XCD's slice, so a
+  // pass starts on the data the one before touched last, which the
+  // memory-side cache still holds (UFX_SERPENTINE=0: all forward)
+  bool serpentine = true;
   std::vector<int> mc_off;                  // multicolour sweeps: positions of each colour
   std::vector<int> mc_pos;                  // and the position of each stored vertex
   hipGraph_t sweep_graph = nullptr;         // diagonal + both sweeps, captured once
@@ -137,11 +142,11 @@ int run_save_old_r(ufx_nc* x) {
 template <typename R>
 int run_grad_lim_r(ufx_nc* x) {
   bool lsq = x->prm.grad == UFX_GRAD_LSQR, bj = x->prm.limiter == UFX_NC_LIM_BJ;
-  const int nb = x->nblk + (x->d.nbnd + BLK - 1) / BLK;
-  if (lsq && bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 1>), nb, x->d, x->g);
-  else if (lsq) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 0>), nb, x->d, x->g);
-  else if (bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 1>), nb, x->d, x->g);
-  else NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 0>), nb, x->d, x->g);
+  const int nb = x->nblk + (x->d.nbnd + BLK - 1) / BLK, rev = x->serpentine ? 1 : 0;
+  if (lsq && bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 1>), nb, x->d, x->g, rev);
+  else if (lsq) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 2, 0>), nb, x->d, x->g, rev);
+  else if (bj) NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 1>), nb, x->d, x->g, rev);
+  else NLAUNCH(x, NK_GRAD, (k_nc_grad_lim<R, 1, 0>), nb, x->d, x->g, rev);
   return launch_ok();
 }
 
@@ -216,7 +221,10 @@ int run_lusgs(ufx_nc* x, double cfl) {
     for (int dir = 0; dir < 2; ++dir)
       for (int k = 0; k < nc; ++k) {
         const int c = dir == 0 ? k : nc - 1 - k, s0 = x->mc_off[c], n = x->mc_off[c + 1] - s0;
-        if (n > 0) NLAUNCH(x, NK_LUSGS, k_nc_lusgs_mc, nblk_of(n), x->d, x->g, s0, n, dir);
+        // consecutive colour launches walk their positions in opposite
+        // directions (when serpentine), each starting where the last ended
+        const int rev = x->serpentine ? (dir * nc + k) & 1 : 0;
+        if (n > 0) NLAUNCH(x, NK_LUSGS, k_nc_lusgs_mc, nblk_of(n), x->d, x->g, s0, n, dir, rev);
       }
     return launch_ok();
   }
@@ -511,6 +519,8 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
                       (float)G.u_inf, (float)G.v_inf, (float)G.p_inf, (float)G.mref,
                       (float)G.igamma1};
   x->mixed = prm->precision == UFX_PREC_MIXED;
+  if (const char* r = std::getenv("UFX_SERPENTINE"); r && std::strcmp(r, "0") == 0)
+    x->serpentine = false;
   // vertices are stored in Hilbert-curve order of their coordinates, so a
   // block of consecutive vertices is a compact patch of the mesh and most of
   // the neighbours it gathers lie in the same block (in the breadth-first
