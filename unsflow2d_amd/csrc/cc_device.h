@@ -79,12 +79,13 @@ struct DevT {
   const R* bf_rec;    // per block-list entry: the face record
   const int* bf_fref; // per block-list entry: the reference face number (error reports)
   int bf_max;
-  const double *r11, *r12, *r22, *cv, *cov;  // noc (cov: ncell)
+  const R *r11, *r12, *r22;  // noc: least-squares factors, in R (the gradient runs in R)
+  const double *cv, *cov;    // noc (cov: ncell)
   const int* nc2c;
   const int* cref;           // storage cell -> reference cell number (ncell)
   const int *v2c_ptr, *v2c;  // point storage order
   const int* avg_w0;         // per block of BLK vertices: its lowest interior cell
-  const double* wt;
+  const R* wt;                // pseudo-Laplacian weights per v2c entry, in R
   const double *px, *py;     // point storage order
   const int *fin, *fout;     // face sides (storage numbering of faces)
   const int* fref;           // storage face -> reference face number (error reports)

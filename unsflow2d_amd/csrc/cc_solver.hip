@@ -810,13 +810,19 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     for (int s = 0; s < n; ++s) o[s] = a[x->ref_of_sto[s]];
     return o;
   };
-  std::vector<double> r11 = perm(m.r11, noc), r12 = perm(m.r12, noc), r22 = perm(m.r22, noc);
+  // the least-squares factors and vertex weights in R: the gradient runs in
+  // R, and the mixed form's vertex averages take FP32 weights (half the
+  // bytes of the pass's largest stream)
+  auto to_r = [](const std::vector<double>& a) { return std::vector<R>(a.begin(), a.end()); };
+  std::vector<R> r11 = to_r(perm(m.r11, noc)), r12 = to_r(perm(m.r12, noc)),
+                 r22 = to_r(perm(m.r22, noc));
   std::vector<double> cv = perm(m.ccv, noc), cov = perm(m.ccov, ncell);
   std::vector<int> nc2c(noc);
   for (int s = 0; s < noc; ++s) nc2c[s] = m.nc2c[x->ref_of_sto[s]];
   // vertices in storage order
   std::vector<int> v2c_ptr(nop + 1, 0), v2c;
-  std::vector<double> wt, px(nop), py(nop);
+  std::vector<R> wt;
+  std::vector<double> px(nop), py(nop);
   std::vector<int> ref_of_pt(nop);
   for (int i = 0; i < nop; ++i) ref_of_pt[x->pt_of_ref[i]] = i;
   v2c.reserve(m.v2c.size());
@@ -825,7 +831,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     int i = ref_of_pt[s];
     for (int j = m.v2c_ptr[i]; j < m.v2c_ptr[i + 1]; ++j) {
       v2c.push_back(sto[m.v2c[j]]);
-      wt.push_back(m.wt[j]);
+      wt.push_back((R)m.wt[j]);
     }
     v2c_ptr[s + 1] = (int)v2c.size();
     px[s] = m.px[i];
