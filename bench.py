@@ -279,7 +279,8 @@ def main_nc(args) -> int:
     mesh = nc.NCMesh.from_raw(raw)
     cnt = mesh.counts
     solver = nc.NCSolver(mesh, mach=0.4, cfl_max=50.0, grad="gg", timemode="lusgs",
-                         lusgs_order=args.lusgs_order, precision=args.precision)
+                         lusgs_order=args.lusgs_order, precision=args.precision,
+                         flux=args.nc_flux, limiter=args.nc_limiter)
     ncol = int(mesh.get("pt_colour").max()) + 1
     print(f"# setup {time.time() - t0:.1f}s: {cnt['nop']} vertices, {cnt['nof']} edges",
           file=sys.stderr, flush=True)
@@ -326,6 +327,8 @@ def main_nc(args) -> int:
     d_n = passes[dom]["launches_per_iteration"]
     achieved = passes[dom]["achieved_gbs"]
     cpu = None if args.no_cpu else nc_cpu_baseline(args.cpu_budget)
+    flux_name = {"vanleer": "van Leer", "ausm": "AUSM+-up"}[args.nc_flux]
+    lim_name = {"vkn": "Venkatakrishnan", "bj": "Barth-Jespersen"}[args.nc_limiter]
     sweep = ("multicolour: one launch per colour per sweep, %d colours" % ncol
              if args.lusgs_order == "multicolour" else
              "level-scheduled, reference order, one workgroup per sweep")
@@ -338,10 +341,10 @@ def main_nc(args) -> int:
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"NCFVM median dual, seeded O-grid 2000x1000 points around the "
                                f"unit circle ({nop} vertices, {nof} edges), M=0.4, GG + "
-                               f"Venkatakrishnan, van Leer, LU-SGS ({sweep}), tanh CFL ramp to 50",
+                               f"{lim_name}, {flux_name}, LU-SGS ({sweep}), tanh CFL ramp to 50",
                    "vertices": nop, "edges": nof, "lusgs_order": args.lusgs_order,
-                   "precision": args.precision,
-                   "colours": ncol},
+                   "precision": args.precision, "flux": args.nc_flux,
+                   "limiter": args.nc_limiter, "colours": ncol},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("nc-" + args.precision, "ufx::" + kname),
@@ -604,6 +607,10 @@ def main() -> int:
                     help="cc: skip the mixed-precision leg and its errors against FP64")
     ap.add_argument("--error-iters", type=int, default=2000,
                     help="cc: iterations from the free stream before comparing mixed with FP64")
+    ap.add_argument("--nc-flux", choices=["vanleer", "ausm"], default="vanleer",
+                    help="nc: dual-face flux (AUSM+-up is build-defined for the vertex-centred path)")
+    ap.add_argument("--nc-limiter", choices=["vkn", "bj"], default="vkn",
+                    help="nc: Venkatakrishnan or Barth-Jespersen (build-defined)")
     ap.add_argument("--lusgs-order", choices=["reference", "multicolour"], default="multicolour",
                     help="nc: LU-SGS sweep order (reference: level schedule of the reference's "
                          "numbering; multicolour: one launch per colour)")
