@@ -105,7 +105,7 @@ struct ufx_cc {
   int nblk_cell = 0, nblk_all = 0, nblk_pt = 0, nblk_face = 0;
   double* d_hist = nullptr;
   int hist_cap = 0;
-  double* cirs_tmp = nullptr;    // [2][noc][4]
+  double* cirs_tmp = nullptr;    // [noc][4] the first smoothing sweep's result
   double* bfluxv = nullptr;      // viscous wall force terms [2][nbf]
   std::vector<int> sto_of_ref;   // reference cell -> storage cell (ncell)
   std::vector<int> ref_of_sto;   // storage cell -> reference cell
@@ -444,16 +444,16 @@ int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int 
 
 // Two Jacobi sweeps of the residual smoothing with the smoothing factor of
 // the current CFL, CCFVM/ImpResiSmoothening.F90:13-44 (OldRes = residual on
-// entry).
+// entry).  A sweep reads OldRes only at its own cell, before writing that
+// cell, so the second sweep reads it from the residual it overwrites and
+// no copy of it is kept.
 template <typename R>
 int run_cirs(ufx_cc* x, DevT<R>& d) {
   double rr = x->cfl / x->p.cfl_max;
   double eps = std::max(0.25 * (rr * rr - 1.0), 0.0);
-  double* old = x->cirs_tmp;
-  double* tmp = x->cirs_tmp + (size_t)4 * x->noc;
-  HIPCHK(hipMemcpyAsync(old, d.res, (size_t)32 * x->noc, hipMemcpyDeviceToDevice, x->stream));
-  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, d.res, old, tmp);
-  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, tmp, old, d.res);
+  double* tmp = x->cirs_tmp;
+  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, d.res, d.res, tmp);
+  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, tmp, d.res, d.res);
   return launch_check();
 }
 
@@ -1194,7 +1194,7 @@ int ufx_cc_create(const ufx_mesh* mesh, const ufx_cc_params* prm, ufx_cc** out) 
     x->d64.ccol = dc;
     x->d32.ccol = dc;
   }
-  if (!rc) rc = dalloc(x, &x->cirs_tmp, 8 * (size_t)noc);
+  if (!rc) rc = dalloc(x, &x->cirs_tmp, 4 * (size_t)noc);
   if (!rc) rc = dalloc(x, &x->bfluxv, 2 * (size_t)std::max(1, m.nbf));
   if (!rc) {
     e = hipStreamSynchronize(x->stream);
