@@ -1155,9 +1155,19 @@ __global__ __launch_bounds__(BLK) void k_visc_res(DevT<R> d, Gas<double> g, doub
   ld4d(d.res + (size_t)c * 4, r);
   const double two = 2.0, two_third = two / 3.0;
   double mc = (double)d.cmul[c];
-  for (int s = 0; s < nf; ++s) {
-    int f = d.sl_face[s * noc + c];
-    int n = d.sl_nbr[s * noc + c];
+  // the slot lists first (independent loads), then the faces' gathers
+  int sf[MAXF], sn[MAXF];
+#pragma unroll
+  for (int s = 0; s < MAXF; ++s)
+    if (s < nf) {
+      sf[s] = d.sl_face[s * noc + c];
+      sn[s] = d.sl_nbr[s * noc + c];
+    }
+#pragma unroll
+  for (int s = 0; s < MAXF; ++s) {
+    if (s >= nf) break;
+    int f = sf[s];
+    int n = sn[s];
     bool in_side = n >= 0;
     n = in_side ? n : ~n;
     R fr[Layout<R>::FW];
