@@ -34,6 +34,14 @@ __device__ __forceinline__ VBlock case_block(const int4* map, int& c) {
   c = m.x;
   return VBlock{m.y, m.z};
 }
+// The same walking the table from its far end when `rev` (the residual
+// stages of the serpentine traversal: each starts on the cases the stage
+// before touched last).
+__device__ __forceinline__ VBlock case_block_dir(const int4* map, int& c, bool rev) {
+  const int4 m = map[rev ? (int)gridDim.x - 1 - (int)blockIdx.x : (int)blockIdx.x];
+  c = m.x;
+  return VBlock{m.y, m.z};
+}
 
 template <typename R>
 __global__ __launch_bounds__(BLK) void km_prim(const CaseDesc<R>* cs, const int4* map,
@@ -94,7 +102,7 @@ template <typename R, int FLUX, bool SL>
 __global__ __launch_bounds__(BLK) void km_stage(const CaseDesc<R>* cs, const int4* map, int mode,
                                                 double ak, double bk, int write_res, int epi) {
   int c;
-  const VBlock vb = case_block(map, c);
+  const VBlock vb = case_block_dir(map, c, (mode & 16) != 0);
   const CaseDesc<R> e = cs[c];
   stage_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, epi);
 }
@@ -104,7 +112,7 @@ __global__ __launch_bounds__(BLK) void km_stage_blk(const CaseDesc<R>* cs, const
                                                     int mode, double ak, double bk,
                                                     int write_res, int epi) {
   int c;
-  const VBlock vb = case_block(map, c);
+  const VBlock vb = case_block_dir(map, c, (mode & 16) != 0);
   const CaseDesc<R> e = cs[c];
   stage_blk_body<R, FLUX, SL>(e.d, vb, e.gr, mode, ak, bk, write_res, e.g, epi);
 }
