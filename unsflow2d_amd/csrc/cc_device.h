@@ -98,6 +98,7 @@ struct DevT {
   R* qpc;                    // [noc][4] the interior cells' record primitives, compact
   R* qT;
   R* qTc;                    // [noc] the interior cells' temperatures beside qpc
+  double* fvis;              // [nof][4] viscous face fluxes times the face length
   R* phc;                    // [noc][4] the interior cells' limiter values, compact (the
                              // ratchet's previous values; the records hold a copy)
   const R* cenc;             // [ncell][2] centroids, compact

@@ -1141,21 +1141,58 @@ __global__ __launch_bounds__(BLK) void k_visc_mul(DevT<R> d, Gas<double> g) {
   d.cmul[c] = (R)((num / den) * pow(p[4], 1.5));
 }
 
-// Viscous face fluxes added to the residual, CCFVM/viscous_flux.F90:24-76:
-// all faces of the cell in ascending order, after the inviscid ones; the
-// `in` side also stores the face viscosity, wall faces their force terms.
+// The viscous flux of every face once (the two cells of an interior face
+// see the same face viscosity, velocities and gradients,
+// CCFVM/viscous_flux.F90:24-76): the flux times the face length into fvis
+// (the residual adds it at `out`, subtracts it at `in`), the face viscosity,
+// and for boundary faces the wall force terms.
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_visc_res(DevT<R> d, Gas<double> g, double rey,
-                                                  double m_inf, double* bfluxv) {
-  int c = cell_index();
-  if (c >= d.noc || d.err[0]) return;
-  const int noc = d.noc;
-  int nf = d.cnf[c] & 0xff;
-  double r[4];
-  ld4d(d.res + (size_t)c * 4, r);
+__global__ __launch_bounds__(BLK) void k_visc_face(DevT<R> d, Gas<double> g, double rey,
+                                                   double m_inf, double* bfluxv) {
+  const int f = cell_index();
+  if (f >= d.nof || d.err[0]) return;
+  const int noc = d.noc, ci = d.fin[f], co = d.fout[f];
+  const double mu = co < noc ? 0.5 * ((double)d.cmul[ci] + (double)d.cmul[co]) : (double)d.cmul[ci];
+  R fr[Layout<R>::FW];
+  face_rec(d, f, fr);
+  const double nx = fr[F_NX], ny = fr[F_NY], area = fr[F_AREA];
+  R fq[5];
+  face_prims<R, 3>(d, f, fq);
+  R fg[10];
+  ldv<R, 10>(d.fgrad + (size_t)f * 10, fg);
   const double two = 2.0, two_third = two / 3.0;
-  double mc = (double)d.cmul[c];
-  // the slot lists first (independent loads), then the faces' gathers
+  double uvel = fq[1], vvel = fq[2];
+  double ux = fg[2], uy = fg[3], vx = fg[4], vy = fg[5], tx = fg[8], ty = fg[9];
+  double tauxx = two_third * (two * ux - vy);
+  double tauxy = uy + vx;
+  double tauyy = two_third * (two * vy - ux);
+  double factor = (g.gamma - 1.0) * m_inf * m_inf * 0.72;
+  double qx = -(tx / factor), qy = -(ty / factor);
+  double mr = mu / rey;
+  double F[4];
+  F[0] = mr * ((uvel * tauxx + vvel * tauxy - qx) * nx + (uvel * tauxy + vvel * tauyy - qy) * ny);
+  F[1] = 0.0;
+  F[2] = mr * (tauxx * nx + tauxy * ny);
+  F[3] = mr * (tauxy * nx + tauyy * ny);
+  double o[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) o[i] = F[i] * area;
+  st4d(d.fvis + (size_t)f * 4, o);
+  d.fmu[f] = mu;
+  if (co >= noc) {
+    const int k = co - noc;
+    bfluxv[k] = -o[2];
+    bfluxv[d.nbf + k] = -o[3];
+  }
+}
+
+// The viscous fluxes summed into each cell's residual over its faces in
+// ascending order, after the inviscid ones.
+template <typename R>
+__global__ __launch_bounds__(BLK) void k_visc_sum(DevT<R> d) {
+  const int c = cell_index();
+  if (c >= d.noc || d.err[0]) return;
+  const int noc = d.noc, nf = d.cnf[c] & 0xff;
   int sf[MAXF], sn[MAXF];
 #pragma unroll
   for (int s = 0; s < MAXF; ++s)
@@ -1163,44 +1200,19 @@ __global__ __launch_bounds__(BLK) void k_visc_res(DevT<R> d, Gas<double> g, doub
       sf[s] = d.sl_face[s * noc + c];
       sn[s] = d.sl_nbr[s * noc + c];
     }
+  double r[4];
+  ld4d(d.res + (size_t)c * 4, r);
 #pragma unroll
   for (int s = 0; s < MAXF; ++s) {
     if (s >= nf) break;
-    int f = sf[s];
-    int n = sn[s];
-    bool in_side = n >= 0;
-    n = in_side ? n : ~n;
-    R fr[Layout<R>::FW];
-    face_rec(d, f, fr);
-    double nx = fr[F_NX], ny = fr[F_NY], area = fr[F_AREA];
-    double mu = n < noc ? 0.5 * (in_side ? mc + (double)d.cmul[n] : (double)d.cmul[n] + mc) : mc;
-    R fq[5];
-    face_prims<R, 3>(d, f, fq);
-    R fg[10];
-    ldv<R, 10>(d.fgrad + (size_t)f * 10, fg);
-    double uvel = fq[1], vvel = fq[2];
-    double ux = fg[2], uy = fg[3], vx = fg[4], vy = fg[5], tx = fg[8], ty = fg[9];
-    double tauxx = two_third * (two * ux - vy);
-    double tauxy = uy + vx;
-    double tauyy = two_third * (two * vy - ux);
-    double factor = (g.gamma - 1.0) * m_inf * m_inf * 0.72;
-    double qx = -(tx / factor), qy = -(ty / factor);
-    double mr = mu / rey;
-    double F[4];
-    F[0] = mr * ((uvel * tauxx + vvel * tauxy - qx) * nx + (uvel * tauxy + vvel * tauyy - qy) * ny);
-    F[1] = 0.0;
-    F[2] = mr * (tauxx * nx + tauxy * ny);
-    F[3] = mr * (tauxy * nx + tauyy * ny);
-    if (in_side) {
-      for (int i = 0; i < 4; ++i) r[i] = r[i] - F[i] * area;
-      d.fmu[f] = mu;
-      if (n >= noc) {
-        int k = n - noc;
-        bfluxv[k] = -(F[2] * area);
-        bfluxv[d.nbf + k] = -(F[3] * area);
-      }
+    double o[4];
+    ld4d(d.fvis + (size_t)sf[s] * 4, o);
+    if (sn[s] >= 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = r[i] - o[i];
     } else {
-      for (int i = 0; i < 4; ++i) r[i] = r[i] + F[i] * area;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) r[i] = r[i] + o[i];
     }
   }
   st4d(d.res + (size_t)c * 4, r);

@@ -472,7 +472,9 @@ int run_residual_full(ufx_cc* x, DevT<R>& d) {
       if (int rc = run_cell_g<R, G_GGFC>(x, d, true, x->p.ilimit != 0, false, 0.0)) return rc;
     }
     LAUNCH(x, UFX_K_STAGE, k_visc_mul<R>, x->nblk_cell, d, x->g);
-    LAUNCH(x, UFX_K_STAGE, k_visc_res<R>, x->nblk_cell, d, x->g, x->p.rey, x->p.m_inf, x->bfluxv);
+    LAUNCH(x, UFX_K_STAGE, k_visc_face<R>, x->nblk_face, d, x->g, x->p.rey, x->p.m_inf,
+           x->bfluxv);
+    LAUNCH(x, UFX_K_STAGE, k_visc_sum<R>, x->nblk_cell, d);
     if (int rc = launch_check()) return rc;
   }
   if (x->p.irs)
@@ -905,6 +907,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   }
   AL(fw, nof); AL(qT, ncell); AL(qTc, noc); AL(gT, 2 * (size_t)noc); AL(qold, 4 * (size_t)noc); AL(qs, 4 * (size_t)noc);
   AL(res, 4 * (size_t)noc); AL(dt, noc); AL(dta, noc); AL(la, noc); AL(fla, nof); AL(fmu, nof);
+  if (x->p.flow != UFX_FLOW_INVISCID) AL(fvis, 4 * (size_t)nof);
   AL(cmul, ncell); AL(prec, (size_t)nop * PW); AL(fgrad, (size_t)nof * 10);
   AL(bflux, 2 * (size_t)std::max(1, nbf)); AL(bres, 4 * (size_t)std::max(1, nbf));
   AL(part, (size_t)std::max(x->nblk_all, 1) * 8); AL(rpart, (size_t)std::max(x->nblk_all, 1) * 8);
