@@ -1220,9 +1220,12 @@ __global__ __launch_bounds__(BLK) void k_visc_sum(DevT<R> d) {
 
 // One Jacobi sweep of the central implicit residual smoothing,
 // CCFVM/ImpResiSmoothening.F90:27-44.
+// upd: also the stage update of CCFVM/main.F90:92-101 with the smoothed
+// residual (the last sweep), as k_rk_update does it.
 template <typename R>
 __global__ __launch_bounds__(BLK) void k_cirs(DevT<R> d, double eps, const double* src,
-                                              const double* old, double* dst) {
+                                              const double* old, double* dst, int upd,
+                                              double ak, double bk) {
   int c = cell_index();
   if (c >= d.noc || d.err[0]) return;
   const int noc = d.noc;
@@ -1240,6 +1243,13 @@ __global__ __launch_bounds__(BLK) void k_cirs(DevT<R> d, double eps, const doubl
   ld4d(old + (size_t)c * 4, o);
   for (int i = 0; i < 4; ++i) out[i] = (it[i] * eps + o[i]) * den;
   st4d(dst + (size_t)c * 4, out);
+  if (upd) {
+    double q[4], qo[4], dta = d.dta[c];
+    ld4d(d.qc + (size_t)c * 4, q);
+    ld4d(d.qold + (size_t)c * 4, qo);
+    for (int i = 0; i < 4; ++i) q[i] = rk_update1(ak, qo[i], bk, q[i], dta, out[i]);
+    st4d(d.qc + (size_t)c * 4, q);
+  }
 }
 
 // Stage update alone, CCFVM/main.F90:92-101.

@@ -447,13 +447,14 @@ int run_stage_kernel(ufx_cc* x, DevT<R>& d, int mode, double ak, double bk, int 
 // entry).  A sweep reads OldRes only at its own cell, before writing that
 // cell, so the second sweep reads it from the residual it overwrites and
 // no copy of it is kept.
+// upd: the second sweep also applies the stage update (ak, bk).
 template <typename R>
-int run_cirs(ufx_cc* x, DevT<R>& d) {
+int run_cirs(ufx_cc* x, DevT<R>& d, bool upd = false, double ak = 0.0, double bk = 0.0) {
   double rr = x->cfl / x->p.cfl_max;
   double eps = std::max(0.25 * (rr * rr - 1.0), 0.0);
   double* tmp = x->cirs_tmp;
-  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, d.res, d.res, tmp);
-  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, tmp, d.res, d.res);
+  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, d.res, d.res, tmp, 0, 0.0, 0.0);
+  LAUNCH(x, UFX_K_STAGE, k_cirs<R>, x->nblk_cell, d, eps, tmp, d.res, d.res, upd ? 1 : 0, ak, bk);
   return launch_check();
 }
 
@@ -461,8 +462,11 @@ int run_cirs(ufx_cc* x, DevT<R>& d) {
 // flux (which first recomputes the diamond-path gradient and the limiter
 // unless that is the gradient method, CCFVM/viscous_flux.F90:17) and the
 // residual smoothing.
+// upd: a stage of a whole iteration with residual smoothing, whose last
+// smoothing sweep also applies the stage update (ak, bk); returns with *upd_done set.
 template <typename R>
-int run_residual_full(ufx_cc* x, DevT<R>& d) {
+int run_residual_full(ufx_cc* x, DevT<R>& d, bool upd = false, double ak = 0.0, double bk = 0.0,
+                      bool* upd_done = nullptr) {
   if (int rc = run_stage_kernel(x, d, 0, 0.0, 0.0, 1)) return rc;
   if (x->p.flow != UFX_FLOW_INVISCID) {
     if (x->p.grad != UFX_GRAD_GGFC) {
@@ -477,8 +481,10 @@ int run_residual_full(ufx_cc* x, DevT<R>& d) {
     LAUNCH(x, UFX_K_STAGE, k_visc_sum<R>, x->nblk_cell, d);
     if (int rc = launch_check()) return rc;
   }
-  if (x->p.irs)
-    if (int rc = run_cirs(x, d)) return rc;
+  if (x->p.irs) {
+    if (int rc = run_cirs(x, d, upd, ak, bk)) return rc;
+    if (upd_done) *upd_done = upd;
+  }
   return UFX_OK;
 }
 
@@ -504,8 +510,10 @@ int run_stage(ufx_cc* x, DevT<R>& d, int stage, bool norm, bool write_res, bool 
     return run_stage_kernel(x, d, (norm ? 2 : 1) | ((whole ? stage : 0) << 2) |
                                       ((x->stage_rev >> stage) & 1 ? 16 : 0),
                             ak, bk, write_res ? 1 : 0, epi ? 1 : 0);
-  if (int rc = run_residual_full(x, d)) return rc;
-  if (int rc = run_rk_update(x, d, ak, bk)) return rc;
+  bool updated = false;
+  if (int rc = run_residual_full(x, d, true, ak, bk, &updated)) return rc;
+  if (!updated)
+    if (int rc = run_rk_update(x, d, ak, bk)) return rc;
   if (norm) {
     LAUNCH(x, UFX_K_NORM, k_sumsq<R>, x->nblk_cell, d);
     return launch_check();
