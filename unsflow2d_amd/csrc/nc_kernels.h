@@ -813,9 +813,21 @@ void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode
   const R* recs = NCRec<R>::rec(d);
   const R* erec = NCRec<R>::edge(d);
   const int e0 = d.be_ptr[blk], ne = d.be_ptr[blk + 1] - e0;
+  // each thread's next edge and its end points are fetched while the
+  // current edge's flux is evaluated
+  int e_nx = 0;
+  int2 pp_nx = make_int2(0, 0);
+  if ((int)threadIdx.x < ne) {
+    e_nx = d.be_edge[e0 + threadIdx.x];
+    pp_nx = d.ep[e_nx];
+  }
   for (int k = threadIdx.x; k < ne; k += BLK) {
-    const int e = d.be_edge[e0 + k];
-    const int2 pp = d.ep[e];
+    const int e = e_nx;
+    const int2 pp = pp_nx;
+    if (k + BLK < ne) {
+      e_nx = d.be_edge[e0 + k + BLK];
+      pp_nx = d.ep[e_nx];
+    }
     R a[W], b[W], er[4];
     ldv<R, W>(recs + (size_t)pp.x * W, a);
     ldv<R, W>(recs + (size_t)pp.y * W, b);
@@ -853,8 +865,7 @@ void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode
   const bool have = p < d.nop;
   double r[4] = {0.0, 0.0, 0.0, 0.0}, la = 0.0, dt = 0.0;
   if (have) {
-    for (int j = d.v2f_ptr[p]; j < d.v2f_ptr[p + 1]; ++j) {
-      const int loc = d.sl_loc[j];
+    auto add = [&](int loc) {
       const bool first = loc >= 0;
       const int k = first ? loc : ~loc;
       const double* o = s_f + (size_t)k * 5;
@@ -867,7 +878,17 @@ void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode
       }
       la = la + o[4];
       nc_add_closure(d, s_bi[k], first, r, la);
-    }
+    };
+    // the first NCU list positions loaded together, then the sums in order
+    const int j0 = d.v2f_ptr[p], n = d.v2f_ptr[p + 1] - j0;
+    int locs[NCU];
+#pragma unroll
+    for (int u = 0; u < NCU; ++u)
+      if (u < n) locs[u] = d.sl_loc[j0 + u];
+#pragma unroll
+    for (int u = 0; u < NCU; ++u)
+      if (u < n) add(locs[u]);
+    for (int j = j0 + NCU; j < j0 + n; ++j) add(d.sl_loc[j]);
     dt = nc_vertex_finish(d, p, r, la, cfl, mode, ak, bk);
   }
   __syncthreads();
