@@ -815,23 +815,29 @@ void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode
   const int e0 = d.be_ptr[blk], ne = d.be_ptr[blk + 1] - e0;
   // each thread's next edge and its end points are fetched while the
   // current edge's flux is evaluated
-  int e_nx = 0;
+  int e_nx = 0, bi_nx = 0;
   int2 pp_nx = make_int2(0, 0);
+  R er_nx[4] = {R(0), R(0), R(0), R(0)};
   if ((int)threadIdx.x < ne) {
     e_nx = d.be_edge[e0 + threadIdx.x];
     pp_nx = d.ep[e_nx];
+    ldv<R, 4>(erec + (size_t)e_nx * 4, er_nx);
+    bi_nx = d.ebc[e_nx];
   }
   for (int k = threadIdx.x; k < ne; k += BLK) {
-    const int e = e_nx;
+    const int e = e_nx, bi = bi_nx;
     const int2 pp = pp_nx;
+    R a[W], b[W], er[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) er[i] = er_nx[i];
+    ldv<R, W>(recs + (size_t)pp.x * W, a);
+    ldv<R, W>(recs + (size_t)pp.y * W, b);
     if (k + BLK < ne) {
       e_nx = d.be_edge[e0 + k + BLK];
       pp_nx = d.ep[e_nx];
+      ldv<R, 4>(erec + (size_t)e_nx * 4, er_nx);
+      bi_nx = d.ebc[e_nx];
     }
-    R a[W], b[W], er[4];
-    ldv<R, W>(recs + (size_t)pp.x * W, a);
-    ldv<R, W>(recs + (size_t)pp.y * W, b);
-    ldv<R, 4>(erec + (size_t)e * 4, er);
     const double nx = er[0], ny = er[1], ds = er[2];
     const double d0 = (double)b[R_CEN] - (double)a[R_CEN];
     const double d1 = (double)b[R_CEN + 1] - (double)a[R_CEN + 1];
@@ -858,7 +864,7 @@ void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode
 #pragma unroll
     for (int i = 0; i < 4; ++i) o[i] = F[i] * ds;
     o[4] = ll;
-    s_bi[k] = d.ebc[e];
+    s_bi[k] = bi;
     d.fla[e] = ll;
   }
   __syncthreads();
