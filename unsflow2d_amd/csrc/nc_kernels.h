@@ -46,6 +46,7 @@ struct DevNC {
   double *qc, *qold, *res, *la, *dt, *dd, *dqc;
   const double* cv;
   const double* erec;
+  const double* egg;         // [nof][2] nx ds, ny ds: the Green-Gauss pass's edge terms
   const int2* ep;
   const int* ebc;            // per edge: its index in the boundary-edge list, or -1
   // block-shared residual: per block of BLK vertices the edges they touch
@@ -66,6 +67,7 @@ struct DevNC {
   // the mixed form's records (FP32), used instead of vrec / erec / wxy
   float* vrec32;          // [nop][NW32]
   const float* erec32;    // [nof][4]
+  const float* egg32;     // [nof][2]
   const float* wxy32;     // [..][2]
   const float2* ggs32;    // [nop] sum over the vertex's dual faces of +-n ds (FP64 sum)
   const double2* pxy;     // vertex coordinates in FP64 (wall forces)
@@ -153,6 +155,7 @@ template <> struct NCRec<double> {
   __device__ static double* phc(const DevNC& d) { return d.vphc; }
   __device__ static const double* xy(const DevNC& d) { return d.vxy; }
   __device__ static const double* edge(const DevNC& d) { return d.erec; }
+  __device__ static const double* gg(const DevNC& d) { return d.egg; }
   __device__ static const double* lsq(const DevNC& d) { return d.wxy; }
 };
 template <> struct NCRec<float> {
@@ -162,6 +165,7 @@ template <> struct NCRec<float> {
   __device__ static float* phc(const DevNC& d) { return d.vphc32; }
   __device__ static const float* xy(const DevNC& d) { return d.vxy32; }
   __device__ static const float* edge(const DevNC& d) { return d.erec32; }
+  __device__ static const float* gg(const DevNC& d) { return d.egg32; }
   __device__ static const float* lsq(const DevNC& d) { return d.wxy32; }
 };
 
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g, int rev)
   const int cnt = min(BLK, d.nop - base);
   const R* qps = NCRec<R>::qp(d);
   const R* xys = NCRec<R>::xy(d);
-  const R* erec = NCRec<R>::edge(d);
+  const R* egg = NCRec<R>::gg(d);
   R q[4], xy[2];
   if (p < d.nop) {
     ldv<R, 4>(qps + (size_t)p * 4, q);
@@ -344,10 +348,11 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g, int rev)
     for_edges(d, j0, n, nb, [&](int2 eo) {
       bool first = eo.x >= 0;
       int e = first ? eo.x : ~eo.x, o = eo.y;
-      R qo[4], er[4];
+      R qo[4], dd[2];
       nbr_qp(o, qo);
-      ldv<R, 4>(erec + (size_t)e * 4, er);
-      R dx = er[0] * er[2], dy = er[1] * er[2];
+      // nx ds and ny ds, the products formed once when the edges were stored
+      ldv<R, 2>(egg + (size_t)e * 2, dd);
+      const R dx = dd[0], dy = dd[1];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         R var = first ? R(0.5) * (q[k] + qo[k]) : R(0.5) * (qo[k] + q[k]);

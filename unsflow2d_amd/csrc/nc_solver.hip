@@ -853,6 +853,14 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     for (int p = 0; p < nop; ++p)
       for (int k = 0; k < NW; ++k) vr32[(size_t)p * NW32 + k] = (float)vrec[(size_t)p * NW + k];
     for (size_t i = 0; i < erec.size(); ++i) er32[i] = (float)erec[i];
+    // the Green-Gauss terms nx ds, ny ds as the FP32 pass forms them (one
+    // single-precision product each)
+    std::vector<float> gg32((size_t)nof * 2);
+    for (int s = 0; s < nof; ++s) {
+      gg32[(size_t)s * 2] = er32[(size_t)s * 4] * er32[(size_t)s * 4 + 2];
+      gg32[(size_t)s * 2 + 1] = er32[(size_t)s * 4 + 1] * er32[(size_t)s * 4 + 2];
+    }
+    NUP(egg32, gg32);
     for (size_t i = 0; i < wxy.size(); ++i) w32[i] = (float)wxy[i];
     // primitives as deviations from the free stream (the state starts
     // there: zero), and the open Green-Gauss sum's +-n ds per vertex
@@ -878,6 +886,12 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     NUP(vqp32, q32); NUP(vphc32, p32); NUP(vxy32, x32);
   } else {
     NUP(vrec, vrec); NUP(erec, erec); NUP(wxy, wxy);
+    std::vector<double> gg((size_t)nof * 2);
+    for (int s = 0; s < nof; ++s) {
+      gg[(size_t)s * 2] = erec[(size_t)s * 4] * erec[(size_t)s * 4 + 2];
+      gg[(size_t)s * 2 + 1] = erec[(size_t)s * 4 + 1] * erec[(size_t)s * 4 + 2];
+    }
+    NUP(egg, gg);
     std::vector<double> q64((size_t)nop * 4, 0.0), p64((size_t)nop * 4, phi0), x64((size_t)nop * 2);
     for (int p = 0; p < nop; ++p) {
       x64[(size_t)p * 2] = m.px[p];
