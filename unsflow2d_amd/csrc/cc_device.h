@@ -18,6 +18,7 @@
 //                        the iteration's start state until the last stage)
 //   frec [nof][FW] R     face record nx, ny, area, centre x, y
 //   fcen [nof][2] R      face centre alone (for the gradient/limiter pass)
+//   fnl [nof][4] R       normal and length alone (for the face spectral radii)
 //   fpt  [nof] int2      face endpoints (storage numbering of points)
 //   prec [nop][PW] R     vertex record: averaged primitives rho,u,v,p,T, mu
 //   sl_face/sl_nbr [MAXF][noc] int  the faces of each cell in ascending
@@ -64,6 +65,7 @@ struct DevT {
   // mesh
   const R* frec;
   const R* fcen;             // [nof][2] face centres (the limiter's only face data)
+  const R* fnl;              // [nof][4] nx, ny, area, 0 (the face spectral radii's face data)
   const int2* fpt;
   const int* fbc;            // bc code per boundary ordinal (nbf)
   const int *bface, *bcell;  // storage face and inner cell per boundary ordinal

@@ -59,9 +59,12 @@ __device__ __forceinline__ void face_lambda_body(const DevT<R>& d, VBlock vb, Ga
                                                  double vll) {
   int f = vb.cell();
   if (f >= d.nof || d.err[0]) return;
-  R fq[4], fr[Layout<R>::FW];
+  // the normal and length from their compact copy (F_NX, F_NY, F_AREA are
+  // its first three values)
+  static_assert(F_NX == 0 && F_NY == 1 && F_AREA == 2, "compact face data layout");
+  R fq[4], fr[4];
   face_prims<R, 4>(d, f, fq);
-  face_rec(d, f, fr);
+  ldv<R, 4>(d.fnl + (size_t)f * 4, fr);
   d.fla[f] = face_lambda_val<R>(g, fq, fr, vll, vll != 0.0 ? d.fmu[f] : 0.0);
 }
 template <typename R>

@@ -794,7 +794,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   d.nop = nop; d.noc = noc; d.nfi = m.nfi; d.nbf = nbf; d.nof = nof; d.ncell = ncell;
   const std::vector<int>& sto = x->sto_of_ref;
   // faces in storage order
-  std::vector<R> frec((size_t)nof * FW, R(0)), fcen((size_t)nof * 2);
+  std::vector<R> frec((size_t)nof * FW, R(0)), fcen((size_t)nof * 2), fnl((size_t)nof * 4, R(0));
   std::vector<int2> fpt(nof);
   std::vector<int> fin(nof), fout(nof);
   for (int f = 0; f < nof; ++f) {
@@ -804,6 +804,9 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
     r[F_CX] = (R)m.fcx[f]; r[F_CY] = (R)m.fcy[f];
     fcen[(size_t)s * 2] = r[F_CX];
     fcen[(size_t)s * 2 + 1] = r[F_CY];
+    fnl[(size_t)s * 4] = r[F_NX];
+    fnl[(size_t)s * 4 + 1] = r[F_NY];
+    fnl[(size_t)s * 4 + 2] = r[F_AREA];
     fpt[s] = make_int2(x->pt_of_ref[m.fp1[f]], x->pt_of_ref[m.fp2[f]]);
     fin[s] = sto[m.fin[f]];
     fout[s] = sto[m.fout[f]];
@@ -875,7 +878,7 @@ int build_dev(ufx_cc* x, DevT<R>& d, const Mesh& m, const std::vector<int>& sl_f
   int rc;
 #define UP(dst, src) if ((rc = dupload(x, &d.dst, src))) return rc
 #define AL(dst, n) if ((rc = dalloc(x, &d.dst, (size_t)(n)))) return rc
-  UP(frec, frec); UP(fcen, fcen); UP(phc, phc); UP(cenc, cenc);
+  UP(frec, frec); UP(fcen, fcen); UP(fnl, fnl); UP(phc, phc); UP(cenc, cenc);
   {
     int2* p = nullptr;
     if ((rc = dupload(x, &p, fpt))) return rc;
