@@ -11,7 +11,7 @@
 //                       (d/dx, d/dy per variable) | x, y
 //   qc, qold, res [nop][4] f64 (conserved order E, rho, rho u, rho v)
 //   la, dt, cv [nop] f64;  fla [nof] f64
-//   erec [nof][4] f64   dual-face nx, ny, length, pad;  ep [nof] int2 (p1, p2);
+//   erec [nof][3] f64   dual-face nx, ny, length;  ep [nof] int2 (p1, p2);
 //                       edges stored in first-touch vertex order
 //   v2f [..] int        edges of a vertex, bit-inverted when it is p2
 //   v2fo [..] int2      the same with the vertex at the other end
@@ -66,7 +66,7 @@ struct DevNC {
   const int* vref;           // reference number of each stored vertex
   // the mixed form's records (FP32), used instead of vrec / erec / wxy
   float* vrec32;          // [nop][NW32]
-  const float* erec32;    // [nof][4]
+  const float* erec32;    // [nof][4] nx, ny, length, pad
   const float* egg32;     // [nof][2]
   const float* wxy32;     // [..][2]
   const float2* ggs32;    // [nop] sum over the vertex's dual faces of +-n ds (FP64 sum)
@@ -168,6 +168,18 @@ template <> struct NCRec<float> {
   __device__ static const float* gg(const DevNC& d) { return d.egg32; }
   __device__ static const float* lsq(const DevNC& d) { return d.wxy32; }
 };
+
+// An edge's nx, ny, length (and 0): three doubles per edge in FP64, four
+// floats (one 16-byte load) in FP32.
+template <typename R> __device__ __forceinline__ void ld_edge(const R* erec, int e, R* er);
+template <> __device__ __forceinline__ void ld_edge<double>(const double* erec, int e, double* er) {
+  const double* p = erec + (size_t)e * 3;
+  er[0] = p[0]; er[1] = p[1]; er[2] = p[2]; er[3] = 0.0;
+}
+template <> __device__ __forceinline__ void ld_edge<float>(const float* erec, int e, float* er) {
+  const float4 v = reinterpret_cast<const float4*>(erec)[e];
+  er[0] = v.x; er[1] = v.y; er[2] = v.z; er[3] = v.w;
+}
 
 template <typename R> __device__ __forceinline__ R nc_abs(R x);
 template <> __device__ __forceinline__ double nc_abs<double>(double x) { return fabs(x); }
@@ -655,7 +667,7 @@ void k_nc_residual(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode, do
     R ot[W];
     ldv<R, W>(op, ot);
     R er[4];
-    ldv<R, 4>(erec + (size_t)e * 4, er);
+    ld_edge<R>(erec, e, er);
     // reconstruction and flux in FP64 from the records (FP32 in the mixed
     // form: rounding absolute states to FP32 would put a noise floor of a
     // few 1e-7 under the residual).  a = p1's record, b = p2's: the edge
@@ -826,7 +838,7 @@ void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode
   if ((int)threadIdx.x < ne) {
     e_nx = d.be_edge[e0 + threadIdx.x];
     pp_nx = d.ep[e_nx];
-    ldv<R, 4>(erec + (size_t)e_nx * 4, er_nx);
+    ld_edge<R>(erec, e_nx, er_nx);
     bi_nx = d.ebc[e_nx];
   }
   for (int k = threadIdx.x; k < ne; k += BLK) {
@@ -840,7 +852,7 @@ void k_nc_residual_blk(DevNC d, NCFree g, Gas<double> ga64, double cfl, int mode
     if (k + BLK < ne) {
       e_nx = d.be_edge[e0 + k + BLK];
       pp_nx = d.ep[e_nx];
-      ldv<R, 4>(erec + (size_t)e_nx * 4, er_nx);
+      ld_edge<R>(erec, e_nx, er_nx);
       bi_nx = d.ebc[e_nx];
     }
     const double nx = er[0], ny = er[1], ds = er[2];
@@ -951,9 +963,9 @@ __device__ __forceinline__ void nc_lusgs_vertex(const DevNC& d, const NCFree& g,
     int e = first ? eo.x : ~eo.x, o = eo.y;
     double sign = first ? 1.0 : -1.0;
     if (dir == 0 ? d.vref[o] < d.vref[it] : d.vref[o] > d.vref[it]) {
-      double nx, ny, ds, pad, f1[4], f2[4], qo[4], qn[4], dq[4];
-      ld2d(d.erec + (size_t)e * 4, nx, ny);
-      ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
+      double f1[4], f2[4], qo[4], qn[4], dq[4];
+      const double nx = d.erec[(size_t)e * 3], ny = d.erec[(size_t)e * 3 + 1];
+      const double ds = d.erec[(size_t)e * 3 + 2];
       ld4d(d.qold + (size_t)o * 4, qo);
       ld4d(d.qc + (size_t)o * 4, qn);
       ld4d(d.dqc + (size_t)o * 4, dq);
@@ -1277,9 +1289,9 @@ __device__ __forceinline__ void nc_sweep_slot(const DevNC& d, const NCFree& g, c
       bool first = eo.x >= 0;
       int e = first ? eo.x : ~eo.x, o = eo.y;
       double sign = first ? 1.0 : -1.0;
-      double nx, ny, ds, pad, f1[4], f2[4], qo[4], qn[4], dq[4];
-      ld2d(d.erec + (size_t)e * 4, nx, ny);
-      ld2d(d.erec + (size_t)e * 4 + 2, ds, pad);
+      double f1[4], f2[4], qo[4], qn[4], dq[4];
+      const double nx = d.erec[(size_t)e * 3], ny = d.erec[(size_t)e * 3 + 1];
+      const double ds = d.erec[(size_t)e * 3 + 2];
       ld4d(d.qold + (size_t)o * 4, qo);
       ld4d(d.qc + (size_t)o * 4, qn);
       ld4d(d.dqc + (size_t)o * 4, dq);

@@ -885,7 +885,11 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
     }
     NUP(vqp32, q32); NUP(vphc32, p32); NUP(vxy32, x32);
   } else {
-    NUP(vrec, vrec); NUP(erec, erec); NUP(wxy, wxy);
+    // edge records packed to their three values
+    std::vector<double> er64((size_t)nof * 3);
+    for (int s = 0; s < nof; ++s)
+      for (int k = 0; k < 3; ++k) er64[(size_t)s * 3 + k] = erec[(size_t)s * 4 + k];
+    NUP(vrec, vrec); NUP(erec, er64); NUP(wxy, wxy);
     std::vector<double> gg((size_t)nof * 2);
     for (int s = 0; s < nof; ++s) {
       gg[(size_t)s * 2] = erec[(size_t)s * 4] * erec[(size_t)s * 4 + 2];
