@@ -46,7 +46,7 @@ constexpr int WALL = 1001, FARF = 2001;
 
 template <typename R> struct Layout;
 template <> struct Layout<double> { static constexpr int RW = 18, FW = 5, PW = 6; };
-template <> struct Layout<float> { static constexpr int RW = 20, FW = 6, PW = 6; };
+template <> struct Layout<float> { static constexpr int RW = 20, FW = 5, PW = 6; };
 // record offsets
 constexpr int R_QP = 0, R_PHI = 4, R_G = 8, R_CEN = 16;
 constexpr int F_NX = 0, F_NY = 1, F_AREA = 2, F_CX = 3, F_CY = 4;
@@ -147,12 +147,11 @@ __device__ __forceinline__ void ldv(const R* __restrict__ p, R* out) {
         out[4 * i] = v.x; out[4 * i + 1] = v.y; out[4 * i + 2] = v.z; out[4 * i + 3] = v.w;
       }
     }
-  } else if constexpr (sizeof(R) == 8) {
-    // records of an odd number of doubles (8-byte aligned): one load each
+  } else if constexpr (sizeof(R) == 8 || N % 2 != 0) {
+    // records of an odd number of values (aligned to one value): one load each
 #pragma unroll
     for (int i = 0; i < N; ++i) out[i] = p[i];
   } else {
-    static_assert(N % 2 == 0, "8-byte vector loads");
     const float2* q = reinterpret_cast<const float2*>(p);
 #pragma unroll
     for (int i = 0; i < N / 2; ++i) { float2 v = q[i]; out[2 * i] = v.x; out[2 * i + 1] = v.y; }
