@@ -356,6 +356,20 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g, int rev)
 #pragma unroll
   for (int k = 0; k < NCU; ++k)
     if (k < n) nb[k] = d.v2fo[j0 + k];
+  // the limiter's difference range over the edges, in edge order: in the
+  // Green-Gauss loop (the same neighbour values), else a loop of its own
+  const R eps = (R)NC_EPS_SP;
+  R dumax[4], dumin[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { dumin[k] = eps; dumax[k] = -eps; }
+  auto range = [&](bool first, const R* qo) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      R du = first ? qo[k] - q[k] : -(q[k] - qo[k]);
+      dumax[k] = dumax[k] > du ? dumax[k] : du;
+      dumin[k] = dumin[k] < du ? dumin[k] : du;
+    }
+  };
   if (GRAD == 1) {
     for_edges(d, j0, n, nb, [&](int2 eo) {
       bool first = eo.x >= 0;
@@ -376,6 +390,7 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g, int rev)
           gr[2 * k + 1] = gr[2 * k + 1] - var * dy;
         }
       }
+      range(first, qo);
     });
     if constexpr (sizeof(R) == 4) {
       // the free-stream part of the open Green-Gauss sum
@@ -404,22 +419,12 @@ __global__ __launch_bounds__(BLK) void k_nc_grad_lim(DevNC d, NCFree g, int rev)
       }
     }
   }
-  // difference range over the edges
-  const R eps = (R)NC_EPS_SP;
-  R dumax[4], dumin[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) { dumin[k] = eps; dumax[k] = -eps; }
-  for_edges(d, j0, n, nb, [&](int2 eo) {
-    bool first = eo.x >= 0;
-    R qo[4];
-    nbr_qp(eo.y, qo);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      R du = first ? qo[k] - q[k] : -(q[k] - qo[k]);
-      dumax[k] = dumax[k] > du ? dumax[k] : du;
-      dumin[k] = dumin[k] < du ? dumin[k] : du;
-    }
-  });
+  if (GRAD != 1)
+    for_edges(d, j0, n, nb, [&](int2 eo) {
+      R qo[4];
+      nbr_qp(eo.y, qo);
+      range(eo.x >= 0, qo);
+    });
   R phi[4];
   ldv<R, 4>(NCRec<R>::phc(d) + (size_t)p * 4, phi);
   const R kappa = (R)0.01f;
