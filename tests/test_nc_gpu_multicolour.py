@@ -87,3 +87,30 @@ def test_multicolour_20k_vertices_equals_oracle():
         orc.iteration_ordered(it, order)
     fol.check("qc it10", dev.get("qc"), orc.qc)
     fol.check("res it10", dev.get("res"), orc.res)
+
+
+def test_multicolour_iterations_keep_qold_for_readers():
+    """Whole multicolour iterations leave the old state by vertex to be
+    formed when something reads it: get("qold") is the state the last
+    iteration started from, and a routine-by-routine RK stage after them
+    (which reads qold) equals the same stage after an explicit save_old."""
+    from unsflow2d_amd import nc, nc_mesh
+    print("case: nc multicolour qold after whole iterations, o-grid 30x12 seed 4", flush=True)
+    m = nc.NCMesh.from_raw(nc_mesh.o_grid(30, 12, 4, stretch=1.2))
+    kw = dict(mach=0.5, aoa_deg=1.5, cfl_max=50.0, grad="gg")
+    a = nc.NCSolver(m, timemode="lusgs", lusgs_order="multicolour", **kw)
+    b = nc.NCSolver(m, timemode="lusgs", lusgs_order="multicolour", **kw)
+    a.iterate(4)
+    b.iterate(4)
+    start = a.get("qc")
+    a.iterate(1)
+    b.iterate(1)
+    assert np.array_equal(a.get("qold"), start)
+    # b: the stage straight after the iterations; a: after save_old of the
+    # same state
+    a.set("qc", start)
+    b.set("qc", start)
+    a.save_old()
+    b.rk_stage(1, 1.0)
+    a.rk_stage(1, 1.0)
+    assert np.array_equal(a.get("qc"), b.get("qc"))

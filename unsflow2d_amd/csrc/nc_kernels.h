@@ -187,13 +187,16 @@ template <> __device__ __forceinline__ float nc_abs<float>(float x) { return fab
 
 // save_old, NCFVM/common.f90:88-104: primitives of every vertex (the only
 // primitive refresh of an iteration, from the FP64 state) and qold = qc.
+// wq = 0 (whole multicolour LU-SGS iterations, whose sweeps read the old
+// state by colour position only): qold by vertex is left to
+// k_nc_qold_from_mc, run when something else reads it.
 template <typename R>
-__global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
+__global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g, int wq) {
   int p = cell_index();
   if (p >= d.nop) return;
   double q[4];
   ld4d(d.qc + (size_t)p * 4, q);
-  st4d(d.qold + (size_t)p * 4, q);
+  if (wq) st4d(d.qold + (size_t)p * 4, q);
   // the multicolour sweeps read the old state by colour position
   if (d.mc_pos) st4d(d.mc_qold + (size_t)d.mc_pos[p] * 4, q);
   double rho = q[1], u = q[2] / q[1], v = q[3] / q[1], et = q[0];
@@ -207,6 +210,15 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g) {
     o[3] = (R)(pr - g.p_inf);
   }
   stv<R, 4>(NCRec<R>::qp(d) + (size_t)p * 4, o);
+}
+
+// qold by vertex from its copy by colour position.
+__global__ __launch_bounds__(BLK) void k_nc_qold_from_mc(DevNC d) {
+  int p = cell_index();
+  if (p >= d.nop) return;
+  double q[4];
+  ld4d(d.mc_qold + (size_t)d.mc_pos[p] * 4, q);
+  st4d(d.qold + (size_t)p * 4, q);
 }
 
 // The boundary closures of one wall or farfield edge, for both of its ends

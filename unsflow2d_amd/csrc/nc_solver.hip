@@ -32,6 +32,8 @@ struct ufx_nc {
   Gas<double> ga{};  // for the AUSM+-up option (mref as CCFVM/ausm_up_flux.F90:79)
   Gas<float> gaf{};  // the same for the mixed form
   bool mixed = false;  // FP32 records, gradients, limiter and fluxes
+  bool qold_pending = false;  // the last save_old (of a whole multicolour LU-SGS
+                              // iteration) left qold by vertex to mc_qold
   DevNC d{};
   int nop = 0, nof = 0, nblk = 0;
   hipStream_t stream = nullptr;
@@ -133,8 +135,8 @@ template <> const Gas<double>& nc_gas<double>(const ufx_nc* x) { return x->ga; }
 template <> const Gas<float>& nc_gas<float>(const ufx_nc* x) { return x->gaf; }
 
 template <typename R>
-int run_save_old_r(ufx_nc* x) {
-  NLAUNCH(x, NK_SAVE, k_nc_save_old<R>, x->nblk, x->d, x->g);
+int run_save_old_r(ufx_nc* x, int wq) {
+  NLAUNCH(x, NK_SAVE, k_nc_save_old<R>, x->nblk, x->d, x->g, wq);
   return launch_ok();
 }
 
@@ -172,8 +174,21 @@ int run_fvresidual_r(ufx_nc* x, double cfl, int mode, double ak, double bk) {
   return launch_ok();
 }
 
-int run_save_old(ufx_nc* x) {
-  return x->mixed ? run_save_old_r<float>(x) : run_save_old_r<double>(x);
+// whole: the save_old of a whole iteration; in multicolour LU-SGS the sweeps
+// read the old state by colour position only, so qold by vertex is written
+// when something reads it (restore_qold)
+int run_save_old(ufx_nc* x, bool whole = false) {
+  const bool skip = whole && x->prm.timemode != UFX_TIME_RK3 &&
+                    x->prm.lusgs_order == UFX_LUSGS_MULTICOLOUR;
+  x->qold_pending = skip;
+  return x->mixed ? run_save_old_r<float>(x, skip ? 0 : 1) : run_save_old_r<double>(x, skip ? 0 : 1);
+}
+
+int restore_qold(ufx_nc* x) {
+  if (!x->qold_pending) return UFX_OK;
+  x->qold_pending = false;
+  NLAUNCH(x, NK_SAVE, k_nc_qold_from_mc, x->nblk, x->d);
+  return launch_ok();
 }
 
 int run_fvresidual(ufx_nc* x, double cfl, int mode, double ak, double bk) {
@@ -965,12 +980,14 @@ int ufx_nc_rk_stage(ufx_nc* x, int stage, double cfl) {
   static const double airk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
   static const double birk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
   if (stage < 1 || stage > 3) { set_error("stage must be 1..3"); return UFX_ERR_ARG; }
+  if (int rc = restore_qold(x)) return rc;
   x->cfl = cfl;
   return run_fvresidual(x, cfl, 1, airk[stage - 1], birk[stage - 1]);
 }
 
 int ufx_nc_lusgs(ufx_nc* x, double cfl) {
   if (int rc = check(x)) return rc;
+  if (int rc = restore_qold(x)) return rc;
   x->cfl = cfl;
   return run_lusgs(x, cfl);
 }
@@ -1021,7 +1038,7 @@ int ufx_nc_iterate(ufx_nc* x, int iter0, int niter, double* hist) {
     int iter = iter0 + k + 1;
     double cfl = nc_cfl(x->prm.cfl_max, iter);
     x->cfl = cfl;
-    if (int rc = run_save_old(x)) return rc;
+    if (int rc = run_save_old(x, true)) return rc;
     if (x->prm.timemode == UFX_TIME_RK3) {
       for (int s = 1; s <= 3; ++s)
         if (int rc = ufx_nc_rk_stage(x, s, cfl)) return rc;
@@ -1052,6 +1069,7 @@ int ufx_nc_set(ufx_nc* x, const char* name, const double* src) {
     NCCHK(hipMemcpyAsync(n == "qc" ? x->d.qc : x->d.qold, t.data(), sizeof(double) * 4 * nop,
                          hipMemcpyHostToDevice, x->stream));
     NCCHK(hipStreamSynchronize(x->stream));
+    if (n == "qold") x->qold_pending = false;
     if (n == "qold" && !x->mc_pos.empty()) {
       // the multicolour sweeps' copy by colour position
       std::vector<double> u(4 * (size_t)nop);
@@ -1096,7 +1114,10 @@ int ufx_nc_get(ufx_nc* x, const char* name, double* dst) {
     return UFX_OK;
   };
   if (n == "qc") return vertex_array(x->d.qc, 4);
-  if (n == "qold") return vertex_array(x->d.qold, 4);
+  if (n == "qold") {
+    if (int rc = restore_qold(x)) return rc;
+    return vertex_array(x->d.qold, 4);
+  }
   if (n == "res") return vertex_array(x->d.res, 4);
   if (n == "la") return vertex_array(x->d.la, 1);
   if (n == "dt") return vertex_array(x->d.dt, 1);
