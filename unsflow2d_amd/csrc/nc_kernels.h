@@ -212,6 +212,18 @@ __global__ __launch_bounds__(BLK) void k_nc_save_old(DevNC d, NCFree g, int wq) 
   stv<R, 4>(NCRec<R>::qp(d) + (size_t)p * 4, o);
 }
 
+// The residual and the LU-SGS diagonal by vertex from their copies by colour
+// position.
+__global__ __launch_bounds__(BLK) void k_nc_res_from_mc(DevNC d) {
+  int p = cell_index();
+  if (p >= d.nop) return;
+  const size_t s = (size_t)d.mc_pos[p];
+  double r[4];
+  ld4d(d.mc_res + s * 4, r);
+  st4d(d.res + (size_t)p * 4, r);
+  d.dd[p] = d.mc_dd[s];
+}
+
 // qold by vertex from its copy by colour position.
 __global__ __launch_bounds__(BLK) void k_nc_qold_from_mc(DevNC d) {
   int p = cell_index();
@@ -614,7 +626,10 @@ __device__ __forceinline__ void nc_add_closure(const DevNC& d, int bi, bool firs
 // LU-SGS diagonal D = V/dt + omega/2 lambda (omega = 1.5,
 // NCFVM/lusgs.f90:4-131; the multicolour sweeps read it once per vertex
 // instead of gathering its three inputs), the residual, and for mode 1 the
-// RK stage update of NCFVM/main.f90:59-70 in place.  Returns dt.
+// RK stage update of NCFVM/main.f90:59-70 in place.  Mode 2 (multicolour
+// LU-SGS) stores residual and diagonal by colour position only
+// (k_nc_res_from_mc forms them by vertex when something reads them).
+// Returns dt.
 __device__ __forceinline__ double nc_vertex_finish(const DevNC& d, int p, const double* r,
                                                    double la, double cfl, int mode, double ak,
                                                    double bk) {
@@ -622,13 +637,14 @@ __device__ __forceinline__ double nc_vertex_finish(const DevNC& d, int p, const 
   const double dt = cfl * cvp / la;
   d.la[p] = la;
   d.dt[p] = dt;
-  if (mode == 0) d.dd[p] = cvp / dt + 0.5 * 1.5 * la;
-  st4d(d.res + (size_t)p * 4, r);
-  if (d.mc_pos && mode == 0) {
+  const double ddv = cvp / dt + 0.5 * 1.5 * la;
+  if (mode == 0) d.dd[p] = ddv;
+  if (mode != 2) st4d(d.res + (size_t)p * 4, r);
+  if (d.mc_pos && mode != 1) {
     // the multicolour sweeps read residual and diagonal by colour position
     const size_t s = (size_t)d.mc_pos[p];
     st4d(d.mc_res + s * 4, r);
-    d.mc_dd[s] = d.dd[p];
+    d.mc_dd[s] = ddv;
   }
   if (mode == 1) {
     double dtbyarea = dt / cvp, q[4], o[4];

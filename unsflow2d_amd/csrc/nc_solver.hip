@@ -34,6 +34,8 @@ struct ufx_nc {
   bool mixed = false;  // FP32 records, gradients, limiter and fluxes
   bool qold_pending = false;  // the last save_old (of a whole multicolour LU-SGS
                               // iteration) left qold by vertex to mc_qold
+  bool res_pending = false;   // the last residual (of a multicolour LU-SGS step)
+                              // left res and dd by vertex to mc_res, mc_dd
   DevNC d{};
   int nop = 0, nof = 0, nblk = 0;
   hipStream_t stream = nullptr;
@@ -157,6 +159,7 @@ int run_grad_lim_r(ufx_nc* x) {
 template <typename R>
 int run_fvresidual_r(ufx_nc* x, double cfl, int mode, double ak, double bk) {
   if (int rc = run_grad_lim_r<R>(x)) return rc;
+  x->res_pending = mode == 2;
   const Gas<double>& ga = x->ga;
   if (x->res_lds > 0) {
     const size_t lds = x->res_lds;
@@ -182,6 +185,13 @@ int run_save_old(ufx_nc* x, bool whole = false) {
                     x->prm.lusgs_order == UFX_LUSGS_MULTICOLOUR;
   x->qold_pending = skip;
   return x->mixed ? run_save_old_r<float>(x, skip ? 0 : 1) : run_save_old_r<double>(x, skip ? 0 : 1);
+}
+
+int restore_res(ufx_nc* x) {
+  if (!x->res_pending) return UFX_OK;
+  x->res_pending = false;
+  NLAUNCH(x, NK_NORM, k_nc_res_from_mc, x->nblk, x->d);
+  return launch_ok();
 }
 
 int restore_qold(ufx_nc* x) {
@@ -229,8 +239,9 @@ int build_sweep_graph(ufx_nc* x) {
 }
 
 int run_lusgs(ufx_nc* x, double cfl) {
-  if (int rc = run_fvresidual(x, cfl, 0, 0.0, 0.0)) return rc;
-  if (x->prm.lusgs_order == UFX_LUSGS_MULTICOLOUR) {
+  const bool mc = x->prm.lusgs_order == UFX_LUSGS_MULTICOLOUR;
+  if (int rc = run_fvresidual(x, cfl, mc ? 2 : 0, 0.0, 0.0)) return rc;
+  if (mc) {
     // one launch per colour: forward in ascending, reverse in descending order
     const int nc = (int)x->mc_off.size() - 1;
     for (int dir = 0; dir < 2; ++dir)
@@ -994,6 +1005,7 @@ int ufx_nc_lusgs(ufx_nc* x, double cfl) {
 
 int ufx_nc_residue(ufx_nc* x, int iter, double* out) {
   if (int rc = check(x)) return rc;
+  if (int rc = restore_res(x)) return rc;
   if (int rc = ensure_hist(x, 1)) return rc;
   if (int rc = run_norm(x, iter, 0, true)) return rc;
   double h[5];
@@ -1118,7 +1130,10 @@ int ufx_nc_get(ufx_nc* x, const char* name, double* dst) {
     if (int rc = restore_qold(x)) return rc;
     return vertex_array(x->d.qold, 4);
   }
-  if (n == "res") return vertex_array(x->d.res, 4);
+  if (n == "res") {
+    if (int rc = restore_res(x)) return rc;
+    return vertex_array(x->d.res, 4);
+  }
   if (n == "la") return vertex_array(x->d.la, 1);
   if (n == "dt") return vertex_array(x->d.dt, 1);
   if (n == "fc_la") {
