@@ -279,7 +279,10 @@ int ufx_cc_forces(ufx_cc* ctx, double* out, double* cp, double* cf);
  * Inviscid RK3 iterations without residual smoothing keep the iteration's
  * start state in "qc" through the stages (the stage states live in a
  * separate buffer) instead of copying it to "qold": after them "qold" still
- * holds what the last save_old (or LU-SGS / viscous iteration) stored. */
+ * holds what the last save_old (or LU-SGS / viscous iteration) stored.
+ * They store no "la" / "dt" either: those of the last time step are formed
+ * from the face radii when get, set, forces or LU-SGS read them (the same
+ * values bit for bit). */
 int ufx_cc_iterate(ufx_cc* ctx, int iter0, int niter, const double* cfl, double* hist);
 
 /* The same iterations replayed from a HIP graph (no reference counterpart;

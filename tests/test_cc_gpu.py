@@ -97,6 +97,9 @@ def test_iterations_match_oracle(flux, ilimit):
     _check("history", hist, ref, 1e-11)
     _check("final qc", dev.get("qc"), orc.qc, 1e-11)
     _check("final phi", dev.get("phi"), orc.phi, 1e-11)
+    # whole iterations leave la and dt to be formed when read
+    _check("final dt", dev.get("dt"), orc.dt, 1e-11)
+    _check("final la", dev.get("la"), orc.la, 1e-11)
 
 
 def test_first_order_start():
@@ -353,6 +356,9 @@ def test_iterate_graph_replays_match_eager(precision):
     a.iterate(10, hist=False)
     b.sync()
     assert np.array_equal(a.get("qc"), b.get("qc"))
+    # the time step of the last replayed iteration, formed when read
+    assert np.array_equal(a.get("dt"), b.get("dt"))
+    assert np.array_equal(a.get("la"), b.get("la"))
 
 
 def test_kept_residual_and_its_norms():

@@ -337,9 +337,12 @@ enum { G_NONE = 0, G_LSQR = 1, G_GG = 2, G_GGFC = 3 };
 //             GGFC: :159-323 cell part, sum of face gradients x co-area / cov;
 //   limiter   Venkatakrishnan with the ratchet, CCFVM/limiter.F90:123-296;
 //   time step spectral02 + time_step02, CCFVM/timestep.F90:98-177.
+// wdt = 0 (whole RK3 iterations, whose stages read only dt/A): the
+// spectral radius and dt are not stored; the same pass with GRAD = G_NONE,
+// TSTEP forms them from the face radii when something reads them.
 template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
 __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<double> g, double vll,
-                                          double cfl) {
+                                          double cfl, int wdt = 1) {
   constexpr int RW = Layout<R>::RW, FW = Layout<R>::FW;
   // the primitives, centroids (and temperatures) of the block's cells are
   // staged in LDS: most neighbours lie in the same block
@@ -539,15 +542,18 @@ __device__ __forceinline__ void cell_body(const DevT<R>& d, VBlock vb, Gas<doubl
   }
   if (LIMIT) stv<R, 4>(d.phc + (size_t)c * 4, own + R_PHI);
   if (TSTEP) {
-    d.la[c] = la;
     double dt = cfl * d.cv[c] / la;
-    d.dt[c] = dt;
+    if (wdt) {
+      d.la[c] = la;
+      d.dt[c] = dt;
+    }
     d.dta[c] = dt / d.cv[c];
   }
 }
 template <typename R, int GRAD, bool WITH_T, bool LIMIT, bool TSTEP>
-__global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl) {
-  cell_body<R, GRAD, WITH_T, LIMIT, TSTEP>(d, hw_block(), g, vll, cfl);
+__global__ __launch_bounds__(BLK) void k_cell(DevT<R> d, Gas<double> g, double vll, double cfl,
+                                              int wdt) {
+  cell_body<R, GRAD, WITH_T, LIMIT, TSTEP>(d, hw_block(), g, vll, cfl, wdt);
 }
 
 // ------------------------------------------------ diamond-path face gradient

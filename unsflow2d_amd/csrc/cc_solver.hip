@@ -97,6 +97,8 @@ struct ufx_cc {
   Chunk chunk;
   bool keep_res = false;           // whole iterations also store the last residual
   bool qold_stale = false;         // fused RK3 iterations ran since qold was last written
+  bool dtla_pending = false;       // fused RK3 iterations left la and dt unstored (formed
+  double dtla_cfl = 0.0;           // from the face radii at this CFL when read)
   bool prims_next = false;         // qpc/qTc hold the next iteration's primitives (written by
                                    // the last stage of a whole iteration; see prim_epilogue)
   bool iter_fusion = true;         // whole RK3 iterations refresh the primitives in their last stage
@@ -368,15 +370,20 @@ int run_avg_range(ufx_cc* x, DevT<R>& d, int nparts) {
 // prec gathers' HBM traffic, the vertex averages no longer staying in L2
 // beside k_cell's other streams: same time, 1.36 instead of 1.09 GB.)
 template <typename R, int GR>
-int run_cell_g(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl) {
+int run_cell_g(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl,
+               int wdt = 1) {
   double vll = x->p.vll;
+  if (tstep) {
+    x->dtla_pending = !wdt;
+    x->dtla_cfl = cfl;
+  }
   int cat = GR != G_NONE ? UFX_K_GRAD : (tstep ? UFX_K_TSTEP : UFX_K_GRAD);
   if (tstep) {
     LAUNCH(x, UFX_K_TSTEP, k_face_lambda<R>, x->nblk_face, d, x->g, vll);
     if (int rc = launch_check()) return rc;
   }
 #define KC(WT, LM, TS) \
-  LAUNCH(x, cat, (k_cell<R, GR, WT, LM, TS>), x->nblk_cell, d, x->g, vll, cfl)
+  LAUNCH(x, cat, (k_cell<R, GR, WT, LM, TS>), x->nblk_cell, d, x->g, vll, cfl, wdt)
   if (with_t) {
     if (limit) { if (tstep) KC(true, true, true); else KC(true, true, false); }
     else { if (tstep) KC(true, false, true); else KC(true, false, false); }
@@ -398,14 +405,27 @@ int run_ggfc_face(ufx_cc* x, DevT<R>& d) {
 // and the time step (the reference calls limit from inside each gradient
 // routine when ILIMIT is set).
 template <typename R>
-int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl) {
+int run_gradient(ufx_cc* x, DevT<R>& d, bool with_t, bool limit, bool tstep, double cfl,
+                 int wdt = 1) {
   switch (x->p.grad) {
-    case UFX_GRAD_GG: return run_cell_g<R, G_GG>(x, d, with_t, limit, tstep, cfl);
+    case UFX_GRAD_GG: return run_cell_g<R, G_GG>(x, d, with_t, limit, tstep, cfl, wdt);
     case UFX_GRAD_GGFC:
       if (int rc = run_ggfc_face(x, d)) return rc;
-      return run_cell_g<R, G_GGFC>(x, d, with_t, limit, tstep, cfl);
-    default: return run_cell_g<R, G_LSQR>(x, d, with_t, limit, tstep, cfl);
+      return run_cell_g<R, G_GGFC>(x, d, with_t, limit, tstep, cfl, wdt);
+    default: return run_cell_g<R, G_LSQR>(x, d, with_t, limit, tstep, cfl, wdt);
   }
+}
+
+// la and dt of the last time step, when fused iterations left them unstored:
+// the time-step pass alone, from the face radii (which nothing has changed
+// since) at that step's CFL, gives them bit for bit.
+template <typename R>
+int restore_dtla(ufx_cc* x, DevT<R>& d) {
+  if (!x->dtla_pending) return UFX_OK;
+  x->dtla_pending = false;
+  LAUNCH(x, UFX_K_TSTEP, (k_cell<R, G_NONE, false, false, true>), x->nblk_cell, d, x->g, x->p.vll,
+         x->dtla_cfl, 1);
+  return launch_check();
 }
 
 template <typename R>
@@ -588,7 +608,7 @@ int run_iteration(ufx_cc* x, DevT<R>& d, int iter, int k) {
     return rc;
   // gradient, limiter and time step in one pass over the cells; the
   // temperature gradient only when the viscous flux needs it
-  if (int rc = run_gradient(x, d, visc, lim, true, x->cfl)) return rc;
+  if (int rc = run_gradient(x, d, visc, lim, true, x->cfl, fused ? 0 : 1)) return rc;
   if (x->p.timemode == UFX_TIME_RK3) {
     for (int s = 1; s <= 3; ++s)
       if (int rc = run_stage(x, d, s, s == 3, s == 3 && x->keep_res, fused, s == 3 && epi))
@@ -623,6 +643,7 @@ int get_field(ufx_cc* x, DevT<R>& d, const std::string& n, double* dst) {
     return UFX_OK;
   }
   if (n == "dt" || n == "la") {
+    if (int rc = restore_dtla(x, d)) return rc;
     std::vector<double> t(noc);
     HIPCHK(copy(t.data(), n == "dt" ? d.dt : d.la, (size_t)noc * 8));
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -713,6 +734,8 @@ int set_field(ufx_cc* x, DevT<R>& d, const std::string& n, const double* src) {
     return UFX_OK;
   }
   if (n == "dt" || n == "la") {
+    // the other of the two stored first, if fused iterations left them
+    if (int rc = restore_dtla(x, d)) return rc;
     std::vector<double> t(noc);
     for (int c = 0; c < noc; ++c) t[sto[c]] = src[c];
     HIPCHK(up(n == "dt" ? d.dt : d.la, t.data(), (size_t)noc * 8));
@@ -1317,7 +1340,10 @@ int ufx_cc_rk_stage(ufx_cc* x, int stage) {
 int ufx_cc_lusgs(ufx_cc* x) {
   if (int rc = check_routine(x)) return rc;
   if (int rc = check_qold(x, "ufx_cc_lusgs")) return rc;
-  return with_dev(x, [&](auto& d) { return run_lusgs(x, d); });
+  return with_dev(x, [&](auto& d) {
+    if (int rc = restore_dtla(x, d)) return rc;
+    return run_lusgs(x, d);
+  });
 }
 
 int ufx_cc_residue(ufx_cc* x, int iter, double* fres, double* fres_raw) {
@@ -1379,6 +1405,7 @@ int ufx_cc_forces(ufx_cc* x, double* out, double* cp, double* cf) {
   int rc = with_dev(x, [&](auto& d) {
     using R = typename std::remove_reference_t<decltype(d)>::Real;
     constexpr int RW = Layout<R>::RW;
+    if (int r = restore_dtla(x, d)) return r;
     LAUNCH(x, UFX_K_OTHER, k_dtmin<R>, 1, d);
     if (int r = launch_check()) return r;
     std::vector<R> rec((size_t)x->ncell * RW);
@@ -1517,6 +1544,10 @@ int ufx_cc_iterate_graph(ufx_cc* x, int iter0, int niter, double cfl, double* hi
   }
   (void)iter0;
   x->qold_stale = x->p.timemode == UFX_TIME_RK3 && x->p.flow == UFX_FLOW_INVISCID && !x->p.irs;
+  // replayed iterations store no la / dt either (their captured time-step
+  // pass ran at the chunk's CFL)
+  x->dtla_pending = x->qold_stale;
+  x->dtla_cfl = cfl;
   if (hist) {
     HIPCHK(hipMemcpyAsync(hist, x->d_hist, (size_t)niter * 8, hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
