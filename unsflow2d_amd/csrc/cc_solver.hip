@@ -1976,6 +1976,7 @@ int batch_iterate(ufx_cc_batch* b, int niter, double cfl) {
     b->cs[i]->cfl = cfl;
     b->cs[i]->qold_stale = true;  // batched cases run the fused RK3 iteration
     b->cs[i]->prims_next = true;  // and its last stage refreshes the primitives
+    b->cs[i]->dtla_pending = false;  // its time-step pass stores la and dt
     if (err[3 * j] != 0) {
       int rc = device_error(b->cs[i]);
       set_error("case " + std::to_string(i) + ": " + ufx_last_error());
