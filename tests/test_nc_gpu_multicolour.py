@@ -114,3 +114,21 @@ def test_multicolour_iterations_keep_qold_for_readers():
     b.rk_stage(1, 1.0)
     a.rk_stage(1, 1.0)
     assert np.array_equal(a.get("qc"), b.get("qc"))
+
+
+def test_fused_first_and_last_colour_equal_separate_launches(monkeypatch):
+    """The first colour's forward update formed by the residual pass and the
+    last colour's reverse update by its forward launch (neither has
+    neighbours to sum) give the same iterates bit for bit as a launch per
+    colour and direction (UFX_NC_FUSE_COLOURS=0)."""
+    from unsflow2d_amd import nc, nc_mesh
+    print("case: nc multicolour fused colour launches, o-grid 40x16 seed 5", flush=True)
+    m = nc.NCMesh.from_raw(nc_mesh.o_grid(40, 16, 5, stretch=1.15))
+    kw = dict(timemode="lusgs", lusgs_order="multicolour", mach=0.45, cfl_max=50.0, grad="gg")
+    a = nc.NCSolver(m, **kw)
+    monkeypatch.setenv("UFX_NC_FUSE_COLOURS", "0")
+    b = nc.NCSolver(m, **kw)
+    ha, hb = a.iterate(12), b.iterate(12)
+    assert np.array_equal(ha, hb)
+    for name in ("qc", "res", "qold"):
+        assert np.array_equal(a.get(name), b.get(name)), name

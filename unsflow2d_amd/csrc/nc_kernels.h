@@ -107,6 +107,8 @@ struct DevNC {
   double* mc_dq;          // [nop][4] dq by position
   double* mc_dd;          // [nop] diagonal by position (the residual pass writes it)
   int mc_q;
+  int mc_n0;              // positions of the first colour, whose forward dq the residual
+                          // pass forms (0: the sweep launch does)
 };
 
 __device__ __forceinline__ void ld2d(const double* p, double& a, double& b) {
@@ -645,6 +647,14 @@ __device__ __forceinline__ double nc_vertex_finish(const DevNC& d, int p, const 
     const size_t s = (size_t)d.mc_pos[p];
     st4d(d.mc_res + s * 4, r);
     d.mc_dd[s] = ddv;
+    if (mode == 2 && s < (size_t)d.mc_n0) {
+      // the first colour has no earlier neighbours: its forward dq is that
+      // of k_nc_lusgs_mc with an empty sum, formed here
+      double dq[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dq[i] = (-r[i] - 0.0) / ddv;
+      st4d(d.mc_dq + s * 4, dq);
+    }
   }
   if (mode == 1) {
     double dtbyarea = dt / cvp, q[4], o[4];
@@ -1046,9 +1056,12 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_level(DevNC d, NCFree g, const
 // Reverse (dir 1, colours descending): the neighbours of higher colour.  A
 // neighbour's state is rebuilt as qold + dq, the sum that defines it, so the
 // arithmetic is the reference loop's, term for term, in that vertex order.
+// fin (forward sweep of the last colour, which has no later neighbours):
+// also the reverse sweep's update of these vertices, whose sum is empty,
+// qc = qold + dq.
 constexpr int MCU = 8;
 __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, int n, int dir,
-                                                     int rev) {
+                                                     int rev, int fin) {
   const int t = xcd_block_dir(blockIdx.x, gridDim.x, rev != 0) * BLK + threadIdx.x;
   if (t >= n) return;
   const int np = d.nop, s = s0 + t, it = d.mc_vert[s];
@@ -1100,6 +1113,13 @@ __global__ __launch_bounds__(BLK) void k_nc_lusgs_mc(DevNC d, NCFree g, int s0, 
     ld4d(d.mc_res + (size_t)s * 4, res);
 #pragma unroll
     for (int i = 0; i < 4; ++i) dq[i] = (-res[i] - cres[i]) / dd;
+    if (fin) {
+      double qo[4], q[4];
+      ld4d(d.mc_qold + (size_t)s * 4, qo);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) q[i] = qo[i] + dq[i];
+      st4d(d.qc + (size_t)it * 4, q);
+    }
   } else {
     dd = d.mc_dd[s];
     ld4d(d.mc_dq + (size_t)s * 4, dq);

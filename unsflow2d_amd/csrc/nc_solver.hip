@@ -242,15 +242,22 @@ int run_lusgs(ufx_nc* x, double cfl) {
   const bool mc = x->prm.lusgs_order == UFX_LUSGS_MULTICOLOUR;
   if (int rc = run_fvresidual(x, cfl, mc ? 2 : 0, 0.0, 0.0)) return rc;
   if (mc) {
-    // one launch per colour: forward in ascending, reverse in descending order
+    // one launch per colour: forward in ascending, reverse in descending
+    // order.  With two colours or more, the first colour's forward dq came
+    // from the residual pass (mc_n0), and the last colour's forward launch
+    // also does its reverse update (fin): neither has neighbours to sum.
     const int nc = (int)x->mc_off.size() - 1;
+    const bool fuse = x->d.mc_n0 > 0;
     for (int dir = 0; dir < 2; ++dir)
       for (int k = 0; k < nc; ++k) {
         const int c = dir == 0 ? k : nc - 1 - k, s0 = x->mc_off[c], n = x->mc_off[c + 1] - s0;
+        if (fuse && ((dir == 0 && c == 0) || (dir == 1 && c == nc - 1))) continue;
+        const int fin = fuse && dir == 0 && c == nc - 1 ? 1 : 0;
         // consecutive colour launches walk their positions in opposite
         // directions (when serpentine), each starting where the last ended
         const int rev = x->serpentine ? (dir * nc + k) & 1 : 0;
-        if (n > 0) NLAUNCH(x, NK_LUSGS, k_nc_lusgs_mc, nblk_of(n), x->d, x->g, s0, n, dir, rev);
+        if (n > 0)
+          NLAUNCH(x, NK_LUSGS, k_nc_lusgs_mc, nblk_of(n), x->d, x->g, s0, n, dir, rev, fin);
       }
     return launch_ok();
   }
@@ -853,6 +860,10 @@ int ufx_nc_create(const ufx_mesh* mesh, const ufx_nc_params* prm, ufx_nc** out) 
         }
       }
       x->d.mc_q = q;
+      // with two colours or more the residual pass forms the first colour's
+      // forward dq (UFX_NC_FUSE_COLOURS=0: the sweep launches do)
+      const char* fz = std::getenv("UFX_NC_FUSE_COLOURS");
+      x->d.mc_n0 = (ncol >= 2 && !(fz && std::string(fz) == "0")) ? x->mc_off[1] : 0;
       if (!rc) rc = nc_upload(x, &x->d.mc_vert, mv);
       if (!rc) rc = nc_upload(x, &x->d.mc_cnt, cnt);
       if (!rc) rc = nc_upload(x, &x->d.mc_eo, eo);
