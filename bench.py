@@ -329,7 +329,9 @@ def main_nc(args) -> int:
     cpu = None if args.no_cpu else nc_cpu_baseline(args.cpu_budget)
     flux_name = {"vanleer": "van Leer", "ausm": "AUSM+-up"}[args.nc_flux]
     lim_name = {"vkn": "Venkatakrishnan", "bj": "Barth-Jespersen"}[args.nc_limiter]
-    sweep = ("multicolour: one launch per colour per sweep, %d colours" % ncol
+    sweep = ("multicolour: one launch per colour per sweep, %d colours; the first colour's "
+             "forward update in the residual pass, the last colour's reverse update in its "
+             "forward launch" % ncol
              if args.lusgs_order == "multicolour" else
              "level-scheduled, reference order, one workgroup per sweep")
     line = {
